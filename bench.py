@@ -1,0 +1,207 @@
+"""Throughput of the derivative-carrying forward pass + exact adjoint on one GPU.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+A step is one pass of the hot path over the workload's whole batch, inputs
+resident in HBM: input layer, forward (linear map + Faa di Bruno step per
+layer), collocation seed of the mean-square residual loss, adjoint
+(pair sums, weight-gradient GEMM, transposed linear map per layer) and
+accumulation of the batch gradient.  It runs once in FP64 (the drop-in
+parity mode) and once per mixed-precision mode; the headline value is the
+mixed-precision (BF16 hi+lo split MFMA) points/s, reported with its error
+against the FP64 run per derivative order.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "fwd+adjoint points/s, mixed-prec vs FP64 on GPU, at stated err vs FP64 oracle"
+PEAK = {"hbm": 8000.0, "fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
+
+
+def algorithmic(wl, na, F, Q, P, N, wbytes):
+    """Per-step algorithmic work of each kernel category (flops, bytes)."""
+    d = wl.dims
+    L = len(d)
+    lin = sum(2.0 * d[l] * (d[l - 1] + 1) * na for l in range(1, L)) * N
+    lin_t = sum(2.0 * d[l] * d[l - 1] * na for l in range(2, L)) * N
+    grad = lin
+    hid = sum(d[l] for l in range(1, L - 1))
+    bf_flops = hid * (3.0 * F + 4.0 * Q + 3.0 * (na - 1)) * N
+    bb_flops = hid * (3.0 * P + 2.0 * na) * N
+    bytes_ = 3.0 * hid * na * wbytes * N
+    return {"linear": ("mfma", lin), "linear_t": ("mfma", lin_t), "grad": ("mfma", grad),
+            "bell_fwd": ("hbm", bytes_, bf_flops), "bell_bwd": ("hbm", bytes_, bb_flops)}
+
+
+def rel_by_order(x, ref, deg):
+    out = {}
+    for p in sorted(set(int(v) for v in deg)):
+        sel = np.asarray(deg) == p
+        r = float(np.abs(ref[..., sel]).max())
+        e = np.abs(x[..., sel] - ref[..., sel])
+        out[str(p)] = {"max": float(e.max() / max(r, 1e-300)), "median": float(np.median(e) / max(r, 1e-300))}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="dense4_k3")
+    ap.add_argument("--mixed", default="bf16x3", choices=["bf16x3", "fp32"])
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the gradient all-reduce across GPUs is a design note, not a measured path)")
+        return 2
+
+    import torch
+    import dnnf90_amd as D
+    from dnnf90_amd.workloads import WORKLOADS
+    from dnnf90_amd._lib import lib
+
+    wl = WORKLOADS[args.workload]
+    N = wl.npt
+    ts = D.TableSet(wl.d0, wl.k, wl.seeds, wl.act)
+    net = D.Net(ts, wl.dims)
+    w, x = wl.inputs()
+    net.weights = w
+    alpha = [a[:wl.d0] for a in D.current_alpha()]
+    terms = wl.terms(alpha)
+    res = D.Residual(terms)
+    deg = np.array([sum(a) for a in alpha])
+    na = len(alpha)
+    counts = np.zeros(4, np.int32)
+    lib().dnnf90_tabset_counts(ts.id, counts.ctypes.data)
+    F, P = int(counts[2]), int(counts[1])
+    Q = int(sum(max(int(p) - 1, 0) for p in deg))
+    dev = torch.device("cuda:0")
+    xd = torch.from_numpy(x).to(dev)
+    fac = 2.0 / N
+
+    def run_mode(prec, name, wbytes, mma_peak):
+        bw = D.BatchWork(net, N, prec)
+        g = D.Grad(net)
+        s = torch.cuda.Stream(device=dev)
+        bw.set_stream(s.cuda_stream)
+        with torch.cuda.stream(s):
+            for _ in range(args.warmup):
+                bw.residual_step(xd, res, fac, g, want_rsq=False)
+            torch.cuda.synchronize()
+            bw.kernel_ms()
+            bw.profile(True)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            t0 = time.perf_counter()
+            e0.record(s)
+            for _ in range(args.steps):
+                bw.residual_step(xd, res, fac, g, want_rsq=False)
+            e1.record(s)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            bw.profile(False)
+            kms = bw.kernel_ms()
+            ms = e0.elapsed_time(e1) / args.steps
+            tout = bw.forward(xd)
+            torch.cuda.synchronize()
+        g.zero()
+        rsq = bw.residual_step(xd, res, fac, g)
+        grad = g.get()
+        t_host = tout.cpu().numpy()
+        bw.set_stream(None)
+        bw.free()
+        g.free()
+        work = algorithmic(wl, na, F, Q, P, N, wbytes)
+        kern = {}
+        for c, (tot, cnt) in kms.items():
+            ent = {"ms_per_step": tot / args.steps, "launches_per_step": cnt / args.steps}
+            if c in work and tot > 0:
+                wk = work[c]
+                if wk[0] == "mfma":
+                    ent.update(bound="mfma", achieved=wk[1] / (tot / args.steps * 1e-3) / 1e12, unit="TFLOP/s",
+                               peak=mma_peak)
+                else:
+                    ent.update(bound="hbm", achieved=wk[1] / (tot / args.steps * 1e-3) / 1e9, unit="GB/s",
+                               peak=PEAK["hbm"])
+                ent["frac"] = ent["achieved"] / ent["peak"]
+                ent["algorithmic_per_step"] = wk[1]
+            kern[c] = ent
+        return {"name": name, "ms_per_step": ms, "wall_ms_per_step": wall * 1e3 / args.steps,
+                "points_per_s": N / (ms * 1e-3), "kernels": kern, "T": t_host, "grad": grad, "rsq": rsq}
+
+    fp64 = run_mode(D.PREC_FP64, "fp64", 8, PEAK["fp64"])
+    mprec = D.PREC_BF16X3 if args.mixed == "bf16x3" else D.PREC_FP32
+    mpeak = PEAK["bf16"] / 3.0 if args.mixed == "bf16x3" else PEAK["fp32"]
+    mixed = run_mode(mprec, args.mixed, 4, mpeak)
+
+    err_T = rel_by_order(mixed["T"], fp64["T"], deg)
+    err_g = float(np.abs(mixed["grad"] - fp64["grad"]).max() / np.abs(fp64["grad"]).max())
+
+    # FP64 GPU against the oracle on a sample of points
+    oracle_check = None
+    cpu = None
+    try:
+        import oracle
+        if not os.path.exists(oracle.LIB):
+            oracle.build(ref=False)
+        o = oracle.Oracle(wl.d0, wl.k, wl.seeds, wl.act)
+        ns = min(N, 256)
+        to = o.eval(wl.dims, w, x[:ns])
+        oracle_check = {"points": ns, "forward_max_rel": max(v["max"] for v in rel_by_order(fp64["T"][:ns], to, deg).values())}
+        if not args.no_cpu:
+            ncores = os.cpu_count() or 1
+            n0 = 64
+            t0 = time.perf_counter()
+            o.pinn_step(wl.dims, w, x[:n0], terms, fac)
+            dt = time.perf_counter() - t0
+            ns = int(min(N, max(n0, n0 * args.cpu_seconds / max(dt, 1e-6))))
+            t0 = time.perf_counter()
+            o.pinn_step(wl.dims, w, x[:ns], terms, fac)
+            dt = time.perf_counter() - t0
+            cpu = {"value": ns / dt, "unit": "points/s", "cores": ncores, "kind": "port",
+                   "sample": f"first {ns} of the {N} points, oracle FP64 fwd+seed+adjoint+accumulate, "
+                             f"OpenMP over points, {ncores} threads, {dt:.1f} s"}
+    except Exception as exc:  # the CPU leg is a reported baseline, never the product
+        cpu = {"error": repr(exc)}
+
+    kk = mixed["kernels"]
+    dom = max((c for c in kk if "frac" in kk[c]), key=lambda c: kk[c]["ms_per_step"])
+    dk = kk[dom]
+    roof = {"bound": dk["bound"], "achieved": dk["achieved"], "peak": dk["peak"], "unit": dk["unit"],
+            "frac": dk["frac"], "traffic": None, "kernel": dom}
+
+    line = {
+        "metric": METRIC, "value": mixed["points_per_s"], "unit": "points/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": mixed["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16x3+f32" if args.mixed == "bf16x3" else "f32", "data": "synthetic",
+        "config": {"workload": f"{wl.name}: MLP {wl.dims} {'tanh' if wl.act == 0 else 'erf-GELU'}, "
+                               f"D0={wl.d0} K={wl.k} |A|={na}, N={N}, loss mean square of {wl.note}",
+                   "precision": args.mixed},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "fp64": {"value": fp64["points_per_s"], "ms_per_step": fp64["ms_per_step"], "kernels": fp64["kernels"],
+                 "vs_oracle": oracle_check},
+        "speedup_vs_fp64": mixed["points_per_s"] / fp64["points_per_s"],
+        "err_vs_fp64": {"forward_by_order": err_T, "grad_max_rel": err_g},
+        "kernels": mixed["kernels"],
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

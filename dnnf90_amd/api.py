@@ -1,0 +1,281 @@
+"""Python mirror of the C ABI (include/dnnf90.h, include/dnnf90_amd.h).
+
+Names and argument meaning follow the reference's C interface
+(lib/dnnf90.h; lib/c_api_module.f90:100-528) and its ctypes example
+(tools/example_py.py); errors become DNNF90Error carrying the C return code
+and message.  Slots are 1-based in the C ABI (dnnf90_alpha) and 0-based here.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import lib
+
+PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
+DEVICE_PTRS = 1
+ACT_TANH, ACT_SIN, ACT_ERF, ACT_J0, ACT_J1, ACT_GELU = 0, 1, 2, 3, 4, 5
+
+
+class DNNF90Error(RuntimeError):
+    def __init__(self, rc, where):
+        msg = lib().dnnf90_last_error().decode(errors="replace")
+        super().__init__(f"{where} returned {rc}: {msg}")
+        self.rc = rc
+
+
+def _chk(rc, where, handle=False):
+    if rc < 0 or (handle and rc == 0):
+        raise DNNF90Error(rc, where)
+    return rc
+
+
+def _host(a, dtype):
+    a = np.ascontiguousarray(a, dtype=dtype)
+    return a, a.ctypes.data
+
+
+def _ptr(a, dtype):
+    """(keepalive, address, is_device) for a numpy array or a torch tensor."""
+    if hasattr(a, "data_ptr"):
+        import torch
+        want = torch.float64 if dtype == np.float64 else (torch.float32 if dtype == np.float32 else torch.int32)
+        if a.dtype != want or not a.is_contiguous():
+            raise TypeError(f"tensor must be contiguous {want}")
+        return a, a.data_ptr(), a.is_cuda
+    arr, p = _host(a, dtype)
+    return arr, p, False
+
+
+class TableSet:
+    """An independent multi-index table family (dnnf90_tabset_dense/closure)."""
+
+    def __init__(self, d0, k, seeds=None, act=ACT_TANH):
+        L = lib()
+        if seeds is None or len(seeds) == 0:
+            self.id = _chk(L.dnnf90_tabset_dense(d0, k), "dnnf90_tabset_dense", True)
+        else:
+            s, p = _host(np.asarray(seeds).reshape(-1), np.int32)
+            self.id = _chk(L.dnnf90_tabset_closure(d0, k, len(seeds), p), "dnnf90_tabset_closure", True)
+        if act != ACT_TANH:
+            _chk(L.dnnf90_tabset_activation(self.id, act), "dnnf90_tabset_activation")
+        self.d0, self.k, self.act = d0, k, act
+        self.na = _chk(L.dnnf90_tabset_nderiv(self.id), "dnnf90_tabset_nderiv")
+
+
+def current_alpha():
+    """Multi-indices of the current family, [nderiv][d0] (dnnf90_alpha)."""
+    L = lib()
+    n = L.dnnf90_nderiv()
+    out = []
+    a = (C.c_int * 64)()
+    for ia in range(1, n + 1):
+        _chk(L.dnnf90_alpha(ia, C.addressof(a)), "dnnf90_alpha")
+        out.append(list(a))
+    return out
+
+
+class Net:
+    """A network bound to a table family; weights live on the host and the GPU."""
+
+    def __init__(self, tabset: TableSet, dims, nid=None):
+        L = lib()
+        self.tabset = tabset
+        if nid is None:
+            d, p = _host(dims, np.int32)
+            nid = _chk(L.dnnf90_net_create_ts(tabset.id, len(dims), p), "dnnf90_net_create_ts", True)
+        self.id = nid
+        nl = L.dnnf90_net_nlayer(nid)
+        dd = np.zeros(nl, np.int32)
+        L.dnnf90_net_dims(nid, dd.ctypes.data)
+        self.dims = [int(v) for v in dd]
+        self.na = L.dnnf90_net_nderiv(nid)
+        self.nw = L.dnnf90_net_nweights(nid)
+
+    @classmethod
+    def load(cls, path, tabset: TableSet):
+        nid = _chk(lib().dnnf90_net_load_ts(tabset.id, str(path).encode()), "dnnf90_net_load_ts", True)
+        return cls(tabset, None, nid)
+
+    def save(self, path):
+        _chk(lib().dnnf90_net_save(self.id, str(path).encode()), "dnnf90_net_save")
+
+    def set_layer(self, l, wrow):
+        a, p = _host(wrow, np.float64)
+        _chk(lib().dnnf90_net_set_layer(self.id, l, p), "dnnf90_net_set_layer")
+
+    def get_layer(self, l):
+        n = self.dims[l - 1] * (self.dims[l - 2] + 1)
+        out = np.zeros(n)
+        _chk(lib().dnnf90_net_get_layer(self.id, l, out.ctypes.data), "dnnf90_net_get_layer")
+        return out
+
+    @property
+    def weights(self):
+        out = np.zeros(self.nw)
+        _chk(lib().dnnf90_net_get_weights(self.id, out.ctypes.data), "dnnf90_net_get_weights")
+        return out
+
+    @weights.setter
+    def weights(self, w):
+        a, p = _host(w, np.float64)
+        if a.size != self.nw:
+            raise ValueError(f"expected {self.nw} weights")
+        _chk(lib().dnnf90_net_set_weights(self.id, p), "dnnf90_net_set_weights")
+
+    def free(self):
+        if self.id:
+            lib().dnnf90_net_free(self.id)
+            self.id = 0
+
+
+class Work:
+    """Per-point evaluation work space (dnnf90_work_create / dnnf90_eval)."""
+
+    def __init__(self, net: Net):
+        self.net = net
+        self.id = _chk(lib().dnnf90_work_create(net.id), "dnnf90_work_create", True)
+
+    def eval(self, x):
+        xa, p = _host(x, np.float64)
+        t = np.zeros(self.net.na)
+        _chk(lib().dnnf90_eval(self.net.id, self.id, p, t.ctypes.data), "dnnf90_eval")
+        return t
+
+    def free(self):
+        lib().dnnf90_work_free(self.id)
+
+
+class Grad:
+    """Gradient accumulator (dnnf90_grad_create); lives on the GPU."""
+
+    def __init__(self, net: Net):
+        self.net = net
+        self.id = _chk(lib().dnnf90_grad_create(net.id), "dnnf90_grad_create", True)
+
+    def zero(self):
+        _chk(lib().dnnf90_grad_zero(self.id), "dnnf90_grad_zero")
+
+    def get(self):
+        out = np.zeros(self.net.nw)
+        _chk(lib().dnnf90_grad_get(self.id, out.ctypes.data), "dnnf90_grad_get")
+        return out
+
+    def set(self, g):
+        a, p = _host(g, np.float64)
+        _chk(lib().dnnf90_grad_set(self.id, p), "dnnf90_grad_set")
+
+    def free(self):
+        lib().dnnf90_grad_free(self.id)
+
+
+class TWork:
+    """Per-point training work space (dnnf90_twork_create / dnnf90_grad_point)."""
+
+    def __init__(self, net: Net):
+        self.net = net
+        self.id = _chk(lib().dnnf90_twork_create(net.id), "dnnf90_twork_create", True)
+
+    def grad_point(self, x, seed, grad: Grad):
+        xa, px = _host(x, np.float64)
+        sa, ps = _host(seed, np.float64)
+        _chk(lib().dnnf90_grad_point(self.net.id, self.id, px, ps, grad.id), "dnnf90_grad_point")
+
+    def free(self):
+        lib().dnnf90_twork_free(self.id)
+
+
+def adam_step(net: Net, grad: Grad, eta, beta1, beta2, eps, nbatch, istep):
+    _chk(lib().dnnf90_adam_step(net.id, grad.id, eta, beta1, beta2, eps, nbatch, istep),
+         "dnnf90_adam_step")
+
+
+class Residual:
+    """R = sum_k coeff_k * [T_0 *] T_slot_k on output 0 (slots 0-based here)."""
+
+    def __init__(self, terms):
+        self.coeff = np.ascontiguousarray([t[0] for t in terms], np.float64)
+        self.slot = np.ascontiguousarray([t[1] + 1 for t in terms], np.int32)
+        self.nonlin = np.ascontiguousarray([1 if t[2] else 0 for t in terms], np.int32)
+        self.nterm = len(terms)
+
+
+class BatchWork:
+    """Batched work space of up to maxpt points (dnnf90_bwork_create)."""
+
+    def __init__(self, net: Net, maxpt, prec=PREC_FP64):
+        self.net, self.maxpt, self.prec = net, maxpt, prec
+        self.id = _chk(lib().dnnf90_bwork_create(net.id, maxpt, prec), "dnnf90_bwork_create", True)
+        self.npt = 0
+
+    def set_stream(self, stream_handle):
+        _chk(lib().dnnf90_bwork_set_stream(self.id, stream_handle), "dnnf90_bwork_set_stream")
+
+    def sync(self):
+        _chk(lib().dnnf90_bwork_sync(self.id), "dnnf90_bwork_sync")
+
+    KERNEL_CATEGORIES = ("linear", "bell_fwd", "bell_bwd", "grad", "linear_t", "other")
+
+    def profile(self, on=True):
+        _chk(lib().dnnf90_bwork_profile(self.id, 1 if on else 0), "dnnf90_bwork_profile")
+
+    def kernel_ms(self):
+        """{category: (total ms, launches)} since the last call (resets)."""
+        ms = np.zeros(6)
+        cnt = np.zeros(6, np.int64)
+        _chk(lib().dnnf90_bwork_kernel_ms(self.id, ms.ctypes.data, cnt.ctypes.data), "dnnf90_bwork_kernel_ms")
+        return {c: (float(ms[i]), int(cnt[i])) for i, c in enumerate(self.KERNEL_CATEGORIES)}
+
+    def forward(self, x, tout=None, want=True):
+        """x [npt][d0]; returns T [npt][nout][na] (numpy, or the given tout)."""
+        xk, px, dev = _ptr(x, np.float64)
+        npt = int(x.shape[0])
+        if want and tout is None:
+            if dev:
+                import torch
+                tout = torch.empty((npt, self.net.dims[-1], self.net.na), dtype=torch.float64, device=x.device)
+            else:
+                tout = np.zeros((npt, self.net.dims[-1], self.net.na))
+        pt = 0
+        if tout is not None:
+            tk, pt, tdev = _ptr(tout, np.float64)
+            if tdev != dev:
+                raise TypeError("x and tout must both be host or both be device")
+        _chk(lib().dnnf90_forward_batch(self.net.id, self.id, npt, px, pt or None,
+                                        DEVICE_PTRS if dev else 0), "dnnf90_forward_batch")
+        self.npt = npt
+        return tout
+
+    def backward(self, seed, grad: Grad):
+        sk, ps, dev = _ptr(seed, np.float64)
+        _chk(lib().dnnf90_backward_batch(self.net.id, self.id, ps, grad.id, DEVICE_PTRS if dev else 0),
+             "dnnf90_backward_batch")
+
+    def residual_step(self, x, res: Residual, fac, grad: Grad, want_rsq=True):
+        """Fused forward + collocation seed + adjoint; returns sum R^2."""
+        xk, px, dev = _ptr(x, np.float64)
+        rsq = C.c_double(0.0)
+        L = lib()
+        args = (res.coeff.ctypes.data, res.slot.ctypes.data, res.nonlin.ctypes.data)
+        _chk(L.dnnf90_residual_step_batch(self.net.id, self.id, int(x.shape[0]), px, res.nterm, *args,
+                                          float(fac), grad.id, C.addressof(rsq) if want_rsq else None,
+                                          DEVICE_PTRS if dev else 0),
+             "dnnf90_residual_step_batch")
+        return rsq.value if want_rsq else None
+
+    def residual_rows(self, x, res: Residual, rows_t, ld):
+        """Gauss-Newton rows dR/dw into the device FP32 buffer rows_t[nw][ld]."""
+        xk, px, dev = _ptr(x, np.float64)
+        rk, pr, rdev = _ptr(rows_t, np.float32)
+        if not rdev:
+            raise TypeError("rows_t must be a device tensor")
+        args = (res.coeff.ctypes.data, res.slot.ctypes.data, res.nonlin.ctypes.data)
+        _chk(lib().dnnf90_residual_rows_batch(self.net.id, self.id, int(x.shape[0]), px, res.nterm, *args,
+                                              pr, int(ld), DEVICE_PTRS if dev else 0),
+             "dnnf90_residual_rows_batch")
+
+    def free(self):
+        if self.id:
+            lib().dnnf90_bwork_free(self.id)
+            self.id = 0

@@ -1,0 +1,62 @@
+"""torch.autograd.Function over the C ABI.
+
+forward(x, w) -> T[N, nout, na]: every carried derivative of every output,
+computed by the gfx950 forward kernels; backward(dL/dT) -> dL/dw by the
+hand-written adjoint kernels (the exact analytical reverse of
+lib/train_module.f90:360-450).  Like the reference, the adjoint stops at the
+first hidden layer, so no gradient flows to x (lib/train_module.f90:430).
+
+This is what the Python comparison scripts (bench/post/compare_frameworks.py,
+bench/compare/grad_cost.py) consume: any loss written in torch on T seeds the
+adjoint through autograd.
+"""
+from __future__ import annotations
+
+import torch
+
+from .api import PREC_FP64, BatchWork, Grad, Net
+
+
+class _DerivForward(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, w, layer):
+        layer._sync_weights(w)
+        T = layer.bwork.forward(x.contiguous())
+        ctx.layer = layer
+        return T
+
+    @staticmethod
+    def backward(ctx, gT):
+        layer = ctx.layer
+        layer.grad.zero()
+        layer.bwork.backward(gT.contiguous().to(torch.float64), layer.grad)
+        g = torch.from_numpy(layer.grad.get()).to(gT.device)
+        return None, g, None
+
+
+class DerivMLP(torch.nn.Module):
+    """MLP whose forward returns all carried input derivatives.
+
+    w is a flat float64 Parameter in the ABI layout: for l = 2..nlayer,
+    ndim(l) rows of ndim(l-1)+1 values, bias first.
+    """
+
+    def __init__(self, net: Net, maxpt: int, prec: int = PREC_FP64, device="cuda"):
+        super().__init__()
+        self.net = net
+        self.w = torch.nn.Parameter(torch.from_numpy(net.weights).to(device))
+        self.bwork = BatchWork(net, maxpt, prec)
+        self.grad = Grad(net)
+        self._wver = None
+
+    def _sync_weights(self, w):
+        ver = (w.data_ptr(), w._version)
+        if ver != self._wver:
+            self.net.weights = w.detach().cpu().numpy()
+            self._wver = ver
+
+    def forward(self, x):
+        if x.shape[0] > self.bwork.maxpt:
+            raise ValueError(f"batch of {x.shape[0]} exceeds maxpt={self.bwork.maxpt}")
+        torch.cuda.synchronize(x.device)
+        return _DerivForward.apply(x.to(torch.float64), self.w, self)

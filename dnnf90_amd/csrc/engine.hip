@@ -1,0 +1,1183 @@
+// engine.hip -- networks, work spaces and the GPU layer loops behind the
+// C ABI (include/dnnf90.h, include/dnnf90_amd.h).
+//
+// The reference runs one point per call through the layer loop of
+// lib/train_module.f90:241-450.  Here a call runs a batch of points: every
+// layer is one linear-map GEMM over the na*Np column axis plus one
+// Faa di Bruno launch, forward; one adjoint launch plus a gradient GEMM and a
+// transposed linear-map GEMM, backward.  The per-point entries of the
+// reference ABI are batches of one.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/dnnf90_amd.h"
+#include "kernels.hpp"
+#include "tables.hpp"
+
+namespace dnnf90 {
+
+static thread_local std::string g_err;
+
+static int fail(int rc, const std::string &m)
+{
+    g_err = m;
+    return rc;
+}
+
+#define HIPCHK(call)                                                                 \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            g_err = std::string(#call) + ": " + hipGetErrorString(e_);               \
+            return e_ == hipErrorOutOfMemory ? -4 : -3;                              \
+        }                                                                            \
+    } while (0)
+
+template <typename T>
+static int dalloc(T **p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) return 0;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        g_err = std::string("hipMalloc: ") + hipGetErrorString(e);
+        return e == hipErrorOutOfMemory ? -4 : -3;
+    }
+    return 0;
+}
+
+template <typename T>
+static void dfree(T *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+template <typename T>
+static int upload(T **dst, const std::vector<T> &src)
+{
+    int rc = dalloc(dst, src.empty() ? 1 : src.size());
+    if (rc) return rc;
+    if (!src.empty()) HIPCHK(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// device copy of one table family's schedules
+struct DevTables {
+    int na = 0, k = 0, d0 = 0, iact = 0, nentry = 0;
+    int *ind_e1 = nullptr, *slot_group = nullptr, *grp_term = nullptr, *grp_store = nullptr;
+    int2 *term_sb = nullptr;
+    double *term_c64 = nullptr, *pbin64 = nullptr, *tpc = nullptr;
+    float *term_c32 = nullptr, *pbin32 = nullptr;
+    int *pstart = nullptr, *pnum = nullptr;
+    int2 *pia_id = nullptr;
+    bool ready = false;
+
+    int build(const TableSet &t, const FwdSchedule &s)
+    {
+        na = t.na; k = t.k; d0 = t.d0; iact = t.iact; nentry = s.nentry;
+        std::vector<int2> tsb(s.term_s.size()), pid(t.pair_ia.size());
+        for (size_t i = 0; i < tsb.size(); ++i) tsb[i] = make_int2(s.term_s[i], s.term_b[i]);
+        for (size_t i = 0; i < pid.size(); ++i) pid[i] = make_int2(t.pair_ia[i], t.pair_id[i]);
+        std::vector<float> c32(s.term_c.begin(), s.term_c.end());
+        std::vector<float> b32(t.pair_binom.begin(), t.pair_binom.end());
+        int rc = 0;
+        if ((rc = upload(&ind_e1, t.ind_e1)) || (rc = upload(&slot_group, s.slot_group)) ||
+            (rc = upload(&grp_term, s.grp_term)) || (rc = upload(&grp_store, s.grp_store)) ||
+            (rc = upload(&term_sb, tsb)) || (rc = upload(&term_c64, s.term_c)) ||
+            (rc = upload(&term_c32, c32)) || (rc = upload(&pstart, t.pair_start)) ||
+            (rc = upload(&pnum, t.pair_num)) || (rc = upload(&pia_id, pid)) ||
+            (rc = upload(&pbin64, t.pair_binom)) || (rc = upload(&pbin32, b32)) ||
+            (rc = upload(&tpc, t.tpc)))
+            return rc;
+        ready = true;
+        return 0;
+    }
+    void release()
+    {
+        dfree(ind_e1); dfree(slot_group); dfree(grp_term); dfree(grp_store); dfree(term_sb);
+        dfree(term_c64); dfree(term_c32); dfree(pstart); dfree(pnum); dfree(pia_id);
+        dfree(pbin64); dfree(pbin32); dfree(tpc);
+        ready = false;
+    }
+    DevFwdSched fwd() const
+    {
+        DevFwdSched f;
+        f.slot_group = slot_group; f.grp_term = grp_term; f.grp_store = grp_store;
+        f.term_sb = term_sb; f.term_c64 = term_c64; f.term_c32 = term_c32;
+        f.na = na; f.k = k; f.nentry = nentry;
+        return f;
+    }
+    DevPairs pairs() const
+    {
+        DevPairs p;
+        p.start = pstart; p.num = pnum; p.ia_id = pia_id; p.binom64 = pbin64; p.binom32 = pbin32;
+        p.na = na;
+        return p;
+    }
+    ActParams act() const
+    {
+        ActParams a;
+        a.iact = iact; a.nq = k + 1; a.tpc = tpc; a.ldt = k + 3;
+        return a;
+    }
+};
+
+struct Net {
+    int nlayer = 0, nw = 0;
+    std::vector<int> dims;
+    std::vector<long> woff;  // per layer l >= 1
+    TableSet tab;
+    FwdSchedule sched;
+    std::vector<double> w;   // host master copy
+    long host_ver = 1, dev_ver = 0, dev32_ver = 0;
+    DevTables dt;
+    double *d_w64 = nullptr;
+    float *d_w32 = nullptr;
+    std::mutex mu;
+
+    int ensure_device(bool need32)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!dt.ready) {
+            int rc = dt.build(tab, sched);
+            if (rc) return rc;
+        }
+        if (!d_w64) {
+            int rc = dalloc(&d_w64, (size_t)nw);
+            if (rc) return rc;
+        }
+        if (dev_ver != host_ver) {
+            HIPCHK(hipMemcpy(d_w64, w.data(), sizeof(double) * nw, hipMemcpyHostToDevice));
+            dev_ver = host_ver;
+        }
+        if (need32 && dev32_ver != dev_ver) {
+            if (!d_w32) {
+                int rc = dalloc(&d_w32, (size_t)nw);
+                if (rc) return rc;
+            }
+            launch_convert_f64_f32(d_w64, d_w32, nw, 0);
+            HIPCHK(hipDeviceSynchronize());
+            dev32_ver = dev_ver;
+        }
+        return 0;
+    }
+    void release()
+    {
+        dt.release();
+        dfree(d_w64);
+        dfree(d_w32);
+    }
+};
+
+struct Grad {
+    int nw = 0;
+    double *d_g = nullptr, *d_m = nullptr, *d_v = nullptr;
+    void release() { dfree(d_g); dfree(d_m); dfree(d_v); }
+};
+
+// kernel categories timed by HIP events when profiling is on
+enum { CAT_LINEAR = 0, CAT_BELL_FWD, CAT_BELL_BWD, CAT_GRAD, CAT_LINEAR_T, CAT_OTHER, NCAT };
+
+struct KProf {
+    bool on = false;
+    std::vector<hipEvent_t> ev;  // begin/end pairs
+    std::vector<int> cat;        // one per pair
+    size_t used = 0;             // events recorded since the last resolve
+    double ms[NCAT] = {};
+    long cnt[NCAT] = {};
+};
+
+// planes of one batched work space: [rows][na*Np] in T
+struct BatchWork {
+    int prec = PREC_FP64, maxpt = 0, npt = 0;
+    long np_cap = 0, np = 0;
+    size_t esz = 8;
+    hipStream_t own = nullptr, stream = nullptr;
+    void *t0 = nullptr, *tl = nullptr, *p0 = nullptr, *p1 = nullptr, *scratch = nullptr;
+    std::vector<void *> tpl, hpl;  // per hidden layer
+    double *partial = nullptr;
+    long partial_cap = 0;
+    double *d_x = nullptr, *d_seed = nullptr, *d_tout = nullptr, *d_rsq = nullptr;
+    double *d_rsq_part = nullptr;
+    double *d_coeff = nullptr;
+    int *d_slot = nullptr, *d_nonlin = nullptr, nterm_cap = 0;
+    std::vector<double> h_coeff;
+    std::vector<int> h_slot, h_nonlin;
+    bool fwd_valid = false;
+    long fwd_ver = -1;
+    KProf prof;
+
+    void mark(int c)
+    {
+        if (prof.used == prof.ev.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return;
+            prof.ev.push_back(e);
+        }
+        if (c >= 0) prof.cat.push_back(c);
+        (void)hipEventRecord(prof.ev[prof.used++], stream);
+    }
+    int resolve()
+    {
+        if (prof.used == 0) return 0;
+        HIPCHK(hipStreamSynchronize(stream));
+        for (size_t i = 0; i + 1 < prof.used; i += 2) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, prof.ev[i], prof.ev[i + 1]));
+            prof.ms[prof.cat[i / 2]] += ms;
+            prof.cnt[prof.cat[i / 2]]++;
+        }
+        prof.used = 0;
+        prof.cat.clear();
+        return 0;
+    }
+
+    void release()
+    {
+        for (auto ev : prof.ev) (void)hipEventDestroy(ev);
+        prof.ev.clear();
+        auto f = [](void *&p) { if (p) (void)hipFree(p); p = nullptr; };
+        f(t0); f(tl); f(p0); f(p1); f(scratch);
+        for (auto &p : tpl) f(p);
+        for (auto &p : hpl) f(p);
+        tpl.clear(); hpl.clear();
+        dfree(partial); dfree(d_x); dfree(d_seed); dfree(d_tout); dfree(d_rsq); dfree(d_rsq_part);
+        dfree(d_coeff); dfree(d_slot); dfree(d_nonlin);
+        if (own) (void)hipStreamDestroy(own);
+        own = stream = nullptr;
+    }
+};
+
+// brackets the launches of one kernel category with events when profiling
+struct PScope {
+    BatchWork &b;
+    PScope(BatchWork &bb, int c) : b(bb)
+    {
+        if (b.prof.on) {
+            if (b.prof.used >= 8192) (void)b.resolve();
+            b.mark(c);
+        }
+    }
+    ~PScope()
+    {
+        if (b.prof.on) b.mark(-1);
+    }
+};
+
+static long round_up(long v, long m) { return (v + m - 1) / m * m; }
+
+static int valloc(void **p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0) return 0;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        g_err = std::string("hipMalloc: ") + hipGetErrorString(e);
+        return e == hipErrorOutOfMemory ? -4 : -3;
+    }
+    return 0;
+}
+
+// gradient GEMM split: about 2048 workgroups over the output tiles
+static void grad_split(long ncol, int M, int Kd, int *nchunk, long *chunk)
+{
+    const int tiles = ((M + 63) / 64) * ((Kd + 63) / 64);
+    long nc = std::max(1, 2048 / tiles);
+    nc = std::min(nc, std::max(1L, ncol / 1024));
+    long ch = round_up((ncol + nc - 1) / nc, 16);
+    nc = (ncol + ch - 1) / ch;
+    *nchunk = (int)nc;
+    *chunk = ch;
+}
+
+static int bwork_init(BatchWork &b, const Net &nt, int maxpt, int prec)
+{
+    b.prec = prec;
+    b.maxpt = maxpt;
+    b.esz = prec == PREC_FP64 ? 8 : 4;
+    b.np_cap = round_up(std::max(maxpt, 1), 64);
+    const int na = nt.tab.na, L = nt.nlayer;
+    const size_t col = (size_t)na * b.np_cap;
+    int nmax = 0;
+    for (int l = 0; l < L; ++l) nmax = std::max(nmax, nt.dims[l]);
+    int rc;
+    HIPCHK(hipStreamCreateWithFlags(&b.own, hipStreamNonBlocking));
+    b.stream = b.own;
+    if ((rc = valloc(&b.t0, (size_t)nt.dims[0] * col * b.esz))) return rc;
+    if ((rc = valloc(&b.tl, (size_t)nt.dims[L - 1] * col * b.esz))) return rc;
+    if ((rc = valloc(&b.p0, (size_t)nmax * col * b.esz))) return rc;
+    if ((rc = valloc(&b.p1, (size_t)nmax * col * b.esz))) return rc;
+    b.tpl.assign(L, nullptr);
+    b.hpl.assign(L, nullptr);
+    for (int l = 1; l < L - 1; ++l) {
+        if ((rc = valloc(&b.tpl[l], (size_t)nt.dims[l] * col * b.esz))) return rc;
+        if ((rc = valloc(&b.hpl[l], (size_t)nt.dims[l] * col * b.esz))) return rc;
+    }
+    // LDS-resident operand store unless the set is too large
+    const int nfwd = bell_fwd_threads(nt.sched.nentry, (int)b.esz);
+    if (nfwd == 0) {
+        if ((rc = valloc(&b.scratch, (size_t)nt.sched.nentry * nmax * b.np_cap * b.esz))) return rc;
+    }
+    long pc = 0;
+    for (int l = 1; l < L; ++l) {
+        int nc;
+        long ch;
+        grad_split((long)col, nt.dims[l], nt.dims[l - 1] + 1, &nc, &ch);
+        pc = std::max(pc, (long)nc * nt.dims[l] * (nt.dims[l - 1] + 1));
+    }
+    b.partial_cap = pc;
+    if ((rc = dalloc(&b.partial, (size_t)pc))) return rc;
+    if ((rc = dalloc(&b.d_x, (size_t)b.np_cap * nt.dims[0]))) return rc;
+    if ((rc = dalloc(&b.d_seed, (size_t)b.np_cap * nt.dims[L - 1] * na))) return rc;
+    if ((rc = dalloc(&b.d_tout, (size_t)b.np_cap * nt.dims[L - 1] * na))) return rc;
+    if ((rc = dalloc(&b.d_rsq_part, (size_t)(b.np_cap / 256 + 1)))) return rc;
+    if ((rc = dalloc(&b.d_rsq, 1))) return rc;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// the layer loops
+
+template <typename T> static const T *weights(const Net &nt);
+template <> const double *weights<double>(const Net &nt) { return nt.d_w64; }
+template <> const float *weights<float>(const Net &nt) { return nt.d_w32; }
+
+template <typename T>
+static void forward_t(const Net &nt, BatchWork &b, const double *d_x, int npt)
+{
+    const int na = nt.tab.na, L = nt.nlayer;
+    const long Np = round_up(std::max(npt, 1), 64);
+    const long ncol = (long)na * Np;
+    b.np = Np;
+    b.npt = npt;
+    hipStream_t s = b.stream;
+    {
+        PScope p(b, CAT_OTHER);
+        launch_input<T>(d_x, npt, nt.dims[0], na, nt.dt.ind_e1, (T *)b.t0, Np, s);
+    }
+    const T *W = weights<T>(nt);
+    const DevFwdSched sch = nt.dt.fwd();
+    const ActParams act = nt.dt.act();
+    for (int l = 1; l < L; ++l) {
+        const int nd = nt.dims[l], ndm = nt.dims[l - 1];
+        const T *X = (const T *)(l == 1 ? b.t0 : b.tpl[l - 1]);
+        T *out = (T *)(l < L - 1 ? b.p0 : b.tl);
+        {
+            PScope p(b, CAT_LINEAR);
+            launch_gemm_wx<T>(b.prec, false, W + nt.woff[l], ndm + 1, X, out, nd, ndm + 1, ncol, Np, s);
+        }
+        if (l < L - 1) {
+            PScope p(b, CAT_BELL_FWD);
+            launch_bell_fwd<T>((const T *)b.p0, (T *)b.tpl[l], (T *)b.hpl[l], nd, Np, sch, act,
+                               (T *)b.scratch, s);
+        }
+    }
+}
+
+// adjoint from the output seed already in p0 (rows < nout).  rows_t == null:
+// gradient added to grad; else the per-point rows are written to rows_t.
+template <typename T>
+static void backward_t(const Net &nt, BatchWork &b, double *grad, float *rows_t, long ldr)
+{
+    const int na = nt.tab.na, L = nt.nlayer;
+    const long Np = b.np, ncol = (long)na * Np;
+    hipStream_t s = b.stream;
+    const T *W = weights<T>(nt);
+    const DevPairs pr = nt.dt.pairs();
+    for (int l = L - 1; l >= 1; --l) {
+        const int nd = nt.dims[l], ndm = nt.dims[l - 1];
+        if (l < L - 1) {
+            PScope p(b, CAT_BELL_BWD);
+            launch_bell_bwd<T>((const T *)b.p1, (const T *)b.hpl[l], (T *)b.p0, nd, Np, pr, s);
+        }
+        const T *X = (const T *)(l == 1 ? b.t0 : b.tpl[l - 1]);
+        {
+            PScope p(b, CAT_GRAD);
+            if (rows_t) {
+                launch_point_rows<T>((const T *)b.p0, X, nd, ndm + 1, na, Np, rows_t + nt.woff[l] * ldr,
+                                     ldr, s);
+            } else {
+                int nc;
+                long ch;
+                grad_split(ncol, nd, ndm + 1, &nc, &ch);
+                launch_gemm_grad<T>(b.prec, (const T *)b.p0, X, nd, ndm + 1, ncol, Np, b.partial, nc, ch,
+                                    grad + nt.woff[l], s);
+            }
+        }
+        if (l > 1) {
+            PScope p(b, CAT_LINEAR_T);
+            launch_gemm_wx<T>(b.prec, true, W + nt.woff[l], ndm + 1, (const T *)b.p0, (T *)b.p1, ndm, nd,
+                              ncol, Np, s);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// registries (lib/c_api_module.f90:38-62)
+
+static constexpr int MAXH = 64;
+
+template <typename Obj>
+struct Registry {
+    Obj *obj[MAXH + 1] = {};
+    int gen[MAXH + 1] = {};      // generation of the slot (networks)
+    int owner[MAXH + 1] = {};    // network a dependent belongs to
+    int ogen[MAXH + 1] = {};     // generation of that network when built
+    int slot()
+    {
+        for (int i = 1; i <= MAXH; ++i)
+            if (!obj[i]) return i;
+        return -1;
+    }
+    Obj *get(int h) const { return (h >= 1 && h <= MAXH) ? obj[h] : nullptr; }
+};
+
+static Registry<Net> g_net;
+static Registry<TableSet> g_tab;
+static Registry<BatchWork> g_work;   // evaluation work spaces (wid)
+static Registry<BatchWork> g_twork;  // training work spaces (tid)
+static Registry<BatchWork> g_bwork;  // batched work spaces (bid)
+static Registry<Grad> g_grad;
+static TableSet g_current;
+static bool g_current_ready = false;
+
+static bool dependent_ok(const Registry<BatchWork> &r, int h, int nid)
+{
+    return r.get(h) && r.owner[h] == nid && r.ogen[h] == g_net.gen[nid];
+}
+static bool grad_ok(int gid, int nid)
+{
+    return g_grad.get(gid) && g_grad.owner[gid] == nid && g_grad.ogen[gid] == g_net.gen[nid];
+}
+
+static int make_net(const TableSet &ts, int nlayer, const int *dims)
+{
+    if (nlayer < 2 || !dims) return fail(-1, "nlayer must be >= 2");
+    for (int l = 0; l < nlayer; ++l)
+        if (dims[l] < 1) return fail(-1, "every width must be >= 1");
+    if (dims[0] != ts.d0) return fail(-1, "input width differs from the table set's d0");
+    int id = g_net.slot();
+    if (id < 0) return fail(-1, "no free network handle");
+    Net *nt = new Net();
+    nt->nlayer = nlayer;
+    nt->dims.assign(dims, dims + nlayer);
+    nt->tab = ts;
+    nt->sched.build(ts);
+    nt->woff.assign(nlayer, 0);
+    long off = 0;
+    for (int l = 1; l < nlayer; ++l) {
+        nt->woff[l] = off;
+        off += (long)dims[l] * (dims[l - 1] + 1);
+    }
+    nt->nw = (int)off;
+    nt->w.assign(off, 0.0);
+    g_net.obj[id] = nt;
+    return id;
+}
+
+static std::string c_path(const char *f) { return f ? std::string(f) : std::string(); }
+
+// nn_weight.dat (lib/net_module.f90:146-215): epoch; "func" code;
+// "Activation_out" code; nlayer; widths; per layer a header record then one
+// list-directed record per neuron (bias, then the incoming weights)
+static int load_weights(const std::string &path, const TableSet &ts, int *nid_out)
+{
+    std::ifstream in(path);
+    if (!in) return fail(-2, "no such file: " + path);
+    std::string line;
+    auto next = [&](std::string &l) { return (bool)std::getline(in, l); };
+    if (!next(line)) return fail(-1, "empty weight file");
+    std::string key;
+    int ifunc = -1, iout = -1, nlayer = 0;
+    if (!next(line)) return fail(-1, "truncated weight file");
+    { std::istringstream is(line); is >> key >> ifunc; }
+    if (!next(line)) return fail(-1, "truncated weight file");
+    { std::istringstream is(line); is >> key >> iout; }
+    if (ifunc < 0 || ifunc > 5) return fail(-2, "weight file names an unknown activation");
+    if (iout != 0) return fail(-2, "weight file has a nonlinear output layer");
+    if (ifunc != ts.iact) return fail(-2, "weight file activation differs from the table set's");
+    if (!next(line)) return fail(-1, "truncated weight file");
+    nlayer = std::atoi(line.c_str());
+    if (nlayer < 2) return fail(-1, "bad layer count");
+    std::vector<int> dims(nlayer);
+    for (int l = 0; l < nlayer; ++l) {
+        if (!next(line)) return fail(-1, "truncated weight file");
+        dims[l] = std::atoi(line.c_str());
+    }
+    int id = make_net(ts, nlayer, dims.data());
+    if (id < 0) return id;
+    Net *nt = g_net.obj[id];
+    for (int l = 1; l < nlayer; ++l) {
+        if (!next(line)) { delete nt; g_net.obj[id] = nullptr; return fail(-1, "truncated weight file"); }
+        for (int j = 0; j < dims[l]; ++j) {
+            int need = dims[l - 1] + 1, got = 0;
+            double *row = &nt->w[nt->woff[l] + (long)j * need];
+            while (got < need) {
+                if (!next(line)) { delete nt; g_net.obj[id] = nullptr; return fail(-1, "truncated weight file"); }
+                for (char &c : line)
+                    if (c == 'D' || c == 'd') c = 'E';
+                std::istringstream is(line);
+                double v;
+                while (got < need && (is >> v)) row[got++] = v;
+            }
+        }
+    }
+    nt->host_ver++;
+    *nid_out = id;
+    return 0;
+}
+
+static const Net *net_or_null(int nid) { return g_net.get(nid); }
+
+template <typename T>
+static int run_forward(Net &nt, BatchWork &b, int npt, const double *x, double *tout, bool dev)
+{
+    if (npt < 0 || npt > b.maxpt) return fail(-1, "npt outside [0, maxpt]");
+    int rc = nt.ensure_device(b.prec != PREC_FP64);
+    if (rc) return rc;
+    const double *dx = x;
+    if (!dev) {
+        if (npt > 0)
+            HIPCHK(hipMemcpyAsync(b.d_x, x, sizeof(double) * npt * nt.dims[0], hipMemcpyHostToDevice, b.stream));
+        dx = b.d_x;
+    }
+    forward_t<T>(nt, b, dx, npt);
+    if (tout) {
+        const int nout = nt.dims[nt.nlayer - 1], na = nt.tab.na;
+        double *dt = dev ? tout : b.d_tout;
+        launch_output<T>((const T *)b.tl, nout, na, npt, b.np, dt, b.stream);
+        if (!dev && npt > 0)
+            HIPCHK(hipMemcpyAsync(tout, b.d_tout, sizeof(double) * npt * nout * na, hipMemcpyDeviceToHost,
+                                  b.stream));
+    }
+    HIPCHK(hipGetLastError());
+    b.fwd_valid = true;
+    b.fwd_ver = nt.dev_ver;
+    return 0;
+}
+
+template <typename T>
+static int run_backward_seed(Net &nt, BatchWork &b, const double *seed, Grad &g, bool dev)
+{
+    if (!b.fwd_valid || b.fwd_ver != nt.dev_ver || nt.host_ver != nt.dev_ver)
+        return fail(-1, "backward needs a forward on the same work space and weights");
+    const int nout = nt.dims[nt.nlayer - 1], na = nt.tab.na;
+    const double *ds = seed;
+    if (!dev) {
+        if (b.npt > 0)
+            HIPCHK(hipMemcpyAsync(b.d_seed, seed, sizeof(double) * b.npt * nout * na, hipMemcpyHostToDevice,
+                                  b.stream));
+        ds = b.d_seed;
+    }
+    {
+        PScope p(b, CAT_OTHER);
+        launch_seed_user<T>(ds, b.npt, nout, na, (T *)b.p0, b.np, b.stream);
+    }
+    backward_t<T>(nt, b, g.d_g, nullptr, 0);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// residual terms are host arrays (they describe the operator); the device
+// copy is refreshed only when they change, so repeated steps never block
+static int set_residual(BatchWork &b, int nterm, const double *coeff, const int *slot,
+                        const int *nonlin, int na, DevResidual *r)
+{
+    if (nterm < 0) return fail(-1, "nterm < 0");
+    std::vector<double> c(nterm);
+    std::vector<int> s(nterm), nl(nterm);
+    for (int i = 0; i < nterm; ++i) {
+        c[i] = coeff[i];
+        if (slot[i] < 1 || slot[i] > na) return fail(-1, "residual slot outside [1, nderiv]");
+        s[i] = slot[i] - 1;
+        nl[i] = (nonlin && nonlin[i]) ? 1 : 0;
+    }
+    if (c != b.h_coeff || s != b.h_slot || nl != b.h_nonlin) {
+        if (nterm > b.nterm_cap) {
+            dfree(b.d_coeff); dfree(b.d_slot); dfree(b.d_nonlin);
+            int rc;
+            if ((rc = dalloc(&b.d_coeff, nterm)) || (rc = dalloc(&b.d_slot, nterm)) ||
+                (rc = dalloc(&b.d_nonlin, nterm)))
+                return rc;
+            b.nterm_cap = nterm;
+        }
+        if (nterm > 0) {
+            HIPCHK(hipStreamSynchronize(b.stream));
+            HIPCHK(hipMemcpy(b.d_coeff, c.data(), sizeof(double) * nterm, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(b.d_slot, s.data(), sizeof(int) * nterm, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(b.d_nonlin, nl.data(), sizeof(int) * nterm, hipMemcpyHostToDevice));
+        }
+        b.h_coeff = c;
+        b.h_slot = s;
+        b.h_nonlin = nl;
+    }
+    r->nterm = nterm;
+    r->coeff = b.d_coeff;
+    r->slot = b.d_slot;
+    r->nonlin = b.d_nonlin;
+    return 0;
+}
+
+template <typename T>
+static int run_residual(Net &nt, BatchWork &b, int npt, const double *x, int nterm,
+                        const double *coeff, const int *slot, const int *nonlin, double fac,
+                        Grad *g, double *rsq, float *rows_t, long ldr, bool dev)
+{
+    const int na = nt.tab.na, nout = nt.dims[nt.nlayer - 1];
+    DevResidual r;
+    int rc = set_residual(b, nterm, coeff, slot, nonlin, na, &r);
+    if (rc) return rc;
+    rc = run_forward<T>(nt, b, npt, x, nullptr, dev);
+    if (rc) return rc;
+    const int nblk = (int)((b.np + 255) / 256);
+    // rows: the seed is dR/dT, set_pinn_seed(t, R=1, fac=1) (sgd_batch_module.f90:742)
+    {
+        PScope p(b, CAT_OTHER);
+        launch_seed_residual<T>((const T *)b.tl, nout, na, npt, b.np, r, rows_t ? 1.0 : fac, rows_t != nullptr,
+                                (T *)b.p0, b.d_rsq_part, b.stream);
+    }
+    if (rows_t) {
+        backward_t<T>(nt, b, nullptr, rows_t, ldr);
+    } else {
+        backward_t<T>(nt, b, g->d_g, nullptr, 0);
+    }
+    if (rsq) {
+        launch_reduce_sum(b.d_rsq_part, nblk, b.d_rsq, b.stream);
+        HIPCHK(hipMemcpyAsync(rsq, b.d_rsq, sizeof(double), hipMemcpyDeviceToHost, b.stream));
+        HIPCHK(hipStreamSynchronize(b.stream));
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace dnnf90
+
+// ===========================================================================
+// C ABI
+using namespace dnnf90;
+
+extern "C" {
+
+const char *dnnf90_last_error(void) { return g_err.c_str(); }
+
+static bool table_args_ok(int d0, int k, int nseed, const int *seeds)
+{
+    if (d0 < 1 || k < 1 || k > 15 || nseed < 0) return false;
+    if (nseed > 0 && !seeds) return false;
+    for (int i = 0; i < d0 * nseed; ++i)
+        if (seeds[i] < 0 || seeds[i] > k) return false;
+    return true;
+}
+
+static int build_current(int d0, int k, int nseed, const int *seeds)
+{
+    if (!table_args_ok(d0, k, nseed, seeds)) return fail(-1, "bad table arguments");
+    TableSet t;
+    std::string err;
+    if (!t.build(d0, k, nseed, seeds, &err)) return fail(-1, err);
+    g_current = std::move(t);
+    g_current_ready = true;
+    return 0;
+}
+
+int dnnf90_tables_init_dense(int d0, int k) { return build_current(d0, k, 0, nullptr); }
+
+int dnnf90_tables_init_closure(int d0, int k, int nseed, const int *seeds)
+{
+    return build_current(d0, k, nseed, seeds);
+}
+
+int dnnf90_nderiv(void) { return g_current_ready ? g_current.na : 0; }
+
+int dnnf90_alpha(int ia, int *a)
+{
+    if (!g_current_ready || ia < 1 || ia > g_current.na || !a) return fail(-1, "slot outside [1, nderiv]");
+    for (int v = 0; v < g_current.d0; ++v) a[v] = g_current.alpha[(size_t)(ia - 1) * g_current.d0 + v];
+    return 0;
+}
+
+static int new_tabset(int d0, int k, int nseed, const int *seeds)
+{
+    if (!table_args_ok(d0, k, nseed, seeds)) return fail(-1, "bad table arguments");
+    int id = g_tab.slot();
+    if (id < 0) return fail(-1, "no free table-set handle");
+    int rc = build_current(d0, k, nseed, seeds);
+    if (rc) return rc;
+    g_tab.obj[id] = new TableSet(g_current);
+    return id;
+}
+
+int dnnf90_tabset_dense(int d0, int k) { return new_tabset(d0, k, 0, nullptr); }
+
+int dnnf90_tabset_closure(int d0, int k, int nseed, const int *seeds)
+{
+    return new_tabset(d0, k, nseed, seeds);
+}
+
+int dnnf90_tabset_nderiv(int tid)
+{
+    TableSet *t = g_tab.get(tid);
+    return t ? t->na : fail(-1, "bad table-set handle");
+}
+
+int dnnf90_tabset_activation(int tid, int iact)
+{
+    TableSet *t = g_tab.get(tid);
+    if (!t) return fail(-1, "bad table-set handle");
+    if (iact < 0 || iact > 5) return fail(-2, "activation code outside 0..5");
+    t->iact = iact;
+    return 0;
+}
+
+int dnnf90_net_create(int nlayer, const int *dims)
+{
+    if (!g_current_ready) return fail(-1, "build the tables first");
+    TableSet ts = g_current;
+    ts.iact = ACT_TANH;
+    return make_net(ts, nlayer, dims);
+}
+
+int dnnf90_net_create_ts(int tid, int nlayer, const int *dims)
+{
+    TableSet *t = g_tab.get(tid);
+    if (!t) return fail(-1, "bad table-set handle");
+    return make_net(*t, nlayer, dims);
+}
+
+int dnnf90_net_load(const char *filename)
+{
+    std::string p = c_path(filename);
+    {
+        std::ifstream f(p);
+        if (!f) return fail(-2, "no such file: " + p);
+    }
+    if (!g_current_ready) return fail(-1, "build the tables first");
+    TableSet ts = g_current;
+    ts.iact = ACT_TANH;
+    int id;
+    int rc = load_weights(p, ts, &id);
+    return rc ? rc : id;
+}
+
+int dnnf90_net_load_ts(int tid, const char *filename)
+{
+    TableSet *t = g_tab.get(tid);
+    if (!t) return fail(-1, "bad table-set handle");
+    std::string p = c_path(filename);
+    {
+        std::ifstream f(p);
+        if (!f) return fail(-2, "no such file: " + p);
+    }
+    int id;
+    int rc = load_weights(p, *t, &id);
+    return rc ? rc : id;
+}
+
+int dnnf90_net_set_layer(int nid, int l, const double *wrow)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt || l < 2 || l > nt->nlayer || !wrow) return fail(-1, "bad network handle or layer");
+    std::lock_guard<std::mutex> lk(nt->mu);
+    const long n = (long)nt->dims[l - 1] * (nt->dims[l - 2] + 1);
+    std::memcpy(&nt->w[nt->woff[l - 1]], wrow, sizeof(double) * n);
+    nt->host_ver++;
+    return 0;
+}
+
+int dnnf90_net_free(int nid)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt) return fail(-1, "bad network handle");
+    nt->release();
+    delete nt;
+    g_net.obj[nid] = nullptr;
+    g_net.gen[nid]++;
+    return 0;
+}
+
+static int make_work(Registry<BatchWork> &reg, int nid, int maxpt, int prec)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt) return fail(-1, "bad network handle");
+    if (maxpt < 1 || prec < 0 || prec > 2) return fail(-1, "maxpt >= 1 and prec in 0..2");
+    int id = reg.slot();
+    if (id < 0) return fail(-1, "no free work-space handle");
+    int rc = nt->ensure_device(prec != PREC_FP64);
+    if (rc) return rc;
+    BatchWork *b = new BatchWork();
+    rc = bwork_init(*b, *nt, maxpt, prec);
+    if (rc) {
+        b->release();
+        delete b;
+        return rc;
+    }
+    reg.obj[id] = b;
+    reg.owner[id] = nid;
+    reg.ogen[id] = g_net.gen[nid];
+    return id;
+}
+
+static int free_work(Registry<BatchWork> &reg, int id)
+{
+    BatchWork *b = reg.get(id);
+    if (!b) return fail(-1, "bad work-space handle");
+    (void)hipStreamSynchronize(b->stream);
+    b->release();
+    delete b;
+    reg.obj[id] = nullptr;
+    return 0;
+}
+
+int dnnf90_work_create(int nid) { return make_work(g_work, nid, 1, PREC_FP64); }
+int dnnf90_work_free(int wid) { return free_work(g_work, wid); }
+
+int dnnf90_eval(int nid, int wid, const double *x, double *t)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt || !dependent_ok(g_work, wid, nid) || !x || !t) return fail(-1, "bad handle or argument");
+    BatchWork &b = *g_work.obj[wid];
+    const int na = nt->tab.na, nout = nt->dims[nt->nlayer - 1];
+    std::vector<double> tout((size_t)nout * na);
+    int rc = run_forward<double>(*nt, b, 1, x, tout.data(), false);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(b.stream));
+    std::memcpy(t, tout.data(), sizeof(double) * na);  // output 1
+    return 0;
+}
+
+int dnnf90_twork_create(int nid) { return make_work(g_twork, nid, 1, PREC_FP64); }
+int dnnf90_twork_free(int tid) { return free_work(g_twork, tid); }
+
+int dnnf90_grad_create(int nid)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt) return fail(-1, "bad network handle");
+    int id = g_grad.slot();
+    if (id < 0) return fail(-1, "no free accumulator handle");
+    Grad *g = new Grad();
+    g->nw = nt->nw;
+    int rc = dalloc(&g->d_g, (size_t)std::max(1, nt->nw));
+    if (rc) { delete g; return rc; }
+    launch_fill(g->d_g, nt->nw, 0.0, 0);
+    if (hipDeviceSynchronize() != hipSuccess) { g->release(); delete g; return fail(-3, "device failure"); }
+    g_grad.obj[id] = g;
+    g_grad.owner[id] = nid;
+    g_grad.ogen[id] = g_net.gen[nid];
+    return id;
+}
+
+int dnnf90_grad_free(int gid)
+{
+    Grad *g = g_grad.get(gid);
+    if (!g) return fail(-1, "bad accumulator handle");
+    (void)hipDeviceSynchronize();
+    g->release();
+    delete g;
+    g_grad.obj[gid] = nullptr;
+    return 0;
+}
+
+int dnnf90_grad_zero(int gid)
+{
+    Grad *g = g_grad.get(gid);
+    if (!g) return fail(-1, "bad accumulator handle");
+    HIPCHK(hipDeviceSynchronize());
+    launch_fill(g->d_g, g->nw, 0.0, 0);
+    HIPCHK(hipDeviceSynchronize());
+    return 0;
+}
+
+int dnnf90_grad_point(int nid, int tid, const double *x, const double *seed, int gid)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt || !dependent_ok(g_twork, tid, nid) || !grad_ok(gid, nid) || !x || !seed)
+        return fail(-1, "bad handle or argument");
+    BatchWork &b = *g_twork.obj[tid];
+    int rc = run_forward<double>(*nt, b, 1, x, nullptr, false);
+    if (rc) return rc;
+    const int na = nt->tab.na, nout = nt->dims[nt->nlayer - 1];
+    std::vector<double> sm((size_t)nout * na, 0.0);
+    std::memcpy(sm.data(), seed, sizeof(double) * na);  // output 1 only
+    rc = run_backward_seed<double>(*nt, b, sm.data(), *g_grad.obj[gid], false);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(b.stream));
+    return 0;
+}
+
+int dnnf90_adam_step(int nid, int gid, double eta, double beta1, double beta2, double eps,
+                     int nbatch, int istep)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt || !grad_ok(gid, nid)) return fail(-1, "bad handle");
+    Grad &g = *g_grad.obj[gid];
+    int rc = nt->ensure_device(false);
+    if (rc) return rc;
+    HIPCHK(hipDeviceSynchronize());
+    if (!g.d_m) {
+        if ((rc = dalloc(&g.d_m, (size_t)std::max(1, g.nw))) || (rc = dalloc(&g.d_v, (size_t)std::max(1, g.nw))))
+            return rc;
+        launch_fill(g.d_m, g.nw, 0.0, 0);
+        launch_fill(g.d_v, g.nw, 0.0, 0);
+    }
+    const int is = std::max(istep, 1);
+    const double bc1 = 1.0 - std::pow(beta1, is), bc2 = 1.0 - std::pow(beta2, is);
+    launch_adam(nt->d_w64, g.d_m, g.d_v, g.d_g, g.nw, eta, beta1, beta2, eps, (double)nbatch, bc1, bc2, 0);
+    HIPCHK(hipDeviceSynchronize());
+    std::lock_guard<std::mutex> lk(nt->mu);
+    HIPCHK(hipMemcpy(nt->w.data(), nt->d_w64, sizeof(double) * nt->nw, hipMemcpyDeviceToHost));
+    nt->host_ver++;
+    nt->dev_ver = nt->host_ver;  // the device copy is the updated one
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// extensions (include/dnnf90_amd.h)
+
+int dnnf90_tabset_counts(int tid, int *counts)
+{
+    TableSet *t = g_tab.get(tid);
+    if (!t || !counts) return fail(-1, "bad table-set handle");
+    counts[0] = t->na;
+    counts[1] = t->npair();
+    counts[2] = t->nfq();
+    counts[3] = t->k;
+    return 0;
+}
+
+int dnnf90_tabset_lists(int tid, int *pair_start, int *pair_num, int *pair_ia, int *pair_id,
+                        double *pair_binom, int *fq_start, int *fq_num, int *fq_ib, int *fq_id,
+                        double *fq_c)
+{
+    TableSet *t = g_tab.get(tid);
+    if (!t) return fail(-1, "bad table-set handle");
+    auto cp = [](auto *dst, const auto &src) {
+        if (dst && !src.empty()) std::memcpy(dst, src.data(), sizeof(src[0]) * src.size());
+    };
+    cp(pair_start, t->pair_start); cp(pair_num, t->pair_num); cp(pair_ia, t->pair_ia);
+    cp(pair_id, t->pair_id); cp(pair_binom, t->pair_binom); cp(fq_start, t->fq_start);
+    cp(fq_num, t->fq_num); cp(fq_ib, t->fq_ib); cp(fq_id, t->fq_id); cp(fq_c, t->fq_c);
+    return 0;
+}
+
+int dnnf90_net_nweights(int nid)
+{
+    Net *nt = g_net.get(nid);
+    return nt ? nt->nw : fail(-1, "bad network handle");
+}
+
+int dnnf90_net_nlayer(int nid)
+{
+    Net *nt = g_net.get(nid);
+    return nt ? nt->nlayer : fail(-1, "bad network handle");
+}
+
+int dnnf90_net_dims(int nid, int *dims)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt || !dims) return fail(-1, "bad network handle");
+    for (int l = 0; l < nt->nlayer; ++l) dims[l] = nt->dims[l];
+    return 0;
+}
+
+int dnnf90_net_nderiv(int nid)
+{
+    Net *nt = g_net.get(nid);
+    return nt ? nt->tab.na : fail(-1, "bad network handle");
+}
+
+int dnnf90_net_get_layer(int nid, int l, double *wrow)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt || l < 2 || l > nt->nlayer || !wrow) return fail(-1, "bad network handle or layer");
+    std::lock_guard<std::mutex> lk(nt->mu);
+    const long n = (long)nt->dims[l - 1] * (nt->dims[l - 2] + 1);
+    std::memcpy(wrow, &nt->w[nt->woff[l - 1]], sizeof(double) * n);
+    return 0;
+}
+
+int dnnf90_net_get_weights(int nid, double *w)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt || !w) return fail(-1, "bad network handle");
+    std::lock_guard<std::mutex> lk(nt->mu);
+    std::memcpy(w, nt->w.data(), sizeof(double) * nt->nw);
+    return 0;
+}
+
+int dnnf90_net_set_weights(int nid, const double *w)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt || !w) return fail(-1, "bad network handle");
+    std::lock_guard<std::mutex> lk(nt->mu);
+    std::memcpy(nt->w.data(), w, sizeof(double) * nt->nw);
+    nt->host_ver++;
+    return 0;
+}
+
+int dnnf90_net_save(int nid, const char *filename)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt || !filename) return fail(-1, "bad network handle");
+    FILE *f = std::fopen(filename, "w");
+    if (!f) return fail(-2, "cannot open " + std::string(filename));
+    std::fprintf(f, "0\nfunc %d\nActivation_out 0\n%d\n", nt->tab.iact, nt->nlayer);
+    for (int l = 0; l < nt->nlayer; ++l) std::fprintf(f, "%d\n", nt->dims[l]);
+    for (int l = 1; l < nt->nlayer; ++l) {
+        std::fprintf(f, " #l=           %d\n", l + 1);
+        const int nrow = nt->dims[l], ncol = nt->dims[l - 1] + 1;
+        for (int j = 0; j < nrow; ++j) {
+            for (int i = 0; i < ncol; ++i) {
+                std::fprintf(f, "%25.16E", nt->w[nt->woff[l] + (long)j * ncol + i]);
+                if ((i + 1) % 100 == 0 && i + 1 < ncol) std::fprintf(f, "\n");
+            }
+            std::fprintf(f, "\n");
+        }
+    }
+    std::fclose(f);
+    return 0;
+}
+
+int dnnf90_grad_get(int gid, double *g)
+{
+    Grad *gr = g_grad.get(gid);
+    if (!gr || !g) return fail(-1, "bad accumulator handle");
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(g, gr->d_g, sizeof(double) * gr->nw, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int dnnf90_grad_set(int gid, const double *g)
+{
+    Grad *gr = g_grad.get(gid);
+    if (!gr || !g) return fail(-1, "bad accumulator handle");
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(gr->d_g, g, sizeof(double) * gr->nw, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int dnnf90_bwork_create(int nid, int maxpt, int prec) { return make_work(g_bwork, nid, maxpt, prec); }
+int dnnf90_bwork_free(int bid) { return free_work(g_bwork, bid); }
+
+int dnnf90_bwork_set_stream(int bid, void *stream)
+{
+    BatchWork *b = g_bwork.get(bid);
+    if (!b) return fail(-1, "bad work-space handle");
+    b->stream = stream ? (hipStream_t)stream : b->own;
+    return 0;
+}
+
+int dnnf90_bwork_profile(int bid, int on)
+{
+    BatchWork *b = g_bwork.get(bid);
+    if (!b) return fail(-1, "bad work-space handle");
+    int rc = b->resolve();
+    if (rc) return rc;
+    b->prof.on = on != 0;
+    return 0;
+}
+
+int dnnf90_bwork_kernel_ms(int bid, double *ms, long *count)
+{
+    BatchWork *b = g_bwork.get(bid);
+    if (!b || !ms || !count) return fail(-1, "bad work-space handle");
+    int rc = b->resolve();
+    if (rc) return rc;
+    for (int c = 0; c < NCAT; ++c) {
+        ms[c] = b->prof.ms[c];
+        count[c] = b->prof.cnt[c];
+        b->prof.ms[c] = 0.0;
+        b->prof.cnt[c] = 0;
+    }
+    return 0;
+}
+
+int dnnf90_bwork_sync(int bid)
+{
+    BatchWork *b = g_bwork.get(bid);
+    if (!b) return fail(-1, "bad work-space handle");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int dnnf90_forward_batch(int nid, int bid, int npt, const double *x, double *tout, int flags)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt || !dependent_ok(g_bwork, bid, nid) || (!x && npt > 0)) return fail(-1, "bad handle or argument");
+    BatchWork &b = *g_bwork.obj[bid];
+    const bool dev = flags & DNNF90_DEVICE_PTRS;
+    int rc = b.prec == PREC_FP64 ? run_forward<double>(*nt, b, npt, x, tout, dev)
+                                 : run_forward<float>(*nt, b, npt, x, tout, dev);
+    if (rc) return rc;
+    if (!dev && tout) HIPCHK(hipStreamSynchronize(b.stream));
+    return 0;
+}
+
+int dnnf90_backward_batch(int nid, int bid, const double *seed, int gid, int flags)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt || !dependent_ok(g_bwork, bid, nid) || !grad_ok(gid, nid) || !seed)
+        return fail(-1, "bad handle or argument");
+    BatchWork &b = *g_bwork.obj[bid];
+    const bool dev = flags & DNNF90_DEVICE_PTRS;
+    int rc = b.prec == PREC_FP64 ? run_backward_seed<double>(*nt, b, seed, *g_grad.obj[gid], dev)
+                                 : run_backward_seed<float>(*nt, b, seed, *g_grad.obj[gid], dev);
+    if (rc) return rc;
+    if (!dev) HIPCHK(hipStreamSynchronize(b.stream));
+    return 0;
+}
+
+int dnnf90_residual_step_batch(int nid, int bid, int npt, const double *x, int nterm,
+                               const double *coeff, const int *slot, const int *nonlin, double fac,
+                               int gid, double *rsq, int flags)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt || !dependent_ok(g_bwork, bid, nid) || !grad_ok(gid, nid) || (!x && npt > 0) ||
+        (nterm > 0 && (!coeff || !slot)))
+        return fail(-1, "bad handle or argument");
+    BatchWork &b = *g_bwork.obj[bid];
+    const bool dev = flags & DNNF90_DEVICE_PTRS;
+    int rc = b.prec == PREC_FP64
+                 ? run_residual<double>(*nt, b, npt, x, nterm, coeff, slot, nonlin, fac, g_grad.obj[gid], rsq,
+                                        nullptr, 0, dev)
+                 : run_residual<float>(*nt, b, npt, x, nterm, coeff, slot, nonlin, fac, g_grad.obj[gid], rsq,
+                                       nullptr, 0, dev);
+    if (rc) return rc;
+    if (!dev) HIPCHK(hipStreamSynchronize(b.stream));
+    return 0;
+}
+
+int dnnf90_residual_rows_batch(int nid, int bid, int npt, const double *x, int nterm,
+                               const double *coeff, const int *slot, const int *nonlin, float *rows_t,
+                               long ld, int flags)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt || !dependent_ok(g_bwork, bid, nid) || (!x && npt > 0) || !rows_t ||
+        (nterm > 0 && (!coeff || !slot)))
+        return fail(-1, "bad handle or argument");
+    BatchWork &b = *g_bwork.obj[bid];
+    if (ld < round_up(std::max(npt, 1), 64)) return fail(-1, "row stride below the padded point count");
+    const bool dev = flags & DNNF90_DEVICE_PTRS;
+    int rc = b.prec == PREC_FP64
+                 ? run_residual<double>(*nt, b, npt, x, nterm, coeff, slot, nonlin, 1.0, nullptr, nullptr,
+                                        rows_t, ld, dev)
+                 : run_residual<float>(*nt, b, npt, x, nterm, coeff, slot, nonlin, 1.0, nullptr, nullptr,
+                                       rows_t, ld, dev);
+    if (rc) return rc;
+    if (!dev) HIPCHK(hipStreamSynchronize(b.stream));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,111 @@
+// kernels.hpp -- launch wrappers for the gfx950 kernels of the derivative-
+// carrying forward pass and its adjoint.
+//
+// Device layout (all layers): a layer's plane is row-major
+// [neuron][slot * Np + point], Np = padded point count, so the whole
+// multi-index axis and the point axis form one contiguous column axis of
+// length na*Np.  The linear maps are then single GEMMs over na*Np columns,
+// and one lane per (point, neuron) reads its slots coalesced across points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace dnnf90 {
+
+enum Precision : int {
+    PREC_FP64 = 0,    // everything in FP64 (drop-in parity mode)
+    PREC_FP32 = 1,    // FP32 storage, FP32 MFMA linear maps, FP32 recursion
+    PREC_BF16X3 = 2,  // FP32 storage, BF16 hi+lo split MFMA linear maps
+};
+
+// forward schedule on the device (see FwdSchedule in tables.hpp)
+struct DevFwdSched {
+    const int *slot_group;   // [na+1]
+    const int *grp_term;     // [ngroup+1]
+    const int *grp_store;    // [ngroup]
+    const int2 *term_sb;     // [F] (operand entry of S_b, of B(a-b,q-1))
+    const double *term_c64;  // [F]
+    const float *term_c32;   // [F]
+    int na, k, nentry;
+};
+
+struct DevPairs {
+    const int *start, *num;  // [na]
+    const int2 *ia_id;       // [P]
+    const double *binom64;   // [P]
+    const float *binom32;    // [P]
+    int na;
+};
+
+struct ActParams {
+    int iact, nq;            // nq = k+1 derivatives beyond sigma itself
+    const double *tpc;       // [(k+2)][(k+3)]
+    int ldt;                 // k+3
+};
+
+// LDS bytes per thread needed by the forward/adjoint kernels (0 = global path)
+int bell_fwd_threads(int nentry, int elem_bytes);
+int bell_bwd_threads(int na, int elem_bytes);
+
+template <typename T>
+void launch_input(const double *x, int npt, int d0, int na, const int *ind_e1, T *T0,
+                  long Np, hipStream_t s);
+
+template <typename T>
+void launch_bell_fwd(const T *S, T *Tout, T *H, int nrow, long Np, const DevFwdSched &sch,
+                     const ActParams &act, T *scratch, hipStream_t s);
+
+template <typename T>
+void launch_bell_bwd(const T *Tb, const T *H, T *Sb, int nrow, long Np, const DevPairs &pr,
+                     hipStream_t s);
+
+// C[M][ncol] = A[M][Kd] * Bext[Kd][ncol]
+//   trans_a == false: A = W[M][Kd] (ldw = Kd, bias first); Bext row 0 is the
+//                     bias channel (1 on the first Np columns), rows 1.. = B
+//   trans_a == true : A[m][k] = W[k][m+1] (ldw = M+1); Bext = B
+template <typename T>
+void launch_gemm_wx(int prec, bool trans_a, const T *W, int ldw, const T *B, T *C, int M, int Kd,
+                    long ncol, long Np, hipStream_t s);
+
+// partial[chunk][M][Kd] = sum over the chunk's columns of A[m][c] * Bext[k][c]
+// (Bext row 0 = bias channel, rows 1.. = B); then grad[M][Kd] += sum_chunk
+template <typename T>
+void launch_gemm_grad(int prec, const T *A, const T *B, int M, int Kd, long ncol, long Np,
+                      double *partial, int nchunk, long chunk_cols, double *grad, hipStream_t s);
+
+// per-point rows: rows[(j*Kd + k)][n] = sum_a A[j][a*Np+n] * Bext[k][a*Np+n]
+template <typename T>
+void launch_point_rows(const T *A, const T *B, int M, int Kd, int na, long Np, float *rowsT,
+                       long ldr, hipStream_t s);
+
+template <typename T>
+void launch_seed_user(const double *seed, int npt, int nout, int na, T *Tb, long Np, hipStream_t s);
+
+struct DevResidual {
+    int nterm;
+    const double *coeff;
+    const int *slot;
+    const int *nonlin;
+};
+
+// R_n = sum_k c_k [T_0 *] T_slot (output 0); seed = fac*R*dR/dT on output 0
+// (unit_r: seed = fac*dR/dT, the Gauss-Newton row seed); rsq_part[block] =
+// sum of R^2 over the block's points; rows > 0 zeroed
+template <typename T>
+void launch_seed_residual(const T *TL, int nout, int na, int npt, long Np, const DevResidual &r,
+                          double fac, bool unit_r, T *Tb, double *rsq_part, hipStream_t s);
+
+template <typename T>
+void launch_output(const T *TL, int nout, int na, int npt, long Np, double *tout, hipStream_t s);
+
+void launch_convert_f64_f32(const double *src, float *dst, long n, hipStream_t s);
+void launch_reduce_sum(const double *part, int n, double *out, hipStream_t s);
+void launch_adam(double *w, double *m, double *v, const double *g, long n, double eta,
+                 double beta1, double beta2, double eps, double nbatch, double bc1,
+                 double bc2, hipStream_t s);
+void launch_fill(double *p, long n, double v, hipStream_t s);
+void launch_axpy(double *y, const double *x, long n, double a, hipStream_t s);
+
+}  // namespace dnnf90

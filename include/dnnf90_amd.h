@@ -1,0 +1,103 @@
+/* dnnf90_amd.h -- batched and device-pointer entries of the MI355X engine.
+ *
+ * The reference ABI (dnnf90.h) is per point and has no gradient read-out,
+ * no split forward/backward and no multi-output entry (lib/dnnf90.h:40-54).
+ * These entries add them for GPU throughput; every one has the same return
+ * conventions as dnnf90.h.  Arrays are host arrays unless `flags` has
+ * DNNF90_DEVICE_PTRS, in which case every array argument is a device
+ * pointer (for example a torch tensor's data pointer) on the current device.
+ *
+ * Layouts: points x[npt][d0]; outputs and seeds [npt][nout][nderiv];
+ * weights and gradients concatenate, for l = 2..nlayer, ndim(l) rows of
+ * ndim(l-1)+1 doubles, bias first (one "#l=" block of nn_weight.dat).
+ */
+#ifndef DNNF90_AMD_H
+#define DNNF90_AMD_H
+#include "dnnf90.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DNNF90_DEVICE_PTRS 1
+
+/* arithmetic of a batched work space */
+#define DNNF90_PREC_FP64 0   /* everything FP64: reference parity mode */
+#define DNNF90_PREC_FP32 1   /* FP32 storage, FP32 MFMA, FP32 recursion */
+#define DNNF90_PREC_BF16X3 2 /* FP32 storage, BF16 hi+lo split MFMA */
+
+/* message of the last failure on the calling thread ("" if none) */
+const char *dnnf90_last_error(void);
+
+/* table-set introspection (host only, no GPU needed): counts[0..3] =
+ * nderiv, number of adjoint pairs P, number of forward terms F, k; then the
+ * lists of lib/multi_index_bell_module.f90:267-332 with 0-based slots:
+ * pair_start/num [nderiv], pair_ia/id/binom [P], fq_start/num
+ * [nderiv][k+1], fq_ib/id/c [F] (any pointer may be NULL) */
+int dnnf90_tabset_counts(int tid, int *counts);
+int dnnf90_tabset_lists(int tid, int *pair_start, int *pair_num, int *pair_ia,
+                        int *pair_id, double *pair_binom, int *fq_start,
+                        int *fq_num, int *fq_ib, int *fq_id, double *fq_c);
+
+int dnnf90_net_nweights(int nid);
+int dnnf90_net_nlayer(int nid);
+int dnnf90_net_dims(int nid, int *dims);
+int dnnf90_net_nderiv(int nid);
+/* inverse of dnnf90_net_set_layer: reads layer l (2..nlayer) */
+int dnnf90_net_get_layer(int nid, int l, double *wrow);
+/* whole weight vector (nweights doubles), read and write */
+int dnnf90_net_get_weights(int nid, double *w);
+int dnnf90_net_set_weights(int nid, const double *w);
+/* nn_weight.dat writer (the format lib/net_module.f90:219-243 writes) */
+int dnnf90_net_save(int nid, const char *filename);
+
+/* gradient read-out and write (nweights doubles) */
+int dnnf90_grad_get(int gid, double *g);
+int dnnf90_grad_set(int gid, const double *g);
+
+/* batched work space for up to maxpt points in arithmetic prec */
+int dnnf90_bwork_create(int nid, int maxpt, int prec);
+int dnnf90_bwork_free(int bid);
+/* launch on the caller's HIP stream instead of the work space's own
+ * (stream is a hipStream_t; NULL restores the work space's own stream) */
+int dnnf90_bwork_set_stream(int bid, void *stream);
+/* wait for everything launched on the work space's stream */
+int dnnf90_bwork_sync(int bid);
+/* kernel timing: with profiling on, every kernel category is bracketed by
+ * HIP events on the work space's stream; dnnf90_bwork_kernel_ms returns and
+ * resets the accumulated milliseconds and launch counts of the six
+ * categories: 0 linear map (W*T), 1 Faa di Bruno forward, 2 adjoint pair
+ * sums, 3 weight gradient (or per-point rows), 4 transposed linear map
+ * (W^T*Sbar), 5 input/seed/output kernels */
+int dnnf90_bwork_profile(int bid, int on);
+int dnnf90_bwork_kernel_ms(int bid, double *ms, long *count);
+
+/* forward over npt points; tout [npt][nout][nderiv] (may be NULL); the
+ * work space keeps every layer's state for a following backward */
+int dnnf90_forward_batch(int nid, int bid, int npt, const double *x,
+                         double *tout, int flags);
+/* adjoint of seed [npt][nout][nderiv] (dL/dT) after a forward on the same
+ * work space; the weight gradient is ADDED to gid */
+int dnnf90_backward_batch(int nid, int bid, const double *seed, int gid,
+                          int flags);
+/* fused collocation step on output 1: R = sum_k coeff[k] * [T_1 *] T_slot[k]
+ * (slot 1-based; nonlin[k] != 0 multiplies by the value T_1), seed =
+ * fac * R * dR/dT (app/pinn_module.f90:32-65), adjoint, gradient ADDED to
+ * gid.  coeff, slot and nonlin are host arrays whatever the flags; *rsq
+ * (host, may be NULL; when given the call waits for the result) receives
+ * sum_n R_n^2 */
+int dnnf90_residual_step_batch(int nid, int bid, int npt, const double *x,
+                               int nterm, const double *coeff,
+                               const int *slot, const int *nonlin,
+                               double fac, int gid, double *rsq, int flags);
+/* Gauss-Newton rows dR_n/dw of the same residual (seed dR/dT), written
+ * transposed as FP32 rows_t[w][n], w < nweights, row stride ld >= npt
+ * (device pointer always) */
+int dnnf90_residual_rows_batch(int nid, int bid, int npt, const double *x,
+                               int nterm, const double *coeff,
+                               const int *slot, const int *nonlin,
+                               float *rows_t, long ld, int flags);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,139 @@
+"""The C-ABI library on the host (no GPU): it loads, exports every symbol
+include/*.h declares, builds tables identical to the oracle's, and honours
+the reference's argument checks and return codes
+(lib/c_api_module.f90:72-96, :100-205, :476-528)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def L():
+    from dnnf90_amd._lib import LIBPATH, lib
+    if not os.path.exists(LIBPATH):
+        from dnnf90_amd import build
+        build.build()
+    return lib()
+
+
+def declared_symbols():
+    names = []
+    for h in ("dnnf90.h", "dnnf90_amd.h"):
+        src = open(os.path.join(ROOT, "include", h)).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        names += re.findall(r"\b(dnnf90_\w+)\s*\(", src)
+    return sorted(set(names))
+
+
+def test_exports_every_declared_symbol(L):
+    names = declared_symbols()
+    assert len(names) >= 24
+    for n in names:
+        assert hasattr(L, n), n
+    ref = re.findall(r"\b(dnnf90_\w+)\s*\(", re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "dnnf90.h")).read(), flags=re.S))
+    assert len(set(ref)) == 24   # the reference's lib/dnnf90.h:16-54
+
+
+def _lists(L, tid):
+    c = np.zeros(4, np.int32)
+    assert L.dnnf90_tabset_counts(tid, c.ctypes.data) == 0
+    na, P, F, k = (int(v) for v in c)
+    arr = [np.zeros(na, np.int32), np.zeros(na, np.int32), np.zeros(P, np.int32), np.zeros(P, np.int32),
+           np.zeros(P), np.zeros(na * (k + 1), np.int32), np.zeros(na * (k + 1), np.int32),
+           np.zeros(F, np.int32), np.zeros(F, np.int32), np.zeros(F)]
+    assert L.dnnf90_tabset_lists(tid, *[a.ctypes.data for a in arr]) == 0
+    return arr
+
+
+TAB_CASES = [(2, 3, [[0, 1], [1, 0], [3, 0]]), (4, 3, None), (4, 7, [[7, 0, 0, 0], [1, 0, 0, 0], [0, 0, 0, 1], [3, 2, 2, 0]]),
+             (3, 4, None), (2, 4, [[4, 0], [0, 2], [1, 0]]), (4, 7, None), (6, 3, None), (1, 15, None),
+             (5, 5, [[2, 1, 0, 1, 1], [0, 0, 5, 0, 0]])]
+
+
+@pytest.mark.parametrize("d0,k,seeds", TAB_CASES)
+def test_tables_bitwise_vs_oracle(L, oracle_mod, d0, k, seeds):
+    import dnnf90_amd as D
+    ts = D.TableSet(d0, k, seeds)
+    o = oracle_mod.Oracle(d0, k, seeds)
+    assert ts.na == o.na
+    got = _lists(L, ts.id)
+    st, nm, ia, idd, b = o.pairs()
+    fst, fnm, fib, fid, fc = o.fq()
+    for g, r in zip(got, [st, nm, ia, idd, b, fst.reshape(-1), fnm.reshape(-1), fib, fid, fc]):
+        assert np.array_equal(g, r)
+    al = np.array([a[:d0] for a in D.current_alpha()])
+    assert np.array_equal(al, o.alpha())
+
+
+def test_force_field_sized_set(L):
+    import dnnf90_amd as D
+    ts = D.TableSet(100, 2)
+    assert ts.na == 1 + 100 + 5050
+
+
+def test_argument_checks(L):
+    s = np.array([0, 1, 1, 0, 3, 0], np.int32)
+    assert L.dnnf90_tables_init_dense(0, 3) == -1
+    assert L.dnnf90_tables_init_dense(2, 16) == -1
+    assert L.dnnf90_tables_init_dense(2, 0) == -1
+    bad = np.array([0, 1, -1, 0], np.int32)
+    assert L.dnnf90_tables_init_closure(2, 3, 2, bad.ctypes.data) == -1
+    big = np.array([0, 4], np.int32)
+    assert L.dnnf90_tables_init_closure(2, 3, 1, big.ctypes.data) == -1
+    assert L.dnnf90_tables_init_closure(2, 3, 3, s.ctypes.data) == 0
+    assert L.dnnf90_nderiv() == 5
+    a = np.zeros(2, np.int32)
+    assert L.dnnf90_alpha(0, a.ctypes.data) == -1
+    assert L.dnnf90_alpha(6, a.ctypes.data) == -1
+    assert L.dnnf90_alpha(5, a.ctypes.data) == 0 and list(a) == [3, 0]
+    tid = L.dnnf90_tabset_dense(2, 3)
+    assert tid > 0
+    assert L.dnnf90_tabset_activation(tid, 9) == -2
+    assert L.dnnf90_tabset_activation(tid + 50, 0) == -1
+    assert L.dnnf90_tabset_nderiv(tid) == 10
+    dims = np.array([2, 5, 1], np.int32)
+    assert L.dnnf90_net_create_ts(tid, 1, dims.ctypes.data) == -1
+    wrongd0 = np.array([3, 5, 1], np.int32)
+    assert L.dnnf90_net_create_ts(tid, 3, wrongd0.ctypes.data) == -1
+    zero = np.array([2, 0, 1], np.int32)
+    assert L.dnnf90_net_create_ts(tid, 3, zero.ctypes.data) == -1
+    assert L.dnnf90_net_load(b"/nonexistent/nn_weight.dat") == -2
+    nid = L.dnnf90_net_create_ts(tid, 3, dims.ctypes.data)
+    assert nid > 0
+    row = np.zeros(5 * 3)
+    assert L.dnnf90_net_set_layer(nid, 1, row.ctypes.data) == -1
+    assert L.dnnf90_net_set_layer(nid, 4, row.ctypes.data) == -1
+    assert L.dnnf90_net_set_layer(nid, 2, row.ctypes.data) == 0
+    assert L.dnnf90_net_free(nid) == 0
+    assert L.dnnf90_net_free(nid) == -1
+    assert L.dnnf90_net_set_layer(nid, 2, row.ctypes.data) == -1
+
+
+def test_weight_file_round_trip(L, tmp_path):
+    """nn_weight.dat written by dnnf90_net_save is read back exactly, and the
+    reference's own weight files load with their shapes."""
+    import dnnf90_amd as D
+    ts = D.TableSet(2, 3, [[0, 1], [1, 0], [3, 0]])
+    net = D.Net(ts, [2, 7, 130, 1])   # a row longer than 100 values wraps
+    w = np.random.default_rng(0).normal(size=net.nw)
+    net.weights = w
+    p = tmp_path / "nn_weight.dat"
+    net.save(p)
+    back = D.Net.load(p, ts)
+    assert back.dims == [2, 7, 130, 1]
+    assert np.array_equal(back.weights, w)
+    assert np.array_equal(back.get_layer(3), w[7 * 3:7 * 3 + 130 * 8])
+    # a file trained with another activation is refused against a tanh set
+    erf = D.TableSet(2, 3, [[0, 1], [1, 0], [3, 0]], act=D.ACT_ERF)
+    net2 = D.Net(erf, [2, 4, 1])
+    q = tmp_path / "erf.dat"
+    net2.save(q)
+    with pytest.raises(D.DNNF90Error) as e:
+        D.Net.load(q, ts)
+    assert e.value.rc == -2
+    assert D.Net.load(q, erf).dims == [2, 4, 1]

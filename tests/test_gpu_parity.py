@@ -1,0 +1,160 @@
+"""GPU parity: the gfx950 path through the C ABI against the oracle.
+
+The oracle (oracle/) is the C FP64 restatement of lib/train_module.f90:241-450
+that agrees bitwise with the reference's own lib/ (tests/test_oracle.py).
+FP64 mode must match it to 1e-12 relative, per derivative order, norm-wise
+(max |diff| over points and slots of the order / max |oracle| of the order),
+the metric of app/hod_check_module.f90:216, 302.
+"""
+import numpy as np
+import pytest
+
+from helpers import GOLDEN_CASES, check_weights, read_golden, rel, rel_by_order
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+
+
+@pytest.fixture(scope="module")
+def D():
+    import dnnf90_amd
+    return dnnf90_amd
+
+
+def _setup(D, oracle_mod, wl, npt):
+    ts = D.TableSet(wl.d0, wl.k, wl.seeds, wl.act)
+    net = D.Net(ts, wl.dims)
+    w, x = wl.inputs(npt)
+    net.weights = w
+    o = oracle_mod.Oracle(wl.d0, wl.k, wl.seeds, wl.act)
+    return ts, net, w, x, o
+
+
+WL_SMALL = [("kdv", 300), ("dense4_k3", 197), ("closure4_k7", 130), ("gelu3_k4", 70), ("ngd2_k4", 257)]
+
+
+@pytest.mark.parametrize("case", list(GOLDEN_CASES))
+def test_golden_per_point_abi(D, case):
+    """dnnf90_eval on the GPU against the reference's golden files."""
+    d0, k, seeds, dims = GOLDEN_CASES[case]
+    ts = D.TableSet(d0, k, seeds)
+    net = D.Net(ts, dims)
+    net.weights = check_weights(dims)
+    x, gold = read_golden(case)
+    wk = D.Work(net)
+    t = wk.eval(x)
+    print(case, "max|gpu - golden| =", np.abs(t - gold).max())
+    assert np.abs(t - gold).max() <= 1e-13
+    wk.free()
+    net.free()
+
+
+@pytest.mark.parametrize("name,npt", WL_SMALL)
+def test_forward_fp64(D, oracle_mod, name, npt):
+    from dnnf90_amd.workloads import WORKLOADS
+    wl = WORKLOADS[name]
+    ts, net, w, x, o = _setup(D, oracle_mod, wl, npt)
+    bw = D.BatchWork(net, npt, D.PREC_FP64)
+    t = bw.forward(x)
+    tr = o.eval(wl.dims, w, x)
+    err = rel_by_order(t, tr, o.deg())
+    print(name, "forward rel err by order", err)
+    assert max(err.values()) <= TOL64
+
+
+@pytest.mark.parametrize("name,npt", WL_SMALL)
+def test_adjoint_fp64(D, oracle_mod, name, npt):
+    from dnnf90_amd.workloads import WORKLOADS
+    wl = WORKLOADS[name]
+    ts, net, w, x, o = _setup(D, oracle_mod, wl, npt)
+    bw = D.BatchWork(net, npt, D.PREC_FP64)
+    g = D.Grad(net)
+    seed = np.random.default_rng(7).standard_normal((npt, wl.dims[-1], o.na))
+    bw.forward(x, want=False)
+    bw.backward(seed, g)
+    gg = g.get()
+    gr = o.grad(wl.dims, w, x, seed)
+    e = rel(gg, gr)
+    print(name, "gradient rel err", e)
+    assert e <= TOL64
+
+
+@pytest.mark.parametrize("name,npt", WL_SMALL)
+def test_residual_step_fp64(D, oracle_mod, name, npt):
+    from dnnf90_amd.workloads import WORKLOADS
+    wl = WORKLOADS[name]
+    ts, net, w, x, o = _setup(D, oracle_mod, wl, npt)
+    terms = wl.terms(o.alpha())
+    bw = D.BatchWork(net, npt, D.PREC_FP64)
+    g = D.Grad(net)
+    rsq = bw.residual_step(x, D.Residual(terms), 2.0 / npt, g)
+    gr, rsq_r, _ = o.pinn_step(wl.dims, w, x, terms, 2.0 / npt)
+    print(name, "rsq", rsq, rsq_r, "grad rel", rel(g.get(), gr))
+    assert abs(rsq - rsq_r) <= 1e-12 * abs(rsq_r)
+    assert rel(g.get(), gr) <= TOL64
+
+
+def test_per_point_abi_and_adam(D, oracle_mod):
+    """dnnf90_grad_point and dnnf90_adam_step (per-point reference ABI)."""
+    from dnnf90_amd.workloads import WORKLOADS
+    wl = WORKLOADS["kdv"]
+    ts, net, w, x, o = _setup(D, oracle_mod, wl, 9)
+    tw = D.TWork(net)
+    g = D.Grad(net)
+    seed = np.random.default_rng(3).standard_normal((9, 1, o.na))
+    for n in range(9):
+        tw.grad_point(x[n], seed[n, 0], g)
+    gr = o.grad(wl.dims, w, x, seed)
+    assert rel(g.get(), gr) <= TOL64
+    # Adam (lib/train_module.f90:656-683) against the same update in numpy
+    eta, b1, b2, eps, nb = 1e-3, 0.9, 0.999, 1e-8, 9
+    m = np.zeros_like(w)
+    v = np.zeros_like(w)
+    ww = w.copy()
+    gv = g.get()
+    for it in (1, 2):
+        D.adam_step(net, g, eta, b1, b2, eps, nb, it)
+        q = gv / nb
+        m = b1 * m + (1 - b1) * q
+        v = b2 * v + (1 - b2) * q ** 2
+        ww = ww - eta * (m / (1 - b1 ** it)) / (np.sqrt(v / (1 - b2 ** it)) + eps)
+    assert np.abs(net.weights - ww).max() <= 1e-15
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-4), (2, 1e-4)])
+@pytest.mark.parametrize("name,npt", [("dense4_k3", 197), ("closure4_k7", 130), ("gelu3_k4", 70)])
+def test_mixed_precision(D, oracle_mod, name, npt, prec, tol):
+    from dnnf90_amd.workloads import WORKLOADS
+    wl = WORKLOADS[name]
+    ts, net, w, x, o = _setup(D, oracle_mod, wl, npt)
+    bw = D.BatchWork(net, npt, prec)
+    t = bw.forward(x)
+    tr = o.eval(wl.dims, w, x)
+    err = rel_by_order(t, tr, o.deg())
+    g = D.Grad(net)
+    seed = np.random.default_rng(7).standard_normal((npt, wl.dims[-1], o.na))
+    bw.backward(seed, g)
+    ge = rel(g.get(), o.grad(wl.dims, w, x, seed))
+    print(name, "prec", prec, "forward err by order", err, "grad err", ge)
+    assert max(err.values()) <= tol and ge <= tol
+
+
+def test_rows_fp64(D, oracle_mod):
+    """Gauss-Newton rows dR_n/dw (the NGD metric rows) against the oracle."""
+    import torch
+    from dnnf90_amd.workloads import WORKLOADS
+    wl = WORKLOADS["ngd2_k4"]
+    npt = 100
+    ts, net, w, x, o = _setup(D, oracle_mod, wl, npt)
+    terms = wl.terms(o.alpha())
+    bw = D.BatchWork(net, npt, D.PREC_FP64)
+    ld = 128
+    rows_t = torch.zeros((net.nw, ld), dtype=torch.float32, device="cuda")
+    bw.residual_rows(x, D.Residual(terms), rows_t, ld)
+    torch.cuda.synchronize()
+    rows = rows_t[:, :npt].double().cpu().numpy().T
+    rr = o.pinn_rows(wl.dims, w, x, terms)
+    e = rel(rows, rr)
+    print("rows rel err", e)
+    assert e <= 1e-6
