@@ -55,7 +55,33 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
+    build_fortran(verbose)
     return LIB
+
+
+FORTRAN_EXAMPLE = os.path.join(HERE, "fortran", "example_fd")
+
+
+def build_fortran(verbose: bool = False) -> str | None:
+    """The Fortran binding (fortran/dnnf90_gpu.f90) and its example program,
+    compiled with amdflang and linked against the library."""
+    fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+    if not os.path.exists(fc):
+        return None
+    bdir = os.path.join(HERE, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    fdir = os.path.join(HERE, "fortran")
+    cmds = [[fc, "-O2", "-module-dir", bdir, "-c", os.path.join(fdir, "dnnf90_gpu.f90"),
+             "-o", os.path.join(bdir, "dnnf90_gpu.o")],
+            [fc, "-O2", "-I", bdir, os.path.join(fdir, "example_fd.f90"), os.path.join(bdir, "dnnf90_gpu.o"),
+             "-L", HERE, "-ldnnf90_amd", "-Wl,-rpath,$ORIGIN/..", "-o", FORTRAN_EXAMPLE]]
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("amdflang failed:\n" + r.stdout + r.stderr)
+    return FORTRAN_EXAMPLE
 
 
 if __name__ == "__main__":
