@@ -275,6 +275,23 @@ class BatchWork:
                                               pr, int(ld), DEVICE_PTRS if dev else 0),
              "dnnf90_residual_rows_batch")
 
+    def ngd_solve(self, rows_t, ld, nrow, b, nb, mu, mode=1):
+        """u = (R R^T/nb + cI)^{-1} (b/nb); c = mu (mode 1) or mu tr(G) (mode 0)."""
+        rk, pr, rdev = _ptr(rows_t, np.float32)
+        if not rdev:
+            raise TypeError("rows_t must be a device tensor")
+        bk, pb, bdev = _ptr(b, np.float64)
+        if bdev:
+            import torch
+            u = torch.empty_like(b)
+            pu = u.data_ptr()
+        else:
+            u = np.zeros(self.net.nw)
+            pu = u.ctypes.data
+        _chk(lib().dnnf90_ngd_solve(self.net.id, self.id, int(nrow), pr, int(ld), pb, int(nb), float(mu),
+                                    int(mode), pu, DEVICE_PTRS if bdev else 0), "dnnf90_ngd_solve")
+        return u
+
     def free(self):
         if self.id:
             lib().dnnf90_bwork_free(self.id)

@@ -77,28 +77,30 @@ static int upload(T **dst, const std::vector<T> &src)
 struct DevTables {
     int na = 0, k = 0, d0 = 0, iact = 0, nentry = 0;
     int *ind_e1 = nullptr, *slot_group = nullptr, *grp_term = nullptr, *grp_store = nullptr;
-    int2 *term_sb = nullptr;
-    double *term_c64 = nullptr, *pbin64 = nullptr, *tpc = nullptr;
-    float *term_c32 = nullptr, *pbin32 = nullptr;
+    TermRec64 *t64 = nullptr, *p64 = nullptr;
+    TermRec32 *t32 = nullptr, *p32 = nullptr;
+    double *tpc = nullptr;
     int *pstart = nullptr, *pnum = nullptr;
-    int2 *pia_id = nullptr;
     bool ready = false;
 
     int build(const TableSet &t, const FwdSchedule &s)
     {
         na = t.na; k = t.k; d0 = t.d0; iact = t.iact; nentry = s.nentry;
-        std::vector<int2> tsb(s.term_s.size()), pid(t.pair_ia.size());
-        for (size_t i = 0; i < tsb.size(); ++i) tsb[i] = make_int2(s.term_s[i], s.term_b[i]);
-        for (size_t i = 0; i < pid.size(); ++i) pid[i] = make_int2(t.pair_ia[i], t.pair_id[i]);
-        std::vector<float> c32(s.term_c.begin(), s.term_c.end());
-        std::vector<float> b32(t.pair_binom.begin(), t.pair_binom.end());
+        std::vector<TermRec64> h64(s.term_s.size()), q64(t.pair_ia.size());
+        std::vector<TermRec32> h32(s.term_s.size()), q32(t.pair_ia.size());
+        for (size_t i = 0; i < h64.size(); ++i) {
+            h64[i] = TermRec64{s.term_s[i], s.term_b[i], s.term_c[i]};
+            h32[i] = TermRec32{s.term_s[i], s.term_b[i], (float)s.term_c[i], 0};
+        }
+        for (size_t i = 0; i < q64.size(); ++i) {
+            q64[i] = TermRec64{t.pair_ia[i], t.pair_id[i], t.pair_binom[i]};
+            q32[i] = TermRec32{t.pair_ia[i], t.pair_id[i], (float)t.pair_binom[i], 0};
+        }
         int rc = 0;
         if ((rc = upload(&ind_e1, t.ind_e1)) || (rc = upload(&slot_group, s.slot_group)) ||
             (rc = upload(&grp_term, s.grp_term)) || (rc = upload(&grp_store, s.grp_store)) ||
-            (rc = upload(&term_sb, tsb)) || (rc = upload(&term_c64, s.term_c)) ||
-            (rc = upload(&term_c32, c32)) || (rc = upload(&pstart, t.pair_start)) ||
-            (rc = upload(&pnum, t.pair_num)) || (rc = upload(&pia_id, pid)) ||
-            (rc = upload(&pbin64, t.pair_binom)) || (rc = upload(&pbin32, b32)) ||
+            (rc = upload(&t64, h64)) || (rc = upload(&t32, h32)) || (rc = upload(&pstart, t.pair_start)) ||
+            (rc = upload(&pnum, t.pair_num)) || (rc = upload(&p64, q64)) || (rc = upload(&p32, q32)) ||
             (rc = upload(&tpc, t.tpc)))
             return rc;
         ready = true;
@@ -106,23 +108,22 @@ struct DevTables {
     }
     void release()
     {
-        dfree(ind_e1); dfree(slot_group); dfree(grp_term); dfree(grp_store); dfree(term_sb);
-        dfree(term_c64); dfree(term_c32); dfree(pstart); dfree(pnum); dfree(pia_id);
-        dfree(pbin64); dfree(pbin32); dfree(tpc);
+        dfree(ind_e1); dfree(slot_group); dfree(grp_term); dfree(grp_store); dfree(t64); dfree(t32);
+        dfree(pstart); dfree(pnum); dfree(p64); dfree(p32); dfree(tpc);
         ready = false;
     }
     DevFwdSched fwd() const
     {
         DevFwdSched f;
         f.slot_group = slot_group; f.grp_term = grp_term; f.grp_store = grp_store;
-        f.term_sb = term_sb; f.term_c64 = term_c64; f.term_c32 = term_c32;
+        f.t64 = t64; f.t32 = t32;
         f.na = na; f.k = k; f.nentry = nentry;
         return f;
     }
     DevPairs pairs() const
     {
         DevPairs p;
-        p.start = pstart; p.num = pnum; p.ia_id = pia_id; p.binom64 = pbin64; p.binom32 = pbin32;
+        p.start = pstart; p.num = pnum; p.p64 = p64; p.p32 = p32;
         p.na = na;
         return p;
     }
@@ -213,6 +214,10 @@ struct BatchWork {
     double *d_rsq_part = nullptr;
     double *d_coeff = nullptr;
     int *d_slot = nullptr, *d_nonlin = nullptr, nterm_cap = 0;
+    // NGD metric and solve (allocated on first use)
+    double *ngd_G = nullptr, *ngd_part = nullptr, *ngd_u = nullptr, *ngd_trace = nullptr;
+    long ngd_G_cap = 0, ngd_part_cap = 0, ngd_u_cap = 0;
+    int *ngd_info = nullptr;
     std::vector<double> h_coeff;
     std::vector<int> h_slot, h_nonlin;
     bool fwd_valid = false;
@@ -255,6 +260,7 @@ struct BatchWork {
         tpl.clear(); hpl.clear();
         dfree(partial); dfree(d_x); dfree(d_seed); dfree(d_tout); dfree(d_rsq); dfree(d_rsq_part);
         dfree(d_coeff); dfree(d_slot); dfree(d_nonlin);
+        dfree(ngd_G); dfree(ngd_part); dfree(ngd_u); dfree(ngd_trace); dfree(ngd_info);
         if (own) (void)hipStreamDestroy(own);
         own = stream = nullptr;
     }
@@ -291,12 +297,14 @@ static int valloc(void **p, size_t bytes)
     return 0;
 }
 
-// gradient GEMM split: about 2048 workgroups over the output tiles
+// gradient GEMM split: about 1024 workgroups over the output tiles
+// (a tile is up to 128 x 144 of the weight block)
 static void grad_split(long ncol, int M, int Kd, int *nchunk, long *chunk)
 {
-    const int tiles = ((M + 63) / 64) * ((Kd + 63) / 64);
-    long nc = std::max(1, 2048 / tiles);
-    nc = std::min(nc, std::max(1L, ncol / 1024));
+    const int bm = M > 64 ? 128 : 64;
+    const int tiles = ((M + bm - 1) / bm) * ((Kd + 143) / 144);
+    long nc = std::max(1, 1024 / tiles);
+    nc = std::min(nc, std::max(1L, ncol / 512));
     long ch = round_up((ncol + nc - 1) / nc, 16);
     nc = (ncol + ch - 1) / ch;
     *nchunk = (int)nc;
@@ -336,7 +344,7 @@ static int bwork_init(BatchWork &b, const Net &nt, int maxpt, int prec)
         int nc;
         long ch;
         grad_split((long)col, nt.dims[l], nt.dims[l - 1] + 1, &nc, &ch);
-        pc = std::max(pc, (long)nc * nt.dims[l] * (nt.dims[l - 1] + 1));
+        pc = std::max(pc, (long)(nc + grad_reduce_splits(nc)) * nt.dims[l] * (nt.dims[l - 1] + 1));
     }
     b.partial_cap = pc;
     if ((rc = dalloc(&b.partial, (size_t)pc))) return rc;
@@ -1076,6 +1084,65 @@ int dnnf90_bwork_set_stream(int bid, void *stream)
     BatchWork *b = g_bwork.get(bid);
     if (!b) return fail(-1, "bad work-space handle");
     b->stream = stream ? (hipStream_t)stream : b->own;
+    return 0;
+}
+
+// NGD step on the GPU (app/optimizer_module.f90:221-469, primal route):
+// G = R R^T over the first nrow columns of the transposed FP32 rows, then
+// u = (G/nb + cI)^{-1} (b/nb) by an FP64 Cholesky
+int dnnf90_ngd_solve(int nid, int bid, int nrow, const float *rows_t, long ld, const double *b, int nb,
+                     double mu, int mode, double *u, int flags)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt || !dependent_ok(g_bwork, bid, nid) || !rows_t || !b || !u || nrow < 1 || nb < 1 || mu < 0.0 ||
+        (mode != 0 && mode != 1))
+        return fail(-1, "bad handle or argument");
+    const long ncol = round_up(nrow, 16);
+    if (ld < ncol || ld % 16 != 0) return fail(-1, "row stride must be a multiple of 16 and >= nrow");
+    BatchWork &bw = *g_bwork.obj[bid];
+    const int nw = nt->nw;
+    const bool dev = flags & DNNF90_DEVICE_PTRS;
+    const long len = (long)nw * nw;
+    const int tiles = ((nw + 127) / 128) * ((nw + 143) / 144);
+    int nc = (int)std::min<long>(std::max(1, 1024 / tiles), std::max(1L, ncol / 512));
+    long ch = round_up((ncol + nc - 1) / nc, 16);
+    nc = (int)((ncol + ch - 1) / ch);
+    int rc;
+    if (bw.ngd_G_cap < len) {
+        dfree(bw.ngd_G);
+        if ((rc = dalloc(&bw.ngd_G, (size_t)len))) return rc;
+        bw.ngd_G_cap = len;
+    }
+    const long pneed = nc > 1 ? (long)(nc + grad_reduce_splits(nc)) * len : 0;
+    if (bw.ngd_part_cap < pneed) {
+        dfree(bw.ngd_part);
+        if ((rc = dalloc(&bw.ngd_part, (size_t)pneed))) return rc;
+        bw.ngd_part_cap = pneed;
+    }
+    if (bw.ngd_u_cap < nw) {
+        dfree(bw.ngd_u);
+        if ((rc = dalloc(&bw.ngd_u, (size_t)nw))) return rc;
+        bw.ngd_u_cap = nw;
+    }
+    if (!bw.ngd_trace && ((rc = dalloc(&bw.ngd_trace, 1)) || (rc = dalloc(&bw.ngd_info, 1)))) return rc;
+    hipStream_t s = bw.stream;
+    if (nc > 1) launch_fill(bw.ngd_G, len, 0.0, s);
+    {
+        PScope p(bw, CAT_GRAD);
+        launch_gram(rows_t, nw, ld, bw.ngd_part, nc, ch, bw.ngd_G, s);
+    }
+    HIPCHK(hipMemcpyAsync(bw.ngd_u, b, sizeof(double) * nw, dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                          s));
+    {
+        PScope p(bw, CAT_OTHER);
+        ngd_factor_solve(bw.ngd_G, nw, nw, (double)nb, mu, mode, bw.ngd_u, bw.ngd_trace, bw.ngd_info, s);
+    }
+    int info = 0;
+    HIPCHK(hipMemcpyAsync(&info, bw.ngd_info, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(u, bw.ngd_u, sizeof(double) * nw, dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipGetLastError());
+    if (info > 0) return fail(-5, "damped metric not positive definite at column " + std::to_string(info));
     return 0;
 }
 

@@ -20,22 +20,32 @@ enum Precision : int {
     PREC_BF16X3 = 2,  // FP32 storage, BF16 hi+lo split MFMA linear maps
 };
 
+// one term of a table-driven sum, 16 bytes so a wave-uniform term is a
+// single scalar load: operand entries (or slots) s, b and the coefficient
+struct alignas(16) TermRec64 {
+    int s, b;
+    double c;
+};
+struct alignas(16) TermRec32 {
+    int s, b;
+    float c;
+    int pad;
+};
+
 // forward schedule on the device (see FwdSchedule in tables.hpp)
 struct DevFwdSched {
     const int *slot_group;   // [na+1]
     const int *grp_term;     // [ngroup+1]
     const int *grp_store;    // [ngroup]
-    const int2 *term_sb;     // [F] (operand entry of S_b, of B(a-b,q-1))
-    const double *term_c64;  // [F]
-    const float *term_c32;   // [F]
+    const TermRec64 *t64;    // [F] (entry of S_b, entry of B(a-b,q-1), C(a,b)/q)
+    const TermRec32 *t32;    // [F]
     int na, k, nentry;
 };
 
 struct DevPairs {
     const int *start, *num;  // [na]
-    const int2 *ia_id;       // [P]
-    const double *binom64;   // [P]
-    const float *binom32;    // [P]
+    const TermRec64 *p64;    // [P] (slot a, slot a-b, C(a,b))
+    const TermRec32 *p32;    // [P]
     int na;
 };
 
@@ -74,6 +84,22 @@ void launch_gemm_wx(int prec, bool trans_a, const T *W, int ldw, const T *B, T *
 template <typename T>
 void launch_gemm_grad(int prec, const T *A, const T *B, int M, int Kd, long ncol, long Np,
                       double *partial, int nchunk, long chunk_cols, double *grad, hipStream_t s);
+
+// partial buffer length needed by launch_gemm_grad / launch_gram:
+// (nchunk + grad_reduce_splits(nchunk)) * M * Kd doubles
+int grad_reduce_splits(int nchunk);
+
+// NGD: G <- G/nb + cI, b <- b/nb, then G = L L^T (FP64 blocked Cholesky, in
+// place, lower) and b <- (L L^T)^{-1} b; *info > 0 flags a non-positive
+// pivot (1-based column); trace is one device double of scratch
+int ngd_factor_solve(double *G, int n, long lda, double nb, double mu, int mode, double *b, double *trace,
+                     int *info, hipStream_t s);
+
+// G[i][j] (+)= sum_n R[i][n] R[j][n] for the transposed FP32 rows R[nw][ld]
+// (FP32 MFMA, FP32 accumulation per chunk, FP64 reduction); with one chunk
+// G is overwritten, otherwise added to
+void launch_gram(const float *R, int nw, long ld, double *partial, int nchunk, long chunk_cols, double *G,
+                 hipStream_t s);
 
 // per-point rows: rows[(j*Kd + k)][n] = sum_a A[j][a*Np+n] * Bext[k][a*Np+n]
 template <typename T>

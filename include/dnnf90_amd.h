@@ -97,6 +97,19 @@ int dnnf90_residual_rows_batch(int nid, int bid, int npt, const double *x,
                                const int *slot, const int *nonlin,
                                float *rows_t, long ld, int flags);
 
+/* NGD step (app/optimizer_module.f90:221-469, primal route): G = R R^T over
+ * the first nrow columns of the transposed FP32 rows rows_t[nweights][ld]
+ * (device; e.g. from dnnf90_residual_rows_batch; ld a multiple of 16, columns
+ * nrow..ld-1 zero), formed on FP32 MFMA; damping c = mu
+ * (mode 1, ABS) or mu*tr(G)/nb (mode 0, TRACE); u = (G/nb + cI)^{-1} (b/nb)
+ * by an FP64 Cholesky on the GPU.  b and u hold nweights doubles (host, or
+ * device with DNNF90_DEVICE_PTRS).  Returns -5 when G/nb + cI is not
+ * positive definite (the reference stops on a singular pivot,
+ * optimizer_module.f90:489-505).  Work space comes from bid. */
+int dnnf90_ngd_solve(int nid, int bid, int nrow, const float *rows_t, long ld,
+                     const double *b, int nb, double mu, int mode, double *u,
+                     int flags);
+
 #ifdef __cplusplus
 }
 #endif

@@ -158,3 +158,64 @@ def test_rows_fp64(D, oracle_mod):
     e = rel(rows, rr)
     print("rows rel err", e)
     assert e <= 1e-6
+
+
+def _ngd_inputs(D, oracle_mod, dims, npt):
+    import torch
+    from dnnf90_amd.workloads import WORKLOADS
+    wl = WORKLOADS["ngd2_k4"]
+    ts = D.TableSet(wl.d0, wl.k, wl.seeds, wl.act)
+    net = D.Net(ts, dims)
+    rng = np.random.default_rng(5)
+    w = np.concatenate([np.column_stack([np.zeros(dims[l]), rng.normal(0, np.sqrt(2.0 / (dims[l - 1] + dims[l])),
+                                         (dims[l], dims[l - 1]))]).reshape(-1) for l in range(1, len(dims))])
+    net.weights = w
+    x = rng.uniform(-1, 1, (npt, wl.d0))
+    o = oracle_mod.Oracle(wl.d0, wl.k, wl.seeds, wl.act)
+    terms = wl.terms(o.alpha())
+    bw = D.BatchWork(net, npt, D.PREC_FP64)
+    ld = ((npt + 63) // 64) * 64
+    rows_t = torch.zeros((net.nw, ld), dtype=torch.float32, device="cuda")
+    bw.residual_rows(x, D.Residual(terms), rows_t, ld)
+    g = D.Grad(net)
+    bw.residual_step(x, D.Residual(terms), 1.0, g)
+    torch.cuda.synchronize()
+    return net, bw, rows_t, ld, g.get(), o, w, x, terms
+
+
+@pytest.mark.parametrize("mode,mu", [(1, 1e-2), (0, 1e-3)])
+def test_ngd_solve_vs_oracle(D, oracle_mod, mode, mu):
+    """Gram metric (FP32 MFMA) + FP64 Cholesky against the oracle's Gaussian
+    elimination (app/optimizer_module.f90:221-522) on the same FP32 rows."""
+    dims = [2, 16, 16, 1]
+    npt = 200
+    net, bw, rows_t, ld, b, o, w, x, terms = _ngd_inputs(D, oracle_mod, dims, npt)
+    u = bw.ngd_solve(rows_t, ld, npt, b, npt, mu, mode)
+    J = rows_t[:, :npt].double().cpu().numpy().T
+    ur = oracle_mod.Oracle.ngd_solve(J, b, npt, mu, mode)
+    G = J.T @ J / npt
+    c = mu if mode == 1 else mu * np.trace(G)
+    cond = np.linalg.cond(G + c * np.eye(len(b)))
+    e = rel(u, ur)
+    print("ngd mode", mode, "rel err vs oracle", e, "cond", cond)
+    assert e <= 1e-7 * cond
+    # the oracle's rows (FP64) agree with the GPU's FP32 rows to FP32 rounding
+    assert rel(J, o.pinn_rows(dims, w, x, terms)) <= 1e-6
+
+
+def test_ngd_full_size_backward_error(D, oracle_mod):
+    """The configured NGD step: 4096 rows of the [2,64,64,64,1] net (8577
+    weights), ABS damping 1e-3; normwise backward error of the solve."""
+    import torch
+    dims = [2, 64, 64, 64, 1]
+    npt = 4096
+    net, bw, rows_t, ld, b, o, w, x, terms = _ngd_inputs(D, oracle_mod, dims, npt)
+    assert net.nw == 8577
+    u = bw.ngd_solve(rows_t, ld, npt, b, npt, 1e-3, 1)
+    Jt = rows_t[:, :npt].double()
+    ut = torch.from_numpy(u).cuda()
+    r = (Jt @ (Jt.T @ ut)) / npt + 1e-3 * ut - torch.from_numpy(b).cuda() / npt
+    gnorm = (Jt * Jt).sum().item() / npt + 1e-3   # Frobenius-type bound of the damped metric
+    be = r.norm().item() / (gnorm * ut.norm().item() + np.linalg.norm(b) / npt)
+    print("ngd full-size backward error", be)
+    assert be <= 1e-6
