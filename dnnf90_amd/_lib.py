@@ -48,6 +48,8 @@ SIGNATURES = {
     "dnnf90_last_error": (_cs, []),
     "dnnf90_tabset_counts": (_i, [_i, _vp]),
     "dnnf90_tabset_lists": (_i, [_i] + [_vp] * 10),
+    "dnnf90_set_codegen": (_i, [_i]),
+    "dnnf90_net_kernel_info": (_i, [_i, _i, _vp]),
     "dnnf90_net_nweights": (_i, [_i]),
     "dnnf90_net_nlayer": (_i, [_i]),
     "dnnf90_net_dims": (_i, [_i, _vp]),

@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdnnf90_amd.so")
-SOURCES = ["tables.cpp", "bell.hip", "gemm.hip", "ngd.hip", "misc.hip", "engine.hip"]
+SOURCES = ["tables.cpp", "codegen.cpp", "bell.hip", "gemm.hip", "ngd.hip", "misc.hip", "engine.hip"]
 ARCH = os.environ.get("DNNF90_ARCH", "gfx950")
 
 
@@ -50,7 +50,7 @@ def build(verbose: bool = False) -> str:
             print(msg)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
+    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lhiprtc"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)

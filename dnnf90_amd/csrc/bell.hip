@@ -16,84 +16,9 @@
 
 namespace dnnf90 {
 
-// sigma^(q)(a), q = 0..nq (nq < KQ), in FP64
-// (lib/multi_index_bell_module.f90:723-867; erf-GELU x Phi^(q) + q Phi^(q-1))
-// kBes selects the J0/J1 branch, compiled apart because ocml's jn() alone
-// costs ~80 VGPRs that the other activations should not pay
-template <int KQ, bool kBes>
-__device__ __forceinline__ void act_derivs(const ActParams &ap, double a, double (&d)[KQ])
-{
-    const int nq = ap.nq;
-#pragma unroll
-    for (int q = 0; q < KQ; ++q) d[q] = 0.0;
-    if (ap.iact == 1) {  // sin(a + q pi/2)
-        const double pio2 = 1.57079632679489661923;
-#pragma unroll
-        for (int q = 0; q < KQ; ++q)
-            if (q <= nq) d[q] = sin(a + (double)q * pio2);
-    } else if (ap.iact == 2) {  // erf: (2/sqrt(pi)) (-1)^(q-1) H_(q-1)(a) e^(-a^2)
-        const double c2spi = 1.12837916709551257390;
-        double pm = exp(-a * a), pc = 2.0 * a * pm;  // psi_0, psi_1
-        d[0] = erf(a);
-        if (nq >= 1) d[1] = c2spi * pm;
-        if (nq >= 2 && KQ > 2) d[2 % KQ] = -c2spi * pc;
-#pragma unroll
-        for (int n = 1; n + 2 < KQ; ++n) {
-            if (n + 2 <= nq) {
-                double pn = 2.0 * a * pc - 2.0 * (double)n * pm;  // psi_(n+1)
-                d[n + 2] = ((n + 1) & 1 ? -c2spi : c2spi) * pn;
-                pm = pc;
-                pc = pn;
-            }
-        }
-    } else if (ap.iact == 5) {  // erf-GELU
-        const double rs2pi = 0.39894228040143267794, rs2 = 0.70710678118654752440;
-        double ps[KQ];
-        ps[0] = rs2pi * exp(-0.5 * a * a);
-        if (KQ > 1) ps[1 % KQ] = a * ps[0];
-#pragma unroll
-        for (int n = 1; n + 1 < KQ; ++n) ps[n + 1] = a * ps[n] - (double)n * ps[n - 1];
-        double phm = 0.5 * erfc(-a * rs2);  // Phi^(q-1)
-        d[0] = a * phm;
-#pragma unroll
-        for (int q = 1; q < KQ; ++q) {
-            if (q <= nq) {
-                double ph = ((q - 1) & 1 ? -1.0 : 1.0) * ps[q - 1];  // Phi^(q)
-                d[q] = a * ph + (double)q * phm;
-                phm = ph;
-            }
-        }
-    } else if (kBes) {
-        // J0^(q) = (-1)^q 2^-q sum_k (-1)^k C(q,k) J_(q-2k); J1 shifted by one
-        const int sh = ap.iact == 4 ? 1 : 0;
-#pragma unroll
-        for (int q = 0; q < KQ; ++q) {
-            if (q <= nq) {
-                double term = 0.0, c = 1.0;
-                for (int kk = 0; kk <= q; ++kk) {
-                    int m = sh + q - 2 * kk;
-                    double jm = jn(m < 0 ? -m : m, a);
-                    if (m < 0 && ((-m) & 1)) jm = -jm;
-                    term += ((kk & 1) ? -1.0 : 1.0) * c * jm;
-                    c = c * (double)(q - kk) / (double)(kk + 1);
-                }
-                d[q] = ((q & 1) ? -term : term) / ldexp(1.0, q);
-            }
-        }
-    } else {  // tanh: Horner on P_q(t), t = tanh(a)
-        const double t = tanh(a);
-#pragma unroll
-        for (int q = 0; q < KQ; ++q) {
-            if (q <= nq) {
-                const double *c = ap.tpc + q * ap.ldt;
-                double v = c[q + 1];
-#pragma unroll
-                for (int i = q; i >= 0; --i) v = v * t + c[i];
-                d[q] = v;
-            }
-        }
-    }
-}
+#define DNNF90_ACT_SRC(...) __VA_ARGS__
+#include "act.inc"
+#undef DNNF90_ACT_SRC
 
 template <typename T> struct Recs;
 template <> struct Recs<double> {

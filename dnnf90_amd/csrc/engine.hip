@@ -20,10 +20,28 @@
 #include <vector>
 
 #include "../../include/dnnf90_amd.h"
+#include "codegen.hpp"
 #include "kernels.hpp"
 #include "tables.hpp"
 
+#include <map>
+
 namespace dnnf90 {
+
+// generated Faa di Bruno kernels: on unless DNNF90_CODEGEN=0 or switched off
+// through dnnf90_set_codegen; code objects are cached by source text
+static int g_codegen = -1;
+static std::mutex g_gen_mu;
+static std::map<std::string, std::vector<char>> g_gen_cache;
+
+static bool codegen_on()
+{
+    if (g_codegen < 0) {
+        const char *e = std::getenv("DNNF90_CODEGEN");
+        g_codegen = (e && e[0] == '0') ? 0 : 1;
+    }
+    return g_codegen != 0;
+}
 
 static thread_local std::string g_err;
 
@@ -146,7 +164,35 @@ struct Net {
     DevTables dt;
     double *d_w64 = nullptr;
     float *d_w32 = nullptr;
+    GenKernels gen[2];       // [0] FP32, [1] FP64
+    bool gen_tried[2] = {false, false};
     std::mutex mu;
+
+    // the generated kernels of one precision (compiled on first use)
+    const GenKernels *generated(bool fp64)
+    {
+        if (!codegen_on()) return nullptr;
+        const int i = fp64 ? 1 : 0;
+        if (!gen_tried[i]) {
+            gen_tried[i] = true;
+            if (gen_bell_usable(tab, sched)) {
+                const std::string src = gen_bell_source(tab, sched, fp64);
+                std::lock_guard<std::mutex> lk(g_gen_mu);
+                auto it = g_gen_cache.find(src);
+                if (it != g_gen_cache.end()) {
+                    gen[i].code = it->second;
+                    if (hipModuleLoadData(&gen[i].mod, gen[i].code.data()) == hipSuccess &&
+                        hipModuleGetFunction(&gen[i].fwd, gen[i].mod, "dnnf90_bell_fwd") == hipSuccess &&
+                        hipModuleGetFunction(&gen[i].bwd, gen[i].mod, "dnnf90_bell_bwd") == hipSuccess)
+                        gen[i].ok = true;
+                } else {
+                    std::string log;
+                    if (gen_bell_compile(tab, sched, fp64, &gen[i], &log) == 0) g_gen_cache[src] = gen[i].code;
+                }
+            }
+        }
+        return gen[i].ok ? &gen[i] : nullptr;
+    }
 
     int ensure_device(bool need32)
     {
@@ -155,6 +201,8 @@ struct Net {
             int rc = dt.build(tab, sched);
             if (rc) return rc;
         }
+        (void)generated(true);
+        if (need32) (void)generated(false);
         if (!d_w64) {
             int rc = dalloc(&d_w64, (size_t)nw);
             if (rc) return rc;
@@ -179,6 +227,8 @@ struct Net {
         dt.release();
         dfree(d_w64);
         dfree(d_w32);
+        gen[0].release();
+        gen[1].release();
     }
 };
 
@@ -359,6 +409,13 @@ static int bwork_init(BatchWork &b, const Net &nt, int maxpt, int prec)
 // ---------------------------------------------------------------------------
 // the layer loops
 
+template <typename T>
+static const GenKernels *gen_for(const Net &nt)
+{
+    const GenKernels &g = nt.gen[sizeof(T) == 8 ? 1 : 0];
+    return (codegen_on() && g.ok) ? &g : nullptr;
+}
+
 template <typename T> static const T *weights(const Net &nt);
 template <> const double *weights<double>(const Net &nt) { return nt.d_w64; }
 template <> const float *weights<float>(const Net &nt) { return nt.d_w32; }
@@ -389,8 +446,12 @@ static void forward_t(const Net &nt, BatchWork &b, const double *d_x, int npt)
         }
         if (l < L - 1) {
             PScope p(b, CAT_BELL_FWD);
-            launch_bell_fwd<T>((const T *)b.p0, (T *)b.tpl[l], (T *)b.hpl[l], nd, Np, sch, act,
-                               (T *)b.scratch, s);
+            const GenKernels *gk = gen_for<T>(nt);
+            if (gk)
+                (void)gen_launch_fwd(*gk, b.p0, b.tpl[l], b.hpl[l], nd, Np, act, s);
+            else
+                launch_bell_fwd<T>((const T *)b.p0, (T *)b.tpl[l], (T *)b.hpl[l], nd, Np, sch, act,
+                                   (T *)b.scratch, s);
         }
     }
 }
@@ -409,7 +470,11 @@ static void backward_t(const Net &nt, BatchWork &b, double *grad, float *rows_t,
         const int nd = nt.dims[l], ndm = nt.dims[l - 1];
         if (l < L - 1) {
             PScope p(b, CAT_BELL_BWD);
-            launch_bell_bwd<T>((const T *)b.p1, (const T *)b.hpl[l], (T *)b.p0, nd, Np, pr, s);
+            const GenKernels *gk = gen_for<T>(nt);
+            if (gk)
+                (void)gen_launch_bwd(*gk, b.p1, b.hpl[l], b.p0, nd, Np, s);
+            else
+                launch_bell_bwd<T>((const T *)b.p1, (const T *)b.hpl[l], (T *)b.p0, nd, Np, pr, s);
         }
         const T *X = (const T *)(l == 1 ? b.t0 : b.tpl[l - 1]);
         {
@@ -1143,6 +1208,30 @@ int dnnf90_ngd_solve(int nid, int bid, int nrow, const float *rows_t, long ld, c
     HIPCHK(hipStreamSynchronize(s));
     HIPCHK(hipGetLastError());
     if (info > 0) return fail(-5, "damped metric not positive definite at column " + std::to_string(info));
+    return 0;
+}
+
+int dnnf90_set_codegen(int on)
+{
+    const int prev = codegen_on() ? 1 : 0;
+    g_codegen = on ? 1 : 0;
+    return prev;
+}
+
+int dnnf90_net_kernel_info(int nid, int prec, int *info)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt || !info || prec < 0 || prec > 2) return fail(-1, "bad network handle or argument");
+    int rc = nt->ensure_device(prec != PREC_FP64);
+    if (rc) return rc;
+    const GenKernels &g = nt->gen[prec == PREC_FP64 ? 1 : 0];
+    const bool on = codegen_on() && g.ok;
+    info[0] = on ? 1 : 0;
+    info[1] = on ? g.fwd_regs : 0;
+    info[2] = on ? g.bwd_regs : 0;
+    info[3] = on ? g.fwd_scratch : 0;
+    info[4] = on ? g.bwd_scratch : 0;
+    info[5] = nt->sched.nentry;
     return 0;
 }
 

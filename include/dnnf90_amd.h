@@ -38,6 +38,16 @@ int dnnf90_tabset_lists(int tid, int *pair_start, int *pair_num, int *pair_ia,
                         int *pair_id, double *pair_binom, int *fq_start,
                         int *fq_num, int *fq_ib, int *fq_id, double *fq_c);
 
+/* Faa di Bruno kernels generated per table family at run time (hipRTC,
+ * straight-line, operands in registers) are used where the family is small
+ * enough; otherwise, or with codegen off, the table-driven kernels run.
+ * dnnf90_set_codegen switches this process-wide (also DNNF90_CODEGEN=0)
+ * and returns the previous setting.  dnnf90_net_kernel_info fills info[6]:
+ * generated kernels active (0/1), their forward and adjoint VGPR counts and
+ * scratch bytes, and the operand entries per lane of the family. */
+int dnnf90_set_codegen(int on);
+int dnnf90_net_kernel_info(int nid, int prec, int *info);
+
 int dnnf90_net_nweights(int nid);
 int dnnf90_net_nlayer(int nid);
 int dnnf90_net_dims(int nid, int *dims);

@@ -22,6 +22,15 @@ def D():
     return dnnf90_amd
 
 
+@pytest.fixture(params=[1, 0], ids=["generated", "table"])
+def codegen(request, D):
+    """Run with the per-family generated kernels, then with the table-driven ones."""
+    from dnnf90_amd._lib import lib
+    prev = lib().dnnf90_set_codegen(request.param)
+    yield request.param
+    lib().dnnf90_set_codegen(prev)
+
+
 def _setup(D, oracle_mod, wl, npt):
     ts = D.TableSet(wl.d0, wl.k, wl.seeds, wl.act)
     net = D.Net(ts, wl.dims)
@@ -51,7 +60,7 @@ def test_golden_per_point_abi(D, case):
 
 
 @pytest.mark.parametrize("name,npt", WL_SMALL)
-def test_forward_fp64(D, oracle_mod, name, npt):
+def test_forward_fp64(D, oracle_mod, codegen, name, npt):
     from dnnf90_amd.workloads import WORKLOADS
     wl = WORKLOADS[name]
     ts, net, w, x, o = _setup(D, oracle_mod, wl, npt)
@@ -64,7 +73,7 @@ def test_forward_fp64(D, oracle_mod, name, npt):
 
 
 @pytest.mark.parametrize("name,npt", WL_SMALL)
-def test_adjoint_fp64(D, oracle_mod, name, npt):
+def test_adjoint_fp64(D, oracle_mod, codegen, name, npt):
     from dnnf90_amd.workloads import WORKLOADS
     wl = WORKLOADS[name]
     ts, net, w, x, o = _setup(D, oracle_mod, wl, npt)
@@ -93,6 +102,21 @@ def test_residual_step_fp64(D, oracle_mod, name, npt):
     print(name, "rsq", rsq, rsq_r, "grad rel", rel(g.get(), gr))
     assert abs(rsq - rsq_r) <= 1e-12 * abs(rsq_r)
     assert rel(g.get(), gr) <= TOL64
+
+
+@pytest.mark.parametrize("name", ["kdv", "dense4_k3", "closure4_k7", "gelu3_k4", "ngd2_k4"])
+def test_generated_kernels_active(D, name):
+    """The measured families run the generated kernels, without scratch spills."""
+    from dnnf90_amd._lib import lib
+    from dnnf90_amd.workloads import WORKLOADS
+    wl = WORKLOADS[name]
+    net = D.Net(D.TableSet(wl.d0, wl.k, wl.seeds, wl.act), wl.dims)
+    for prec in (D.PREC_FP64, D.PREC_FP32):
+        info = np.zeros(6, np.int32)
+        assert lib().dnnf90_net_kernel_info(net.id, prec, info.ctypes.data) == 0
+        print(name, "prec", prec, "generated", info[0], "vgprs fwd/bwd", info[1], info[2], "scratch", info[3], info[4])
+        assert info[0] == 1
+        assert info[3] == 0 and info[4] == 0
 
 
 def test_per_point_abi_and_adam(D, oracle_mod):
@@ -124,7 +148,7 @@ def test_per_point_abi_and_adam(D, oracle_mod):
 
 @pytest.mark.parametrize("prec,tol", [(1, 1e-4), (2, 1e-4)])
 @pytest.mark.parametrize("name,npt", [("dense4_k3", 197), ("closure4_k7", 130), ("gelu3_k4", 70)])
-def test_mixed_precision(D, oracle_mod, name, npt, prec, tol):
+def test_mixed_precision(D, oracle_mod, codegen, name, npt, prec, tol):
     from dnnf90_amd.workloads import WORKLOADS
     wl = WORKLOADS[name]
     ts, net, w, x, o = _setup(D, oracle_mod, wl, npt)
