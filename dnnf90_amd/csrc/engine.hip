@@ -347,18 +347,10 @@ static int valloc(void **p, size_t bytes)
     return 0;
 }
 
-// gradient GEMM split: about 1024 workgroups over the output tiles
-// (a tile is up to 128 x 144 of the weight block)
-static void grad_split(long ncol, int M, int Kd, int *nchunk, long *chunk)
+// gradient GEMM split over the column axis (chunks of the K5 kernel)
+static void grad_split(long ncol, int M, int Kd, int esz, int *nchunk, long *chunk)
 {
-    const int bm = M > 64 ? 128 : 64;
-    const int tiles = ((M + bm - 1) / bm) * ((Kd + 143) / 144);
-    long nc = std::max(1, 1024 / tiles);
-    nc = std::min(nc, std::max(1L, ncol / 512));
-    long ch = round_up((ncol + nc - 1) / nc, 16);
-    nc = (ncol + ch - 1) / ch;
-    *nchunk = (int)nc;
-    *chunk = ch;
+    grad_chunks(ncol, M, Kd, esz, nchunk, chunk);
 }
 
 static int bwork_init(BatchWork &b, const Net &nt, int maxpt, int prec)
@@ -393,7 +385,7 @@ static int bwork_init(BatchWork &b, const Net &nt, int maxpt, int prec)
     for (int l = 1; l < L; ++l) {
         int nc;
         long ch;
-        grad_split((long)col, nt.dims[l], nt.dims[l - 1] + 1, &nc, &ch);
+        grad_split((long)col, nt.dims[l], nt.dims[l - 1] + 1, (int)b.esz, &nc, &ch);
         pc = std::max(pc, (long)(nc + grad_reduce_splits(nc)) * nt.dims[l] * (nt.dims[l - 1] + 1));
     }
     b.partial_cap = pc;
@@ -485,7 +477,7 @@ static void backward_t(const Net &nt, BatchWork &b, double *grad, float *rows_t,
             } else {
                 int nc;
                 long ch;
-                grad_split(ncol, nd, ndm + 1, &nc, &ch);
+                grad_split(ncol, nd, ndm + 1, (int)b.esz, &nc, &ch);
                 launch_gemm_grad<T>(b.prec, (const T *)b.p0, X, nd, ndm + 1, ncol, Np, b.partial, nc, ch,
                                     grad + nt.woff[l], s);
             }

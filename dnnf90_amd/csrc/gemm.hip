@@ -16,6 +16,8 @@
 // (measured on gfx950).
 #include <hip/hip_bf16.h>
 
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace dnnf90 {
@@ -52,108 +54,123 @@ __device__ __forceinline__ T bext(const T *__restrict__ B, long ncol, long Np, i
 }
 
 // ---------------------------------------------------------------------------
-// K1 / K6, FP64 and FP32.  Block tile BM x 64, k-tile 16, four waves in 2x2;
-// a wave owns (BM/2) x 32 = (BM/32) x 2 MFMA blocks.  The next k-tile is
-// loaded into registers while the current one is multiplied.
-static constexpr int BN = 64, BK = 16;
-
-template <typename T, bool kTrans, int BM>
-__global__ __launch_bounds__(256) void k_gemm_wx(const T *__restrict__ W, int ldw, const T *__restrict__ B,
-                                                 T *__restrict__ C, int M, int Kd, long ncol, long Np)
-{
-    constexpr int RB = BM / 32;            // row blocks per wave
-    constexpr int AE = BM * BK / 256;      // A elements per thread per k-tile
-    __shared__ T As[BK][BM];
-    __shared__ T Bs[BK][BN];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
-    const int m0 = blockIdx.y * BM;
-    const long n0 = (long)blockIdx.x * BN;
-    typename Mfma<T>::V acc[RB][2];
+// K1 / K6, FP64 and FP32.  A workgroup owns 16*RBT rows of C.  Their block of
+// A (K1: the weight columns after the bias; K6: the transposed weights) is
+// staged once into LDS in MFMA operand order,
+//     As[rb][s][lane] = A[16 rb + lane%16][4 s + lane/16],
+// zero-padded to a whole number of KC-step chunks, so the A operand of every
+// MFMA is one conflict-free 256-byte LDS row.  The k extent of A is the whole
+// reduction, so B is never staged: each wave streams 16-column blocks of B
+// from HBM straight into registers (lane: column lane%16, rows 4 s + lane/16;
+// sixteen lanes read 64 contiguous bytes of a row), KC k-steps at a time with
+// the next chunk's loads issued before the current chunk's MFMAs, and writes
+// its 16-column strip of C.  K1's bias enters in the epilogue: it feeds slot
+// 0 only, i.e. the first Np columns (Np is a multiple of 64, so a 16-column
+// block lies wholly inside or outside).  The four waves of a workgroup take
+// adjacent column blocks.
+// kAcc64 (FP32 K1 of the mixed mode): every MFMA starts from zero and its
+// four-term sums are accumulated in FP64, which removes the FP32
+// accumulation error of the long k sums (the dominant error of an
+// all-FP32 forward pass).
+template <typename T, bool kAcc64> struct LinAcc {
+    typename Mfma<T>::V v;
+    __device__ __forceinline__ void zero() { v = typename Mfma<T>::V{0, 0, 0, 0}; }
+    __device__ __forceinline__ void mma(T a, T b) { v = Mfma<T>::mma(a, b, v); }
+    __device__ __forceinline__ T get(int r, T add) const { return v[r] + add; }
+};
+template <> struct LinAcc<float, true> {
+    double v[4];
+    __device__ __forceinline__ void zero() { v[0] = v[1] = v[2] = v[3] = 0.0; }
+    __device__ __forceinline__ void mma(float a, float b)
+    {
+        const flt4 d = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, flt4{0, 0, 0, 0}, 0, 0, 0);
 #pragma unroll
-    for (int i = 0; i < RB; ++i)
-#pragma unroll
-        for (int jj = 0; jj < 2; ++jj) acc[i][jj] = typename Mfma<T>::V{0, 0, 0, 0};
-
-    // staging assignment: A: kTrans ? (k = tid / (BM/AE), m run of AE) : (m = tid / (BK/AE)... )
-    T ra[AE], rb[4];
-    auto load = [&](int k0) {
-        if (!kTrans) {
-            // A[m][k] = W[m][k], k contiguous: thread reads AE consecutive k of one row
-            constexpr int TPR = BK / AE;   // threads per row
-            const int m = tid / TPR, kq = (tid % TPR) * AE, gm = m0 + m;
-#pragma unroll
-            for (int e = 0; e < AE; ++e) {
-                const int gk = k0 + kq + e;
-                ra[e] = (gm < M && gk < Kd) ? W[(long)gm * ldw + gk] : T(0);
-            }
-        } else {
-            // A[m][k] = W[k][m+1], m contiguous
-            constexpr int TPR = BM / AE;
-            const int k = tid / TPR, mq = (tid % TPR) * AE, gk = k0 + k;
-#pragma unroll
-            for (int e = 0; e < AE; ++e) {
-                const int gm = m0 + mq + e;
-                ra[e] = (gm < M && gk < Kd) ? W[(long)gk * ldw + 1 + gm] : T(0);
-            }
-        }
-        const int k = tid >> 4, cq = (tid & 15) * 4, gk = k0 + k;
-        const long gc = n0 + cq;
-        if (gk < Kd && !(!kTrans && gk == 0)) {
-            const T *src = B + (long)(kTrans ? gk : gk - 1) * ncol + gc;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) rb[e] = src[e];
-        } else {
-#pragma unroll
-            for (int e = 0; e < 4; ++e) rb[e] = bext<T, !kTrans>(B, ncol, Np, gk, Kd, gc + e);
-        }
-    };
-    auto store = [&]() {
-        if (!kTrans) {
-            constexpr int TPR = BK / AE;
-            const int m = tid / TPR, kq = (tid % TPR) * AE;
-#pragma unroll
-            for (int e = 0; e < AE; ++e) As[kq + e][m] = ra[e];
-        } else {
-            constexpr int TPR = BM / AE;
-            const int k = tid / TPR, mq = (tid % TPR) * AE;
-#pragma unroll
-            for (int e = 0; e < AE; ++e) As[k][mq + e] = ra[e];
-        }
-        const int k = tid >> 4, cq = (tid & 15) * 4;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) Bs[k][cq + e] = rb[e];
-    };
-
-    load(0);
-    for (int k0 = 0; k0 < Kd; k0 += BK) {
-        store();
-        __syncthreads();
-        if (k0 + BK < Kd) load(k0 + BK);
-#pragma unroll
-        for (int kk = 0; kk < BK; kk += 4) {
-            T a[RB], b[2];
-#pragma unroll
-            for (int i = 0; i < RB; ++i) a[i] = As[kk + (lane >> 4)][wm * (BM / 2) + i * 16 + (lane & 15)];
-#pragma unroll
-            for (int jj = 0; jj < 2; ++jj) b[jj] = Bs[kk + (lane >> 4)][wn * 32 + jj * 16 + (lane & 15)];
-#pragma unroll
-            for (int i = 0; i < RB; ++i)
-#pragma unroll
-                for (int jj = 0; jj < 2; ++jj) acc[i][jj] = Mfma<T>::mma(a[i], b[jj], acc[i][jj]);
-        }
-        __syncthreads();
+        for (int r = 0; r < 4; ++r) v[r] += (double)d[r];
     }
+    __device__ __forceinline__ float get(int r, float add) const { return (float)(v[r] + (double)add); }
+};
+
+template <typename T, bool kTrans, bool kAcc64, int RBT, int KC>
+__global__ __launch_bounds__(256) void k_lin(const T *__restrict__ W, int ldw, const T *__restrict__ B,
+                                             T *__restrict__ C, int M, int Kd, long ncol, long Np, int nrt,
+                                             int ncw, long cbpw)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char lin_lds[];
+    T *As = reinterpret_cast<T *>(lin_lds);
+    // work item: consecutive items on one XCD (workgroup ids are dealt to the
+    // eight XCDs in turn), row tiles of one column range adjacent, so the
+    // second row tile's reads of B hit that XCD's L2
+    const int G = gridDim.x, bid = blockIdx.x;
+    const int w = (bid & 7) * (G >> 3) + (bid >> 3);
+    const int rt = w % nrt, cw = w / nrt;
+    if (cw >= ncw) return;
+    const long ncb = ncol >> 4;
+    const long cb0 = (long)cw * cbpw, cb1 = min(ncb, cb0 + cbpw);
+    if (cb0 >= cb1) return;
+    const int ks = (Kd + 3) >> 2, nch = (ks + KC - 1) / KC, ksp = nch * KC;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int m0 = rt * 16 * RBT;
+    for (int e = tid; e < RBT * ksp * 64; e += 256) {
+        const int rb = e / (ksp * 64), rem = e - rb * ksp * 64, s = rem >> 6, ln = rem & 63;
+        const int m = m0 + 16 * rb + (ln & 15), k = 4 * s + (ln >> 4);
+        T v = T(0);
+        if (m < M && k < Kd) v = kTrans ? W[(long)k * ldw + 1 + m] : W[(long)m * ldw + 1 + k];
+        As[e] = v;
+    }
+    T *Bias = As + RBT * ksp * 64;   // K1: the bias column of the tile's rows
+    if (!kTrans && tid < 16 * RBT) Bias[tid] = m0 + tid < M ? W[(long)(m0 + tid) * ldw] : T(0);
+    __syncthreads();
+    long cb = cb0 + wave;
+    if (cb >= cb1) return;
+    // rows of B this lane reads, clamped into B (A is zero past Kd)
+    const int kq = lane >> 4;
+    const T *bl = B + (lane & 15);
+    T bc[KC], bn[KC];
+    auto load = [&](T (&dst)[KC], long cbl, int ch) {
+        const T *src = bl + cbl * 16;
 #pragma unroll
-    for (int i = 0; i < RB; ++i)
+        for (int e = 0; e < KC; ++e) dst[e] = src[(long)min(4 * (ch * KC + e) + kq, Kd - 1) * ncol];
+    };
+    LinAcc<T, kAcc64> acc[RBT];
+    load(bc, cb, 0);
+    int ch = 0;
+    for (;;) {
+        long cbn = cb;
+        int chn = ch + 1;
+        if (chn == nch) {
+            chn = 0;
+            cbn = cb + 4;
+        }
+        const bool more = cbn < cb1;
+        if (more) load(bn, cbn, chn);
+        if (ch == 0) {
 #pragma unroll
-        for (int jj = 0; jj < 2; ++jj)
+            for (int rb = 0; rb < RBT; ++rb) acc[rb].zero();
+        }
+        const T *Ac = As + (long)ch * KC * 64 + lane;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int gm = m0 + wm * (BM / 2) + i * 16 + Mfma<T>::row(lane, r);
-                const long gc = n0 + wn * 32 + jj * 16 + (lane & 15);
-                if (gm < M) C[(long)gm * ncol + gc] = acc[i][jj][r];
-            }
+        for (int e = 0; e < KC; ++e)
+#pragma unroll
+            for (int rb = 0; rb < RBT; ++rb) acc[rb].mma(Ac[(rb * ksp + e) * 64], bc[e]);
+        if (ch == nch - 1) {
+            const long c = cb * 16 + (lane & 15);
+            const bool bias = !kTrans && cb * 16 < Np;
+#pragma unroll
+            for (int rb = 0; rb < RBT; ++rb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int m = m0 + 16 * rb + Mfma<T>::row(lane, r);
+                    const T add = bias ? Bias[16 * rb + Mfma<T>::row(lane, r)] : T(0);
+                    if (m < M) C[(long)m * ncol + c] = acc[rb].get(r, add);
+                }
+        }
+        if (!more) break;
+#pragma unroll
+        for (int e = 0; e < KC; ++e) bc[e] = bn[e];
+        cb = cbn;
+        ch = chn;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -162,7 +179,7 @@ __global__ __launch_bounds__(256) void k_gemm_wx(const T *__restrict__ W, int ld
 // 2^-32 relative) with FP32 accumulation on v_mfma_f32_16x16x32_bf16.  The
 // operands are split while staging into LDS as [row][k] so a lane's eight
 // consecutive k are one 16-byte read.
-static constexpr int XBK = 32, XPAD = 8;
+static constexpr int XBK = 32, XPAD = 8, BN = 64;
 
 __device__ __forceinline__ void split8(const float (&v)[8], bf16x8 &hi, bf16x8 &lo)
 {
@@ -258,16 +275,69 @@ __global__ __launch_bounds__(256) void k_gemm_wx_bf3(const float *__restrict__ W
             }
 }
 
+template <typename T, bool kTrans, bool kAcc64, int RBT, int KC>
+static void lin_go(const T *W, int ldw, const T *B, T *C, int M, int Kd, long ncol, long Np, hipStream_t s)
+{
+    const int ks = (Kd + 3) / 4, ksp = (ks + KC - 1) / KC * KC;
+    const size_t lds = ((size_t)RBT * ksp * 64 + 16 * RBT) * sizeof(T);
+    static size_t lds_set = 64 * 1024;   // dynamic LDS above 64 KiB must be requested
+    if (lds > lds_set) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_lin<T, kTrans, kAcc64, RBT, KC>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        lds_set = lds;
+    }
+    const int nrt = (M + 16 * RBT - 1) / (16 * RBT);
+    const long ncb = ncol / 16;
+    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / std::max<size_t>(lds, 1)));
+    const long want = std::max(1L, (long)256 * per_cu / nrt);
+    long cbpw = (ncb + want - 1) / want;
+    cbpw = (cbpw + 3) / 4 * 4;
+    long ncw = (ncb + cbpw - 1) / cbpw;
+    long g = (long)nrt * ncw;
+    g = (g + 7) / 8 * 8;
+    hipLaunchKernelGGL((k_lin<T, kTrans, kAcc64, RBT, KC>), dim3((unsigned)g), dim3(256), lds, s, W, ldw, B, C, M,
+                       Kd, ncol, Np, nrt, (int)ncw, cbpw);
+}
+
+template <typename T, bool kTrans, bool kAcc64, int RBT>
+static void lin_kc(const T *W, int ldw, const T *B, T *C, int M, int Kd, long ncol, long Np, hipStream_t s)
+{
+    // chunks of 8 k-steps; a reduction of under 32 terms is one exact chunk
+    const int ks = (Kd + 3) / 4;
+    if (ks >= 8) lin_go<T, kTrans, kAcc64, RBT, 8>(W, ldw, B, C, M, Kd, ncol, Np, s);
+    else if (ks >= 4) lin_go<T, kTrans, kAcc64, RBT, 4>(W, ldw, B, C, M, Kd, ncol, Np, s);
+    else lin_go<T, kTrans, kAcc64, RBT, 1>(W, ldw, B, C, M, Kd, ncol, Np, s);
+}
+
+// Kd: length of the reduction (K1: the ndm weight columns after the bias;
+// K6: nd)
+template <typename T, bool kTrans, bool kAcc64>
+static void lin_pick(const T *W, int ldw, const T *B, T *C, int M, int Kd, long ncol, long Np, hipStream_t s)
+{
+    // rows per workgroup: the whole of M up to 128, fewer where the staged
+    // block would pass 144 KiB of LDS
+    const size_t per_rb = (size_t)(((Kd + 3) / 4 + 7) / 8 * 8) * 64 * sizeof(T);
+    int rbt = 8;
+    while (rbt > 1 && 16 * (rbt / 2) >= M) rbt /= 2;
+    while (rbt > 1 && rbt * per_rb > (size_t)144 * 1024) rbt /= 2;
+    switch (rbt) {
+    case 8: lin_kc<T, kTrans, kAcc64, 8>(W, ldw, B, C, M, Kd, ncol, Np, s); break;
+    case 4: lin_kc<T, kTrans, kAcc64, 4>(W, ldw, B, C, M, Kd, ncol, Np, s); break;
+    case 2: lin_kc<T, kTrans, kAcc64, 2>(W, ldw, B, C, M, Kd, ncol, Np, s); break;
+    default: lin_kc<T, kTrans, kAcc64, 1>(W, ldw, B, C, M, Kd, ncol, Np, s); break;
+    }
+}
+
 template <typename T>
 void launch_gemm_wx(int prec, bool trans_a, const T *W, int ldw, const T *B, T *C, int M, int Kd, long ncol,
                     long Np, hipStream_t s)
 {
-    const bool big = M > 64;
-    const int bm = big ? 128 : 64;
-    dim3 grid((unsigned)(ncol / BN), (unsigned)((M + bm - 1) / bm));
-#define LAUNCH(K, ...) hipLaunchKernelGGL((K), grid, dim3(256), 0, s, __VA_ARGS__)
     if constexpr (sizeof(T) == 4) {
         if (prec == PREC_BF16X3) {
+            const bool big = M > 64;
+            const int bm = big ? 128 : 64;
+            dim3 grid((unsigned)(ncol / BN), (unsigned)((M + bm - 1) / bm));
+#define LAUNCH(K, ...) hipLaunchKernelGGL((K), grid, dim3(256), 0, s, __VA_ARGS__)
             if (trans_a) {
                 if (big) LAUNCH((k_gemm_wx_bf3<true, 128>), W, ldw, B, C, M, Kd, ncol, Np);
                 else LAUNCH((k_gemm_wx_bf3<true, 64>), W, ldw, B, C, M, Kd, ncol, Np);
@@ -275,21 +345,165 @@ void launch_gemm_wx(int prec, bool trans_a, const T *W, int ldw, const T *B, T *
                 if (big) LAUNCH((k_gemm_wx_bf3<false, 128>), W, ldw, B, C, M, Kd, ncol, Np);
                 else LAUNCH((k_gemm_wx_bf3<false, 64>), W, ldw, B, C, M, Kd, ncol, Np);
             }
+#undef LAUNCH
             return;
         }
+        // FP32 mode: the forward map accumulates in FP64
+        if (trans_a) lin_pick<T, true, false>(W, ldw, B, C, M, Kd, ncol, Np, s);
+        else lin_pick<T, false, true>(W, ldw, B, C, M, Kd - 1, ncol, Np, s);
+        return;
     }
-    if (trans_a) {
-        if (big) LAUNCH((k_gemm_wx<T, true, 128>), W, ldw, B, C, M, Kd, ncol, Np);
-        else LAUNCH((k_gemm_wx<T, true, 64>), W, ldw, B, C, M, Kd, ncol, Np);
-    } else {
-        if (big) LAUNCH((k_gemm_wx<T, false, 128>), W, ldw, B, C, M, Kd, ncol, Np);
-        else LAUNCH((k_gemm_wx<T, false, 64>), W, ldw, B, C, M, Kd, ncol, Np);
-    }
-#undef LAUNCH
+    if (trans_a) lin_pick<T, true, false>(W, ldw, B, C, M, Kd, ncol, Np, s);
+    else lin_pick<T, false, false>(W, ldw, B, C, M, Kd - 1, ncol, Np, s);
 }
 
 // ---------------------------------------------------------------------------
 // K5: partial[chunk][m][k] = sum over the chunk's columns of A[m][c] Bext[k][c].
+// The reduction runs along the contiguous column axis, so no operand is
+// staged in LDS: the sum over columns may visit them in any order as long as
+// A and B visit them alike, so lane (p = lane%16, q = lane/16) of a 16x16x4
+// MFMA takes, for row p of its A block and row p of its B block, the V
+// consecutive columns c0 + q V .. c0 + q V + V-1, and
+// feeds them to V successive MFMAs as its k = q element (V: one or two
+// 16-byte loads).  A wave owns the
+// whole (16 RB) x (16 CB) output tile and every fourth step of the
+// workgroup's column chunk; the next step's loads are issued before the
+// current step's MFMAs.  The four waves' tiles are summed in LDS in a fixed
+// order and written as one FP64 partial per chunk.
+// With a bias row (K5), B holds the Kd-1 rows after it and the bias column
+// of the gradient, the sum of A over the first Np columns (slot 0), is added
+// up by the lanes of the first k tile from the A values they already hold.
+template <typename T> struct Vec2x16;
+template <> struct Vec2x16<float> { typedef float V __attribute__((ext_vector_type(4))); };
+template <> struct Vec2x16<double> { typedef double V __attribute__((ext_vector_type(2))); };
+
+template <typename T, int RB, int CB, int NH>
+__global__ __launch_bounds__(256) void k_grad_nt(const T *__restrict__ A, const T *__restrict__ B, int M, int Kd,
+                                                 long ncol, long Np, long chunk, int bias_row, int ntm, int ntk,
+                                                 int nchunk, double *__restrict__ partial)
+{
+    constexpr int H = 16 / sizeof(T);   // elements per 16-byte load
+    constexpr int V = NH * H;           // columns per lane per step
+    constexpr int SC = 4 * V;           // columns per wave step
+    constexpr int LD = 16 * CB + 1;     // reduction row: the tile, then the bias column
+    typedef typename Vec2x16<T>::V VT;
+    __shared__ double red[16 * RB * LD];
+    const int G = gridDim.x, bid = blockIdx.x;
+    const int w = (bid & 7) * (G >> 3) + (bid >> 3);
+    const int ntile = ntm * ntk;
+    const int tile = w % ntile, chk = w / ntile;
+    if (chk >= nchunk) return;
+    const int koff = bias_row ? 1 : 0, Kg = Kd - koff;   // GEMM columns k <-> weight column k + koff
+    const int m0 = (tile % ntm) * 16 * RB, k0 = (tile / ntm) * 16 * CB;
+    const bool do_bias = bias_row && k0 == 0;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int p = lane & 15, q = lane >> 4;
+    const long cb = (long)chk * chunk, ce = min(ncol, cb + chunk);
+
+    // per-lane row pointers, clamped into the operands: a row past M or Kg
+    // only feeds output entries that are never written
+    const T *pa[RB];
+    const T *pb[CB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i) pa[i] = A + (long)min(m0 + 16 * i + p, M - 1) * ncol + q * V;
+#pragma unroll
+    for (int j = 0; j < CB; ++j) pb[j] = B + (long)min(k0 + 16 * j + p, Kg - 1) * ncol + q * V;
+    typename Mfma<T>::V acc[RB][CB];
+    T bacc[RB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+        bacc[i] = T(0);
+#pragma unroll
+        for (int j = 0; j < CB; ++j) acc[i][j] = typename Mfma<T>::V{0, 0, 0, 0};
+    }
+
+    VT a0[RB][NH], b0[CB][NH], a1[RB][NH], b1[CB][NH];
+    auto load = [&](VT (&ar)[RB][NH], VT (&br)[CB][NH], long c) {
+#pragma unroll
+        for (int i = 0; i < RB; ++i)
+#pragma unroll
+            for (int h = 0; h < NH; ++h) ar[i][h] = *reinterpret_cast<const VT *>(pa[i] + c + h * H);
+#pragma unroll
+        for (int j = 0; j < CB; ++j)
+#pragma unroll
+            for (int h = 0; h < NH; ++h) br[j][h] = *reinterpret_cast<const VT *>(pb[j] + c + h * H);
+    };
+    auto mul = [&](const VT (&ar)[RB][NH], const VT (&br)[CB][NH], long c) {
+        if (do_bias && c < Np) {   // a step never straddles Np (multiples of 16)
+#pragma unroll
+            for (int i = 0; i < RB; ++i)
+#pragma unroll
+                for (int h = 0; h < NH; ++h)
+#pragma unroll
+                    for (int e = 0; e < H; ++e) bacc[i] += ar[i][h][e];
+        }
+#pragma unroll
+        for (int h = 0; h < NH; ++h)
+#pragma unroll
+            for (int e = 0; e < H; ++e)
+#pragma unroll
+                for (int i = 0; i < RB; ++i)
+#pragma unroll
+                    for (int j = 0; j < CB; ++j) acc[i][j] = Mfma<T>::mma(ar[i][h][e], br[j][h][e], acc[i][j]);
+    };
+    long c = cb + (long)wave * SC;
+    if (c < ce) load(a0, b0, c);
+    while (c < ce) {
+        const long c1 = c + 4 * SC;
+        if (c1 < ce) load(a1, b1, c1);
+        mul(a0, b0, c);
+        if (c1 >= ce) break;
+        const long c2 = c1 + 4 * SC;
+        if (c2 < ce) load(a0, b0, c2);
+        mul(a1, b1, c1);
+        c = c2;
+    }
+    // fixed-order sum of the four waves' tiles (bias: also of the four lanes
+    // q that share a row)
+    for (int wv = 0; wv < 4; ++wv) {
+        if (wave == wv) {
+#pragma unroll
+            for (int i = 0; i < RB; ++i)
+#pragma unroll
+                for (int j = 0; j < CB; ++j)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int idx = (16 * i + Mfma<T>::row(lane, r)) * LD + 16 * j + p;
+                        red[idx] = (wv == 0 ? 0.0 : red[idx]) + (double)acc[i][j][r];
+                    }
+            if (do_bias) {
+                for (int qq = 0; qq < 4; ++qq) {
+                    if (q == qq) {
+#pragma unroll
+                        for (int i = 0; i < RB; ++i) {
+                            const int idx = (16 * i + p) * LD + 16 * CB;
+                            red[idx] = (wv == 0 && qq == 0 ? 0.0 : red[idx]) + (double)bacc[i];
+                        }
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                }
+            }
+        }
+        __syncthreads();
+    }
+    double *P = partial + (long)chk * M * Kd;
+    for (int idx = tid; idx < 16 * RB * LD; idx += 256) {
+        const int r = idx / LD, cidx = idx % LD;
+        const int m = m0 + r;
+        if (m >= M) continue;
+        if (cidx < 16 * CB) {
+            const int k = k0 + cidx;
+            if (k < Kg) P[(long)m * Kd + koff + k] = red[idx];
+        } else if (do_bias) {
+            P[(long)m * Kd] = red[idx];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The Gauss-Newton Gram matrix (G = R R^T over the point axis).
 // Block tile 16*RB*4 rows (four waves stacked along m) x 144 columns of the
 // weight block (nine 16-wide MFMA blocks), so a weight block of up to
 // 128 x 144 is one tile and each operand streams from HBM once.
@@ -397,19 +611,42 @@ __global__ void k_reduce_pass2(const double *__restrict__ part2, int nsplit, lon
 
 int grad_reduce_splits(int nchunk) { return nchunk >= 64 ? 16 : (nchunk >= 16 ? 4 : 1); }
 
+// K5 tile columns: 64, or 128 for FP32 blocks wider than 64
+static int grad_cb(int esz, int Kg) { return (esz == 4 && Kg > 64) ? 8 : 4; }
+
+void grad_chunks(long ncol, int M, int Kd, int esz, int *nchunk, long *chunk)
+{
+    // about 1024 workgroups over tiles x chunks; chunks are whole multiples
+    // of 128 columns (every wave step of k_grad_nt stays inside its chunk)
+    const int cb = grad_cb(esz, Kd - 1);
+    const int tiles = ((M + 63) / 64) * ((Kd - 1 + 16 * cb - 1) / (16 * cb));
+    long nc = std::max(1, 1024 / tiles);
+    nc = std::min(nc, std::max(1L, ncol / 512));
+    const long ch = (((ncol + nc - 1) / nc) + 127) / 128 * 128;
+    *nchunk = (int)((ncol + ch - 1) / ch);
+    *chunk = ch;
+}
+
 template <typename T>
 void launch_gemm_grad(int prec, const T *A, const T *B, int M, int Kd, long ncol, long Np, double *partial,
                       int nchunk, long chunk_cols, double *grad, hipStream_t s)
 {
     (void)prec;
-    const int rb = M > 64 ? 2 : 1;
-    dim3 grid((unsigned)nchunk, (unsigned)((M + 64 * rb - 1) / (64 * rb)), (unsigned)((Kd + GN - 1) / GN));
-    if (rb == 2)
-        hipLaunchKernelGGL((k_gemm_grad<T, 2>), grid, dim3(256), 0, s, A, B, M, Kd, ncol, Np, chunk_cols, 1,
-                           partial);
-    else
-        hipLaunchKernelGGL((k_gemm_grad<T, 1>), grid, dim3(256), 0, s, A, B, M, Kd, ncol, Np, chunk_cols, 1,
-                           partial);
+    const int cb = grad_cb((int)sizeof(T), Kd - 1);
+    const int ntm = (M + 63) / 64, ntk = (Kd - 1 + 16 * cb - 1) / (16 * cb);
+    long g = (long)ntm * ntk * nchunk;
+    g = (g + 7) / 8 * 8;
+    bool done = false;
+    if constexpr (sizeof(T) == 4) {
+        if (cb == 8) {
+            hipLaunchKernelGGL((k_grad_nt<T, 4, 8, 2>), dim3((unsigned)g), dim3(256), 0, s, A, B, M, Kd, ncol, Np,
+                               chunk_cols, 1, ntm, ntk, nchunk, partial);
+            done = true;
+        }
+    }
+    if (!done)
+        hipLaunchKernelGGL((k_grad_nt<T, 4, 4, sizeof(T) == 4 ? 2 : 1>), dim3((unsigned)g), dim3(256), 0, s, A, B, M, Kd, ncol, Np,
+                           chunk_cols, 1, ntm, ntk, nchunk, partial);
     const long len = (long)M * Kd;
     const int ns = grad_reduce_splits(nchunk);
     double *part2 = partial + (long)nchunk * len;

@@ -85,6 +85,10 @@ template <typename T>
 void launch_gemm_grad(int prec, const T *A, const T *B, int M, int Kd, long ncol, long Np,
                       double *partial, int nchunk, long chunk_cols, double *grad, hipStream_t s);
 
+// column chunks of launch_gemm_grad for an M x Kd weight block over ncol
+// columns of esz-byte elements (chunk: a multiple of 128 columns)
+void grad_chunks(long ncol, int M, int Kd, int esz, int *nchunk, long *chunk);
+
 // partial buffer length needed by launch_gemm_grad / launch_gram:
 // (nchunk + grad_reduce_splits(nchunk)) * M * Kd doubles
 int grad_reduce_splits(int nchunk);
