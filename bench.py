@@ -26,6 +26,22 @@ sys.path.insert(0, ROOT)
 
 METRIC = "fwd+adjoint points/s, mixed-prec vs FP64 on GPU, at stated err vs FP64 oracle"
 PEAK = {"hbm": 8000.0, "fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
+DTYPE = {"fp32": "f32 (FP64-accumulated forward map)",
+         "bf16x3": "f32 storage, bf16 hi+lo split MFMA",
+         "bf16x6": "f32 storage, bf16 three-way split MFMA (FP64-accumulated forward map)"}
+TRAFFIC = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic.json")
+
+
+def traffic_of(workload, mode, category):
+    """HBM bytes per launch of a kernel category, as measured by the
+    FETCH_SIZE / WRITE_SIZE counter passes summarised in profiles/ (scaled
+    by the same counters' reading of a plain copy); None if not measured."""
+    try:
+        with open(TRAFFIC) as f:
+            t = json.load(f)
+        return t["workloads"][workload][mode][category]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def algorithmic(wl, na, F, Q, P, N, wbytes):
@@ -107,7 +123,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="dense4_k3")
-    ap.add_argument("--mixed", default="bf16x3", choices=["bf16x3", "fp32"])
+    ap.add_argument("--mixed", default="bf16x6", choices=["bf16x6", "fp32", "bf16x3"])
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only copy a buffer of this many MiB 5 times (HBM counter calibration)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -117,6 +135,15 @@ def main():
         return 2
 
     import torch
+    if args.copy_mib:
+        n = args.copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_mib": args.copy_mib, "launches": 5}), flush=True)
+        return 0
     import dnnf90_amd as D
     from dnnf90_amd.workloads import WORKLOADS
     from dnnf90_amd._lib import lib
@@ -195,7 +222,9 @@ def main():
 
     fp64 = run_mode(D.PREC_FP64, "fp64", 8, PEAK["fp64"])
     others = {}
-    for name, prec, peak in (("fp32", D.PREC_FP32, PEAK["fp32"]), ("bf16x3", D.PREC_BF16X3, PEAK["bf16"] / 3.0)):
+    modes = (("fp32", D.PREC_FP32, PEAK["fp32"]), ("bf16x3", D.PREC_BF16X3, PEAK["bf16"] / 3.0),
+             ("bf16x6", D.PREC_BF16X6, PEAK["bf16"] / 6.0))
+    for name, prec, peak in modes:
         r = run_mode(prec, name, 4, peak)
         r["err_T"] = rel_by_order(r["T"], fp64["T"], deg)
         r["err_g"] = float(np.abs(r["grad"] - fp64["grad"]).max() / np.abs(fp64["grad"]).max())
@@ -234,13 +263,14 @@ def main():
     dom = max((c for c in kk if "frac" in kk[c]), key=lambda c: kk[c]["ms_per_step"])
     dk = kk[dom]
     roof = {"bound": dk["bound"], "achieved": dk["achieved"], "peak": dk["peak"], "unit": dk["unit"],
-            "frac": dk["frac"], "traffic": None, "kernel": dom}
+            "frac": dk["frac"], "traffic": traffic_of(wl.name, args.mixed, dom), "kernel": dom,
+            "launch_ms": dk["ms_per_step"] / max(dk["launches_per_step"], 1e-9)}
 
     line = {
         "metric": METRIC, "value": mixed["points_per_s"], "unit": "points/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": mixed["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "bf16x3+f32" if args.mixed == "bf16x3" else "f32", "data": "synthetic",
+        "dtype": DTYPE[args.mixed], "data": "synthetic",
         "config": {"workload": f"{wl.name}: MLP {wl.dims} {'tanh' if wl.act == 0 else 'erf-GELU'}, "
                                f"D0={wl.d0} K={wl.k} |A|={na}, N={N}, loss mean square of {wl.note}",
                    "precision": args.mixed},

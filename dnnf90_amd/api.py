@@ -13,7 +13,7 @@ import numpy as np
 
 from ._lib import lib
 
-PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
+PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_BF16X6 = 0, 1, 2, 3
 DEVICE_PTRS = 1
 ACT_TANH, ACT_SIN, ACT_ERF, ACT_J0, ACT_J1, ACT_GELU = 0, 1, 2, 3, 4, 5
 
@@ -215,15 +215,15 @@ class BatchWork:
     def sync(self):
         _chk(lib().dnnf90_bwork_sync(self.id), "dnnf90_bwork_sync")
 
-    KERNEL_CATEGORIES = ("linear", "bell_fwd", "bell_bwd", "grad", "linear_t", "other")
+    KERNEL_CATEGORIES = ("linear", "bell_fwd", "bell_bwd", "grad", "linear_t", "other", "grad_reduce")
 
     def profile(self, on=True):
         _chk(lib().dnnf90_bwork_profile(self.id, 1 if on else 0), "dnnf90_bwork_profile")
 
     def kernel_ms(self):
         """{category: (total ms, launches)} since the last call (resets)."""
-        ms = np.zeros(6)
-        cnt = np.zeros(6, np.int64)
+        ms = np.zeros(len(self.KERNEL_CATEGORIES))
+        cnt = np.zeros(len(self.KERNEL_CATEGORIES), np.int64)
         _chk(lib().dnnf90_bwork_kernel_ms(self.id, ms.ctypes.data, cnt.ctypes.data), "dnnf90_bwork_kernel_ms")
         return {c: (float(ms[i]), int(cnt[i])) for i, c in enumerate(self.KERNEL_CATEGORIES)}
 

@@ -239,7 +239,7 @@ struct Grad {
 };
 
 // kernel categories timed by HIP events when profiling is on
-enum { CAT_LINEAR = 0, CAT_BELL_FWD, CAT_BELL_BWD, CAT_GRAD, CAT_LINEAR_T, CAT_OTHER, NCAT };
+enum { CAT_LINEAR = 0, CAT_BELL_FWD, CAT_BELL_BWD, CAT_GRAD, CAT_LINEAR_T, CAT_OTHER, CAT_GRAD_REDUCE, NCAT };
 
 struct KProf {
     bool on = false;
@@ -469,18 +469,19 @@ static void backward_t(const Net &nt, BatchWork &b, double *grad, float *rows_t,
                 launch_bell_bwd<T>((const T *)b.p1, (const T *)b.hpl[l], (T *)b.p0, nd, Np, pr, s);
         }
         const T *X = (const T *)(l == 1 ? b.t0 : b.tpl[l - 1]);
-        {
+        if (rows_t) {
             PScope p(b, CAT_GRAD);
-            if (rows_t) {
-                launch_point_rows<T>((const T *)b.p0, X, nd, ndm + 1, na, Np, rows_t + nt.woff[l] * ldr,
-                                     ldr, s);
-            } else {
-                int nc;
-                long ch;
-                grad_split(ncol, nd, ndm + 1, (int)b.esz, &nc, &ch);
-                launch_gemm_grad<T>(b.prec, (const T *)b.p0, X, nd, ndm + 1, ncol, Np, b.partial, nc, ch,
-                                    grad + nt.woff[l], s);
+            launch_point_rows<T>((const T *)b.p0, X, nd, ndm + 1, na, Np, rows_t + nt.woff[l] * ldr, ldr, s);
+        } else {
+            int nc;
+            long ch;
+            grad_split(ncol, nd, ndm + 1, (int)b.esz, &nc, &ch);
+            {
+                PScope p(b, CAT_GRAD);
+                launch_gemm_grad<T>(b.prec, (const T *)b.p0, X, nd, ndm + 1, ncol, Np, b.partial, nc, ch, s);
             }
+            PScope p(b, CAT_GRAD_REDUCE);
+            launch_grad_reduce(b.partial, nc, (long)nd * (ndm + 1), grad + nt.woff[l], s);
         }
         if (l > 1) {
             PScope p(b, CAT_LINEAR_T);
@@ -877,7 +878,7 @@ static int make_work(Registry<BatchWork> &reg, int nid, int maxpt, int prec)
 {
     Net *nt = g_net.get(nid);
     if (!nt) return fail(-1, "bad network handle");
-    if (maxpt < 1 || prec < 0 || prec > 2) return fail(-1, "maxpt >= 1 and prec in 0..2");
+    if (maxpt < 1 || prec < 0 || prec > 3) return fail(-1, "maxpt >= 1 and prec in 0..3");
     int id = reg.slot();
     if (id < 0) return fail(-1, "no free work-space handle");
     int rc = nt->ensure_device(prec != PREC_FP64);
@@ -1213,7 +1214,7 @@ int dnnf90_set_codegen(int on)
 int dnnf90_net_kernel_info(int nid, int prec, int *info)
 {
     Net *nt = g_net.get(nid);
-    if (!nt || !info || prec < 0 || prec > 2) return fail(-1, "bad network handle or argument");
+    if (!nt || !info || prec < 0 || prec > 3) return fail(-1, "bad network handle or argument");
     int rc = nt->ensure_device(prec != PREC_FP64);
     if (rc) return rc;
     const GenKernels &g = nt->gen[prec == PREC_FP64 ? 1 : 0];

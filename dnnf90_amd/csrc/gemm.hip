@@ -8,41 +8,16 @@
 // The column axis is slot * Np + point for every layer, so each map is one
 // GEMM over na*Np columns.  Row 0 of [1; T] is the bias channel: 1 on the
 // first Np columns (slot 0), 0 elsewhere; it is generated, never stored.
-//
-// MFMA layouts (16x16 family, wave of 64): lane l feeds row l mod 16 of A
-// and column l mod 16 of B; the k index a lane feeds is l/16 (x4 forms) or
-// 8*(l/16)+e (bf16 x32 form).  Result: column l mod 16, rows 4*(l/16)+r for
-// the FP32-accumulating forms and l/16+4r for v_mfma_f64_16x16x4f64
-// (measured on gfx950).
+// The MFMA operand layouts are described in mfma.hpp; the BF16 three-way
+// split forms of K1, K5 and K6 are in gemm_x6.hip.
 #include <hip/hip_bf16.h>
 
 #include <algorithm>
 
 #include "kernels.hpp"
+#include "mfma.hpp"
 
 namespace dnnf90 {
-
-typedef double dbl4 __attribute__((ext_vector_type(4)));
-typedef float flt4 __attribute__((ext_vector_type(4)));
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-
-template <typename T> struct Mfma;
-template <> struct Mfma<double> {
-    typedef dbl4 V;
-    static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
-    static __device__ __forceinline__ V mma(double a, double b, V c)
-    {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-};
-template <> struct Mfma<float> {
-    typedef flt4 V;
-    static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
-    static __device__ __forceinline__ V mma(float a, float b, V c)
-    {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-};
 
 // value of row gk of [1; B] (bias row first) at column gc
 template <typename T, bool kBiasRow>
@@ -348,6 +323,10 @@ void launch_gemm_wx(int prec, bool trans_a, const T *W, int ldw, const T *B, T *
 #undef LAUNCH
             return;
         }
+        if (prec == PREC_BF16X6) {
+            launch_lin_x6(trans_a, W, ldw, B, C, M, Kd, ncol, Np, s);
+            return;
+        }
         // FP32 mode: the forward map accumulates in FP64
         if (trans_a) lin_pick<T, true, false>(W, ldw, B, C, M, Kd, ncol, Np, s);
         else lin_pick<T, false, true>(W, ldw, B, C, M, Kd - 1, ncol, Np, s);
@@ -373,9 +352,6 @@ void launch_gemm_wx(int prec, bool trans_a, const T *W, int ldw, const T *B, T *
 // With a bias row (K5), B holds the Kd-1 rows after it and the bias column
 // of the gradient, the sum of A over the first Np columns (slot 0), is added
 // up by the lanes of the first k tile from the A values they already hold.
-template <typename T> struct Vec2x16;
-template <> struct Vec2x16<float> { typedef float V __attribute__((ext_vector_type(4))); };
-template <> struct Vec2x16<double> { typedef double V __attribute__((ext_vector_type(2))); };
 
 template <typename T, int RB, int CB, int NH>
 __global__ __launch_bounds__(256) void k_grad_nt(const T *__restrict__ A, const T *__restrict__ B, int M, int Kd,
@@ -629,25 +605,30 @@ void grad_chunks(long ncol, int M, int Kd, int esz, int *nchunk, long *chunk)
 
 template <typename T>
 void launch_gemm_grad(int prec, const T *A, const T *B, int M, int Kd, long ncol, long Np, double *partial,
-                      int nchunk, long chunk_cols, double *grad, hipStream_t s)
+                      int nchunk, long chunk_cols, hipStream_t s)
 {
-    (void)prec;
     const int cb = grad_cb((int)sizeof(T), Kd - 1);
     const int ntm = (M + 63) / 64, ntk = (Kd - 1 + 16 * cb - 1) / (16 * cb);
     long g = (long)ntm * ntk * nchunk;
     g = (g + 7) / 8 * 8;
     bool done = false;
     if constexpr (sizeof(T) == 4) {
-        if (cb == 8) {
+        if (prec == PREC_BF16X6) {
+            launch_grad_x6(A, B, M, Kd, ncol, Np, partial, nchunk, chunk_cols, s);
+            done = true;
+        } else if (cb == 8) {
             hipLaunchKernelGGL((k_grad_nt<T, 4, 8, 2>), dim3((unsigned)g), dim3(256), 0, s, A, B, M, Kd, ncol, Np,
                                chunk_cols, 1, ntm, ntk, nchunk, partial);
             done = true;
         }
     }
     if (!done)
-        hipLaunchKernelGGL((k_grad_nt<T, 4, 4, sizeof(T) == 4 ? 2 : 1>), dim3((unsigned)g), dim3(256), 0, s, A, B, M, Kd, ncol, Np,
-                           chunk_cols, 1, ntm, ntk, nchunk, partial);
-    const long len = (long)M * Kd;
+        hipLaunchKernelGGL((k_grad_nt<T, 4, 4, sizeof(T) == 4 ? 2 : 1>), dim3((unsigned)g), dim3(256), 0, s, A, B, M,
+                           Kd, ncol, Np, chunk_cols, 1, ntm, ntk, nchunk, partial);
+}
+
+void launch_grad_reduce(double *partial, int nchunk, long len, double *grad, hipStream_t s)
+{
     const int ns = grad_reduce_splits(nchunk);
     double *part2 = partial + (long)nchunk * len;
     const unsigned gx = (unsigned)((len + 255) / 256);
@@ -708,7 +689,7 @@ void launch_point_rows(const T *A, const T *B, int M, int Kd, int na, long Np, f
     template void launch_gemm_wx<T>(int, bool, const T *, int, const T *, T *, int, int, long, long,         \
                                     hipStream_t);                                                           \
     template void launch_gemm_grad<T>(int, const T *, const T *, int, int, long, long, double *, int, long,  \
-                                      double *, hipStream_t);                                               \
+                                      hipStream_t);                                                         \
     template void launch_point_rows<T>(const T *, const T *, int, int, int, long, float *, long, hipStream_t);
 INST(double)
 INST(float)

@@ -1,0 +1,352 @@
+// gemm_x6.hip -- K1, K6 and K5 on BF16 three-way split products.
+//
+// An FP32 operand x is carried as three bf16 pieces x = h + m + l (mfma.hpp
+// split3), and a product a*b as the six terms whose size is 2^-16 |a b| or
+// more; every term is one v_mfma_f32_16x16x32_bf16, exact products summed in
+// FP32.  Six bf16 MFMAs cover 32 k at 96 cycles where FP32 MFMA needs eight
+// v_mfma_f32_16x16x4f32 at 256 cycles, with product error near FP32's.  The
+// forward map K1 (whose accumulation error feeds every carried derivative)
+// moves each 32-term MFMA sum into an FP64 accumulator, which keeps the
+// forward error of an FP32-storage pass below 1e-6; K6 and K5 accumulate in
+// FP32 (K5's chunk partials are then summed in FP64 like every K5).
+//
+//   K1  S   = W  [1; T]   (lib/train_module.f90:262-278)
+//   K6  Tb  = W' Sbar     (lib/train_module.f90:430-444)
+//   K5  dW += Sbar [1; T]^T (lib/train_module.f90:408-422)
+#include <hip/hip_bf16.h>
+
+#include <algorithm>
+
+#include "kernels.hpp"
+#include "mfma.hpp"
+
+namespace dnnf90 {
+
+// ---------------------------------------------------------------------------
+// K1 / K6.  The workgroup's 16*RBT rows of A (K1: the weight columns after
+// the bias; K6: the transposed weights) are split once into LDS in operand
+// order, Ap[rb][kc][piece][lane] = pieces of A[16 rb + lane%16][32 kc +
+// 8 (lane/16) .. +7], zero past Kd.  A wave streams NCB adjacent 16-column
+// blocks of B: lane (col = lane%16, q = lane/16) reads rows 32 kc + 8 q ..
+// +7 of its column (sixteen lanes read 64 contiguous bytes of a row), splits
+// them and multiplies by every row block; the next chunk's rows are loaded
+// before the current chunk's MFMAs.  K1's bias is added in the epilogue to
+// the first Np columns (slot 0; NCB*16 columns never straddle Np).
+template <bool kTrans, bool kAcc64, int RBT, int NCB>
+__global__ __launch_bounds__(256) void k_lin_x6(const float *__restrict__ W, int ldw, const float *__restrict__ B,
+                                                float *__restrict__ C, int M, int Kd, long ncol, long Np, int nrt,
+                                                int ncw, long cupw)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char x6_lds[];
+    bf16x8 *Ap = reinterpret_cast<bf16x8 *>(x6_lds);
+    const int G = gridDim.x, bid = blockIdx.x;
+    const int w = (bid & 7) * (G >> 3) + (bid >> 3);
+    const int rt = w % nrt, cw = w / nrt;
+    if (cw >= ncw) return;
+    const long ncu = ncol / (16 * NCB);
+    const long cu0 = (long)cw * cupw, cu1 = min(ncu, cu0 + cupw);
+    if (cu0 >= cu1) return;
+    const int nkc = (Kd + 31) >> 5;
+    const int tid = threadIdx.x, lane = tid & 63, q = lane >> 4;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int m0 = rt * 16 * RBT;
+    for (int u = tid; u < RBT * nkc * 64; u += 256) {
+        const int rb = u / (nkc * 64), rem = u - rb * nkc * 64, kc = rem >> 6, ln = rem & 63;
+        const int m = m0 + 16 * rb + (ln & 15), k = 32 * kc + 8 * (ln >> 4);
+        float v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int kk = k + e;
+            v[e] = (m < M && kk < Kd) ? (kTrans ? W[(long)kk * ldw + 1 + m] : W[(long)m * ldw + 1 + kk]) : 0.f;
+        }
+        bf16x8 h, mm, l;
+        split3(v, h, mm, l);
+        bf16x8 *dst = Ap + (long)(rb * nkc + kc) * 3 * 64 + ln;
+        dst[0] = h;
+        dst[64] = mm;
+        dst[128] = l;
+    }
+    float *Bias = reinterpret_cast<float *>(Ap + (long)RBT * nkc * 3 * 64);
+    if (!kTrans && tid < 16 * RBT) Bias[tid] = m0 + tid < M ? W[(long)(m0 + tid) * ldw] : 0.f;
+    __syncthreads();
+    long cu = cu0 + wave;
+    if (cu >= cu1) return;
+
+    const float *bl = B + (lane & 15);
+    float rc[NCB][8], rn[NCB][8];
+    auto load = [&](float (&dst)[NCB][8], long cul, int kc) {
+#pragma unroll
+        for (int j = 0; j < NCB; ++j) {
+            const float *src = bl + cul * 16 * NCB + 16 * j;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) dst[j][e] = src[(long)min(32 * kc + 8 * q + e, Kd - 1) * ncol];
+        }
+    };
+    flt4 cacc[RBT][NCB];
+    double acc64[kAcc64 ? RBT : 1][kAcc64 ? NCB : 1][4];
+    load(rc, cu, 0);
+    int kc = 0;
+    for (;;) {
+        long cun = cu;
+        int kcn = kc + 1;
+        if (kcn == nkc) {
+            kcn = 0;
+            cun = cu + 4;
+        }
+        const bool more = cun < cu1;
+        if (more) load(rn, cun, kcn);
+        if (kc == 0 || kAcc64) {
+#pragma unroll
+            for (int rb = 0; rb < RBT; ++rb)
+#pragma unroll
+                for (int j = 0; j < NCB; ++j) cacc[rb][j] = flt4{0, 0, 0, 0};
+        }
+        if (kAcc64 && kc == 0) {
+#pragma unroll
+            for (int rb = 0; rb < (kAcc64 ? RBT : 1); ++rb)
+#pragma unroll
+                for (int j = 0; j < (kAcc64 ? NCB : 1); ++j)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) acc64[rb][j][r] = 0.0;
+        }
+        bf16x8 bh[NCB], bm[NCB], blo[NCB];
+#pragma unroll
+        for (int j = 0; j < NCB; ++j) split3(rc[j], bh[j], bm[j], blo[j]);
+        const bf16x8 *Ak = Ap + (long)kc * 3 * 64 + lane;
+#pragma unroll
+        for (int rb = 0; rb < RBT; ++rb) {
+            const bf16x8 *a = Ak + (long)rb * nkc * 3 * 64;
+            const bf16x8 ah = a[0], am = a[64], al = a[128];
+#pragma unroll
+            for (int j = 0; j < NCB; ++j) cacc[rb][j] = mma_x6(ah, am, al, bh[j], bm[j], blo[j], cacc[rb][j]);
+        }
+        if (kAcc64) {
+#pragma unroll
+            for (int rb = 0; rb < (kAcc64 ? RBT : 1); ++rb)
+#pragma unroll
+                for (int j = 0; j < (kAcc64 ? NCB : 1); ++j)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) acc64[rb][j][r] += (double)cacc[rb][j][r];
+        }
+        if (kc == nkc - 1) {
+            const bool bias = !kTrans && cu * 16 * NCB < Np;
+#pragma unroll
+            for (int rb = 0; rb < RBT; ++rb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int mr = 16 * rb + 4 * q + r, m = m0 + mr;
+                    const float add = bias ? Bias[mr] : 0.f;
+#pragma unroll
+                    for (int j = 0; j < NCB; ++j) {
+                        const long c = cu * 16 * NCB + 16 * j + (lane & 15);
+                        float val;
+                        if (kAcc64)
+                            val = (float)(acc64[kAcc64 ? rb : 0][kAcc64 ? j : 0][r] + (double)add);
+                        else
+                            val = cacc[rb][j][r] + add;
+                        if (m < M) C[(long)m * ncol + c] = val;
+                    }
+                }
+        }
+        if (!more) break;
+#pragma unroll
+        for (int j = 0; j < NCB; ++j)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) rc[j][e] = rn[j][e];
+        cu = cun;
+        kc = kcn;
+    }
+}
+
+template <bool kTrans, bool kAcc64, int RBT>
+static void lin_x6_go(const float *W, int ldw, const float *B, float *C, int M, int Kd, long ncol, long Np,
+                      hipStream_t s)
+{
+    constexpr int NCB = 2;
+    const int nkc = (Kd + 31) / 32;
+    const size_t lds = (size_t)RBT * nkc * 3 * 64 * 16 + 16 * RBT * sizeof(float);
+    static size_t lds_set = 64 * 1024;   // dynamic LDS above 64 KiB must be requested
+    if (lds > lds_set) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_lin_x6<kTrans, kAcc64, RBT, NCB>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        lds_set = lds;
+    }
+    const int nrt = (M + 16 * RBT - 1) / (16 * RBT);
+    const long ncu = ncol / (16 * NCB);
+    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / lds));
+    const long want = std::max(1L, (long)256 * per_cu / nrt);
+    long cupw = (ncu + want - 1) / want;
+    cupw = (cupw + 3) / 4 * 4;
+    const long ncw = (ncu + cupw - 1) / cupw;
+    long g = (long)nrt * ncw;
+    g = (g + 7) / 8 * 8;
+    hipLaunchKernelGGL((k_lin_x6<kTrans, kAcc64, RBT, NCB>), dim3((unsigned)g), dim3(256), lds, s, W, ldw, B, C, M,
+                       Kd, ncol, Np, nrt, (int)ncw, cupw);
+}
+
+template <bool kTrans, bool kAcc64>
+static void lin_x6_pick(const float *W, int ldw, const float *B, float *C, int M, int Kd, long ncol, long Np,
+                        hipStream_t s)
+{
+    const size_t per_rb = (size_t)((Kd + 31) / 32) * 3 * 64 * 16;
+    int rbt = 8;
+    while (rbt > 1 && 16 * (rbt / 2) >= M) rbt /= 2;
+    while (rbt > 1 && rbt * per_rb > (size_t)144 * 1024) rbt /= 2;
+    switch (rbt) {
+    case 8: lin_x6_go<kTrans, kAcc64, 8>(W, ldw, B, C, M, Kd, ncol, Np, s); break;
+    case 4: lin_x6_go<kTrans, kAcc64, 4>(W, ldw, B, C, M, Kd, ncol, Np, s); break;
+    case 2: lin_x6_go<kTrans, kAcc64, 2>(W, ldw, B, C, M, Kd, ncol, Np, s); break;
+    default: lin_x6_go<kTrans, kAcc64, 1>(W, ldw, B, C, M, Kd, ncol, Np, s); break;
+    }
+}
+
+void launch_lin_x6(bool trans_a, const float *W, int ldw, const float *B, float *C, int M, int Kd, long ncol,
+                   long Np, hipStream_t s)
+{
+    if (trans_a) lin_x6_pick<true, false>(W, ldw, B, C, M, Kd, ncol, Np, s);
+    else lin_x6_pick<false, true>(W, ldw, B, C, M, Kd - 1, ncol, Np, s);
+}
+
+// ---------------------------------------------------------------------------
+// K5.  As k_grad_nt (gemm.hip): lane (p = lane%16, q = lane/16) reads the 8
+// consecutive columns c0 + 8 q .. +7 of row p of each A and B block with two
+// 16-byte loads -- exactly the 32-k slice of one bf16 MFMA -- splits them and
+// forms the six-term products; a wave owns the (16 RB) x (16 CB) tile over
+// every fourth 32-column step of the chunk, the next step's rows loaded
+// before the current step's MFMAs.  A rows are split once per step, B rows
+// one block at a time.  The bias column (sum of A over the first Np columns)
+// is added up from the loaded A rows by the first k tile.
+template <int RB, int CB>
+__global__ __launch_bounds__(256) void k_grad_x6(const float *__restrict__ A, const float *__restrict__ B, int M,
+                                                 int Kd, long ncol, long Np, long chunk, int ntm, int ntk,
+                                                 int nchunk, double *__restrict__ partial)
+{
+    constexpr int SC = 32;              // columns per wave step
+    constexpr int LD = 16 * CB + 1;     // reduction row: the tile, then the bias column
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    __shared__ double red[16 * RB * LD];
+    const int G = gridDim.x, bid = blockIdx.x;
+    const int w = (bid & 7) * (G >> 3) + (bid >> 3);
+    const int ntile = ntm * ntk;
+    const int tile = w % ntile, chk = w / ntile;
+    if (chk >= nchunk) return;
+    const int Kg = Kd - 1;   // GEMM column k <-> weight column k + 1
+    const int m0 = (tile % ntm) * 16 * RB, k0 = (tile / ntm) * 16 * CB;
+    const bool do_bias = k0 == 0;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int p = lane & 15, q = lane >> 4;
+    const long cb = (long)chk * chunk, ce = min(ncol, cb + chunk);
+
+    const float *pa[RB];
+    const float *pb[CB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i) pa[i] = A + (long)min(m0 + 16 * i + p, M - 1) * ncol + 8 * q;
+#pragma unroll
+    for (int j = 0; j < CB; ++j) pb[j] = B + (long)min(k0 + 16 * j + p, Kg - 1) * ncol + 8 * q;
+    flt4 acc[RB][CB];
+    float bacc[RB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+        bacc[i] = 0.f;
+#pragma unroll
+        for (int j = 0; j < CB; ++j) acc[i][j] = flt4{0, 0, 0, 0};
+    }
+    float a0[RB][8], b0[CB][8], a1[RB][8], b1[CB][8];
+    auto ld8 = [](float (&d)[8], const float *src) {
+        const f4 x = *reinterpret_cast<const f4 *>(src), y = *reinterpret_cast<const f4 *>(src + 4);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            d[e] = x[e];
+            d[4 + e] = y[e];
+        }
+    };
+    auto load = [&](float (&ar)[RB][8], float (&br)[CB][8], long c) {
+#pragma unroll
+        for (int i = 0; i < RB; ++i) ld8(ar[i], pa[i] + c);
+#pragma unroll
+        for (int j = 0; j < CB; ++j) ld8(br[j], pb[j] + c);
+    };
+    auto mul = [&](const float (&ar)[RB][8], const float (&br)[CB][8], long c) {
+        if (do_bias && c < Np) {   // a step never straddles Np (multiples of 32)
+#pragma unroll
+            for (int i = 0; i < RB; ++i)
+#pragma unroll
+                for (int e = 0; e < 8; ++e) bacc[i] += ar[i][e];
+        }
+        bf16x8 ah[RB], am[RB], al[RB];
+#pragma unroll
+        for (int i = 0; i < RB; ++i) split3(ar[i], ah[i], am[i], al[i]);
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            bf16x8 bh, bm, bl;
+            split3(br[j], bh, bm, bl);
+#pragma unroll
+            for (int i = 0; i < RB; ++i) acc[i][j] = mma_x6(ah[i], am[i], al[i], bh, bm, bl, acc[i][j]);
+        }
+    };
+    long c = cb + (long)wave * SC;
+    if (c < ce) load(a0, b0, c);
+    while (c < ce) {
+        const long c1 = c + 4 * SC;
+        if (c1 < ce) load(a1, b1, c1);
+        mul(a0, b0, c);
+        if (c1 >= ce) break;
+        const long c2 = c1 + 4 * SC;
+        if (c2 < ce) load(a0, b0, c2);
+        mul(a1, b1, c1);
+        c = c2;
+    }
+    for (int wv = 0; wv < 4; ++wv) {
+        if (wave == wv) {
+#pragma unroll
+            for (int i = 0; i < RB; ++i)
+#pragma unroll
+                for (int j = 0; j < CB; ++j)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int idx = (16 * i + 4 * q + r) * LD + 16 * j + p;
+                        red[idx] = (wv == 0 ? 0.0 : red[idx]) + (double)acc[i][j][r];
+                    }
+            if (do_bias) {
+                for (int qq = 0; qq < 4; ++qq) {
+                    if (q == qq) {
+#pragma unroll
+                        for (int i = 0; i < RB; ++i) {
+                            const int idx = (16 * i + p) * LD + 16 * CB;
+                            red[idx] = (wv == 0 && qq == 0 ? 0.0 : red[idx]) + (double)bacc[i];
+                        }
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                }
+            }
+        }
+        __syncthreads();
+    }
+    double *P = partial + (long)chk * M * Kd;
+    for (int idx = tid; idx < 16 * RB * LD; idx += 256) {
+        const int r = idx / LD, cidx = idx % LD;
+        const int m = m0 + r;
+        if (m >= M) continue;
+        if (cidx < 16 * CB) {
+            const int k = k0 + cidx;
+            if (k < Kg) P[(long)m * Kd + 1 + k] = red[idx];
+        } else if (do_bias) {
+            P[(long)m * Kd] = red[idx];
+        }
+    }
+}
+
+// tile of the split K5: 64 x 64 of the weight block (a larger tile spills)
+void launch_grad_x6(const float *A, const float *B, int M, int Kd, long ncol, long Np, double *partial,
+                    int nchunk, long chunk, hipStream_t s)
+{
+    const int ntm = (M + 63) / 64, ntk = (Kd - 1 + 63) / 64;
+    long g = (long)ntm * ntk * nchunk;
+    g = (g + 7) / 8 * 8;
+    hipLaunchKernelGGL((k_grad_x6<4, 4>), dim3((unsigned)g), dim3(256), 0, s, A, B, M, Kd, ncol, Np, chunk, ntm, ntk,
+                       nchunk, partial);
+}
+
+}  // namespace dnnf90

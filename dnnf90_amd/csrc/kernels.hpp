@@ -18,6 +18,7 @@ enum Precision : int {
     PREC_FP64 = 0,    // everything in FP64 (drop-in parity mode)
     PREC_FP32 = 1,    // FP32 storage, FP32 MFMA linear maps, FP32 recursion
     PREC_BF16X3 = 2,  // FP32 storage, BF16 hi+lo split MFMA linear maps
+    PREC_BF16X6 = 3,  // FP32 storage, BF16 three-way split MFMA linear maps
 };
 
 // one term of a table-driven sum, 16 bytes so a wave-uniform term is a
@@ -80,10 +81,21 @@ void launch_gemm_wx(int prec, bool trans_a, const T *W, int ldw, const T *B, T *
                     long ncol, long Np, hipStream_t s);
 
 // partial[chunk][M][Kd] = sum over the chunk's columns of A[m][c] * Bext[k][c]
-// (Bext row 0 = bias channel, rows 1.. = B); then grad[M][Kd] += sum_chunk
+// (Bext row 0 = bias channel, rows 1.. = B)
 template <typename T>
 void launch_gemm_grad(int prec, const T *A, const T *B, int M, int Kd, long ncol, long Np,
-                      double *partial, int nchunk, long chunk_cols, double *grad, hipStream_t s);
+                      double *partial, int nchunk, long chunk_cols, hipStream_t s);
+// grad[len] += sum over chunks of partial[chunk][len], in a fixed order
+void launch_grad_reduce(double *partial, int nchunk, long len, double *grad, hipStream_t s);
+
+// K1 (trans_a false) / K6 on BF16 three-way split products (gemm_x6.hip);
+// arguments as launch_gemm_wx
+void launch_lin_x6(bool trans_a, const float *W, int ldw, const float *B, float *C, int M, int Kd, long ncol,
+                   long Np, hipStream_t s);
+// K5 chunk partials on BF16 three-way split products (gemm_x6.hip);
+// partial[chunk][M][Kd] as launch_gemm_grad, without the final reduction
+void launch_grad_x6(const float *A, const float *B, int M, int Kd, long ncol, long Np, double *partial,
+                    int nchunk, long chunk, hipStream_t s);
 
 // column chunks of launch_gemm_grad for an M x Kd weight block over ncol
 // columns of esz-byte elements (chunk: a multiple of 128 columns)

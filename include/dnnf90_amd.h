@@ -22,8 +22,13 @@ extern "C" {
 
 /* arithmetic of a batched work space */
 #define DNNF90_PREC_FP64 0   /* everything FP64: reference parity mode */
-#define DNNF90_PREC_FP32 1   /* FP32 storage, FP32 MFMA, FP32 recursion */
-#define DNNF90_PREC_BF16X3 2 /* FP32 storage, BF16 hi+lo split MFMA */
+#define DNNF90_PREC_FP32 1   /* FP32 storage and recursion, FP32 MFMA; the
+                                forward linear map accumulates in FP64 */
+#define DNNF90_PREC_BF16X3 2 /* FP32 storage, BF16 hi+lo split MFMA (about
+                                16 significant bits per operand) */
+#define DNNF90_PREC_BF16X6 3 /* FP32 storage and recursion, linear maps on
+                                BF16 three-way split MFMA (six products);
+                                the forward map accumulates in FP64 */
 
 /* message of the last failure on the calling thread ("" if none) */
 const char *dnnf90_last_error(void);
@@ -74,10 +79,11 @@ int dnnf90_bwork_set_stream(int bid, void *stream);
 int dnnf90_bwork_sync(int bid);
 /* kernel timing: with profiling on, every kernel category is bracketed by
  * HIP events on the work space's stream; dnnf90_bwork_kernel_ms returns and
- * resets the accumulated milliseconds and launch counts of the six
- * categories: 0 linear map (W*T), 1 Faa di Bruno forward, 2 adjoint pair
- * sums, 3 weight gradient (or per-point rows), 4 transposed linear map
- * (W^T*Sbar), 5 input/seed/output kernels */
+ * resets the accumulated milliseconds and launch counts of the seven
+ * categories (ms[7], count[7]): 0 linear map (W*T), 1 Faa di Bruno
+ * forward, 2 adjoint pair sums, 3 weight gradient chunk partials (or
+ * per-point rows), 4 transposed linear map (W^T*Sbar), 5 input/seed/output
+ * kernels, 6 fixed-order reduction of the gradient partials */
 int dnnf90_bwork_profile(int bid, int on);
 int dnnf90_bwork_kernel_ms(int bid, double *ms, long *count);
 
