@@ -170,8 +170,9 @@ def main():
     if args.workload == "ngd2_k4":
         return bench_ngd(args, D, torch, wl, net, xd, res, N, dev)
 
-    def run_mode(prec, name, wbytes, mma_peak):
-        bw = D.BatchWork(net, N, prec)
+    def run_mode(prec, name, wbytes, mma_peak, budget=0):
+        bw = D.BatchWork(net, N, prec, budget=budget)
+        info = bw.info()
         g = D.Grad(net)
         s = torch.cuda.Stream(device=dev)
         bw.set_stream(s.cuda_stream)
@@ -217,7 +218,7 @@ def main():
                 ent["frac"] = ent["achieved"] / ent["peak"]
                 ent["algorithmic_per_step"] = wk[1]
             kern[c] = ent
-        return {"name": name, "ms_per_step": ms, "wall_ms_per_step": wall * 1e3 / args.steps,
+        return {"name": name, "ms_per_step": ms, "wall_ms_per_step": wall * 1e3 / args.steps, "info": info,
                 "points_per_s": N / (ms * 1e-3), "kernels": kern, "T": t_host, "grad": grad, "rsq": rsq}
 
     fp64 = run_mode(D.PREC_FP64, "fp64", 8, PEAK["fp64"])
@@ -231,6 +232,14 @@ def main():
         others[name] = r
     mixed = others[args.mixed]
     err_T, err_g = mixed["err_T"], mixed["err_g"]
+    # the same mixed step with per-layer recompute of h in the reverse pass
+    mprec = {m[0]: m[1] for m in modes}[args.mixed]
+    mpeak = {m[0]: m[2] for m in modes}[args.mixed]
+    rc_run = run_mode(mprec, args.mixed + "+recompute", 4, mpeak, budget=1)
+    recompute = {"mode": args.mixed, "ms_per_step": rc_run["ms_per_step"],
+                 "points_per_s": rc_run["points_per_s"],
+                 "identical_grad": bool(np.array_equal(rc_run["grad"], mixed["grad"])),
+                 "device_bytes": rc_run["info"]["bytes"], "device_bytes_all_kept": rc_run["info"]["bytes_stored"]}
 
     # FP64 GPU against the oracle on a sample of points
     oracle_check = None
@@ -281,6 +290,7 @@ def main():
         "speedup_vs_fp64": mixed["points_per_s"] / fp64["points_per_s"],
         "err_vs_fp64": {"forward_by_order": err_T, "grad_max_rel": err_g},
         "kernels": mixed["kernels"],
+        "recompute": recompute,
         "modes": {k: {"points_per_s": r["points_per_s"], "ms_per_step": r["ms_per_step"],
                       "speedup_vs_fp64": r["points_per_s"] / fp64["points_per_s"],
                       "max_err_by_order": {o: v["max"] for o, v in r["err_T"].items()},

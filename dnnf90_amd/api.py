@@ -202,12 +202,27 @@ class Residual:
 
 
 class BatchWork:
-    """Batched work space of up to maxpt points (dnnf90_bwork_create)."""
+    """Batched work space of up to maxpt points (dnnf90_bwork_create).
 
-    def __init__(self, net: Net, maxpt, prec=PREC_FP64):
+    budget: device bytes; when keeping every hidden layer's h would pass it,
+    the reverse pass recomputes each hidden layer's S and h instead
+    (dnnf90_bwork_create_budget).  0 keeps everything."""
+
+    def __init__(self, net: Net, maxpt, prec=PREC_FP64, budget=0):
         self.net, self.maxpt, self.prec = net, maxpt, prec
-        self.id = _chk(lib().dnnf90_bwork_create(net.id, maxpt, prec), "dnnf90_bwork_create", True)
+        if budget and budget > 0:
+            self.id = _chk(lib().dnnf90_bwork_create_budget(net.id, maxpt, prec, float(budget)),
+                           "dnnf90_bwork_create_budget", True)
+        else:
+            self.id = _chk(lib().dnnf90_bwork_create(net.id, maxpt, prec), "dnnf90_bwork_create", True)
         self.npt = 0
+
+    def info(self):
+        """{recompute, bytes, bytes_stored, maxpt, prec} (dnnf90_bwork_info)."""
+        v = np.zeros(5)
+        _chk(lib().dnnf90_bwork_info(self.id, v.ctypes.data), "dnnf90_bwork_info")
+        return {"recompute": bool(v[0]), "bytes": float(v[1]), "bytes_stored": float(v[2]),
+                "maxpt": int(v[3]), "prec": int(v[4])}
 
     def set_stream(self, stream_handle):
         _chk(lib().dnnf90_bwork_set_stream(self.id, stream_handle), "dnnf90_bwork_set_stream")

@@ -18,10 +18,19 @@ from .api import PREC_FP64, BatchWork, Grad, Net
 
 
 class _DerivForward(torch.autograd.Function):
+    # the work space launches on its own torch stream, which waits for the
+    # caller's stream before (x, the seed) and is waited for after (T): the
+    # ABI's NULL stream means "the work space's own stream", so torch's
+    # default stream cannot be handed over directly
     @staticmethod
     def forward(ctx, x, w, layer):
         layer._sync_weights(w)
-        T = layer.bwork.forward(x.contiguous())
+        cur = torch.cuda.current_stream(x.device)
+        x = x.contiguous()
+        T = torch.empty((x.shape[0], layer.net.dims[-1], layer.net.na), dtype=torch.float64, device=x.device)
+        layer.stream.wait_stream(cur)
+        layer.bwork.forward(x, T)
+        cur.wait_stream(layer.stream)
         ctx.layer = layer
         return T
 
@@ -29,7 +38,11 @@ class _DerivForward(torch.autograd.Function):
     def backward(ctx, gT):
         layer = ctx.layer
         layer.grad.zero()
-        layer.bwork.backward(gT.contiguous().to(torch.float64), layer.grad)
+        cur = torch.cuda.current_stream(gT.device)
+        seed = gT.contiguous().to(torch.float64)
+        layer.stream.wait_stream(cur)
+        layer.bwork.backward(seed, layer.grad)
+        cur.wait_stream(layer.stream)
         g = torch.from_numpy(layer.grad.get()).to(gT.device)
         return None, g, None
 
@@ -38,14 +51,18 @@ class DerivMLP(torch.nn.Module):
     """MLP whose forward returns all carried input derivatives.
 
     w is a flat float64 Parameter in the ABI layout: for l = 2..nlayer,
-    ndim(l) rows of ndim(l-1)+1 values, bias first.
+    ndim(l) rows of ndim(l-1)+1 values, bias first.  budget (device bytes)
+    switches the adjoint to per-layer recompute of h once keeping every
+    layer's state would pass it.
     """
 
-    def __init__(self, net: Net, maxpt: int, prec: int = PREC_FP64, device="cuda"):
+    def __init__(self, net: Net, maxpt: int, prec: int = PREC_FP64, device="cuda", budget=0):
         super().__init__()
         self.net = net
         self.w = torch.nn.Parameter(torch.from_numpy(net.weights).to(device))
-        self.bwork = BatchWork(net, maxpt, prec)
+        self.bwork = BatchWork(net, maxpt, prec, budget)
+        self.stream = torch.cuda.Stream(device=device)
+        self.bwork.set_stream(self.stream.cuda_stream)
         self.grad = Grad(net)
         self._wver = None
 
@@ -58,5 +75,4 @@ class DerivMLP(torch.nn.Module):
     def forward(self, x):
         if x.shape[0] > self.bwork.maxpt:
             raise ValueError(f"batch of {x.shape[0]} exceeds maxpt={self.bwork.maxpt}")
-        torch.cuda.synchronize(x.device)
         return _DerivForward.apply(x.to(torch.float64), self.w, self)

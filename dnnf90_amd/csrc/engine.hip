@@ -208,6 +208,8 @@ struct Net {
             if (rc) return rc;
         }
         if (dev_ver != host_ver) {
+            // kernels of any stream may still read the previous weights
+            HIPCHK(hipDeviceSynchronize());
             HIPCHK(hipMemcpy(d_w64, w.data(), sizeof(double) * nw, hipMemcpyHostToDevice));
             dev_ver = host_ver;
         }
@@ -257,7 +259,14 @@ struct BatchWork {
     size_t esz = 8;
     hipStream_t own = nullptr, stream = nullptr;
     void *t0 = nullptr, *tl = nullptr, *p0 = nullptr, *p1 = nullptr, *scratch = nullptr;
-    std::vector<void *> tpl, hpl;  // per hidden layer
+    std::vector<void *> tpl, hpl;  // per hidden layer (hpl unused when recomputing)
+    // recompute: h of a hidden layer is not kept from the forward pass; the
+    // reverse pass recomputes S and h of that layer (K1 + Faa di Bruno
+    // forward) from the stored T of the layer below, into the plane hs
+    bool recompute = false;
+    void *hs = nullptr;
+    double bytes = 0.0, bytes_stored = 0.0;   // device bytes held / needed with every h kept
+    void *hplane(int l) const { return recompute ? hs : hpl[l]; }
     double *partial = nullptr;
     long partial_cap = 0;
     double *d_x = nullptr, *d_seed = nullptr, *d_tout = nullptr, *d_rsq = nullptr;
@@ -304,7 +313,7 @@ struct BatchWork {
         for (auto ev : prof.ev) (void)hipEventDestroy(ev);
         prof.ev.clear();
         auto f = [](void *&p) { if (p) (void)hipFree(p); p = nullptr; };
-        f(t0); f(tl); f(p0); f(p1); f(scratch);
+        f(t0); f(tl); f(p0); f(p1); f(scratch); f(hs);
         for (auto &p : tpl) f(p);
         for (auto &p : hpl) f(p);
         tpl.clear(); hpl.clear();
@@ -353,7 +362,7 @@ static void grad_split(long ncol, int M, int Kd, int esz, int *nchunk, long *chu
     grad_chunks(ncol, M, Kd, esz, nchunk, chunk);
 }
 
-static int bwork_init(BatchWork &b, const Net &nt, int maxpt, int prec)
+static int bwork_init(BatchWork &b, const Net &nt, int maxpt, int prec, double budget)
 {
     b.prec = prec;
     b.maxpt = maxpt;
@@ -361,9 +370,26 @@ static int bwork_init(BatchWork &b, const Net &nt, int maxpt, int prec)
     b.np_cap = round_up(std::max(maxpt, 1), 64);
     const int na = nt.tab.na, L = nt.nlayer;
     const size_t col = (size_t)na * b.np_cap;
-    int nmax = 0;
+    int nmax = 0, hsum = 0;
     for (int l = 0; l < L; ++l) nmax = std::max(nmax, nt.dims[l]);
+    for (int l = 1; l < L - 1; ++l) hsum += nt.dims[l];
     int rc;
+    // device bytes: a plane of n rows is n * col * esz
+    auto plane = [&](int n) { return (double)n * col * b.esz; };
+    long pc = 0;
+    for (int l = 1; l < L; ++l) {
+        int nc;
+        long ch;
+        grad_split((long)col, nt.dims[l], nt.dims[l - 1] + 1, (int)b.esz, &nc, &ch);
+        pc = std::max(pc, (long)(nc + grad_reduce_splits(nc)) * nt.dims[l] * (nt.dims[l - 1] + 1));
+    }
+    const int nfwd = bell_fwd_threads(nt.sched.nentry, (int)b.esz);
+    const double fixed = plane(nt.dims[0]) + plane(nt.dims[L - 1]) + 2 * plane(nmax) + 8.0 * pc +
+                         (nfwd == 0 ? (double)nt.sched.nentry * nmax * b.np_cap * b.esz : 0.0) +
+                         8.0 * ((double)b.np_cap * (nt.dims[0] + 2.0 * nt.dims[L - 1] * na) + b.np_cap / 256 + 2);
+    b.bytes_stored = fixed + 2 * plane(hsum);
+    b.recompute = budget > 0.0 && b.bytes_stored > budget && L > 2;
+    b.bytes = b.recompute ? fixed + plane(hsum) + plane(nmax) : b.bytes_stored;
     HIPCHK(hipStreamCreateWithFlags(&b.own, hipStreamNonBlocking));
     b.stream = b.own;
     if ((rc = valloc(&b.t0, (size_t)nt.dims[0] * col * b.esz))) return rc;
@@ -374,19 +400,12 @@ static int bwork_init(BatchWork &b, const Net &nt, int maxpt, int prec)
     b.hpl.assign(L, nullptr);
     for (int l = 1; l < L - 1; ++l) {
         if ((rc = valloc(&b.tpl[l], (size_t)nt.dims[l] * col * b.esz))) return rc;
-        if ((rc = valloc(&b.hpl[l], (size_t)nt.dims[l] * col * b.esz))) return rc;
+        if (!b.recompute && (rc = valloc(&b.hpl[l], (size_t)nt.dims[l] * col * b.esz))) return rc;
     }
+    if (b.recompute && (rc = valloc(&b.hs, (size_t)nmax * col * b.esz))) return rc;
     // LDS-resident operand store unless the set is too large
-    const int nfwd = bell_fwd_threads(nt.sched.nentry, (int)b.esz);
     if (nfwd == 0) {
         if ((rc = valloc(&b.scratch, (size_t)nt.sched.nentry * nmax * b.np_cap * b.esz))) return rc;
-    }
-    long pc = 0;
-    for (int l = 1; l < L; ++l) {
-        int nc;
-        long ch;
-        grad_split((long)col, nt.dims[l], nt.dims[l - 1] + 1, (int)b.esz, &nc, &ch);
-        pc = std::max(pc, (long)(nc + grad_reduce_splits(nc)) * nt.dims[l] * (nt.dims[l - 1] + 1));
     }
     b.partial_cap = pc;
     if ((rc = dalloc(&b.partial, (size_t)pc))) return rc;
@@ -412,6 +431,31 @@ template <typename T> static const T *weights(const Net &nt);
 template <> const double *weights<double>(const Net &nt) { return nt.d_w64; }
 template <> const float *weights<float>(const Net &nt) { return nt.d_w32; }
 
+// K1 of layer l from the stored T of the layer below, then (hidden layers)
+// the Faa di Bruno step writing T and h of layer l; S lives in p0
+template <typename T>
+static void layer_forward(const Net &nt, BatchWork &b, int l, long ncol, long Np)
+{
+    const int L = nt.nlayer, nd = nt.dims[l], ndm = nt.dims[l - 1];
+    hipStream_t s = b.stream;
+    const T *W = weights<T>(nt);
+    const T *X = (const T *)(l == 1 ? b.t0 : b.tpl[l - 1]);
+    T *out = (T *)(l < L - 1 ? b.p0 : b.tl);
+    {
+        PScope p(b, CAT_LINEAR);
+        launch_gemm_wx<T>(b.prec, false, W + nt.woff[l], ndm + 1, X, out, nd, ndm + 1, ncol, Np, s);
+    }
+    if (l < L - 1) {
+        PScope p(b, CAT_BELL_FWD);
+        const GenKernels *gk = gen_for<T>(nt);
+        if (gk)
+            (void)gen_launch_fwd(*gk, b.p0, b.tpl[l], b.hplane(l), nd, Np, nt.dt.act(), s);
+        else
+            launch_bell_fwd<T>((const T *)b.p0, (T *)b.tpl[l], (T *)b.hplane(l), nd, Np, nt.dt.fwd(), nt.dt.act(),
+                               (T *)b.scratch, s);
+    }
+}
+
 template <typename T>
 static void forward_t(const Net &nt, BatchWork &b, const double *d_x, int npt)
 {
@@ -425,27 +469,7 @@ static void forward_t(const Net &nt, BatchWork &b, const double *d_x, int npt)
         PScope p(b, CAT_OTHER);
         launch_input<T>(d_x, npt, nt.dims[0], na, nt.dt.ind_e1, (T *)b.t0, Np, s);
     }
-    const T *W = weights<T>(nt);
-    const DevFwdSched sch = nt.dt.fwd();
-    const ActParams act = nt.dt.act();
-    for (int l = 1; l < L; ++l) {
-        const int nd = nt.dims[l], ndm = nt.dims[l - 1];
-        const T *X = (const T *)(l == 1 ? b.t0 : b.tpl[l - 1]);
-        T *out = (T *)(l < L - 1 ? b.p0 : b.tl);
-        {
-            PScope p(b, CAT_LINEAR);
-            launch_gemm_wx<T>(b.prec, false, W + nt.woff[l], ndm + 1, X, out, nd, ndm + 1, ncol, Np, s);
-        }
-        if (l < L - 1) {
-            PScope p(b, CAT_BELL_FWD);
-            const GenKernels *gk = gen_for<T>(nt);
-            if (gk)
-                (void)gen_launch_fwd(*gk, b.p0, b.tpl[l], b.hpl[l], nd, Np, act, s);
-            else
-                launch_bell_fwd<T>((const T *)b.p0, (T *)b.tpl[l], (T *)b.hpl[l], nd, Np, sch, act,
-                                   (T *)b.scratch, s);
-        }
-    }
+    for (int l = 1; l < L; ++l) layer_forward<T>(nt, b, l, ncol, Np);
 }
 
 // adjoint from the output seed already in p0 (rows < nout).  rows_t == null:
@@ -461,12 +485,15 @@ static void backward_t(const Net &nt, BatchWork &b, double *grad, float *rows_t,
     for (int l = L - 1; l >= 1; --l) {
         const int nd = nt.dims[l], ndm = nt.dims[l - 1];
         if (l < L - 1) {
+            // p0 is free here (its S-bar was consumed by layer l+1); the
+            // recomputed T of layer l equals the stored one bit for bit
+            if (b.recompute) layer_forward<T>(nt, b, l, ncol, Np);
             PScope p(b, CAT_BELL_BWD);
             const GenKernels *gk = gen_for<T>(nt);
             if (gk)
-                (void)gen_launch_bwd(*gk, b.p1, b.hpl[l], b.p0, nd, Np, s);
+                (void)gen_launch_bwd(*gk, b.p1, b.hplane(l), b.p0, nd, Np, s);
             else
-                launch_bell_bwd<T>((const T *)b.p1, (const T *)b.hpl[l], (T *)b.p0, nd, Np, pr, s);
+                launch_bell_bwd<T>((const T *)b.p1, (const T *)b.hplane(l), (T *)b.p0, nd, Np, pr, s);
         }
         const T *X = (const T *)(l == 1 ? b.t0 : b.tpl[l - 1]);
         if (rows_t) {
@@ -874,7 +901,7 @@ int dnnf90_net_free(int nid)
     return 0;
 }
 
-static int make_work(Registry<BatchWork> &reg, int nid, int maxpt, int prec)
+static int make_work(Registry<BatchWork> &reg, int nid, int maxpt, int prec, double budget = 0.0)
 {
     Net *nt = g_net.get(nid);
     if (!nt) return fail(-1, "bad network handle");
@@ -884,7 +911,7 @@ static int make_work(Registry<BatchWork> &reg, int nid, int maxpt, int prec)
     int rc = nt->ensure_device(prec != PREC_FP64);
     if (rc) return rc;
     BatchWork *b = new BatchWork();
-    rc = bwork_init(*b, *nt, maxpt, prec);
+    rc = bwork_init(*b, *nt, maxpt, prec, budget);
     if (rc) {
         b->release();
         delete b;
@@ -1135,6 +1162,23 @@ int dnnf90_grad_set(int gid, const double *g)
 }
 
 int dnnf90_bwork_create(int nid, int maxpt, int prec) { return make_work(g_bwork, nid, maxpt, prec); }
+
+int dnnf90_bwork_create_budget(int nid, int maxpt, int prec, double budget_bytes)
+{
+    return make_work(g_bwork, nid, maxpt, prec, budget_bytes);
+}
+
+int dnnf90_bwork_info(int bid, double *info)
+{
+    BatchWork *b = g_bwork.get(bid);
+    if (!b || !info) return fail(-1, "bad work-space handle or argument");
+    info[0] = b->recompute ? 1.0 : 0.0;
+    info[1] = b->bytes;
+    info[2] = b->bytes_stored;
+    info[3] = (double)b->maxpt;
+    info[4] = (double)b->prec;
+    return 0;
+}
 int dnnf90_bwork_free(int bid) { return free_work(g_bwork, bid); }
 
 int dnnf90_bwork_set_stream(int bid, void *stream)

@@ -71,6 +71,17 @@ int dnnf90_grad_set(int gid, const double *g);
 
 /* batched work space for up to maxpt points in arithmetic prec */
 int dnnf90_bwork_create(int nid, int maxpt, int prec);
+/* the same with a device-memory budget in bytes: when keeping every hidden
+ * layer's h (the second Faa di Bruno output, needed only by the adjoint)
+ * would pass the budget, the work space keeps the layers' T only and the
+ * reverse pass recomputes S and h of each hidden layer from the stored T of
+ * the layer below, into one scratch plane -- one more linear map and Faa di
+ * Bruno step per hidden layer, results identical bit for bit.
+ * budget_bytes <= 0: keep everything (dnnf90_bwork_create). */
+int dnnf90_bwork_create_budget(int nid, int maxpt, int prec, double budget_bytes);
+/* info[5]: recompute (0/1), device bytes held, bytes needed with every h
+ * kept, maxpt, prec */
+int dnnf90_bwork_info(int bid, double *info);
 int dnnf90_bwork_free(int bid);
 /* launch on the caller's HIP stream instead of the work space's own
  * (stream is a hipStream_t; NULL restores the work space's own stream) */

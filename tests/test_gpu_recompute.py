@@ -1,0 +1,105 @@
+"""Per-layer recompute of the derivative stack, and the adjoint behind the
+torch.autograd.Function (the differentiation workload: MLP [4,64x4,1], tanh,
+closure{t, x, x^7, x^3 y^2 z^2}, loss mean square of u_t + u u_x + d_x^7 u +
+d_x^3 d_y^2 d_z^2 u).
+
+Recompute keeps each hidden layer's T and rebuilds S and h in the reverse
+pass; the kernels are deterministic, so the results must be identical bit
+for bit to the keep-everything work space.  The autograd gradient is checked
+against the oracle (the C restatement of lib/train_module.f90:241-450 and
+app/pinn_module.f90:32-65) and against central differences with h = 1e-4 of
+the loss written in torch.
+"""
+import numpy as np
+import pytest
+
+from helpers import rel
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def D():
+    import dnnf90_amd
+    return dnnf90_amd
+
+
+def _setup(D, oracle_mod, name, npt):
+    from dnnf90_amd.workloads import WORKLOADS
+    wl = WORKLOADS[name]
+    ts = D.TableSet(wl.d0, wl.k, wl.seeds, wl.act)
+    net = D.Net(ts, wl.dims)
+    w, x = wl.inputs(npt)
+    net.weights = w
+    o = oracle_mod.Oracle(wl.d0, wl.k, wl.seeds, wl.act)
+    return wl, ts, net, w, x, o
+
+
+@pytest.mark.parametrize("prec", [0, 3], ids=["fp64", "bf16x6"])
+@pytest.mark.parametrize("name,npt", [("closure4_k7", 300), ("dense4_k3", 150)])
+def test_recompute_identical(D, oracle_mod, name, npt, prec):
+    wl, ts, net, w, x, o = _setup(D, oracle_mod, name, npt)
+    res = D.Residual(wl.terms(o.alpha()))
+    out = []
+    for budget in (0, 1):
+        bw = D.BatchWork(net, npt, prec, budget=budget)
+        info = bw.info()
+        g = D.Grad(net)
+        rsq = bw.residual_step(x, res, 2.0 / npt, g)
+        out.append((info, rsq, g.get()))
+        print(name, "prec", prec, "budget", budget, info)
+        bw.free()
+        g.free()
+    (i0, r0, g0), (i1, r1, g1) = out
+    assert not i0["recompute"] and i1["recompute"]
+    assert i1["bytes"] < i0["bytes"] == i0["bytes_stored"]
+    assert r0 == r1
+    assert np.array_equal(g0, g1)
+
+
+def test_autograd_recompute_closure4_k7(D, oracle_mod):
+    import torch
+    from dnnf90_amd.autograd import DerivMLP
+    npt = 64
+    wl, ts, net, w, x, o = _setup(D, oracle_mod, "closure4_k7", npt)
+    terms = wl.terms(o.alpha())
+    model = DerivMLP(net, npt, D.PREC_FP64, budget=1)
+    assert model.bwork.info()["recompute"]
+    xt = torch.from_numpy(x).cuda()
+    co = torch.tensor([t[0] for t in terms], dtype=torch.float64, device="cuda")
+    sl = [t[1] for t in terms]
+    nl = [bool(t[2]) for t in terms]
+
+    def loss_of(T):
+        u = T[:, 0, 0]
+        R = torch.zeros_like(u)
+        for k in range(len(terms)):
+            R = R + co[k] * (u if nl[k] else 1.0) * T[:, 0, sl[k]]
+        return (R * R).mean()
+
+    loss = loss_of(model(xt))
+    loss.backward()
+    g = model.w.grad.detach().cpu().numpy()
+    go, rsq, _ = o.pinn_step(wl.dims, w, x, terms, 2.0 / npt)
+    e_or = rel(g, go)
+    print("autograd + recompute vs oracle: rel", e_or, "loss", float(loss), rsq / npt)
+    assert e_or <= 1e-12
+    assert abs(float(loss) - rsq / npt) <= 1e-12 * abs(rsq / npt)
+
+    # central differences, h = 1e-4, on weights spread over every layer
+    h = 1e-4
+    idx = np.random.default_rng(11).choice(net.nw, 8, replace=False)
+    gmax = np.abs(g).max()
+    worst = 0.0
+    with torch.no_grad():
+        for i in idx:
+            w0 = float(model.w[i])
+            model.w[i] = w0 + h
+            lp = float(loss_of(model(xt)))
+            model.w[i] = w0 - h
+            lm = float(loss_of(model(xt)))
+            model.w[i] = w0
+            fd = (lp - lm) / (2 * h)
+            worst = max(worst, abs(fd - g[i]) / gmax)
+    print("central FD (h=1e-4) vs autograd: max |fd - g| / max|g| =", worst)
+    assert worst <= 1e-5
