@@ -587,16 +587,22 @@ __global__ void k_reduce_pass2(const double *__restrict__ part2, int nsplit, lon
 
 int grad_reduce_splits(int nchunk) { return nchunk >= 64 ? 16 : (nchunk >= 16 ? 4 : 1); }
 
+#ifndef GRAD_WGS
+#define GRAD_WGS 256
+#endif
+
 // K5 tile columns: 64, or 128 for FP32 blocks wider than 64
 static int grad_cb(int esz, int Kg) { return (esz == 4 && Kg > 64) ? 8 : 4; }
 
 void grad_chunks(long ncol, int M, int Kd, int esz, int *nchunk, long *chunk)
 {
-    // about 1024 workgroups over tiles x chunks; chunks are whole multiples
-    // of 128 columns (every wave step of k_grad_nt stays inside its chunk)
+    // one resident round of workgroups (one per CU: the gradient kernels
+    // hold a whole output tile per wave) over tiles x chunks, so every
+    // workgroup streams one long column range; chunks are whole multiples of
+    // 128 columns (every wave step stays inside its chunk)
     const int cb = grad_cb(esz, Kd - 1);
     const int tiles = ((M + 63) / 64) * ((Kd - 1 + 16 * cb - 1) / (16 * cb));
-    long nc = std::max(1, 1024 / tiles);
+    long nc = std::max(1, GRAD_WGS / tiles);
     nc = std::min(nc, std::max(1L, ncol / 512));
     const long ch = (((ncol + nc - 1) / nc) + 127) / 128 * 128;
     *nchunk = (int)((ncol + ch - 1) / ch);

@@ -22,6 +22,12 @@
 
 namespace dnnf90 {
 
+// register prefetch depth (chunks of 32 k) of the split linear maps and
+// workgroups per CU of the split weight gradient
+#ifndef LIN_X6_KP
+#define LIN_X6_KP 3
+#endif
+
 // ---------------------------------------------------------------------------
 // K1 / K6.  The workgroup's 16*RBT rows of A (K1: the weight columns after
 // the bias; K6: the transposed weights) are split once into LDS in operand
@@ -32,7 +38,7 @@ namespace dnnf90 {
 // them and multiplies by every row block; the next chunk's rows are loaded
 // before the current chunk's MFMAs.  K1's bias is added in the epilogue to
 // the first Np columns (slot 0; NCB*16 columns never straddle Np).
-template <bool kTrans, bool kAcc64, int RBT, int NCB>
+template <bool kTrans, bool kAcc64, int RBT, int NCB, int KP>
 __global__ __launch_bounds__(256) void k_lin_x6(const float *__restrict__ W, int ldw, const float *__restrict__ B,
                                                 float *__restrict__ C, int M, int Kd, long ncol, long Np, int nrt,
                                                 int ncw, long cupw)
@@ -69,12 +75,18 @@ __global__ __launch_bounds__(256) void k_lin_x6(const float *__restrict__ W, int
     float *Bias = reinterpret_cast<float *>(Ap + (long)RBT * nkc * 3 * 64);
     if (!kTrans && tid < 16 * RBT) Bias[tid] = m0 + tid < M ? W[(long)(m0 + tid) * ldw] : 0.f;
     __syncthreads();
-    long cu = cu0 + wave;
-    if (cu >= cu1) return;
-
+    if (cu0 + wave >= cu1) return;
+    // the wave's iterations: it -> (unit cu0 + wave + 4 (it / nkc), chunk
+    // it % nkc); the rows of iteration it + KP - 1 are loaded while
+    // iteration it is multiplied (a ring of KP register buffers, indexed
+    // statically by unrolling the loop KP times)
+    const long nu = (cu1 - cu0 - wave + 3) / 4;
+    const long nit = nu * nkc;
     const float *bl = B + (lane & 15);
-    float rc[NCB][8], rn[NCB][8];
-    auto load = [&](float (&dst)[NCB][8], long cul, int kc) {
+    float rb_[KP][NCB][8];
+    auto load = [&](float (&dst)[NCB][8], long it) {
+        const long cul = cu0 + wave + 4 * (it / nkc);
+        const int kc = (int)(it % nkc);
 #pragma unroll
         for (int j = 0; j < NCB; ++j) {
             const float *src = bl + cul * 16 * NCB + 16 * j;
@@ -84,17 +96,9 @@ __global__ __launch_bounds__(256) void k_lin_x6(const float *__restrict__ W, int
     };
     flt4 cacc[RBT][NCB];
     double acc64[kAcc64 ? RBT : 1][kAcc64 ? NCB : 1][4];
-    load(rc, cu, 0);
-    int kc = 0;
-    for (;;) {
-        long cun = cu;
-        int kcn = kc + 1;
-        if (kcn == nkc) {
-            kcn = 0;
-            cun = cu + 4;
-        }
-        const bool more = cun < cu1;
-        if (more) load(rn, cun, kcn);
+    auto step = [&](const float (&rc)[NCB][8], long it) {
+        const long cu = cu0 + wave + 4 * (it / nkc);
+        const int kc = (int)(it % nkc);
         if (kc == 0 || kAcc64) {
 #pragma unroll
             for (int rb = 0; rb < RBT; ++rb)
@@ -148,13 +152,16 @@ __global__ __launch_bounds__(256) void k_lin_x6(const float *__restrict__ W, int
                     }
                 }
         }
-        if (!more) break;
+    };
 #pragma unroll
-        for (int j = 0; j < NCB; ++j)
+    for (int j = 0; j < KP - 1; ++j)
+        if (j < nit) load(rb_[j], j);
+    for (long it = 0; it < nit; it += KP) {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) rc[j][e] = rn[j][e];
-        cu = cun;
-        kc = kcn;
+        for (int j = 0; j < KP; ++j) {
+            if (it + j + KP - 1 < nit) load(rb_[(j + KP - 1) % KP], it + j + KP - 1);
+            if (it + j < nit) step(rb_[j], it + j);
+        }
     }
 }
 
@@ -162,12 +169,12 @@ template <bool kTrans, bool kAcc64, int RBT>
 static void lin_x6_go(const float *W, int ldw, const float *B, float *C, int M, int Kd, long ncol, long Np,
                       hipStream_t s)
 {
-    constexpr int NCB = 2;
+    constexpr int NCB = 2, KP = LIN_X6_KP;
     const int nkc = (Kd + 31) / 32;
     const size_t lds = (size_t)RBT * nkc * 3 * 64 * 16 + 16 * RBT * sizeof(float);
     static size_t lds_set = 64 * 1024;   // dynamic LDS above 64 KiB must be requested
     if (lds > lds_set) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_lin_x6<kTrans, kAcc64, RBT, NCB>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_lin_x6<kTrans, kAcc64, RBT, NCB, KP>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         lds_set = lds;
     }
@@ -180,7 +187,7 @@ static void lin_x6_go(const float *W, int ldw, const float *B, float *C, int M, 
     const long ncw = (ncu + cupw - 1) / cupw;
     long g = (long)nrt * ncw;
     g = (g + 7) / 8 * 8;
-    hipLaunchKernelGGL((k_lin_x6<kTrans, kAcc64, RBT, NCB>), dim3((unsigned)g), dim3(256), lds, s, W, ldw, B, C, M,
+    hipLaunchKernelGGL((k_lin_x6<kTrans, kAcc64, RBT, NCB, KP>), dim3((unsigned)g), dim3(256), lds, s, W, ldw, B, C, M,
                        Kd, ncol, Np, nrt, (int)ncw, cupw);
 }
 
@@ -252,7 +259,10 @@ __global__ __launch_bounds__(256) void k_grad_x6(const float *__restrict__ A, co
 #pragma unroll
         for (int j = 0; j < CB; ++j) acc[i][j] = flt4{0, 0, 0, 0};
     }
-    float a0[RB][8], b0[CB][8], a1[RB][8], b1[CB][8];
+    // one register stage of raw rows: a step splits them into bf16 pieces,
+    // then reuses the registers for the next step's loads, which are in
+    // flight during the step's MFMAs
+    float ar[RB][8], br[CB][8];
     auto ld8 = [](float (&d)[8], const float *src) {
         const f4 x = *reinterpret_cast<const f4 *>(src), y = *reinterpret_cast<const f4 *>(src + 4);
 #pragma unroll
@@ -261,41 +271,31 @@ __global__ __launch_bounds__(256) void k_grad_x6(const float *__restrict__ A, co
             d[4 + e] = y[e];
         }
     };
-    auto load = [&](float (&ar)[RB][8], float (&br)[CB][8], long c) {
+    auto load = [&](long c) {
 #pragma unroll
         for (int i = 0; i < RB; ++i) ld8(ar[i], pa[i] + c);
 #pragma unroll
         for (int j = 0; j < CB; ++j) ld8(br[j], pb[j] + c);
     };
-    auto mul = [&](const float (&ar)[RB][8], const float (&br)[CB][8], long c) {
+    long c = cb + (long)wave * SC;
+    if (c < ce) load(c);
+    for (; c < ce; c += 4 * SC) {
         if (do_bias && c < Np) {   // a step never straddles Np (multiples of 32)
 #pragma unroll
             for (int i = 0; i < RB; ++i)
 #pragma unroll
                 for (int e = 0; e < 8; ++e) bacc[i] += ar[i][e];
         }
-        bf16x8 ah[RB], am[RB], al[RB];
+        bf16x8 ah[RB], am[RB], al[RB], bh[CB], bm[CB], bl[CB];
 #pragma unroll
         for (int i = 0; i < RB; ++i) split3(ar[i], ah[i], am[i], al[i]);
 #pragma unroll
-        for (int j = 0; j < CB; ++j) {
-            bf16x8 bh, bm, bl;
-            split3(br[j], bh, bm, bl);
+        for (int j = 0; j < CB; ++j) split3(br[j], bh[j], bm[j], bl[j]);
+        if (c + 4 * SC < ce) load(c + 4 * SC);
 #pragma unroll
-            for (int i = 0; i < RB; ++i) acc[i][j] = mma_x6(ah[i], am[i], al[i], bh, bm, bl, acc[i][j]);
-        }
-    };
-    long c = cb + (long)wave * SC;
-    if (c < ce) load(a0, b0, c);
-    while (c < ce) {
-        const long c1 = c + 4 * SC;
-        if (c1 < ce) load(a1, b1, c1);
-        mul(a0, b0, c);
-        if (c1 >= ce) break;
-        const long c2 = c1 + 4 * SC;
-        if (c2 < ce) load(a0, b0, c2);
-        mul(a1, b1, c1);
-        c = c2;
+        for (int j = 0; j < CB; ++j)
+#pragma unroll
+            for (int i = 0; i < RB; ++i) acc[i][j] = mma_x6(ah[i], am[i], al[i], bh[j], bm[j], bl[j], acc[i][j]);
     }
     for (int wv = 0; wv < 4; ++wv) {
         if (wave == wv) {
@@ -338,15 +338,23 @@ __global__ __launch_bounds__(256) void k_grad_x6(const float *__restrict__ A, co
     }
 }
 
-// tile of the split K5: 64 x 64 of the weight block (a larger tile spills)
+// tile of the split K5: (16 RB) x 64 of the weight block, RB = 8 where the
+// block has more than 64 rows (each split element then feeds more MFMAs)
+int grad_x6_rb(int M) { return M > 64 ? 8 : 4; }
+
 void launch_grad_x6(const float *A, const float *B, int M, int Kd, long ncol, long Np, double *partial,
                     int nchunk, long chunk, hipStream_t s)
 {
-    const int ntm = (M + 63) / 64, ntk = (Kd - 1 + 63) / 64;
+    const int rb = grad_x6_rb(M);
+    const int ntm = (M + 16 * rb - 1) / (16 * rb), ntk = (Kd - 1 + 63) / 64;
     long g = (long)ntm * ntk * nchunk;
     g = (g + 7) / 8 * 8;
-    hipLaunchKernelGGL((k_grad_x6<4, 4>), dim3((unsigned)g), dim3(256), 0, s, A, B, M, Kd, ncol, Np, chunk, ntm, ntk,
-                       nchunk, partial);
+    if (rb == 8)
+        hipLaunchKernelGGL((k_grad_x6<8, 4>), dim3((unsigned)g), dim3(256), 0, s, A, B, M, Kd, ncol, Np, chunk, ntm,
+                           ntk, nchunk, partial);
+    else
+        hipLaunchKernelGGL((k_grad_x6<4, 4>), dim3((unsigned)g), dim3(256), 0, s, A, B, M, Kd, ncol, Np, chunk, ntm,
+                           ntk, nchunk, partial);
 }
 
 }  // namespace dnnf90

@@ -40,18 +40,33 @@ template <> struct Vec2x16<double> { typedef double V __attribute__((ext_vector_
 
 // x = h + m + l (bf16 pieces, each the round-to-nearest bf16 of what the
 // previous ones leave; the residuals are exact in FP32), so a product of two
-// split operands keeps the six terms of relative size 2^-16 or larger
+// split operands keeps the six terms of relative size 2^-16 or larger.
+// Elements go in pairs: one packed conversion per pair and level, and a
+// bf16 pair widens back to FP32 by a shift and a mask.
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float flt2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ flt2 widen2(bf16x2 v)
+{
+    const unsigned b = __builtin_bit_cast(unsigned, v);
+    return flt2{__builtin_bit_cast(float, b << 16), __builtin_bit_cast(float, b & 0xffff0000u)};
+}
+
 __device__ __forceinline__ void split3(const float (&v)[8], bf16x8 &h, bf16x8 &m, bf16x8 &l)
 {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        const __bf16 a = (__bf16)v[e];
-        const float r = v[e] - (float)a;
-        const __bf16 b = (__bf16)r;
-        const float r2 = r - (float)b;
-        h[e] = a;
-        m[e] = b;
-        l[e] = (__bf16)r2;
+    for (int p = 0; p < 4; ++p) {
+        const flt2 x = {v[2 * p], v[2 * p + 1]};
+        const bf16x2 a = __builtin_convertvector(x, bf16x2);
+        const flt2 r = x - widen2(a);
+        const bf16x2 b = __builtin_convertvector(r, bf16x2);
+        const bf16x2 c = __builtin_convertvector(r - widen2(b), bf16x2);
+        h[2 * p] = a[0];
+        h[2 * p + 1] = a[1];
+        m[2 * p] = b[0];
+        m[2 * p + 1] = b[1];
+        l[2 * p] = c[0];
+        l[2 * p + 1] = c[1];
     }
 }
 
