@@ -274,8 +274,8 @@ struct BatchWork {
     double *d_coeff = nullptr;
     int *d_slot = nullptr, *d_nonlin = nullptr, nterm_cap = 0;
     // NGD metric and solve (allocated on first use)
-    double *ngd_G = nullptr, *ngd_part = nullptr, *ngd_u = nullptr, *ngd_trace = nullptr;
-    long ngd_G_cap = 0, ngd_part_cap = 0, ngd_u_cap = 0;
+    double *ngd_G = nullptr, *ngd_part = nullptr, *ngd_u = nullptr, *ngd_trace = nullptr, *ngd_work = nullptr;
+    long ngd_G_cap = 0, ngd_part_cap = 0, ngd_u_cap = 0, ngd_work_cap = 0;
     int *ngd_info = nullptr;
     std::vector<double> h_coeff;
     std::vector<int> h_slot, h_nonlin;
@@ -319,7 +319,7 @@ struct BatchWork {
         tpl.clear(); hpl.clear();
         dfree(partial); dfree(d_x); dfree(d_seed); dfree(d_tout); dfree(d_rsq); dfree(d_rsq_part);
         dfree(d_coeff); dfree(d_slot); dfree(d_nonlin);
-        dfree(ngd_G); dfree(ngd_part); dfree(ngd_u); dfree(ngd_trace); dfree(ngd_info);
+        dfree(ngd_G); dfree(ngd_part); dfree(ngd_u); dfree(ngd_trace); dfree(ngd_info); dfree(ngd_work);
         if (own) (void)hipStreamDestroy(own);
         own = stream = nullptr;
     }
@@ -1227,6 +1227,12 @@ int dnnf90_ngd_solve(int nid, int bid, int nrow, const float *rows_t, long ld, c
         bw.ngd_u_cap = nw;
     }
     if (!bw.ngd_trace && ((rc = dalloc(&bw.ngd_trace, 1)) || (rc = dalloc(&bw.ngd_info, 1)))) return rc;
+    const long wneed = ngd_work_doubles(nw);
+    if (bw.ngd_work_cap < wneed) {
+        dfree(bw.ngd_work);
+        if ((rc = dalloc(&bw.ngd_work, (size_t)wneed))) return rc;
+        bw.ngd_work_cap = wneed;
+    }
     hipStream_t s = bw.stream;
     if (nc > 1) launch_fill(bw.ngd_G, len, 0.0, s);
     {
@@ -1237,7 +1243,8 @@ int dnnf90_ngd_solve(int nid, int bid, int nrow, const float *rows_t, long ld, c
                           s));
     {
         PScope p(bw, CAT_OTHER);
-        ngd_factor_solve(bw.ngd_G, nw, nw, (double)nb, mu, mode, bw.ngd_u, bw.ngd_trace, bw.ngd_info, s);
+        ngd_factor_solve(bw.ngd_G, nw, nw, (double)nb, mu, mode, bw.ngd_u, bw.ngd_trace, bw.ngd_info, bw.ngd_work,
+                         s);
     }
     int info = 0;
     HIPCHK(hipMemcpyAsync(&info, bw.ngd_info, sizeof(int), hipMemcpyDeviceToHost, s));

@@ -488,13 +488,14 @@ static constexpr int GN = 144, GC = 16;
 template <typename T, int RB>
 __global__ __launch_bounds__(256) void k_gemm_grad(const T *__restrict__ A, const T *__restrict__ B, int M,
                                                    int Kd, long ncol, long Np, long chunk, int bias_row,
-                                                   double *__restrict__ partial)
+                                                   int lower, double *__restrict__ partial)
 {
     constexpr int BMg = 64 * RB;
     __shared__ T As[GC][BMg + 1];
     __shared__ T Bs[GC][GN + 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m0 = blockIdx.y * BMg, k0 = blockIdx.z * GN;
+    if (lower && k0 > m0 + BMg - 1) return;   // tile wholly above the diagonal
     const long cb = (long)blockIdx.x * chunk;
     const long ce = min(ncol, cb + chunk);
     typename Mfma<T>::V acc[RB][9];
@@ -643,17 +644,18 @@ void launch_grad_reduce(double *partial, int nchunk, long len, double *grad, hip
 }
 
 // the Gram matrix of Gauss-Newton rows: G[i][j] = sum_n R[i][n] R[j][n]
-// (rows stored transposed, R[w][n]); same tile as K5 without a bias row
+// (rows stored transposed, R[w][n]), 128 x 144 tiles; only tiles holding
+// lower-triangle entries are formed (the factorization reads no other)
 void launch_gram(const float *R, int nw, long ld, double *partial, int nchunk, long chunk_cols, double *G,
                  hipStream_t s)
 {
     dim3 grid((unsigned)nchunk, (unsigned)((nw + 127) / 128), (unsigned)((nw + GN - 1) / GN));
     if (nchunk == 1) {  // one chunk: the tiles write G directly
-        hipLaunchKernelGGL((k_gemm_grad<float, 2>), grid, dim3(256), 0, s, R, R, nw, nw, ld, 0L, chunk_cols, 0, G);
+        hipLaunchKernelGGL((k_gemm_grad<float, 2>), grid, dim3(256), 0, s, R, R, nw, nw, ld, 0L, chunk_cols, 0, 1, G);
         return;
     }
     hipLaunchKernelGGL((k_gemm_grad<float, 2>), grid, dim3(256), 0, s, R, R, nw, nw, ld, 0L, chunk_cols, 0,
-                       partial);
+                       1, partial);
     const long len = (long)nw * nw;
     const int ns = grad_reduce_splits(nchunk);
     double *part2 = partial + (long)nchunk * len;

@@ -109,9 +109,13 @@ int grad_reduce_splits(int nchunk);
 // place, lower) and b <- (L L^T)^{-1} b; *info > 0 flags a non-positive
 // pivot (1-based column); trace is one device double of scratch
 int ngd_factor_solve(double *G, int n, long lda, double nb, double mu, int mode, double *b, double *trace,
-                     int *info, hipStream_t s);
+                     int *info, double *work, hipStream_t s);
+// doubles of device work space ngd_factor_solve needs for order n (the
+// inverses of the diagonal blocks and one packed panel)
+long ngd_work_doubles(int n);
 
-// G[i][j] (+)= sum_n R[i][n] R[j][n] for the transposed FP32 rows R[nw][ld]
+// G[i][j] (+)= sum_n R[i][n] R[j][n] for the transposed FP32 rows R[nw][ld],
+// on every tile that holds lower-triangle entries
 // (FP32 MFMA, FP32 accumulation per chunk, FP64 reduction); with one chunk
 // G is overwritten, otherwise added to
 void launch_gram(const float *R, int nw, long ld, double *partial, int nchunk, long chunk_cols, double *G,

@@ -8,12 +8,21 @@
 // G + cI = L L^T, followed by the two triangular solves; for an SPD system
 // the two routes agree to rounding.
 //
-//   gram        G = R R^T, R the transposed FP32 rows (gemm.hip)
-//   k_damp      G <- G / nb + c I (lower triangle), b <- b / nb
-//   k_potrf     Cholesky of one NB x NB diagonal block in LDS
-//   k_trsm      L21 = A21 L11^-T for the panel under it
-//   k_syrk      A22 -= L21 L21^T on v_mfma_f64_16x16x4f64, lower tiles only
-//   k_trsv_*    blocked forward and backward substitution
+// Right-looking blocked factorization over panels of PB = 128 columns:
+//
+//   gram          G = R R^T, lower triangle, R the transposed FP32 rows (gemm.hip)
+//   k_damp        G <- G / nb + c I (lower triangle), b <- b / nb
+//   k_chol_diag   one workgroup: Cholesky of the PB x PB diagonal block in LDS,
+//                 then its inverse Linv (kept for the panel and the solves)
+//   k_chol_panel  L21 = A21 Linv^T for the rows under the block: an FP64
+//                 MFMA product instead of a row-sequential triangular solve;
+//                 also packs L21 row-contiguously for the update
+//   k_chol_syrk   A22 -= L21 L21^T on v_mfma_f64_16x16x4f64, lower 64 x 64
+//                 tiles, the panel streamed through LDS in 32-column chunks
+//   k_chol_fwd / k_chol_bwd
+//                 one launch per panel: y_rest -= L21 y_k (or the transposed
+//                 update), and the workgroup that owns the next diagonal
+//                 block solves it with that block's Linv
 #include <algorithm>
 
 #include "kernels.hpp"
@@ -21,8 +30,6 @@
 namespace dnnf90 {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
-
-static constexpr int NB = 64;
 
 // tr(G) of the raw Gram matrix (before scaling); one block, fixed order
 __global__ void k_trace(const double *__restrict__ G, int n, long lda, double *__restrict__ out)
@@ -55,111 +62,169 @@ __global__ void k_damp(double *__restrict__ G, int n, long lda, double nb, doubl
     G[(long)i * lda + idx] = v;
 }
 
-// Cholesky of the diagonal block at (k, k), in place (lower); info > 0 marks
-// the first non-positive pivot (global index + 1)
-__global__ __launch_bounds__(256) void k_potrf(double *__restrict__ A, long lda, int k, int nb,
-                                               int *__restrict__ info)
+static constexpr int PB = 128;   // panel width
+static constexpr int PBL = PB + 1;
+
+// sum_{m < len} x[m] y[m] over LDS rows, four partial sums
+__device__ __forceinline__ double ldot(const double *__restrict__ x, const double *__restrict__ y, int len)
 {
-    __shared__ double S[NB][NB + 1];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int m = 0;
+    for (; m + 4 <= len; m += 4) {
+        s0 += x[m] * y[m];
+        s1 += x[m + 1] * y[m + 1];
+        s2 += x[m + 2] * y[m + 2];
+        s3 += x[m + 3] * y[m + 3];
+    }
+    for (; m < len; ++m) s0 += x[m] * y[m];
+    return (s0 + s1) + (s2 + s3);
+}
+
+// Cholesky of the diagonal block at (k, k) (nb <= PB rows), in place in G
+// (lower), then its inverse and the inverse's transpose, written to
+// Linv[PB][PB] and LinvT[PB][PB] (zero off the triangle; rows past nb hold
+// the identity).  Left-looking: column j is one dot product per row of the
+// rows above it (two barriers a column).  The inverse is forward
+// substitution of L X = I, one column of X per thread.  info > 0 marks the
+// first non-positive pivot (index + 1).
+__global__ __launch_bounds__(256) void k_chol_diag(double *__restrict__ A, long lda, int k, int nb,
+                                                   double *__restrict__ Linv, double *__restrict__ LinvT,
+                                                   int *__restrict__ info)
+{
+    __shared__ double S[PB][PBL];    // L, row-major
+    __shared__ double dg[PB];        // 1 / L[j][j]
     const int tid = threadIdx.x;
-    for (int u = tid; u < nb * nb; u += 256) {
-        const int r = u / nb, c = u % nb;
-        S[r][c] = c <= r ? A[(long)(k + r) * lda + k + c] : 0.0;
+    for (int u = tid; u < PB * PB; u += 256) {
+        const int r = u / PB, c = u % PB;
+        S[r][c] = (r < nb && c <= r) ? A[(long)(k + r) * lda + k + c] : (r == c ? 1.0 : 0.0);
     }
     __syncthreads();
     for (int j = 0; j < nb; ++j) {
+        // thread i >= j: t_i = A[i][j] - sum_{m<j} L[i][m] L[j][m]
+        const int i = j + tid;
+        double t = 0.0;
+        if (i < nb) t = S[i][j] - ldot(&S[i][0], &S[j][0], j);
         if (tid == 0) {
-            const double d = S[j][j];
-            if (!(d > 0.0) && *info == 0) *info = k + j + 1;
-            S[j][j] = sqrt(d > 0.0 ? d : 1.0);
+            if (!(t > 0.0) && *info == 0) *info = k + j + 1;
+            const double l = sqrt(t > 0.0 ? t : 1.0);
+            S[j][j] = l;
+            dg[j] = 1.0 / l;
         }
         __syncthreads();
-        const double djj = S[j][j];
-        for (int i = j + 1 + tid; i < nb; i += 256) S[i][j] /= djj;
-        __syncthreads();
-        // trailing update of the block: S[i][c] -= S[i][j] S[c][j], j < c <= i
-        const int m = nb - j - 1;
-        for (int u = tid; u < m * m; u += 256) {
-            const int i = j + 1 + u / m, c = j + 1 + u % m;
-            if (c <= i) S[i][c] -= S[i][j] * S[c][j];
-        }
+        if (tid > 0 && i < nb) S[i][j] = t * dg[j];
         __syncthreads();
     }
+    for (int j = nb; j < PB; ++j)
+        if (tid == 0) dg[j] = 1.0;
     for (int u = tid; u < nb * nb; u += 256) {
         const int r = u / nb, c = u % nb;
         if (c <= r) A[(long)(k + r) * lda + k + c] = S[r][c];
     }
-}
-
-// rows i >= k+nb of the panel: x L11^T = a, one lane per row, L11 in LDS
-__global__ __launch_bounds__(64) void k_trsm(double *__restrict__ A, long lda, int k, int nb, int n)
-{
-    // L is read by broadcast; X is [column][lane] so lane accesses are
-    // conflict free; 2 x 32 KiB of LDS
-    __shared__ double L[NB][NB];
-    __shared__ double X[NB][64];
-    const int tid = threadIdx.x;
-    for (int u = tid; u < nb * nb; u += 64) {
-        const int r = u / nb, c = u % nb;
-        L[r][c] = c <= r ? A[(long)(k + r) * lda + k + c] : 0.0;
-    }
-    const int row0 = k + nb + blockIdx.x * 64;
-    for (int u = tid; u < 64 * nb; u += 64) {
-        const int r = u / nb, c = u % nb;
-        X[c][r] = row0 + r < n ? A[(long)(row0 + r) * lda + k + c] : 0.0;
+    __syncthreads();
+    // X = L^-1 by columns: thread c < PB solves L x = e_c.  X[i][c] (i > c)
+    // is kept at S[c][i], the free upper triangle; X[c][c] = dg[c].  Column
+    // c of X is row c of S, which only thread c writes:
+    //   X[i][c] = -(L[i][c] dg[c] + sum_{c < m < i} L[i][m] X[m][c]) dg[i]
+    if (tid < PB) {
+        const int c = tid;
+        for (int i = c + 1; i < PB; ++i) {
+            const double sum = S[i][c] * dg[c] + ldot(&S[i][c + 1], &S[c][c + 1], i - c - 1);
+            S[c][i] = -sum * dg[i];
+        }
     }
     __syncthreads();
-    for (int j = 0; j < nb; ++j) {
-        double s = X[j][tid];
-        for (int m = 0; m < j; ++m) s -= X[m][tid] * L[j][m];
-        X[j][tid] = s / L[j][j];
-    }
-    __syncthreads();
-    for (int u = tid; u < 64 * nb; u += 64) {
-        const int r = u / nb, c = u % nb;
-        if (row0 + r < n) A[(long)(row0 + r) * lda + k + c] = X[c][r];
+    for (int u = tid; u < PB * PB; u += 256) {
+        const int r = u / PB, c = u % PB;
+        const double x = r == c ? dg[r] : (r > c ? S[c][r] : 0.0);   // X[r][c]
+        Linv[u] = x;
+        LinvT[c * PB + r] = x;
     }
 }
 
-// trailing update A22 -= L21 L21^T over 64 x 64 tiles with ti >= tj; the
-// panel width is NB (the last panel may be narrower: nb)
-__global__ __launch_bounds__(256) void k_syrk(double *__restrict__ A, long lda, int k, int nb, int n)
+// L21 = A21 Linv^T for 64 rows per workgroup (rows k+nb.. of the panel's
+// columns k..k+nb-1); the rows are staged in LDS, Linv is read through L2;
+// the result is written back into G and packed into P[row][PB]
+__global__ __launch_bounds__(256) void k_chol_panel(double *__restrict__ A, long lda, int k, int nb, int n,
+                                                    const double *__restrict__ Linv, double *__restrict__ P)
 {
-    __shared__ double Pi[NB][64];  // [kk][row of tile i]; 2 x 32 KiB
-    __shared__ double Pj[NB][64];
-    // tile index t -> (ti, tj) with ti >= tj in the trailing matrix
+    __shared__ double Ar[64][PBL];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int r0 = k + nb + blockIdx.x * 64;
+    for (int u = tid; u < 64 * PB; u += 256) {
+        const int r = u / PB, c = u % PB;
+        Ar[r][c] = (r0 + r < n && c < nb) ? A[(long)(r0 + r) * lda + k + c] : 0.0;
+    }
+    __syncthreads();
+    // wave: rows 16 wave .. +15, all PB columns (8 blocks of 16)
+    dbl4 acc[PB / 16];
+#pragma unroll
+    for (int jb = 0; jb < PB / 16; ++jb) acc[jb] = dbl4{0, 0, 0, 0};
+    const int q = lane >> 4, p = lane & 15;
+    for (int kk = 0; kk < nb; kk += 4) {
+        const double a = Ar[16 * wave + p][kk + q];
+#pragma unroll
+        for (int jb = 0; jb < PB / 16; ++jb) {
+            const double bv = Linv[(long)(16 * jb + p) * PB + kk + q];   // Linv^T[kk+q][col]
+            acc[jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[jb], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int jb = 0; jb < PB / 16; ++jb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int gr = r0 + 16 * wave + q + 4 * r;   // f64 MFMA result rows
+            const int c = 16 * jb + p;
+            if (gr < n && c < nb) {
+                A[(long)gr * lda + k + c] = acc[jb][r];
+                P[(long)gr * PB + c] = acc[jb][r];
+            }
+        }
+}
+
+// A22 -= L21 L21^T over lower 64 x 64 tiles of the trailing matrix (rows
+// and columns >= base); L21 packed in P[row][PB], nb columns used
+__global__ __launch_bounds__(256) void k_chol_syrk(double *__restrict__ A, long lda, const double *__restrict__ P,
+                                                   int base, int nb, int n)
+{
+    __shared__ double Pi[32][64];
+    __shared__ double Pj[32][64];
     const long t = blockIdx.x;
     int ti = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) / 2.0);
     while ((long)(ti + 1) * (ti + 2) / 2 <= t) ++ti;
     while ((long)ti * (ti + 1) / 2 > t) --ti;
     const int tj = (int)(t - (long)ti * (ti + 1) / 2);
-    const int base = k + nb;
     const int r0 = base + ti * 64, c0 = base + tj * 64;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int u = tid; u < 64 * nb; u += 256) {
-        const int r = u / nb, kk = u % nb;
-        Pi[kk][r] = r0 + r < n ? A[(long)(r0 + r) * lda + k + kk] : 0.0;
-        Pj[kk][r] = c0 + r < n ? A[(long)(c0 + r) * lda + k + kk] : 0.0;
-    }
-    __syncthreads();
     const int wm = wave >> 1, wn = wave & 1;
     dbl4 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int jj = 0; jj < 2; ++jj) acc[i][jj] = dbl4{0, 0, 0, 0};
-    for (int kk = 0; kk < nb; kk += 4) {
-        const int kr = kk + (lane >> 4);
-        double a[2], b[2];
+    for (int k0 = 0; k0 < nb; k0 += 32) {
+        for (int u = tid; u < 64 * 32; u += 256) {
+            const int r = u >> 5, kk = u & 31;
+            const bool ok = k0 + kk < nb;
+            Pi[kk][r] = (ok && r0 + r < n) ? P[(long)(r0 + r) * PB + k0 + kk] : 0.0;
+            Pj[kk][r] = (ok && c0 + r < n) ? P[(long)(c0 + r) * PB + k0 + kk] : 0.0;
+        }
+        __syncthreads();
 #pragma unroll
-        for (int i = 0; i < 2; ++i) a[i] = kr < nb ? Pi[kr][wm * 32 + i * 16 + (lane & 15)] : 0.0;
+        for (int kk = 0; kk < 32; kk += 4) {
+            const int kr = kk + (lane >> 4);
+            double a[2], b[2];
 #pragma unroll
-        for (int jj = 0; jj < 2; ++jj) b[jj] = kr < nb ? Pj[kr][wn * 32 + jj * 16 + (lane & 15)] : 0.0;
+            for (int i = 0; i < 2; ++i) a[i] = Pi[kr][wm * 32 + i * 16 + (lane & 15)];
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+            for (int jj = 0; jj < 2; ++jj) b[jj] = Pj[kr][wn * 32 + jj * 16 + (lane & 15)];
 #pragma unroll
-            for (int jj = 0; jj < 2; ++jj)
-                acc[i][jj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[jj], acc[i][jj], 0, 0, 0);
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int jj = 0; jj < 2; ++jj)
+                    acc[i][jj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[jj], acc[i][jj], 0, 0, 0);
+        }
+        __syncthreads();
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -167,100 +232,144 @@ __global__ __launch_bounds__(256) void k_syrk(double *__restrict__ A, long lda, 
         for (int jj = 0; jj < 2; ++jj)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int gr = r0 + wm * 32 + i * 16 + (lane >> 4) + 4 * r;  // f64 MFMA result rows
+                const int gr = r0 + wm * 32 + i * 16 + (lane >> 4) + 4 * r;
                 const int gc = c0 + wn * 32 + jj * 16 + (lane & 15);
                 if (gr < n && gc <= gr) A[(long)gr * lda + gc] -= acc[i][jj][r];
             }
 }
 
-// forward substitution on one diagonal block: y_k = L_kk^-1 y_k (one wave)
-__global__ void k_trsv_lower_diag(const double *__restrict__ A, long lda, int k, int nb, double *__restrict__ y)
+// y_blk <- M y_blk for the block at k (nb rows), M = Linv or Linv^T given
+// row-major (Mr = Linv, or LinvT for the transposed solve): one wave per
+// row, lanes along the row, 256 threads
+__device__ void blk_solve(const double *__restrict__ Mr, int k, int nb, double *__restrict__ y,
+                          double *__restrict__ v)
 {
-    __shared__ double v[NB];
-    const int t = threadIdx.x;
-    if (t < nb) v[t] = y[k + t];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid < PB) v[tid] = tid < nb ? y[k + tid] : 0.0;
     __syncthreads();
-    for (int j = 0; j < nb; ++j) {
-        if (t == 0) v[j] /= A[(long)(k + j) * lda + k + j];
-        __syncthreads();
-        if (t > j && t < nb) v[t] -= A[(long)(k + t) * lda + k + j] * v[j];
-        __syncthreads();
+    const double v0 = v[lane], v1 = v[lane + 64];
+    for (int r = wave; r < nb; r += 4) {
+        double s = Mr[r * PB + lane] * v0 + Mr[r * PB + lane + 64] * v1;
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+        if (lane == 0) y[k + r] = s;
     }
-    if (t < nb) y[k + t] = v[t];
 }
 
-// y_i -= sum_{m < nb} L[i][k+m] y[k+m] for i >= k+nb: one wave per row
-__global__ __launch_bounds__(256) void k_trsv_lower_update(const double *__restrict__ A, long lda, int k, int nb,
-                                                            int n, double *__restrict__ y)
+__global__ __launch_bounds__(256) void k_chol_solve_first(const double *__restrict__ Mr, int k, int nb,
+                                                          double *__restrict__ y)
 {
-    const int lane = threadIdx.x & 63;
-    const int i = k + nb + blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= n) return;
-    double s = lane < nb ? A[(long)i * lda + k + lane] * y[k + lane] : 0.0;
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    if (lane == 0) y[i] -= s;
+    __shared__ double v[PB];
+    blk_solve(Mr, k, nb, y, v);
 }
 
-// backward substitution on one diagonal block: u_k = L_kk^-T u_k
-__global__ void k_trsv_upper_diag(const double *__restrict__ A, long lda, int k, int nb, double *__restrict__ y)
+// forward step of panel k (y_k already solved): rows i >= k+nb get
+// y_i -= sum_m L[i][k+m] y[k+m], one wave per row; workgroup 0 owns the next
+// diagonal block [k+nb, k+nb+nb2) and then solves it with its Linv
+__global__ __launch_bounds__(256) void k_chol_fwd(const double *__restrict__ A, long lda, int k, int nb, int n,
+                                                  const double *__restrict__ Linv_next, int nb2,
+                                                  double *__restrict__ y)
 {
-    __shared__ double v[NB];
-    const int t = threadIdx.x;
-    if (t < nb) v[t] = y[k + t];
+    __shared__ double v[PB];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r0 = k + nb + blockIdx.x * PB;
+    const double y0 = lane < nb ? y[k + lane] : 0.0, y1 = lane + 64 < nb ? y[k + 64 + lane] : 0.0;
+    for (int rr = wave; rr < PB; rr += 4) {
+        const int i = r0 + rr;
+        if (i >= n) break;
+        const double *Li = A + (long)i * lda + k;
+        double s = (lane < nb ? Li[lane] * y0 : 0.0) + (lane + 64 < nb ? Li[lane + 64] * y1 : 0.0);
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+        if (lane == 0) y[i] -= s;
+    }
+    if (blockIdx.x == 0 && nb2 > 0) {
+        __threadfence_block();
+        __syncthreads();
+        blk_solve(Linv_next, k + nb, nb2, y, v);
+    }
+}
+
+// backward step of panel k (u_k already solved): rows i < k get
+// y_i -= sum_m L[k+m][i] u[k+m] (lanes along i: row reads coalesce);
+// workgroup 0 owns the previous block [k-PB, k) and solves it with Linv^T
+// (LinvT_prev, row-major); the m range is split over the two halves of the
+// workgroup
+__global__ __launch_bounds__(256) void k_chol_bwd(const double *__restrict__ A, long lda, int k, int nb,
+                                                  const double *__restrict__ LinvT_prev, double *__restrict__ y)
+{
+    __shared__ double v[PB];
+    __shared__ double uk[PB];
+    const int tid = threadIdx.x;
+    if (tid < PB) uk[tid] = tid < nb ? y[k + tid] : 0.0;
     __syncthreads();
-    for (int j = nb - 1; j >= 0; --j) {
-        if (t == 0) v[j] /= A[(long)(k + j) * lda + k + j];
-        __syncthreads();
-        if (t < j) v[t] -= A[(long)(k + j) * lda + k + t] * v[j];
-        __syncthreads();
+    const int hi = k - blockIdx.x * PB;   // rows [hi - PB, hi)
+    const int t = tid & (PB - 1), half = tid >> 7;
+    const int i = hi - PB + t;
+    const int m0 = half * (nb / 2), m1 = half ? nb : nb / 2;
+    double s0 = 0.0, s1 = 0.0;
+    if (i >= 0) {
+        int m = m0;
+        for (; m + 2 <= m1; m += 2) {
+            s0 += A[(long)(k + m) * lda + i] * uk[m];
+            s1 += A[(long)(k + m + 1) * lda + i] * uk[m + 1];
+        }
+        for (; m < m1; ++m) s0 += A[(long)(k + m) * lda + i] * uk[m];
     }
-    if (t < nb) y[k + t] = v[t];
-}
-
-// y_i -= sum_m L[k+m][i] y[k+m] for i < k: lanes along i (row reads coalesce)
-__global__ void k_trsv_upper_update(const double *__restrict__ A, long lda, int k, int nb, double *__restrict__ y)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= k) return;
-    double s = 0.0;
-    for (int m = 0; m < nb; ++m) s += A[(long)(k + m) * lda + i] * y[k + m];
-    y[i] -= s;
+    if (half) v[t] = s0 + s1;
+    __syncthreads();
+    if (!half && i >= 0) y[i] -= (s0 + s1) + v[t];
+    if (blockIdx.x == 0) {
+        __threadfence_block();
+        __syncthreads();
+        blk_solve(LinvT_prev, k - PB, PB, y, v);
+    }
 }
 
 __global__ void k_clear_info(int *info) { *info = 0; }
 
-int ngd_factor_solve(double *G, int n, long lda, double nb, double mu, int mode, double *b, double *trace,
-                     int *info, hipStream_t s)
+long ngd_work_doubles(int n)
 {
+    const long np = (n + PB - 1) / PB;
+    return 2 * np * PB * PB + (long)n * PB;
+}
+
+int ngd_factor_solve(double *G, int n, long lda, double nb, double mu, int mode, double *b, double *trace,
+                     int *info, double *work, hipStream_t s)
+{
+    const int np = (n + PB - 1) / PB;
+    double *Linv = work, *LinvT = work + (long)np * PB * PB, *P = work + 2 * (long)np * PB * PB;
     hipLaunchKernelGGL(k_clear_info, dim3(1), dim3(1), 0, s, info);
     hipLaunchKernelGGL(k_trace, dim3(1), dim3(256), 0, s, G, n, lda, trace);
     hipLaunchKernelGGL(k_damp, dim3((unsigned)((n + 255) / 256), (unsigned)n), dim3(256), 0, s, G, n, lda, nb, mu,
                        mode, trace, b);
-    for (int k = 0; k < n; k += NB) {
-        const int nb_ = std::min(NB, n - k);
-        hipLaunchKernelGGL(k_potrf, dim3(1), dim3(256), 0, s, G, lda, k, nb_, info);
+    for (int p = 0; p < np; ++p) {
+        const int k = p * PB, nb_ = std::min(PB, n - k);
+        double *Li = Linv + (long)p * PB * PB;
+        hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(256), 0, s, G, lda, k, nb_, Li, LinvT + (long)p * PB * PB,
+                           info);
         const int rest = n - k - nb_;
         if (rest > 0) {
-            hipLaunchKernelGGL(k_trsm, dim3((unsigned)((rest + 63) / 64)), dim3(64), 0, s, G, lda, k, nb_, n);
+            hipLaunchKernelGGL(k_chol_panel, dim3((unsigned)((rest + 63) / 64)), dim3(256), 0, s, G, lda, k, nb_, n,
+                               (const double *)Li, P);
             const long T = (rest + 63) / 64;
-            hipLaunchKernelGGL(k_syrk, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, s, G, lda, k, nb_, n);
+            hipLaunchKernelGGL(k_chol_syrk, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, s, G, lda,
+                               (const double *)P, k + nb_, nb_, n);
         }
     }
-    for (int k = 0; k < n; k += NB) {
-        const int nb_ = std::min(NB, n - k);
-        hipLaunchKernelGGL(k_trsv_lower_diag, dim3(1), dim3(64), 0, s, G, lda, k, nb_, b);
-        const int rest = n - k - nb_;
-        if (rest > 0)
-            hipLaunchKernelGGL(k_trsv_lower_update, dim3((unsigned)((rest + 3) / 4)), dim3(256), 0, s, G, lda, k,
-                               nb_, n, b);
+    // L y = b, then L^T u = y (in place in b)
+    hipLaunchKernelGGL(k_chol_solve_first, dim3(1), dim3(256), 0, s, (const double *)Linv, 0, std::min(PB, n), b);
+    for (int p = 0; p + 1 < np; ++p) {
+        const int k = p * PB, nb_ = PB, rest = n - k - nb_;
+        const int nb2 = std::min(PB, rest);
+        hipLaunchKernelGGL(k_chol_fwd, dim3((unsigned)((rest + PB - 1) / PB)), dim3(256), 0, s, (const double *)G, lda,
+                           k, nb_, n, (const double *)(Linv + (long)(p + 1) * PB * PB), nb2, b);
     }
-    const int last = ((n - 1) / NB) * NB;
-    for (int k = last; k >= 0; k -= NB) {
-        const int nb_ = std::min(NB, n - k);
-        hipLaunchKernelGGL(k_trsv_upper_diag, dim3(1), dim3(64), 0, s, G, lda, k, nb_, b);
-        if (k > 0)
-            hipLaunchKernelGGL(k_trsv_upper_update, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, s, G, lda, k,
-                               nb_, b);
+    const int kl = (np - 1) * PB;
+    hipLaunchKernelGGL(k_chol_solve_first, dim3(1), dim3(256), 0, s, (const double *)(LinvT + (long)(np - 1) * PB * PB),
+                       kl, n - kl, b);
+    for (int p = np - 1; p >= 1; --p) {
+        const int k = p * PB, nb_ = std::min(PB, n - k);
+        hipLaunchKernelGGL(k_chol_bwd, dim3((unsigned)((k + PB - 1) / PB)), dim3(256), 0, s, (const double *)G, lda, k,
+                           nb_, (const double *)(LinvT + (long)(p - 1) * PB * PB), b);
     }
     return 0;
 }
