@@ -53,7 +53,8 @@ std::string gen_bell_source(const TableSet &t, const FwdSchedule &s, bool fp64)
       << "typedef " << (fp64 ? "double" : "float") << " real;\n";
 
     // ---- forward: T_a and h_a of every slot from the S_a of one lane
-    o << "extern \"C\" __global__ __launch_bounds__(256) void dnnf90_bell_fwd(const real *__restrict__ S, "
+    const char *sfx = fp64 ? "_f64" : "_f32";
+    o << "extern \"C\" __global__ __launch_bounds__(256) void dnnf90_bell_fwd" << sfx << "(const real *__restrict__ S, "
          "real *__restrict__ Tout, real *__restrict__ H, int nrow, long Np, ActParams ap)\n{\n"
       << "  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;\n"
       << "  const int j = blockIdx.y * (blockDim.x >> 6) + wave;\n"
@@ -87,7 +88,7 @@ std::string gen_bell_source(const TableSet &t, const FwdSchedule &s, bool fp64)
     o << "}\n";
 
     // ---- adjoint: Sbar from Tbar and h of one lane
-    o << "extern \"C\" __global__ __launch_bounds__(256) void dnnf90_bell_bwd(const real *__restrict__ Tb, "
+    o << "extern \"C\" __global__ __launch_bounds__(256) void dnnf90_bell_bwd" << sfx << "(const real *__restrict__ Tb, "
          "const real *__restrict__ H, real *__restrict__ Sb, int nrow, long Np)\n{\n"
       << "  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;\n"
       << "  const int j = blockIdx.y * (blockDim.x >> 6) + wave;\n"
@@ -144,8 +145,8 @@ int gen_bell_compile(const TableSet &t, const FwdSchedule &s, bool fp64, GenKern
     hiprtcGetCode(prog, g->code.data());
     hiprtcDestroyProgram(&prog);
     if (hipModuleLoadData(&g->mod, g->code.data()) != hipSuccess) return -3;
-    if (hipModuleGetFunction(&g->fwd, g->mod, "dnnf90_bell_fwd") != hipSuccess ||
-        hipModuleGetFunction(&g->bwd, g->mod, "dnnf90_bell_bwd") != hipSuccess)
+    if (hipModuleGetFunction(&g->fwd, g->mod, fp64 ? "dnnf90_bell_fwd_f64" : "dnnf90_bell_fwd_f32") != hipSuccess ||
+        hipModuleGetFunction(&g->bwd, g->mod, fp64 ? "dnnf90_bell_bwd_f64" : "dnnf90_bell_bwd_f32") != hipSuccess)
         return -3;
     (void)hipFuncGetAttribute(&g->fwd_scratch, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, g->fwd);
     (void)hipFuncGetAttribute(&g->bwd_scratch, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, g->bwd);
