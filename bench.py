@@ -89,7 +89,7 @@ def bench_ngd(args, D, torch, wl, net, xd, res, N, dev, mu=1e-3):
     ld = ((N + 63) // 64) * 64
     rows = torch.zeros((net.nw, ld), dtype=torch.float32, device=dev)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-    acc = np.zeros(3)
+    acc = np.zeros(4)
     with torch.cuda.stream(s):
         for it in range(args.warmup + args.steps):
             ev[0].record(s)
@@ -104,10 +104,13 @@ def bench_ngd(args, D, torch, wl, net, xd, res, N, dev, mu=1e-3):
             u = bw.ngd_solve(rows, ld, N, b, N, mu, 1)
             t_solve = time.perf_counter() - t0
             torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ud = bw.ngd_solve(rows, ld, N, b, N, mu, 1, dual=True)
+            t_dual = time.perf_counter() - t0
             if it >= args.warmup:
-                acc += [ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]), t_solve * 1e3]
+                acc += [ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]), t_solve * 1e3, t_dual * 1e3]
     acc /= args.steps
-    ms = float(acc.sum())
+    ms = float(acc[:3].sum())
     nw = net.nw
     gram_flop = 2.0 * nw * nw * N
     chol_flop = nw ** 3 / 3.0
@@ -118,6 +121,10 @@ def bench_ngd(args, D, torch, wl, net, xd, res, N, dev, mu=1e-3):
             "config": {"workload": f"{wl.name}: MLP {wl.dims} tanh, D0={wl.d0} K={wl.k}, N={N}, nw={nw}, "
                                    f"rows of {wl.note}, ABS damping {mu}"},
             "breakdown_ms": {"rows": acc[0], "gradient": acc[1], "metric_and_solve": acc[2]},
+            "dual_route": {"metric_and_solve_ms": acc[3], "ms_per_step": float(acc[0] + acc[1] + acc[3]),
+                           "rows_per_s": N / (float(acc[0] + acc[1] + acc[3]) * 1e-3),
+                           "rel_diff_to_primal": float((torch.as_tensor(ud) - torch.as_tensor(u)).norm() /
+                                                       torch.as_tensor(u).norm())},
             "metric_solve_flops": {"gram": gram_flop, "cholesky": chol_flop},
             "u_norm": float(torch.as_tensor(u).norm())}
     print(json.dumps(line), flush=True)

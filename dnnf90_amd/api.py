@@ -15,6 +15,7 @@ from ._lib import lib
 
 PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_BF16X6 = 0, 1, 2, 3
 DEVICE_PTRS = 1
+NGD_DUAL = 2
 ACT_TANH, ACT_SIN, ACT_ERF, ACT_J0, ACT_J1, ACT_GELU = 0, 1, 2, 3, 4, 5
 
 
@@ -290,8 +291,9 @@ class BatchWork:
                                               pr, int(ld), DEVICE_PTRS if dev else 0),
              "dnnf90_residual_rows_batch")
 
-    def ngd_solve(self, rows_t, ld, nrow, b, nb, mu, mode=1):
-        """u = (R R^T/nb + cI)^{-1} (b/nb); c = mu (mode 1) or mu tr(G) (mode 0)."""
+    def ngd_solve(self, rows_t, ld, nrow, b, nb, mu, mode=1, dual=False):
+        """u = (R R^T/nb + cI)^{-1} (b/nb); c = mu (mode 1) or mu tr(G) (mode 0).
+        dual: the same step through the nrow x nrow Gram matrix."""
         rk, pr, rdev = _ptr(rows_t, np.float32)
         if not rdev:
             raise TypeError("rows_t must be a device tensor")
@@ -304,7 +306,8 @@ class BatchWork:
             u = np.zeros(self.net.nw)
             pu = u.ctypes.data
         _chk(lib().dnnf90_ngd_solve(self.net.id, self.id, int(nrow), pr, int(ld), pb, int(nb), float(mu),
-                                    int(mode), pu, DEVICE_PTRS if bdev else 0), "dnnf90_ngd_solve")
+                                    int(mode), pu, (DEVICE_PTRS if bdev else 0) | (NGD_DUAL if dual else 0)),
+             "dnnf90_ngd_solve")
         return u
 
     def free(self):

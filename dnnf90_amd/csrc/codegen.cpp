@@ -141,13 +141,25 @@ int gen_bell_compile(const TableSet &t, const FwdSchedule &s, bool fp64, GenKern
     }
     size_t sz = 0;
     hiprtcGetCodeSize(prog, &sz);
-    g->code.resize(sz);
-    hiprtcGetCode(prog, g->code.data());
+    std::vector<char> code(sz);
+    hiprtcGetCode(prog, code.data());
     hiprtcDestroyProgram(&prog);
-    if (hipModuleLoadData(&g->mod, g->code.data()) != hipSuccess) return -3;
-    if (hipModuleGetFunction(&g->fwd, g->mod, fp64 ? "dnnf90_bell_fwd_f64" : "dnnf90_bell_fwd_f32") != hipSuccess ||
-        hipModuleGetFunction(&g->bwd, g->mod, fp64 ? "dnnf90_bell_bwd_f64" : "dnnf90_bell_bwd_f32") != hipSuccess)
+    return gen_bell_load(code, fp64, g);
+}
+
+int gen_bell_load(const std::vector<char> &code, bool fp64, GenKernels *g)
+{
+    *g = GenKernels();
+    g->code = code;
+    if (hipModuleLoadData(&g->mod, g->code.data()) != hipSuccess ||
+        hipModuleGetFunction(&g->fwd, g->mod, fp64 ? "dnnf90_bell_fwd_f64" : "dnnf90_bell_fwd_f32") != hipSuccess ||
+        hipModuleGetFunction(&g->bwd, g->mod, fp64 ? "dnnf90_bell_bwd_f64" : "dnnf90_bell_bwd_f32") != hipSuccess) {
+        // the table-driven kernels take over; leave no error behind for
+        // the next launch check
+        (void)hipGetLastError();
+        g->release();
         return -3;
+    }
     (void)hipFuncGetAttribute(&g->fwd_scratch, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, g->fwd);
     (void)hipFuncGetAttribute(&g->bwd_scratch, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, g->bwd);
     (void)hipFuncGetAttribute(&g->fwd_regs, HIP_FUNC_ATTRIBUTE_NUM_REGS, g->fwd);

@@ -26,6 +26,9 @@ std::string gen_bell_source(const TableSet &t, const FwdSchedule &s, bool fp64);
 bool gen_bell_usable(const TableSet &t, const FwdSchedule &s);
 // compile with hipRTC and load on the current device; 0 on success
 int gen_bell_compile(const TableSet &t, const FwdSchedule &s, bool fp64, GenKernels *g, std::string *log);
+// load compiled code (from gen_bell_compile or a cache of it) into g;
+// 0, or -3 with g released and the HIP error state cleared
+int gen_bell_load(const std::vector<char> &code, bool fp64, GenKernels *g);
 
 hipError_t gen_launch_fwd(const GenKernels &g, const void *S, void *Tout, void *H, int nrow, long Np,
                           const ActParams &ap, hipStream_t s);

@@ -180,11 +180,7 @@ struct Net {
                 std::lock_guard<std::mutex> lk(g_gen_mu);
                 auto it = g_gen_cache.find(src);
                 if (it != g_gen_cache.end()) {
-                    gen[i].code = it->second;
-                    if (hipModuleLoadData(&gen[i].mod, gen[i].code.data()) == hipSuccess &&
-                        hipModuleGetFunction(&gen[i].fwd, gen[i].mod, "dnnf90_bell_fwd") == hipSuccess &&
-                        hipModuleGetFunction(&gen[i].bwd, gen[i].mod, "dnnf90_bell_bwd") == hipSuccess)
-                        gen[i].ok = true;
+                    (void)gen_bell_load(it->second, fp64, &gen[i]);
                 } else {
                     std::string log;
                     if (gen_bell_compile(tab, sched, fp64, &gen[i], &log) == 0) g_gen_cache[src] = gen[i].code;
@@ -276,6 +272,9 @@ struct BatchWork {
     // NGD metric and solve (allocated on first use)
     double *ngd_G = nullptr, *ngd_part = nullptr, *ngd_u = nullptr, *ngd_trace = nullptr, *ngd_work = nullptr;
     long ngd_G_cap = 0, ngd_part_cap = 0, ngd_u_cap = 0, ngd_work_cap = 0;
+    double *ngd_J = nullptr;         // dual route: rows row-contiguous, FP64
+    double *ngd_v = nullptr;         // dual route: b, J b, R y
+    long ngd_J_cap = 0, ngd_v_cap = 0;
     int *ngd_info = nullptr;
     std::vector<double> h_coeff;
     std::vector<int> h_slot, h_nonlin;
@@ -320,6 +319,7 @@ struct BatchWork {
         dfree(partial); dfree(d_x); dfree(d_seed); dfree(d_tout); dfree(d_rsq); dfree(d_rsq_part);
         dfree(d_coeff); dfree(d_slot); dfree(d_nonlin);
         dfree(ngd_G); dfree(ngd_part); dfree(ngd_u); dfree(ngd_trace); dfree(ngd_info); dfree(ngd_work);
+        dfree(ngd_J); dfree(ngd_v);
         if (own) (void)hipStreamDestroy(own);
         own = stream = nullptr;
     }
@@ -1192,6 +1192,30 @@ int dnnf90_bwork_set_stream(int bid, void *stream)
 // NGD step on the GPU (app/optimizer_module.f90:221-469, primal route):
 // G = R R^T over the first nrow columns of the transposed FP32 rows, then
 // u = (G/nb + cI)^{-1} (b/nb) by an FP64 Cholesky
+extern "C++" {
+// grow a device buffer to n elements
+template <typename T>
+static int grow(T **p, long *cap, long n)
+{
+    if (*cap >= n) return 0;
+    dfree(*p);
+    int rc = dalloc(p, (size_t)n);
+    if (rc) return rc;
+    *cap = n;
+    return 0;
+}
+
+// Gram product chunking over ncol columns of an n x n result
+static void gram_split(int n, long ncol, int *nc, long *ch)
+{
+    const int tiles = ((n + 127) / 128) * ((n + 143) / 144);
+    int c = (int)std::min<long>(std::max(1, 1024 / tiles), std::max(1L, ncol / 512));
+    long h = round_up((ncol + c - 1) / c, 16);
+    *nc = (int)((ncol + h - 1) / h);
+    *ch = h;
+}
+}  // extern "C++"
+
 int dnnf90_ngd_solve(int nid, int bid, int nrow, const float *rows_t, long ld, const double *b, int nb,
                      double mu, int mode, double *u, int flags)
 {
@@ -1201,50 +1225,56 @@ int dnnf90_ngd_solve(int nid, int bid, int nrow, const float *rows_t, long ld, c
         return fail(-1, "bad handle or argument");
     const long ncol = round_up(nrow, 16);
     if (ld < ncol || ld % 16 != 0) return fail(-1, "row stride must be a multiple of 16 and >= nrow");
+    const bool dual = flags & DNNF90_NGD_DUAL;
+    if (dual && mu <= 0.0) return fail(-1, "the dual route needs a positive damping");
     BatchWork &bw = *g_bwork.obj[bid];
     const int nw = nt->nw;
     const bool dev = flags & DNNF90_DEVICE_PTRS;
-    const long len = (long)nw * nw;
-    const int tiles = ((nw + 127) / 128) * ((nw + 143) / 144);
-    int nc = (int)std::min<long>(std::max(1, 1024 / tiles), std::max(1L, ncol / 512));
-    long ch = round_up((ncol + nc - 1) / nc, 16);
-    nc = (int)((ncol + ch - 1) / ch);
+    // the factored system: nw x nw (primal) or nrow x nrow (dual)
+    const int n = dual ? nrow : nw;
+    const long ldj = round_up(nw, 16);          // dual: J row stride
+    const long gcol = dual ? ldj : ncol;        // Gram reduction length
+    const long len = (long)n * n;
+    int nc;
+    long ch;
+    gram_split(n, gcol, &nc, &ch);
     int rc;
-    if (bw.ngd_G_cap < len) {
-        dfree(bw.ngd_G);
-        if ((rc = dalloc(&bw.ngd_G, (size_t)len))) return rc;
-        bw.ngd_G_cap = len;
-    }
-    const long pneed = nc > 1 ? (long)(nc + grad_reduce_splits(nc)) * len : 0;
-    if (bw.ngd_part_cap < pneed) {
-        dfree(bw.ngd_part);
-        if ((rc = dalloc(&bw.ngd_part, (size_t)pneed))) return rc;
-        bw.ngd_part_cap = pneed;
-    }
-    if (bw.ngd_u_cap < nw) {
-        dfree(bw.ngd_u);
-        if ((rc = dalloc(&bw.ngd_u, (size_t)nw))) return rc;
-        bw.ngd_u_cap = nw;
-    }
+    if ((rc = grow(&bw.ngd_G, &bw.ngd_G_cap, len))) return rc;
+    if ((rc = grow(&bw.ngd_part, &bw.ngd_part_cap, nc > 1 ? (long)(nc + grad_reduce_splits(nc)) * len : 0)))
+        return rc;
+    if ((rc = grow(&bw.ngd_u, &bw.ngd_u_cap, (long)nw))) return rc;
+    if ((rc = grow(&bw.ngd_work, &bw.ngd_work_cap, ngd_work_doubles(n)))) return rc;
     if (!bw.ngd_trace && ((rc = dalloc(&bw.ngd_trace, 1)) || (rc = dalloc(&bw.ngd_info, 1)))) return rc;
-    const long wneed = ngd_work_doubles(nw);
-    if (bw.ngd_work_cap < wneed) {
-        dfree(bw.ngd_work);
-        if ((rc = dalloc(&bw.ngd_work, (size_t)wneed))) return rc;
-        bw.ngd_work_cap = wneed;
+    if (dual) {
+        if ((rc = grow(&bw.ngd_J, &bw.ngd_J_cap, (long)nrow * ldj))) return rc;
+        if ((rc = grow(&bw.ngd_v, &bw.ngd_v_cap, 2L * nw + nrow))) return rc;
     }
     hipStream_t s = bw.stream;
-    if (nc > 1) launch_fill(bw.ngd_G, len, 0.0, s);
-    {
-        PScope p(bw, CAT_GRAD);
-        launch_gram(rows_t, nw, ld, bw.ngd_part, nc, ch, bw.ngd_G, s);
-    }
-    HIPCHK(hipMemcpyAsync(bw.ngd_u, b, sizeof(double) * nw, dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
-                          s));
-    {
+    if (nc > 1 && !dual) launch_fill(bw.ngd_G, len, 0.0, s);
+    if (!dual) {
+        {
+            PScope p(bw, CAT_GRAD);
+            launch_gram(rows_t, nw, ld, bw.ngd_part, nc, ch, bw.ngd_G, s);
+        }
+        HIPCHK(hipMemcpyAsync(bw.ngd_u, b, sizeof(double) * nw,
+                              dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
         PScope p(bw, CAT_OTHER);
         ngd_factor_solve(bw.ngd_G, nw, nw, (double)nb, mu, mode, bw.ngd_u, bw.ngd_trace, bw.ngd_info, bw.ngd_work,
                          s);
+    } else {
+        double *db = bw.ngd_v, *jv = bw.ngd_v + nw, *ry = bw.ngd_v + nw + nrow;
+        HIPCHK(hipMemcpyAsync(db, b, sizeof(double) * nw, dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+        {
+            PScope p(bw, CAT_GRAD);
+            launch_rows_transpose(rows_t, ld, nw, nrow, bw.ngd_J, ldj, s);
+            launch_gram64(bw.ngd_J, nrow, ldj, bw.ngd_G, s);
+        }
+        PScope p(bw, CAT_OTHER);
+        launch_rows_matvec(bw.ngd_J, ldj, nrow, nw, db, jv, s);          // J b
+        ngd_factor_solve(bw.ngd_G, nrow, nrow, (double)nb, mu, mode, jv, bw.ngd_trace, bw.ngd_info, bw.ngd_work,
+                         s);                                             // y = K^-1 J b / nb
+        launch_rows_matvec(rows_t, ld, nw, nrow, jv, ry, s);            // R y = J^T y
+        launch_dual_combine(nw, db, ry, (double)nb, mu, mode, bw.ngd_trace, bw.ngd_u, s);
     }
     int info = 0;
     HIPCHK(hipMemcpyAsync(&info, bw.ngd_info, sizeof(int), hipMemcpyDeviceToHost, s));

@@ -664,6 +664,12 @@ void launch_gram(const float *R, int nw, long ld, double *partial, int nchunk, l
     hipLaunchKernelGGL(k_reduce_pass2, dim3(gx), dim3(256), 0, s, part2, ns, len, G);
 }
 
+void launch_gram64(const double *R, int n, long ld, double *G, hipStream_t s)
+{
+    dim3 grid(1u, (unsigned)((n + 127) / 128), (unsigned)((n + GN - 1) / GN));
+    hipLaunchKernelGGL((k_gemm_grad<double, 2>), grid, dim3(256), 0, s, R, R, n, n, ld, 0L, ld, 0, 1, G);
+}
+
 // per-point gradient rows (no reduction over points): the Jacobian rows of
 // the NGD metric, stored transposed, rowsT[w][n]
 template <typename T>

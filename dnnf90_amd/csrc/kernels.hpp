@@ -110,6 +110,17 @@ int grad_reduce_splits(int nchunk);
 // pivot (1-based column); trace is one device double of scratch
 int ngd_factor_solve(double *G, int n, long lda, double nb, double mu, int mode, double *b, double *trace,
                      int *info, double *work, hipStream_t s);
+// NGD dual route helpers (ngd.hip): Jm[n][w] = R[w][n] widened to FP64 with
+// zero columns up to ldj; out = M x with FP64 sums; u = (b - R y)/nb / c
+void launch_rows_transpose(const float *R, long ldr, int nw, int nrow, double *Jm, long ldj, hipStream_t s);
+template <typename T>
+void launch_rows_matvec(const T *M, long ld, int nr, int nc, const double *x, double *out, hipStream_t s);
+// G = R R^T over FP64 rows R[n][ld] on FP64 MFMA, tiles holding
+// lower-triangle entries (the dual route's K, where the 1/c of the step
+// amplifies any error of K)
+void launch_gram64(const double *R, int n, long ld, double *G, hipStream_t s);
+void launch_dual_combine(int nw, const double *b, const double *ry, double nb, double mu, int mode,
+                         const double *trace, double *u, hipStream_t s);
 // doubles of device work space ngd_factor_solve needs for order n (the
 // inverses of the diagonal blocks and one packed panel)
 long ngd_work_doubles(int n);

@@ -374,4 +374,78 @@ int ngd_factor_solve(double *G, int n, long lda, double nb, double mu, int mode,
     return 0;
 }
 
+
+// ---------------------------------------------------------------------------
+// Dual (Gram) route, app/optimizer_module.f90:408-446: with G = J^T J / N_b
+// and c the damping, (G + cI)^{-1} b1 = (b1 - J^T K^{-1} (J b1) / N_b) / c,
+// K = J J^T / N_b + cI (nrow x nrow).  The rows arrive transposed,
+// R[w][n] = J[n][w]; J is formed row-contiguous for the Gram product.
+
+// Jm[n][w] = R[w][n] (n < nrow, w < nw), zero for nw <= w < ldj, widened
+// to FP64 (exactly); 32 x 32 tiles through LDS so both sides are coalesced
+__global__ __launch_bounds__(256) void k_rows_transpose(const float *__restrict__ R, long ldr, int nw, int nrow,
+                                                        double *__restrict__ Jm, long ldj)
+{
+    __shared__ float t[32][33];
+    const int w0 = blockIdx.x * 32, n0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+    for (int r = ty; r < 32; r += 8) {
+        const int w = w0 + r, n = n0 + tx;
+        t[r][tx] = (w < nw && n < nrow) ? R[(long)w * ldr + n] : 0.f;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int n = n0 + r, w = w0 + tx;
+        if (n < nrow && w < ldj) Jm[(long)n * ldj + w] = (double)t[tx][r];
+    }
+}
+
+// out[r] = sum_c M[r][c] x[c] for the matrix M[nr][ld] (c < nc), FP64
+// sums; one wave per row, lanes along the row
+template <typename T>
+__global__ __launch_bounds__(256) void k_rows_matvec(const T *__restrict__ M, long ld, int nr, int nc,
+                                                     const double *__restrict__ x, double *__restrict__ out)
+{
+    const int lane = threadIdx.x & 63;
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= nr) return;
+    const T *row = M + (long)r * ld;
+    double s = 0.0;
+    for (int c = lane; c < nc; c += 64) s += (double)row[c] * x[c];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) out[r] = s;
+}
+
+// u[w] = (b[w] / nb - (R y)[w] / nb) / c with c = mu (ABS) or mu tr / nb
+// (TRACE), tr the raw trace left by ngd_factor_solve; (R y) precomputed
+__global__ void k_dual_combine(int nw, const double *__restrict__ b, const double *__restrict__ ry, double nb,
+                               double mu, int mode, const double *__restrict__ trace, double *__restrict__ u)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nw) return;
+    const double c = mode == 1 ? mu : mu * (*trace / nb);
+    u[w] = (b[w] / nb - ry[w] / nb) / c;
+}
+
+void launch_rows_transpose(const float *R, long ldr, int nw, int nrow, double *Jm, long ldj, hipStream_t s)
+{
+    dim3 grid((unsigned)((ldj + 31) / 32), (unsigned)((nrow + 31) / 32));
+    hipLaunchKernelGGL(k_rows_transpose, grid, dim3(256), 0, s, R, ldr, nw, nrow, Jm, ldj);
+}
+
+template <typename T>
+void launch_rows_matvec(const T *M, long ld, int nr, int nc, const double *x, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_rows_matvec<T>), dim3((unsigned)((nr + 3) / 4)), dim3(256), 0, s, M, ld, nr, nc, x, out);
+}
+template void launch_rows_matvec<float>(const float *, long, int, int, const double *, double *, hipStream_t);
+template void launch_rows_matvec<double>(const double *, long, int, int, const double *, double *, hipStream_t);
+
+void launch_dual_combine(int nw, const double *b, const double *ry, double nb, double mu, int mode,
+                         const double *trace, double *u, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dual_combine, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, nw, b, ry, nb, mu, mode,
+                       trace, u);
+}
+
 }  // namespace dnnf90

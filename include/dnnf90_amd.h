@@ -19,6 +19,7 @@ extern "C" {
 #endif
 
 #define DNNF90_DEVICE_PTRS 1
+#define DNNF90_NGD_DUAL 2     /* dnnf90_ngd_solve: Gram (dual) route */
 
 /* arithmetic of a batched work space */
 #define DNNF90_PREC_FP64 0   /* everything FP64: reference parity mode */
@@ -132,7 +133,11 @@ int dnnf90_residual_rows_batch(int nid, int bid, int npt, const double *x,
  * by an FP64 Cholesky on the GPU.  b and u hold nweights doubles (host, or
  * device with DNNF90_DEVICE_PTRS).  Returns -5 when G/nb + cI is not
  * positive definite (the reference stops on a singular pivot,
- * optimizer_module.f90:489-505).  Work space comes from bid. */
+ * optimizer_module.f90:489-505).  Work space comes from bid.
+ * flags & DNNF90_NGD_DUAL takes the Gram route of
+ * app/optimizer_module.f90:408-446 instead: K = R^T R / nb + cI (nrow x
+ * nrow), y = K^{-1} (R^T b) / nb, u = (b - R y) / nb / c -- the same step
+ * to rounding, cheaper when nrow < nweights (needs mu > 0). */
 int dnnf90_ngd_solve(int nid, int bid, int nrow, const float *rows_t, long ld,
                      const double *b, int nb, double mu, int mode, double *u,
                      int flags);

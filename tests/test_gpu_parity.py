@@ -207,14 +207,17 @@ def _ngd_inputs(D, oracle_mod, dims, npt):
     return net, bw, rows_t, ld, g.get(), o, w, x, terms
 
 
+@pytest.mark.parametrize("dual", [False, True], ids=["primal", "dual"])
 @pytest.mark.parametrize("mode,mu", [(1, 1e-2), (0, 1e-3)])
-def test_ngd_solve_vs_oracle(D, oracle_mod, mode, mu):
-    """Gram metric (FP32 MFMA) + FP64 Cholesky against the oracle's Gaussian
-    elimination (app/optimizer_module.f90:221-522) on the same FP32 rows."""
+def test_ngd_solve_vs_oracle(D, oracle_mod, mode, mu, dual):
+    """Gram metric (FP32 MFMA) + FP64 Cholesky, primal (nw x nw) or dual
+    (nrow x nrow, app/optimizer_module.f90:408-446), against the oracle's
+    primal Gaussian elimination (app/optimizer_module.f90:221-522) on the
+    same FP32 rows; both routes give the same step to rounding."""
     dims = [2, 16, 16, 1]
     npt = 200
     net, bw, rows_t, ld, b, o, w, x, terms = _ngd_inputs(D, oracle_mod, dims, npt)
-    u = bw.ngd_solve(rows_t, ld, npt, b, npt, mu, mode)
+    u = bw.ngd_solve(rows_t, ld, npt, b, npt, mu, mode, dual=dual)
     J = rows_t[:, :npt].double().cpu().numpy().T
     ur = oracle_mod.Oracle.ngd_solve(J, b, npt, mu, mode)
     G = J.T @ J / npt
@@ -243,3 +246,10 @@ def test_ngd_full_size_backward_error(D, oracle_mod):
     be = r.norm().item() / (gnorm * ut.norm().item() + np.linalg.norm(b) / npt)
     print("ngd full-size backward error", be)
     assert be <= 1e-6
+    # the dual (Gram, 4096 x 4096) route solves the same system
+    ud = bw.ngd_solve(rows_t, ld, npt, b, npt, 1e-3, 1, dual=True)
+    udt = torch.from_numpy(ud).cuda()
+    rd = (Jt @ (Jt.T @ udt)) / npt + 1e-3 * udt - torch.from_numpy(b).cuda() / npt
+    bed = rd.norm().item() / (gnorm * udt.norm().item() + np.linalg.norm(b) / npt)
+    print("ngd dual backward error", bed, "rel diff to primal", rel(ud, u))
+    assert bed <= 1e-6
