@@ -48,6 +48,8 @@ SIGNATURES = {
     "dnnf90_last_error": (_cs, []),
     "dnnf90_tabset_counts": (_i, [_i, _vp]),
     "dnnf90_tabset_lists": (_i, [_i] + [_vp] * 10),
+    "dnnf90_tabset_write_order": (_i, [_i, _cs]),
+    "dnnf90_tabset_free": (_i, [_i]),
     "dnnf90_set_codegen": (_i, [_i]),
     "dnnf90_net_kernel_info": (_i, [_i, _i, _vp]),
     "dnnf90_net_nweights": (_i, [_i]),
@@ -70,6 +72,7 @@ SIGNATURES = {
     "dnnf90_bwork_kernel_ms": (_i, [_i, _vp, _vp]),
     "dnnf90_forward_batch": (_i, [_i, _i, _i, _vp, _vp, _i]),
     "dnnf90_backward_batch": (_i, [_i, _i, _vp, _i, _i]),
+    "dnnf90_grad_batch": (_i, [_i, _i, _i, _vp, _vp, _d, _i, _i]),
     "dnnf90_residual_step_batch": (_i, [_i, _i, _i, _vp, _i, _vp, _vp, _vp, _d, _i, _vp, _i]),
     "dnnf90_residual_rows_batch": (_i, [_i, _i, _i, _vp, _i, _vp, _vp, _vp, _vp, _l, _i]),
     "dnnf90_ngd_solve": (_i, [_i, _i, _i, _vp, _l, _vp, _i, _d, _i, _vp, _i]),

@@ -8,6 +8,7 @@ and message.  Slots are 1-based in the C ABI (dnnf90_alpha) and 0-based here.
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -63,6 +64,24 @@ class TableSet:
             _chk(L.dnnf90_tabset_activation(self.id, act), "dnnf90_tabset_activation")
         self.d0, self.k, self.act = d0, k, act
         self.na = _chk(L.dnnf90_tabset_nderiv(self.id), "dnnf90_tabset_nderiv")
+
+    def write_order(self, path):
+        """Write hod_alpha_order.dat (the reference's slot-order file)."""
+        _chk(lib().dnnf90_tabset_write_order(self.id, os.fsencode(path)), "dnnf90_tabset_write_order")
+
+    def free(self):
+        """Release the handle (networks built from the set stay valid)."""
+        if getattr(self, "id", 0) > 0:
+            lib().dnnf90_tabset_free(self.id)
+            self.id = 0
+
+    def __del__(self):
+        # the library holds 64 table sets at most; hand the slot back when the
+        # Python object goes (host memory only, no GPU call)
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def current_alpha():
@@ -267,6 +286,16 @@ class BatchWork:
         sk, ps, dev = _ptr(seed, np.float64)
         _chk(lib().dnnf90_backward_batch(self.net.id, self.id, ps, grad.id, DEVICE_PTRS if dev else 0),
              "dnnf90_backward_batch")
+
+    def grad_batch(self, x, y, coef, grad: Grad):
+        """Value-only batched step (net_grad_batch): gradient of
+        sum_n coef/2 (u_n - y_n)^2 added to grad; single-output networks."""
+        xk, px, xdev = _ptr(x, np.float64)
+        yk, py, ydev = _ptr(y, np.float64)
+        if xdev != ydev:
+            raise TypeError("x and y must both be host or both be device")
+        _chk(lib().dnnf90_grad_batch(self.net.id, self.id, int(x.shape[0]), px, py, float(coef), grad.id,
+                                     DEVICE_PTRS if xdev else 0), "dnnf90_grad_batch")
 
     def residual_step(self, x, res: Residual, fac, grad: Grad, want_rsq=True):
         """Fused forward + collocation seed + adjoint; returns sum R^2."""

@@ -263,7 +263,47 @@ void launch_bell_bwd(const T *Tb, const T *H, T *Sb, int nrow, long Np, const De
     }
 }
 
+// ---------------------------------------------------------------------------
+// value-only path (lib/train_module.f90:525-631): Z = sigma(S), Zd =
+// sigma'(S) elementwise over n values; the adjoint is Sbar = Tbar * Zd
+template <typename T, bool kBes>
+__global__ void k_act_value(const T *__restrict__ S, T *__restrict__ Z, T *__restrict__ Zd, long n, ActParams ap)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double d[2];
+    act_derivs<2, kBes>(ap, (double)S[i], d);
+    Z[i] = (T)d[0];
+    Zd[i] = (T)d[1];
+}
+
+template <typename T>
+__global__ void k_mul_value(const T *__restrict__ Tb, const T *__restrict__ Zd, T *__restrict__ Sb, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) Sb[i] = Tb[i] * Zd[i];
+}
+
+template <typename T>
+void launch_act_value(const T *S, T *Z, T *Zd, long n, const ActParams &ap, hipStream_t s)
+{
+    const unsigned g = (unsigned)((n + 255) / 256);
+    const bool bes = ap.iact == 3 || ap.iact == 4;   // J0, J1: the series variant of act_derivs
+    if (bes)
+        hipLaunchKernelGGL((k_act_value<T, true>), dim3(g), dim3(256), 0, s, S, Z, Zd, n, ap);
+    else
+        hipLaunchKernelGGL((k_act_value<T, false>), dim3(g), dim3(256), 0, s, S, Z, Zd, n, ap);
+}
+
+template <typename T>
+void launch_mul_value(const T *Tb, const T *Zd, T *Sb, long n, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_mul_value<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, Tb, Zd, Sb, n);
+}
+
 #define INST(T)                                                                                         \
+    template void launch_act_value<T>(const T *, T *, T *, long, const ActParams &, hipStream_t);         \
+    template void launch_mul_value<T>(const T *, const T *, T *, long, hipStream_t);                     \
     template void launch_bell_fwd<T>(const T *, T *, T *, int, long, const DevFwdSched &, const ActParams &, \
                                      T *, hipStream_t);                                                  \
     template void launch_bell_bwd<T>(const T *, const T *, T *, int, long, const DevPairs &, hipStream_t);

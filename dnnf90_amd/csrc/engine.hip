@@ -684,6 +684,78 @@ static int run_backward_seed(Net &nt, BatchWork &b, const double *seed, Grad &g,
     return 0;
 }
 
+// value-only batched step (lib/train_module.f90:525-631, net_grad_batch):
+// the work space's planes are used with one slot (row stride Np); Zd of
+// hidden layer l sits after Z in the same plane.  Gradient added to g.
+template <typename T>
+static int run_grad_value(Net &nt, BatchWork &b, int npt, const double *x, const double *y, double coef, Grad &g,
+                          bool dev)
+{
+    const int L = nt.nlayer;
+    if (npt < 1 || npt > b.maxpt) return fail(-1, "npt outside [1, maxpt]");
+    if (nt.dims[L - 1] != 1) return fail(-1, "the value path is written for a single output");
+    int rc = nt.ensure_device(b.prec != PREC_FP64);
+    if (rc) return rc;
+    const long Np = round_up(npt, 64);
+    hipStream_t s = b.stream;
+    const double *dx = x, *dy = y;
+    if (!dev) {
+        HIPCHK(hipMemcpyAsync(b.d_x, x, sizeof(double) * npt * nt.dims[0], hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(b.d_seed, y, sizeof(double) * npt, hipMemcpyHostToDevice, s));
+        dx = b.d_x;
+        dy = b.d_seed;
+    }
+    const T *W = weights<T>(nt);
+    const ActParams act = nt.dt.act();
+    auto Z = [&](int l) { return (T *)(l == 0 ? b.t0 : (l == L - 1 ? b.tl : b.tpl[l])); };
+    auto Zd = [&](int l) { return (T *)b.tpl[l] + (long)nt.dims[l] * Np; };
+    {
+        PScope p(b, CAT_OTHER);
+        launch_input<T>(dx, npt, nt.dims[0], 1, nt.dt.ind_e1, Z(0), Np, s);
+    }
+    for (int l = 1; l < L; ++l) {
+        const int nd = nt.dims[l], ndm = nt.dims[l - 1];
+        T *out = l < L - 1 ? (T *)b.p0 : Z(l);
+        {
+            PScope p(b, CAT_LINEAR);
+            launch_gemm_wx<T>(b.prec, false, W + nt.woff[l], ndm + 1, Z(l - 1), out, nd, ndm + 1, Np, Np, s);
+        }
+        if (l < L - 1) {
+            PScope p(b, CAT_BELL_FWD);
+            launch_act_value<T>(out, Z(l), Zd(l), (long)nd * Np, act, s);
+        }
+    }
+    {
+        PScope p(b, CAT_OTHER);
+        launch_seed_value<T>(Z(L - 1), dy, coef, npt, Np, (T *)b.p0, s);
+    }
+    for (int l = L - 1; l >= 1; --l) {
+        const int nd = nt.dims[l], ndm = nt.dims[l - 1];
+        if (l < L - 1) {
+            PScope p(b, CAT_BELL_BWD);
+            launch_mul_value<T>((const T *)b.p1, Zd(l), (T *)b.p0, (long)nd * Np, s);
+        }
+        int nc;
+        long ch;
+        grad_split(Np, nd, ndm + 1, (int)b.esz, &nc, &ch);
+        {
+            PScope p(b, CAT_GRAD);
+            launch_gemm_grad<T>(b.prec, (const T *)b.p0, Z(l - 1), nd, ndm + 1, Np, Np, b.partial, nc, ch, s);
+        }
+        {
+            PScope p(b, CAT_GRAD_REDUCE);
+            launch_grad_reduce(b.partial, nc, (long)nd * (ndm + 1), g.d_g + nt.woff[l], s);
+        }
+        if (l > 1) {
+            PScope p(b, CAT_LINEAR_T);
+            launch_gemm_wx<T>(b.prec, true, W + nt.woff[l], ndm + 1, (const T *)b.p0, (T *)b.p1, ndm, nd, Np, Np, s);
+        }
+    }
+    b.fwd_valid = false;   // the planes now hold the one-slot layout
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 // residual terms are host arrays (they describe the operator); the device
 // copy is refreshed only when they change, so repeated steps never block
 static int set_residual(BatchWork &b, int nterm, const double *coeff, const int *slot,
@@ -1050,6 +1122,35 @@ int dnnf90_tabset_counts(int tid, int *counts)
     return 0;
 }
 
+int dnnf90_tabset_free(int tid)
+{
+    TableSet *t = g_tab.get(tid);
+    if (!t) return fail(-1, "bad table-set handle");
+    // networks hold their own copy of the tables, so none depends on the slot
+    delete t;
+    g_tab.obj[tid] = nullptr;
+    return 0;
+}
+
+int dnnf90_tabset_write_order(int tid, const char *filename)
+{
+    TableSet *t = g_tab.get(tid);
+    if (!t || !filename) return fail(-1, "bad table-set handle or file name");
+    FILE *f = std::fopen(filename, "w");
+    if (!f) return fail(-2, std::string("cannot write ") + filename);
+    // lib/multi_index_bell_module.f90:451-471: two comment lines, then
+    // "ia  |alpha|  alpha(1:D0)" one line per slot, 1-based ia
+    std::fprintf(f, "# column order of high-order derivative targets\n");
+    std::fprintf(f, "# ia  |alpha|  alpha(1:D0)\n");
+    for (int a = 0; a < t->na; ++a) {
+        std::fprintf(f, "%d  %d  ", a + 1, t->deg[a]);
+        for (int i = 0; i < t->d0; ++i) std::fprintf(f, " %d", t->alpha[(size_t)a * t->d0 + i]);
+        std::fprintf(f, "\n");
+    }
+    const bool ok = std::fclose(f) == 0;
+    return ok ? 0 : fail(-2, std::string("cannot write ") + filename);
+}
+
 int dnnf90_tabset_lists(int tid, int *pair_start, int *pair_num, int *pair_ia, int *pair_id,
                         double *pair_binom, int *fq_start, int *fq_num, int *fq_ib, int *fq_id,
                         double *fq_c)
@@ -1366,6 +1467,20 @@ int dnnf90_backward_batch(int nid, int bid, const double *seed, int gid, int fla
                                  : run_backward_seed<float>(*nt, b, seed, *g_grad.obj[gid], dev);
     if (rc) return rc;
     if (!dev) HIPCHK(hipStreamSynchronize(b.stream));
+    return 0;
+}
+
+int dnnf90_grad_batch(int nid, int bid, int npt, const double *x, const double *y, double coef, int gid, int flags)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt || !dependent_ok(g_bwork, bid, nid) || !grad_ok(gid, nid) || !x || !y)
+        return fail(-1, "bad handle or argument");
+    BatchWork &b = *g_bwork.obj[bid];
+    const bool dev = flags & DNNF90_DEVICE_PTRS;
+    int rc = b.prec == PREC_FP64 ? run_grad_value<double>(*nt, b, npt, x, y, coef, *g_grad.obj[gid], dev)
+                                 : run_grad_value<float>(*nt, b, npt, x, y, coef, *g_grad.obj[gid], dev);
+    if (rc) return rc;
+    if (!dev) HIPCHK(hipStreamSynchronize(b.stream));   // the host x and y are read asynchronously
     return 0;
 }
 

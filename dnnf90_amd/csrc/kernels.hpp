@@ -137,6 +137,15 @@ template <typename T>
 void launch_point_rows(const T *A, const T *B, int M, int Kd, int na, long Np, float *rowsT,
                        long ldr, hipStream_t s);
 
+// value-only path (bell.hip, misc.hip): Z = sigma(S), Zd = sigma'(S);
+// Sbar = Tbar * Zd; D = coef (Z - y) on npt points, 0 on the padding
+template <typename T>
+void launch_act_value(const T *S, T *Z, T *Zd, long n, const ActParams &ap, hipStream_t s);
+template <typename T>
+void launch_mul_value(const T *Tb, const T *Zd, T *Sb, long n, hipStream_t s);
+template <typename T>
+void launch_seed_value(const T *Z, const double *y, double coef, int npt, long Np, T *D, hipStream_t s);
+
 template <typename T>
 void launch_seed_user(const double *seed, int npt, int nout, int na, T *Tb, long Np, hipStream_t s);
 

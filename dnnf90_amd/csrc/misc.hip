@@ -206,12 +206,30 @@ void launch_axpy(double *y, const double *x, long n, double a, hipStream_t s)
     if (n > 0) hipLaunchKernelGGL(k_axpy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, y, x, n, a);
 }
 
+// value-only output seed (lib/train_module.f90:585-586): D = coef (Z - Y)
+// on the npt points of the single output, 0 on the padding
+template <typename T>
+__global__ void k_seed_value(const T *__restrict__ Z, const double *__restrict__ y, double coef, int npt, long Np,
+                             T *__restrict__ D)
+{
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= Np) return;
+    D[n] = n < npt ? (T)(coef * ((double)Z[n] - y[n])) : T(0);
+}
+
+template <typename T>
+void launch_seed_value(const T *Z, const double *y, double coef, int npt, long Np, T *D, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_seed_value<T>), dim3((unsigned)((Np + 255) / 256)), dim3(256), 0, s, Z, y, coef, npt, Np, D);
+}
+
 #define INST(T)                                                                                     \
     template void launch_input<T>(const double *, int, int, int, const int *, T *, long, hipStream_t); \
     template void launch_seed_user<T>(const double *, int, int, int, T *, long, hipStream_t);         \
     template void launch_seed_residual<T>(const T *, int, int, int, long, const DevResidual &, double, \
                                           bool, T *, double *, hipStream_t);                         \
-    template void launch_output<T>(const T *, int, int, int, long, double *, hipStream_t);
+    template void launch_output<T>(const T *, int, int, int, long, double *, hipStream_t);              \
+    template void launch_seed_value<T>(const T *, const double *, double, int, long, T *, hipStream_t);
 INST(double)
 INST(float)
 #undef INST

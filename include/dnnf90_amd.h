@@ -44,6 +44,17 @@ int dnnf90_tabset_lists(int tid, int *pair_start, int *pair_num, int *pair_ia,
                         int *pair_id, double *pair_binom, int *fq_start,
                         int *fq_num, int *fq_ib, int *fq_id, double *fq_c);
 
+/* Release a table-set handle.  No counterpart in the reference: its table
+ * sets (lib/c_api_module.f90:61, rtab(MAXH)) live until the process ends, so a
+ * host may create at most 64.  Networks built from the set keep their own copy
+ * of the tables and stay valid.  Returns 0, or -1 for a bad handle. */
+int dnnf90_tabset_free(int tid);
+
+/* the slot-order file hod_alpha_order.dat of the family
+ * (lib/multi_index_bell_module.f90:451-471, same text); -2 if it cannot be
+ * written */
+int dnnf90_tabset_write_order(int tid, const char *filename);
+
 /* Faa di Bruno kernels generated per table family at run time (hipRTC,
  * straight-line, operands in registers) are used where the family is small
  * enough; otherwise, or with codegen off, the table-driven kernels run.
@@ -107,6 +118,14 @@ int dnnf90_forward_batch(int nid, int bid, int npt, const double *x,
  * work space; the weight gradient is ADDED to gid */
 int dnnf90_backward_batch(int nid, int bid, const double *seed, int gid,
                           int flags);
+/* value-only batched step of a single-output network (net_grad_batch,
+ * lib/train_module.f90:525-631): forward with sigma only, output seed
+ * coef * (u - y) per point, deltas (W^T D) sigma', gradient blocks with the
+ * bias added to gid.  x [npt][d0], y [npt].  Uses the work space's memory
+ * with one slot; a following dnnf90_backward_batch needs a new forward. */
+int dnnf90_grad_batch(int nid, int bid, int npt, const double *x, const double *y, double coef, int gid,
+                      int flags);
+
 /* fused collocation step on output 1: R = sum_k coeff[k] * [T_1 *] T_slot[k]
  * (slot 1-based; nonlin[k] != 0 multiplies by the value T_1), seed =
  * fac * R * dR/dT (app/pinn_module.f90:32-65), adjoint, gradient ADDED to

@@ -137,3 +137,24 @@ def test_weight_file_round_trip(L, tmp_path):
         D.Net.load(q, ts)
     assert e.value.rc == -2
     assert D.Net.load(q, erf).dims == [2, 4, 1]
+
+
+def test_tabset_free_returns_handles(L):
+    """The pool holds 64 table sets (lib/c_api_module.f90:38); releasing them
+    lets a host create more, and a network built from a released set keeps its
+    tables."""
+    import dnnf90_amd as D
+    ts = D.TableSet(2, 3, [[0, 1], [1, 0], [3, 0]])
+    dims = np.array([2, 8, 1], np.int32)
+    nid = L.dnnf90_net_create_ts(ts.id, 3, dims.ctypes.data)
+    assert nid > 0
+    na = ts.na
+    ts.free()
+    assert ts.id == 0
+    assert L.dnnf90_tabset_free(0) == -1
+    assert L.dnnf90_net_nderiv(nid) == na
+    for _ in range(3 * 64):
+        t = D.TableSet(4, 3)
+        assert t.id > 0
+        t.free()
+    assert L.dnnf90_net_free(nid) == 0

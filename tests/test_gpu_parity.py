@@ -253,3 +253,47 @@ def test_ngd_full_size_backward_error(D, oracle_mod):
     bed = rd.norm().item() / (gnorm * udt.norm().item() + np.linalg.norm(b) / npt)
     print("ngd dual backward error", bed, "rel diff to primal", rel(ud, u))
     assert bed <= 1e-6
+
+
+@pytest.mark.parametrize("prec,tol", [(0, TOL64), (3, 2e-6)], ids=["fp64", "bf16x6"])
+def test_grad_batch_value_only(D, oracle_mod, prec, tol):
+    """The value-only batched step (net_grad_batch, lib/train_module.f90:525-631)
+    against the oracle's adjoint seeded on the value slot only: with T-bar
+    zero on every derivative slot, every pair sum for b > 0 vanishes, so
+    the derivative-carrying adjoint gives exactly the value-only gradient."""
+    from dnnf90_amd.workloads import WORKLOADS
+    wl = WORKLOADS["closure4_k7"]
+    npt = 300
+    ts, net, w, x, o = _setup(D, oracle_mod, wl, npt)
+    y = np.random.default_rng(5).standard_normal(npt)
+    coef = 0.37
+    bw = D.BatchWork(net, npt, prec)
+    g = D.Grad(net)
+    bw.grad_batch(x, y, coef, g)
+    gv = g.get()
+    t = o.eval(wl.dims, w, x)
+    seed = np.zeros((npt, 1, o.na))
+    seed[:, 0, 0] = coef * (t[:, 0, 0] - y)
+    gr = o.grad(wl.dims, w, x, seed)
+    e = rel(gv, gr)
+    print("value-only grad prec", prec, "rel err", e)
+    assert e <= tol
+    # the work space still serves the derivative-carrying path afterwards
+    tt = bw.forward(x)
+    assert max(rel_by_order(tt, t, o.deg()).values()) <= (TOL64 if prec == 0 else 2e-6)
+
+
+def test_write_c_ref(D, oracle_mod, tmp_path):
+    """c_ref.dat (tools/gen_c_ref.f90) of the KdV family through the GPU."""
+    from dnnf90_amd.formats import c_ref_points, read_c_ref, write_c_ref
+    from dnnf90_amd.workloads import WORKLOADS
+    wl = WORKLOADS["kdv"]
+    ts, net, w, x, o = _setup(D, oracle_mod, wl, 25)
+    path = tmp_path / "c_ref.dat"
+    write_c_ref(net, str(path))
+    xr, tr = read_c_ref(str(path))
+    assert np.array_equal(xr, c_ref_points())
+    ref = o.eval(wl.dims, w, c_ref_points())[:, 0, :]
+    e = rel(tr, ref)
+    print("c_ref.dat values rel err", e)
+    assert e <= TOL64
