@@ -50,6 +50,17 @@ SIGNATURES = {
     "dnnf90_tabset_lists": (_i, [_i] + [_vp] * 10),
     "dnnf90_tabset_write_order": (_i, [_i, _cs]),
     "dnnf90_tabset_free": (_i, [_i]),
+    "dnnf90_kf_create": (_i, [_i, _d, _d, _d, _i]),
+    "dnnf90_kf_set": (_i, [_i, _d, _d]),
+    "dnnf90_kf_free": (_i, [_i]),
+    "dnnf90_kf_update_grad": (_i, [_i, _i, _i, _d, _d]),
+    "dnnf90_kf_update_obs": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _i]),
+    "dnnf90_kf_update_resid": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _i]),
+    "dnnf90_kf_update_residual": (_i, [_i, _i, _i, _i, _vp, _i, _vp, _vp, _vp, _vp, _i]),
+    "dnnf90_kf_iekf_begin": (_i, [_i, _i]),
+    "dnnf90_kf_iekf_iter": (_i, [_i, _i, _i, _vp, _vp, _d, _d, _i, _i]),
+    "dnnf90_kf_get": (_i, [_i, _vp, _vp]),
+    "dnnf90_comm_eval_batch": (_i, [_i, _vp, _i, _i, _vp, _vp, _vp, _i]),
     "dnnf90_set_codegen": (_i, [_i]),
     "dnnf90_net_kernel_info": (_i, [_i, _i, _vp]),
     "dnnf90_net_nweights": (_i, [_i]),

@@ -343,3 +343,107 @@ class BatchWork:
         if self.id:
             lib().dnnf90_bwork_free(self.id)
             self.id = 0
+
+
+class KalmanFilter:
+    """Extended Kalman filter over a network's weights (lib/kalman_module.f90),
+    covariance on the device.  Method names follow the reference's kf_*
+    routines; observations are processed in order within one call."""
+
+    def __init__(self, net: Net, p0, lam_ini, lam0, decoupled=False, gate=0.0, q=0.0):
+        self.net = net
+        self.id = _chk(lib().dnnf90_kf_create(net.id, float(p0), float(lam_ini), float(lam0),
+                                              1 if decoupled else 0), "dnnf90_kf_create", True)
+        self.decoupled = bool(decoupled)
+        if gate or q:
+            self.set(gate, q)
+
+    def set(self, gate=0.0, q=0.0):
+        _chk(lib().dnnf90_kf_set(self.id, float(gate), float(q)), "dnnf90_kf_set")
+
+    def update_grad(self, grad: Grad, xi, scale=1.0):
+        """kf_update_grad: the row held by grad, residual xi."""
+        _chk(lib().dnnf90_kf_update_grad(self.net.id, self.id, grad.id, float(xi), float(scale)),
+             "dnnf90_kf_update_grad")
+
+    def update_obs(self, bw: "BatchWork", x, seed, target, pred=None):
+        """kf_update_obs over the observations x[n], seed[n][nout][na]."""
+        xk, px, dev = _ptr(x, np.float64)
+        sk, ps, sdev = _ptr(seed, np.float64)
+        t, pt = _host(np.atleast_1d(target), np.float64)
+        pr = _host(np.atleast_1d(pred), np.float64) if pred is not None else (None, None)
+        _chk(lib().dnnf90_kf_update_obs(self.net.id, self.id, bw.id, int(x.shape[0]), px, ps, pt, pr[1],
+                                        DEVICE_PTRS if dev else 0), "dnnf90_kf_update_obs")
+
+    def update_resid(self, bw: "BatchWork", x, seed, resid, rnoise=None):
+        """kf_update_resid_multi (kf_update_resid when rnoise is None)."""
+        xk, px, dev = _ptr(x, np.float64)
+        sk, ps, sdev = _ptr(seed, np.float64)
+        r, pr = _host(np.atleast_1d(resid), np.float64)
+        rn = _host(np.atleast_1d(rnoise), np.float64) if rnoise is not None else (None, None)
+        _chk(lib().dnnf90_kf_update_resid(self.net.id, self.id, bw.id, int(x.shape[0]), px, ps, pr, rn[1],
+                                          DEVICE_PTRS if dev else 0), "dnnf90_kf_update_resid")
+
+    def update_residual(self, bw: "BatchWork", x, res: Residual, src=None):
+        """The trainer's collocation observation: R = src + sum_k c_k [T_0 *] T_slot
+        at the running weights, seed dR/dT, then kf_update_resid."""
+        xk, px, dev = _ptr(x, np.float64)
+        s = _host(np.atleast_1d(src), np.float64) if src is not None else (None, None)
+        if dev and src is not None:
+            raise TypeError("src must be on the device when x is")
+        _chk(lib().dnnf90_kf_update_residual(self.net.id, self.id, bw.id, int(x.shape[0]), px, res.nterm,
+                                             res.coeff.ctypes.data, res.slot.ctypes.data, res.nonlin.ctypes.data,
+                                             s[1], DEVICE_PTRS if dev else 0), "dnnf90_kf_update_residual")
+
+    def iekf_begin(self):
+        _chk(lib().dnnf90_kf_iekf_begin(self.net.id, self.id), "dnnf90_kf_iekf_begin")
+
+    def iekf_iter(self, bw: "BatchWork", x, seed, resid, rnoise, final):
+        xk, px, dev = _ptr(x, np.float64)
+        sk, ps, sdev = _ptr(seed, np.float64)
+        _chk(lib().dnnf90_kf_iekf_iter(self.net.id, self.id, bw.id, px, ps, float(resid), float(rnoise),
+                                       1 if final else 0, DEVICE_PTRS if dev else 0), "dnnf90_kf_iekf_iter")
+
+    def state(self):
+        info = np.zeros(10)
+        _chk(lib().dnnf90_kf_get(self.id, None, info.ctypes.data), "dnnf90_kf_get")
+        keys = ("lam", "lam0", "gate", "q", "ngated", "nw", "nb", "maxb", "bad", "bytes")
+        return {k: (int(v) if k in ("ngated", "nw", "nb", "maxb", "bad") else float(v)) for k, v in zip(keys, info)}
+
+    def covariance(self):
+        """P(nw, nw) or Pb(maxb, maxb, nb), flattened column-major as the reference holds it."""
+        st = self.state()
+        n = st["maxb"] * st["maxb"] * st["nb"] if self.decoupled else st["nw"] * st["nw"]
+        P = np.zeros(n)
+        _chk(lib().dnnf90_kf_get(self.id, P.ctypes.data, None), "dnnf90_kf_get")
+        return P
+
+    def free(self):
+        if getattr(self, "id", 0):
+            lib().dnnf90_kf_free(self.id)
+            self.id = 0
+
+
+class Committee:
+    """Members of one shape (lib/committee_module.f90); eval returns the mean
+    and the sample standard deviation of every slot (comm_eval)."""
+
+    def __init__(self, nets):
+        self.nets = list(nets)
+        self.ids = np.ascontiguousarray([n.id for n in self.nets], np.int32)
+
+    def eval(self, bw: "BatchWork", x):
+        xk, px, dev = _ptr(x, np.float64)
+        n0 = self.nets[0]
+        shp = (int(x.shape[0]), n0.dims[-1], n0.na)
+        if dev:
+            import torch
+            m = torch.empty(shp, dtype=torch.float64, device=x.device)
+            s = torch.empty(shp, dtype=torch.float64, device=x.device)
+            pm, ps = m.data_ptr(), s.data_ptr()
+        else:
+            m, s = np.zeros(shp), np.zeros(shp)
+            pm, ps = m.ctypes.data, s.ctypes.data
+        _chk(lib().dnnf90_comm_eval_batch(len(self.nets), self.ids.ctypes.data, bw.id, shp[0], px, pm, ps,
+                                          DEVICE_PTRS if dev else 0), "dnnf90_comm_eval_batch")
+        return m, s

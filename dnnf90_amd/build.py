@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdnnf90_amd.so")
-SOURCES = ["tables.cpp", "codegen.cpp", "bell.hip", "gemm.hip", "gemm_x6.hip", "ngd.hip", "misc.hip", "engine.hip"]
+SOURCES = ["tables.cpp", "codegen.cpp", "bell.hip", "gemm.hip", "gemm_x6.hip", "ngd.hip", "misc.hip", "kalman.hip", "engine.hip"]
 ARCH = os.environ.get("DNNF90_ARCH", "gfx950")
 
 

@@ -275,6 +275,8 @@ struct BatchWork {
     double *ngd_J = nullptr;         // dual route: rows row-contiguous, FP64
     double *ngd_v = nullptr;         // dual route: b, J b, R y
     long ngd_J_cap = 0, ngd_v_cap = 0;
+    double *comm_buf = nullptr;      // committee: running mean and M2
+    long comm_cap = 0;
     int *ngd_info = nullptr;
     std::vector<double> h_coeff;
     std::vector<int> h_slot, h_nonlin;
@@ -319,7 +321,7 @@ struct BatchWork {
         dfree(partial); dfree(d_x); dfree(d_seed); dfree(d_tout); dfree(d_rsq); dfree(d_rsq_part);
         dfree(d_coeff); dfree(d_slot); dfree(d_nonlin);
         dfree(ngd_G); dfree(ngd_part); dfree(ngd_u); dfree(ngd_trace); dfree(ngd_info); dfree(ngd_work);
-        dfree(ngd_J); dfree(ngd_v);
+        dfree(ngd_J); dfree(ngd_v); dfree(comm_buf);
         if (own) (void)hipStreamDestroy(own);
         own = stream = nullptr;
     }
@@ -825,6 +827,198 @@ static int run_residual(Net &nt, BatchWork &b, int npt, const double *x, int nte
         HIPCHK(hipStreamSynchronize(b.stream));
     }
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+
+// committee buffers: running mean and M2 of len values each
+static int grow_comm(BatchWork &b, long len)
+{
+    if (2 * len <= b.comm_cap) return 0;
+    HIPCHK(hipStreamSynchronize(b.stream));
+    dfree(b.comm_buf);
+    b.comm_cap = 0;
+    int rc = dalloc(&b.comm_buf, (size_t)(2 * len));
+    if (!rc) b.comm_cap = 2 * len;
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// extended Kalman filter (lib/kalman_module.f90).  The filter's row is the
+// gradient vector in this library's weight order, which is the reference's
+// flatten order l = 2..nlayer, neuron, bias then fan-in (:378-386).
+
+struct KFilter {
+    int nw = 0, decoupled = 0;
+    int nt = 0;                       // dense: tiles per side
+    int nb = 0, maxb = 0;             // decoupled: blocks, largest block
+    long plen = 0;                    // doubles held for P
+    double *P = nullptr;
+    std::vector<int> bsz, boff;
+    std::vector<long> poff;
+    long *d_poff = nullptr;
+    int *d_bsz = nullptr, *d_boff = nullptr;
+    double *pj = nullptr, *kv = nullptr, *part = nullptr, *g = nullptr, *wprior = nullptr, *r = nullptr;
+    double *d_x = nullptr, *d_seed = nullptr, *d_r0 = nullptr;
+    long x_cap = 0, seed_cap = 0, r0_cap = 0;
+    KfScalars *st = nullptr;
+    bool prior = false;
+    void release()
+    {
+        dfree(P); dfree(d_poff); dfree(d_bsz); dfree(d_boff); dfree(pj); dfree(kv); dfree(part); dfree(g);
+        dfree(wprior); dfree(r); dfree(d_x); dfree(d_seed); dfree(d_r0); dfree(st);
+    }
+};
+
+static Registry<KFilter> g_kf;
+
+static bool kf_ok(int kid, int nid)
+{
+    return g_kf.get(kid) && g_kf.owner[kid] == nid && g_kf.ogen[kid] == g_net.gen[nid];
+}
+
+// grow a device buffer to n doubles
+static int kf_grow(double **p, long *cap, long n)
+{
+    if (n <= *cap) return 0;
+    dfree(*p);
+    *cap = 0;
+    int rc = dalloc(p, (size_t)std::max(1L, n));
+    if (!rc) *cap = n;
+    return rc;
+}
+
+// kf_init (kalman_module.f90:107-167, kf_block_map :171-185)
+static int kf_build(const Net &nt, KFilter &kf, double p0, double lam_ini, double lam0, int decoupled)
+{
+    kf.nw = nt.nw;
+    kf.decoupled = decoupled != 0;
+    int rc;
+    const long padw = kf.decoupled ? kf.nw : (long)round_up(std::max(kf.nw, 1), 64);
+    if (!kf.decoupled) {
+        kf.nt = (int)(padw / 64);
+        kf.plen = (long)kf.nt * (kf.nt + 1) / 2 * 64 * 64;
+        if ((rc = dalloc(&kf.P, (size_t)kf.plen)) || (rc = dalloc(&kf.part, (size_t)kf.nt * padw))) return rc;
+        launch_kf_init_tiles(kf.P, kf.nt, kf.nw, p0, 0);
+    } else {
+        long m = 0, po = 0;
+        for (int l = 1; l < nt.nlayer; ++l)
+            for (int j = 0; j < nt.dims[l]; ++j) {
+                const int n = nt.dims[l - 1] + 1;
+                kf.bsz.push_back(n);
+                kf.boff.push_back((int)m);
+                kf.poff.push_back(po);
+                m += n;
+                po += (long)n * n;
+                kf.maxb = std::max(kf.maxb, n);
+            }
+        kf.nb = (int)kf.bsz.size();
+        kf.plen = po;
+        if (kf.maxb > 8192) return fail(-1, "decoupled filter blocks are limited to fan-in 8191");
+        if ((rc = dalloc(&kf.P, (size_t)kf.plen)) || (rc = upload(&kf.d_poff, kf.poff)) ||
+            (rc = upload(&kf.d_bsz, kf.bsz)) || (rc = upload(&kf.d_boff, kf.boff)))
+            return rc;
+        launch_kf_init_blocks(kf.P, kf.d_poff, kf.d_bsz, kf.nb, p0, 0);
+    }
+    if ((rc = dalloc(&kf.pj, (size_t)padw)) || (rc = dalloc(&kf.kv, (size_t)padw)) ||
+        (rc = dalloc(&kf.g, (size_t)std::max(1, kf.nw))) || (rc = dalloc(&kf.r, 1)) || (rc = dalloc(&kf.st, 1)))
+        return rc;
+    launch_fill(kf.pj, padw, 0.0, 0);
+    launch_fill(kf.kv, padw, 0.0, 0);
+    KfScalars h{};
+    h.lam = lam_ini;
+    h.lam0 = lam0;
+    h.gs = 1.0;
+    HIPCHK(hipMemcpy(kf.st, &h, sizeof(h), hipMemcpyHostToDevice));
+    HIPCHK(hipDeviceSynchronize());
+    return 0;
+}
+
+// the pass over P: the pending downdate, then (g) pj = P (scale g)
+static void kf_pass(KFilter &kf, const double *g, double scale, hipStream_t s)
+{
+    if (!kf.decoupled) {
+        launch_kf_tiles(kf.P, kf.nt, kf.nw, g, scale, kf.kv, kf.pj, kf.st, g != nullptr, kf.part, s);
+        if (g) launch_kf_sum_part(kf.part, kf.nt, kf.pj, s);
+    } else {
+        launch_kf_blocks(kf.P, kf.d_poff, kf.d_bsz, kf.d_boff, kf.nb, kf.maxb, g, scale, kf.kv, kf.pj, kf.st,
+                         g != nullptr, s);
+    }
+}
+
+// one update from the row g (kf_update_grad :353-493, kf_iekf_iter :532-648)
+static void kf_step(Net &nt, KFilter &kf, const double *g, double scale, int xmode, double v, const double *seed,
+                    const double *T, int nsl, bool iekf, bool final, hipStream_t s)
+{
+    kf_pass(kf, g, scale, s);
+    launch_kf_gain(kf.st, g, scale, kf.pj, kf.nw, xmode, v, seed, T, nsl, kf.r, kf.wprior, nt.d_w64, final, s);
+    launch_kf_apply(kf.st, kf.pj, kf.kv, nt.d_w64, kf.wprior, kf.nw, iekf, s);
+}
+
+// one observation at the device point dx: the row dy/dw from the seed (or
+// the collocation residual's dR/dT, res), then the update
+static void kf_observe(Net &nt, BatchWork &b, KFilter &kf, const double *dx, const double *dseed,
+                       const DevResidual *res, const double *r0, int xmode, double v, double scale, bool iekf,
+                       bool final)
+{
+    hipStream_t s = b.stream;
+    const int nout = nt.dims[nt.nlayer - 1], na = nt.tab.na;
+    launch_fill(kf.g, kf.nw, 0.0, s);
+    forward_t<double>(nt, b, dx, 1);
+    if (xmode == KF_XI_LINEAR) launch_output<double>((const double *)b.tl, nout, na, 1, b.np, b.d_tout, s);
+    if (res)
+        launch_seed_residual<double>((const double *)b.tl, nout, na, 1, b.np, *res, 1.0, true, (double *)b.p0,
+                                     b.d_rsq_part, s, r0, kf.r);
+    else
+        launch_seed_user<double>(dseed, 1, nout, na, (double *)b.p0, b.np, s);
+    backward_t<double>(nt, b, kf.g, nullptr, 0);
+    kf_step(nt, kf, kf.g, scale, xmode, v, dseed, b.d_tout, nout * na, iekf, final, s);
+}
+
+// after the updates: weights back to the host copy, the stream drained and
+// the filter's state checked
+static int kf_finish(Net &nt, KFilter &kf, hipStream_t s)
+{
+    KfScalars h;
+    HIPCHK(hipMemcpyAsync(&h, kf.st, sizeof(h), hipMemcpyDeviceToHost, s));
+    {
+        std::lock_guard<std::mutex> lk(nt.mu);
+        HIPCHK(hipMemcpyAsync(nt.w.data(), nt.d_w64, sizeof(double) * nt.nw, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        nt.host_ver++;
+        nt.dev_ver = nt.host_ver;   // the device copy is the updated one
+    }
+    HIPCHK(hipGetLastError());
+    if (h.bad)
+        return fail(-5, "nonpositive innovation denominator: the covariance has lost positive definiteness");
+    return 0;
+}
+
+// what an observation sequence needs: an FP64 work space, the filter's net,
+// a sound filter, the points (and seeds) on the device
+static int kf_begin(Net &nt, BatchWork &b, KFilter &kf, int nobs, const double *x, const double *seed,
+                    bool dev, const double **dx, const double **dseed)
+{
+    if (b.prec != PREC_FP64) return fail(-1, "the filter runs on an FP64 work space");
+    if (nobs < 0 || (nobs > 0 && !x)) return fail(-1, "bad observation arguments");
+    int rc = nt.ensure_device(false);
+    if (rc) return rc;
+    KfScalars h;
+    HIPCHK(hipMemcpy(&h, kf.st, sizeof(h), hipMemcpyDeviceToHost));
+    if (h.bad) return fail(-5, "the filter met a nonpositive denominator earlier; create it again");
+    const int d0 = nt.dims[0], nsl = nt.dims[nt.nlayer - 1] * nt.tab.na;
+    *dx = x;
+    if (dseed) *dseed = seed;
+    if (!dev && nobs > 0) {
+        if ((rc = kf_grow(&kf.d_x, &kf.x_cap, (long)nobs * d0))) return rc;
+        HIPCHK(hipMemcpyAsync(kf.d_x, x, sizeof(double) * nobs * d0, hipMemcpyHostToDevice, b.stream));
+        *dx = kf.d_x;
+        if (seed && dseed) {
+            if ((rc = kf_grow(&kf.d_seed, &kf.seed_cap, (long)nobs * nsl))) return rc;
+            HIPCHK(hipMemcpyAsync(kf.d_seed, seed, sizeof(double) * nobs * nsl, hipMemcpyHostToDevice, b.stream));
+            *dseed = kf.d_seed;
+        }
+    }
     return 0;
 }
 
@@ -1522,6 +1716,280 @@ int dnnf90_residual_rows_batch(int nid, int bid, int npt, const double *x, int n
                                        rows_t, ld, dev);
     if (rc) return rc;
     if (!dev) HIPCHK(hipStreamSynchronize(b.stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// extended Kalman filter and committee (include/dnnf90_amd.h)
+
+int dnnf90_kf_create(int nid, double p0, double lam_ini, double lam0, int decoupled)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt) return fail(-1, "bad network handle");
+    if (!(p0 > 0.0)) return fail(-1, "p0 must be positive");
+    if (!(lam_ini > 0.0 && lam_ini <= 1.0 && lam0 > 0.0 && lam0 <= 1.0))
+        return fail(-1, "forgetting parameters must lie in (0, 1]");
+    int id = g_kf.slot();
+    if (id < 0) return fail(-1, "no free filter handle");
+    KFilter *kf = new KFilter();
+    int rc = kf_build(*nt, *kf, p0, lam_ini, lam0, decoupled);
+    if (rc) {
+        kf->release();
+        delete kf;
+        return rc;
+    }
+    g_kf.obj[id] = kf;
+    g_kf.owner[id] = nid;
+    g_kf.ogen[id] = g_net.gen[nid];
+    return id;
+}
+
+int dnnf90_kf_set(int kid, double gate, double q)
+{
+    KFilter *kf = g_kf.get(kid);
+    if (!kf) return fail(-1, "bad filter handle");
+    if (gate < 0.0 || q < 0.0) return fail(-1, "gate and q must be >= 0");
+    HIPCHK(hipDeviceSynchronize());
+    KfScalars h;
+    HIPCHK(hipMemcpy(&h, kf->st, sizeof(h), hipMemcpyDeviceToHost));
+    h.gate = gate;
+    h.q = q;
+    HIPCHK(hipMemcpy(kf->st, &h, sizeof(h), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int dnnf90_kf_free(int kid)
+{
+    KFilter *kf = g_kf.get(kid);
+    if (!kf) return fail(-1, "bad filter handle");
+    (void)hipDeviceSynchronize();
+    kf->release();
+    delete kf;
+    g_kf.obj[kid] = nullptr;
+    return 0;
+}
+
+int dnnf90_kf_update_grad(int nid, int kid, int gid, double xi, double scale)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt || !kf_ok(kid, nid) || !grad_ok(gid, nid)) return fail(-1, "bad handle");
+    KFilter &kf = *g_kf.obj[kid];
+    int rc = nt->ensure_device(false);
+    if (rc) return rc;
+    HIPCHK(hipDeviceSynchronize());
+    kf_step(*nt, kf, g_grad.obj[gid]->d_g, scale, KF_XI_GIVEN, xi, nullptr, nullptr, 0, false, true, 0);
+    return kf_finish(*nt, kf, 0);
+}
+
+int dnnf90_kf_update_obs(int nid, int kid, int bid, int nobs, const double *x, const double *seed,
+                         const double *target, const double *pred, int flags)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt || !kf_ok(kid, nid) || !dependent_ok(g_bwork, bid, nid) || (nobs > 0 && (!seed || !target)))
+        return fail(-1, "bad handle or argument");
+    KFilter &kf = *g_kf.obj[kid];
+    BatchWork &b = *g_bwork.obj[bid];
+    const double *dx, *ds;
+    int rc = kf_begin(*nt, b, kf, nobs, x, seed, flags & DNNF90_DEVICE_PTRS, &dx, &ds);
+    if (rc) return rc;
+    const int d0 = nt->dims[0], nsl = nt->dims[nt->nlayer - 1] * nt->tab.na;
+    for (int n = 0; n < nobs; ++n) {
+        // xi = target - pred, pred either given or the linear functional seed.T
+        const int xm = pred ? KF_XI_GIVEN : KF_XI_LINEAR;
+        const double v = pred ? target[n] - pred[n] : target[n];
+        kf_observe(*nt, b, kf, dx + (long)n * d0, ds + (long)n * nsl, nullptr, nullptr, xm, v, 1.0, false, true);
+    }
+    b.fwd_valid = false;
+    return kf_finish(*nt, kf, b.stream);
+}
+
+int dnnf90_kf_update_resid(int nid, int kid, int bid, int nobs, const double *x, const double *seed,
+                           const double *resid, const double *rnoise, int flags)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt || !kf_ok(kid, nid) || !dependent_ok(g_bwork, bid, nid) || (nobs > 0 && (!seed || !resid)))
+        return fail(-1, "bad handle or argument");
+    for (int n = 0; rnoise && n < nobs; ++n)
+        if (!(rnoise[n] > 0.0)) return fail(-1, "observation noise must be > 0");
+    KFilter &kf = *g_kf.obj[kid];
+    BatchWork &b = *g_bwork.obj[bid];
+    const double *dx, *ds;
+    int rc = kf_begin(*nt, b, kf, nobs, x, seed, flags & DNNF90_DEVICE_PTRS, &dx, &ds);
+    if (rc) return rc;
+    const int d0 = nt->dims[0], nsl = nt->dims[nt->nlayer - 1] * nt->tab.na;
+    for (int n = 0; n < nobs; ++n) {
+        const double sc = rnoise ? 1.0 / std::sqrt(rnoise[n]) : 1.0;
+        kf_observe(*nt, b, kf, dx + (long)n * d0, ds + (long)n * nsl, nullptr, nullptr, KF_XI_GIVEN, -resid[n], sc,
+                   false, true);
+    }
+    b.fwd_valid = false;
+    return kf_finish(*nt, kf, b.stream);
+}
+
+int dnnf90_kf_update_residual(int nid, int kid, int bid, int nobs, const double *x, int nterm,
+                              const double *coeff, const int *slot, const int *nonlin, const double *rsrc,
+                              int flags)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt || !kf_ok(kid, nid) || !dependent_ok(g_bwork, bid, nid) || (nterm > 0 && (!coeff || !slot)))
+        return fail(-1, "bad handle or argument");
+    KFilter &kf = *g_kf.obj[kid];
+    BatchWork &b = *g_bwork.obj[bid];
+    if (b.prec != PREC_FP64) return fail(-1, "the filter runs on an FP64 work space");
+    DevResidual res;
+    int rc = set_residual(b, nterm, coeff, slot, nonlin, nt->tab.na, &res);
+    if (rc) return rc;
+    const double *dx;
+    rc = kf_begin(*nt, b, kf, nobs, x, nullptr, flags & DNNF90_DEVICE_PTRS, &dx, nullptr);
+    if (rc) return rc;
+    const double *dr0 = rsrc;
+    if (rsrc && !(flags & DNNF90_DEVICE_PTRS) && nobs > 0) {
+        if ((rc = kf_grow(&kf.d_r0, &kf.r0_cap, nobs))) return rc;
+        HIPCHK(hipMemcpyAsync(kf.d_r0, rsrc, sizeof(double) * nobs, hipMemcpyHostToDevice, b.stream));
+        dr0 = kf.d_r0;
+    }
+    const int d0 = nt->dims[0];
+    for (int n = 0; n < nobs; ++n)
+        kf_observe(*nt, b, kf, dx + (long)n * d0, nullptr, &res, dr0 ? dr0 + n : nullptr, KF_XI_RESID, 0.0, 1.0,
+                   false, true);
+    b.fwd_valid = false;
+    return kf_finish(*nt, kf, b.stream);
+}
+
+int dnnf90_kf_iekf_begin(int nid, int kid)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt || !kf_ok(kid, nid)) return fail(-1, "bad handle");
+    KFilter &kf = *g_kf.obj[kid];
+    int rc = nt->ensure_device(false);
+    if (rc) return rc;
+    if (!kf.wprior && (rc = dalloc(&kf.wprior, (size_t)std::max(1, kf.nw)))) return rc;
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(kf.wprior, nt->d_w64, sizeof(double) * kf.nw, hipMemcpyDeviceToDevice));
+    kf.prior = true;
+    return 0;
+}
+
+int dnnf90_kf_iekf_iter(int nid, int kid, int bid, const double *x, const double *seed, double resid,
+                        double rnoise, int final, int flags)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt || !kf_ok(kid, nid) || !dependent_ok(g_bwork, bid, nid) || !seed)
+        return fail(-1, "bad handle or argument");
+    KFilter &kf = *g_kf.obj[kid];
+    if (!kf.prior) return fail(-1, "dnnf90_kf_iekf_begin first");
+    if (!(rnoise > 0.0)) return fail(-1, "observation noise must be > 0");
+    BatchWork &b = *g_bwork.obj[bid];
+    const double *dx, *ds;
+    int rc = kf_begin(*nt, b, kf, 1, x, seed, flags & DNNF90_DEVICE_PTRS, &dx, &ds);
+    if (rc) return rc;
+    kf_observe(*nt, b, kf, dx, ds, nullptr, nullptr, KF_XI_IEKF, resid, 1.0 / std::sqrt(rnoise), true,
+               final != 0);
+    b.fwd_valid = false;
+    return kf_finish(*nt, kf, b.stream);
+}
+
+int dnnf90_kf_get(int kid, double *P, double *info)
+{
+    KFilter *kfp = g_kf.get(kid);
+    if (!kfp) return fail(-1, "bad filter handle");
+    KFilter &kf = *kfp;
+    HIPCHK(hipDeviceSynchronize());
+    KfScalars h;
+    HIPCHK(hipMemcpy(&h, kf.st, sizeof(h), hipMemcpyDeviceToHost));
+    if (P) {
+        if (h.pending && !h.bad) {   // settle the downdate the last update left owing
+            kf_pass(kf, nullptr, 1.0, 0);
+            launch_kf_clear(kf.st, 0);
+            HIPCHK(hipDeviceSynchronize());
+        }
+        std::vector<double> hp((size_t)kf.plen);
+        HIPCHK(hipMemcpy(hp.data(), kf.P, sizeof(double) * kf.plen, hipMemcpyDeviceToHost));
+        if (!kf.decoupled) {
+            // P(nw, nw) column-major (symmetric), from the lower tiles
+            const long n = kf.nw;
+            for (long c = 0; c < n; ++c)
+                for (long r = 0; r < n; ++r) {
+                    const long gr = std::max(r, c), gc = std::min(r, c);
+                    const long I = gr / 64, J = gc / 64;
+                    P[c * n + r] = hp[(size_t)((I * (I + 1) / 2 + J) * 4096 + (gr % 64) * 64 + gc % 64)];
+                }
+        } else {
+            // Pb(maxb, maxb, nb) column-major, zero outside each block
+            const long mb = kf.maxb;
+            std::fill(P, P + mb * mb * kf.nb, 0.0);
+            for (int b = 0; b < kf.nb; ++b)
+                for (long c = 0; c < kf.bsz[b]; ++c)
+                    for (long r = 0; r < kf.bsz[b]; ++r)
+                        P[b * mb * mb + c * mb + r] = hp[(size_t)(kf.poff[b] + r * kf.bsz[b] + c)];
+        }
+    }
+    if (info) {
+        info[0] = h.lam;
+        info[1] = h.lam0;
+        info[2] = h.gate;
+        info[3] = h.q;
+        info[4] = h.ngated;
+        info[5] = kf.nw;
+        info[6] = kf.decoupled ? kf.nb : 0;
+        info[7] = kf.decoupled ? kf.maxb : 0;
+        info[8] = h.bad;
+        info[9] = (double)kf.plen * sizeof(double);
+    }
+    return 0;
+}
+
+int dnnf90_comm_eval_batch(int nmem, const int *nids, int bid, int npt, const double *x, double *tmean,
+                           double *tstd, int flags)
+{
+    if (nmem < 1 || !nids || (npt > 0 && (!x || !tmean || !tstd))) return fail(-1, "bad argument");
+    Net *n0 = g_net.get(nids[0]);
+    if (!n0 || !dependent_ok(g_bwork, bid, nids[0])) return fail(-1, "bad handle");
+    BatchWork &b = *g_bwork.obj[bid];
+    if (npt < 0 || npt > b.maxpt) return fail(-1, "npt outside [0, maxpt]");
+    std::vector<Net *> mem(nmem);
+    for (int k = 0; k < nmem; ++k) {
+        Net *nk = g_net.get(nids[k]);
+        if (!nk) return fail(-1, "bad member handle");
+        // members share the shape and the table family (committee_module.f90:44-50)
+        if (nk->dims != n0->dims || nk->tab.alpha != n0->tab.alpha || nk->tab.iact != n0->tab.iact)
+            return fail(-1, "committee members differ in shape or tables");
+        mem[k] = nk;
+    }
+    const bool dev = flags & DNNF90_DEVICE_PTRS;
+    const int nout = n0->dims[n0->nlayer - 1], na = n0->tab.na;
+    const long len = (long)npt * nout * na;
+    hipStream_t s = b.stream;
+    int rc;
+    for (Net *nk : mem)
+        if ((rc = nk->ensure_device(b.prec != PREC_FP64))) return rc;
+    if (len > 0 && (rc = grow_comm(b, len))) return rc;
+    double *mean = dev ? tmean : b.comm_buf, *m2 = dev ? tstd : b.comm_buf + len;
+    const double *dx = x;
+    if (!dev && npt > 0) {
+        HIPCHK(hipMemcpyAsync(b.d_x, x, sizeof(double) * npt * n0->dims[0], hipMemcpyHostToDevice, s));
+        dx = b.d_x;
+    }
+    launch_fill(mean, len, 0.0, s);
+    launch_fill(m2, len, 0.0, s);
+    for (int k = 0; k < nmem; ++k) {
+        if (b.prec == PREC_FP64) {
+            forward_t<double>(*mem[k], b, dx, npt);
+            launch_output<double>((const double *)b.tl, nout, na, npt, b.np, b.d_tout, s);
+        } else {
+            forward_t<float>(*mem[k], b, dx, npt);
+            launch_output<float>((const float *)b.tl, nout, na, npt, b.np, b.d_tout, s);
+        }
+        launch_comm_welford(b.d_tout, mean, m2, len, k + 1, s);
+    }
+    launch_comm_finish(m2, len, nmem, s);
+    if (!dev && len > 0) {
+        HIPCHK(hipMemcpyAsync(tmean, mean, sizeof(double) * len, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(tstd, m2, sizeof(double) * len, hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipGetLastError());
+    b.fwd_valid = false;
     return 0;
 }
 

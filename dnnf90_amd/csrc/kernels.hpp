@@ -156,12 +156,14 @@ struct DevResidual {
     const int *nonlin;
 };
 
-// R_n = sum_k c_k [T_0 *] T_slot (output 0); seed = fac*R*dR/dT on output 0
-// (unit_r: seed = fac*dR/dT, the Gauss-Newton row seed); rsq_part[block] =
-// sum of R^2 over the block's points; rows > 0 zeroed
+// R_n = [r0_n +] sum_k c_k [T_0 *] T_slot (output 0); seed = fac*R*dR/dT on
+// output 0 (unit_r: seed = fac*dR/dT, the Gauss-Newton row seed);
+// rsq_part[block] = sum of R^2 over the block's points; rows > 0 zeroed;
+// r0 (optional) starts R_n, r_out (optional) receives R_n
 template <typename T>
 void launch_seed_residual(const T *TL, int nout, int na, int npt, long Np, const DevResidual &r,
-                          double fac, bool unit_r, T *Tb, double *rsq_part, hipStream_t s);
+                          double fac, bool unit_r, T *Tb, double *rsq_part, hipStream_t s,
+                          const double *r0 = nullptr, double *r_out = nullptr);
 
 template <typename T>
 void launch_output(const T *TL, int nout, int na, int npt, long Np, double *tout, hipStream_t s);
@@ -173,5 +175,51 @@ void launch_adam(double *w, double *m, double *v, const double *g, long n, doubl
                  double bc2, hipStream_t s);
 void launch_fill(double *p, long n, double v, hipStream_t s);
 void launch_axpy(double *y, const double *x, long n, double a, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// extended Kalman filter (kalman.hip)
+
+// filter scalars, resident on the device so that a sequence of observations
+// runs without a host round trip
+struct KfScalars {
+    double lam, lam0, gate, q;
+    double lam_dd;   // lambda of the downdate still owed to P
+    double xi;       // innovation of the current observation, scaled
+    double denom;    // lambda + j.P j of the current observation
+    double gs;       // gate scale of the current observation
+    int pending;     // P owes the downdate (kv, pj, lam_dd)
+    int ngated;
+    int gated;       // the current observation tripped the gate
+    int bad;         // a nonpositive denominator was met
+};
+
+// how k_kf_gain forms the innovation xi_in before xi = scale * xi_in
+enum {
+    KF_XI_GIVEN = 0,    // xi_in = v
+    KF_XI_LINEAR = 1,   // xi_in = v - seed.T (target minus the linear prediction)
+    KF_XI_RESID = 2,    // xi_in = v - R, R from the device
+    KF_XI_IEKF = 3,     // xi = -scale v - j.(wprior - w)
+};
+
+void launch_kf_init_tiles(double *P, int nt, int nw, double p0, hipStream_t s);
+void launch_kf_init_blocks(double *P, const long *poff, const int *bsz, int nb, double p0, hipStream_t s);
+// dense pass: pending downdate, then (want) the shares of P (scale g) in part[nt][nt*64]
+void launch_kf_tiles(double *P, int nt, int nw, const double *g, double scale, const double *kv,
+                     const double *pj, const KfScalars *st, bool want, double *part, hipStream_t s);
+void launch_kf_sum_part(const double *part, int nt, double *pj, hipStream_t s);
+// decoupled pass: pending downdate and (want) pj = P (scale g), per block
+void launch_kf_blocks(double *P, const long *poff, const int *bsz, const int *boff, int nb, int maxb,
+                      const double *g, double scale, const double *kv, double *pj, const KfScalars *st, bool want,
+                      hipStream_t s);
+void launch_kf_gain(KfScalars *st, const double *g, double scale, const double *pj, int nw, int xmode, double v,
+                    const double *seed, const double *T, int nsl, const double *rv, const double *wprior,
+                    const double *w, bool final, hipStream_t s);
+void launch_kf_apply(const KfScalars *st, double *pj, double *kv, double *w, const double *wprior, int nw,
+                     bool iekf, hipStream_t s);
+void launch_kf_clear(KfScalars *st, hipStream_t s);
+
+// committee: member k (1-based) folded into mean and m2; then m2 <- std
+void launch_comm_welford(const double *t, double *mean, double *m2, long n, int k, hipStream_t s);
+void launch_comm_finish(double *m2, long n, int nmem, hipStream_t s);
 
 }  // namespace dnnf90

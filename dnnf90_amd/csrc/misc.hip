@@ -55,7 +55,9 @@ __global__ __launch_bounds__(256) void k_seed_residual(const T *__restrict__ TL,
                                                        int npt, long Np, DevResidual rs,
                                                        double fac, int unit_r,
                                                        T *__restrict__ Tb,
-                                                       double *__restrict__ rsq_part)
+                                                       double *__restrict__ rsq_part,
+                                                       const double *__restrict__ r0,
+                                                       double *__restrict__ r_out)
 {
     __shared__ double red[256];
     const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -66,12 +68,13 @@ __global__ __launch_bounds__(256) void k_seed_residual(const T *__restrict__ TL,
         for (int a = 0; a < na; ++a) Tb[(long)a * Np + n] = T(0);
         if (n < npt) {
             const double t0 = (double)TL[n];
-            double R = 0.0;
+            double R = r0 ? r0[n] : 0.0;
             for (int k = 0; k < rs.nterm; ++k) {
                 const double tv = (double)TL[(long)rs.slot[k] * Np + n];
                 R += rs.nonlin[k] ? rs.coeff[k] * t0 * tv : rs.coeff[k] * tv;
             }
             r2 = R * R;
+            if (r_out) r_out[n] = R;
             if (unit_r) R = 1.0;  // Gauss-Newton row seed dR/dT
             double s0 = 0.0;
             for (int k = 0; k < rs.nterm; ++k) {
@@ -99,10 +102,11 @@ __global__ __launch_bounds__(256) void k_seed_residual(const T *__restrict__ TL,
 
 template <typename T>
 void launch_seed_residual(const T *TL, int nout, int na, int npt, long Np, const DevResidual &r,
-                          double fac, bool unit_r, T *Tb, double *rsq_part, hipStream_t s)
+                          double fac, bool unit_r, T *Tb, double *rsq_part, hipStream_t s,
+                          const double *r0, double *r_out)
 {
     hipLaunchKernelGGL((k_seed_residual<T>), dim3((unsigned)((Np + 255) / 256)), dim3(256), 0, s, TL,
-                       nout, na, npt, Np, r, fac, unit_r ? 1 : 0, Tb, rsq_part);
+                       nout, na, npt, Np, r, fac, unit_r ? 1 : 0, Tb, rsq_part, r0, r_out);
 }
 
 template <typename T>
@@ -227,7 +231,7 @@ void launch_seed_value(const T *Z, const double *y, double coef, int npt, long N
     template void launch_input<T>(const double *, int, int, int, const int *, T *, long, hipStream_t); \
     template void launch_seed_user<T>(const double *, int, int, int, T *, long, hipStream_t);         \
     template void launch_seed_residual<T>(const T *, int, int, int, long, const DevResidual &, double, \
-                                          bool, T *, double *, hipStream_t);                         \
+                                          bool, T *, double *, hipStream_t, const double *, double *); \
     template void launch_output<T>(const T *, int, int, int, long, double *, hipStream_t);              \
     template void launch_seed_value<T>(const T *, const double *, double, int, long, T *, hipStream_t);
 INST(double)

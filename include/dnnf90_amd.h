@@ -161,6 +161,73 @@ int dnnf90_ngd_solve(int nid, int bid, int nrow, const float *rows_t, long ld,
                      const double *b, int nb, double mu, int mode, double *u,
                      int flags);
 
+/* ---- extended Kalman filter over the weights (lib/kalman_module.f90) ----
+ * The covariance lives on the device: dense, as the lower triangle of P
+ * (nw (nw + 64) / 2 doubles, no size guard beyond device memory; the
+ * reference stops above 4 GB, :133-141), or decoupled, one (fan-in + 1)^2
+ * block per neuron (:71-83, fan-in <= 8191 here).  The row of an
+ * observation is dy/dw in this library's weight order, which is the
+ * reference's flatten order (:378-386).  Observations are processed in
+ * order, entirely on the device; the weights move on the device and are
+ * copied back to the network once per call.  The observation entries take
+ * an FP64 batched work space bid of the same network and return -5 (where
+ * the reference stops, :399-405) when a denominator lambda + j.Pj is not
+ * positive; the filter then refuses further updates. */
+
+/* kf_init (:107-167): p0 > 0, lam_ini and lam0 in (0, 1]; decoupled != 0
+ * for the node-wise filter.  Returns a filter handle of network nid. */
+int dnnf90_kf_create(int nid, double p0, double lam_ini, double lam0, int decoupled);
+/* the innovation gate and the process noise q (kalman_t%gate :62, %q :70,
+ * set after kf_init by app/lib_net_module.f90:474-475); 0 = off */
+int dnnf90_kf_set(int kid, double gate, double q);
+int dnnf90_kf_free(int kid);
+/* kf_update_grad (:353-493): one update from the row held by the gradient
+ * accumulator gid, residual xi = target - prediction, row and residual
+ * scaled by scale */
+int dnnf90_kf_update_grad(int nid, int kid, int gid, double xi, double scale);
+/* kf_update_obs (:309-334) and kf_update (:201-218, a one-hot seed) over
+ * nobs observations x[nobs][d0], seed[nobs][nout][na] (the reference's
+ * form is nout = 1): xi = target[n] - pred[n], or, pred == NULL,
+ * target[n] - seed.T(x) at the running weights.  x and seed are device
+ * pointers with DNNF90_DEVICE_PTRS; target and pred are host arrays. */
+int dnnf90_kf_update_obs(int nid, int kid, int bid, int nobs, const double *x,
+                         const double *seed, const double *target,
+                         const double *pred, int flags);
+/* kf_update_resid_multi (:280-307) and kf_update_resid (:240-254): seed =
+ * dR/dT of every output, xi = -resid[n], row and residual scaled by
+ * 1/sqrt(rnoise[n]) (rnoise == NULL: 1).  resid, rnoise on the host. */
+int dnnf90_kf_update_resid(int nid, int kid, int bid, int nobs, const double *x,
+                           const double *seed, const double *resid,
+                           const double *rnoise, int flags);
+/* the trainer's collocation observation (app/sgd_batch_module.f90:547-550):
+ * R = rsrc[n] + sum_k coeff_k [T_0 *] T_slot_k on output 0 (calc_pinn_residual,
+ * app/pinn_module.f90:32-47; slots 1-based as in dnnf90_residual_step_batch;
+ * rsrc may be NULL), formed on the device at the running weights, seed
+ * dR/dT (set_pinn_seed with R = fac = 1), then kf_update_resid(x, seed, R). */
+int dnnf90_kf_update_residual(int nid, int kid, int bid, int nobs, const double *x,
+                              int nterm, const double *coeff, const int *slot,
+                              const int *nonlin, const double *rsrc, int flags);
+/* kf_iekf_begin (:512-528) and kf_iekf_iter (:532-648): resid = R at the
+ * current weights (the caller re-evaluates it, e.g. with
+ * dnnf90_forward_batch), seed = dR/dT [nout][na], rnoise > 0; final != 0
+ * downdates the covariance and advances lambda. */
+int dnnf90_kf_iekf_begin(int nid, int kid);
+int dnnf90_kf_iekf_iter(int nid, int kid, int bid, const double *x, const double *seed,
+                        double resid, double rnoise, int final, int flags);
+/* The covariance in the reference's layout -- P(nw, nw) column-major, or
+ * Pb(maxb, maxb, nb) column-major with zeros outside each block -- (P may
+ * be NULL) and info[10] = lambda, lam0, gate, q, gated updates, nw, nb,
+ * maxb (0, 0 for the dense filter), broken flag, bytes held on the device. */
+int dnnf90_kf_get(int kid, double *P, double *info);
+
+/* comm_eval (lib/committee_module.f90:79-111) over npt points: mean and
+ * sample standard deviation (Welford, n - 1 denominator, 0 for a single
+ * member) of every slot of every output over the nmem networks nids, which
+ * share the shape and the table family; tmean, tstd [npt][nout][na].  bid is
+ * a batched work space of nids[0] (its precision is the members'). */
+int dnnf90_comm_eval_batch(int nmem, const int *nids, int bid, int npt, const double *x,
+                           double *tmean, double *tstd, int flags);
+
 #ifdef __cplusplus
 }
 #endif

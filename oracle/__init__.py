@@ -1,9 +1,11 @@
 """ctypes wrappers over the oracle (TEST INFRASTRUCTURE ONLY).
 
 `Oracle` drives liboracle.so, the C FP64 restatement of the reference's
-table build, forward, adjoint, collocation seed and NGD solve
-(lib/multi_index_bell_module.f90:128-351, lib/train_module.f90:241-450,
-app/pinn_module.f90:32-65, app/optimizer_module.f90:221-522).
+table build, forward, adjoint, collocation seed, NGD solve, committee and
+extended Kalman filter (lib/multi_index_bell_module.f90:128-351,
+lib/train_module.f90:241-450, app/pinn_module.f90:32-65,
+app/optimizer_module.f90:221-522, lib/committee_module.f90:79-111,
+lib/kalman_module.f90:107-659).
 
 `RefLib` drives _ref/libdnnf90_ref.so, the reference's own lib/ compiled
 from its sources by `make ref`, through its dnnf90.h C ABI.
@@ -83,12 +85,22 @@ class Oracle:
                                  _pd, _pd, _pd, _i],
                 "ot_pinn_rows": [_i, _i, _pi, _pd, _i, _pd, _i, _pd, _pi, _pi, _pd, _i],
                 "ot_ngd_solve": [_i, _i, _pd, _pd, _i, _d, _i, _pd],
+                "ot_comm_eval": [_i, _i, _pi, _i, _pd, _i, _pd, _pd, _pd],
+                "ot_kf_create": [_i, _pi, _d, _d, _d, _i],
+                "ot_kf_set": [_i, _d, _d],
+                "ot_kf_free": [_i],
+                "ot_kf_update_grad": [_i, _pd, _pd, _d, _d],
+                "ot_kf_iekf_begin": [_i, _pd],
+                "ot_kf_iekf_iter": [_i, _pd, _pd, _d, _d, _i],
+                "ot_kf_state": [_i, _pd, _pd, _pi],
             }.items():
                 f = getattr(L, name)
                 f.argtypes = argt
                 f.restype = _i
             L.ot_act_derivs.argtypes = [_i, _d, _i, _pd]
             L.ot_act_derivs.restype = None
+            L.ot_kf_cov_len.argtypes = [_i]
+            L.ot_kf_cov_len.restype = C.c_long
             cls._lib = L
         return cls._lib
 
@@ -221,6 +233,88 @@ class Oracle:
         return u
 
 
+    def comm_eval(self, dims, W, x):
+        """Committee mean and sample std of every slot, W[nmem][nw]."""
+        dims, W, x = _ia(dims), _da(W), _da(x)
+        shp = (x.shape[0], int(dims[-1]), self.na)
+        m, sd = np.zeros(shp), np.zeros(shp)
+        rc = self.lib().ot_comm_eval(self.h, len(dims), _ip(dims), W.shape[0], _dp(W), x.shape[0],
+                                     _dp(x), _dp(m), _dp(sd))
+        assert rc == 0
+        return m, sd
+
+    def kf_seq(self, dims, w, p0, lam_ini, lam0, decoupled, gate, q, mode, x, seed, val,
+               rnoise=None, niter=1, terms=()):
+        """A sequence of filter updates, observation by observation, in the
+        four forms ref_driver.f90's refx_kf_seq drives on the reference:
+        0 kf_update_obs (target val), 1 kf_update_resid_multi (resid val,
+        rnoise), 2 IEKF on R = seed.T - val, 3 the collocation residual
+        R = val + sum_k c_k [T_0 *] T_slot.  Returns (w, P, lam, ngated) with
+        P as the filter holds it (column-major P or the Pb blocks)."""
+        L = self.lib()
+        dims, w = _ia(dims), _da(w).copy()
+        x, seed, val = _da(x), _da(seed), _da(val)
+        nobs = x.shape[0]
+        rnoise = np.ones(nobs) if rnoise is None else _da(rnoise)
+        h = L.ot_kf_create(len(dims), _ip(dims), p0, lam_ini, lam0, 1 if decoupled else 0)
+        assert h > 0
+        try:
+            L.ot_kf_set(h, gate, q)
+            for n in range(nobs):
+                xn = x[n:n + 1]
+                sn = seed[n:n + 1]
+                if mode == 0:
+                    T = self.eval(dims, w, xn)[0]
+                    pred = 0.0
+                    for a in range(self.na):
+                        pred = pred + float(sn[0, 0, a]) * float(T[0, a])
+                    g = self.grad(dims, w, xn, sn, nthreads=1)
+                    rc = L.ot_kf_update_grad(h, _dp(w), _dp(g), float(val[n]) - pred, 1.0)
+                elif mode == 1:
+                    g = self.grad(dims, w, xn, sn, nthreads=1)
+                    rc = L.ot_kf_update_grad(h, _dp(w), _dp(g), -float(val[n]),
+                                             1.0 / np.sqrt(float(rnoise[n])))
+                elif mode == 2:
+                    L.ot_kf_iekf_begin(h, _dp(w))
+                    for it in range(niter):
+                        T = self.eval(dims, w, xn)[0]
+                        R = 0.0
+                        for j in range(T.shape[0]):
+                            for a in range(self.na):
+                                R = R + float(sn[0, j, a]) * float(T[j, a])
+                        R = R - float(val[n])
+                        g = self.grad(dims, w, xn, sn, nthreads=1)
+                        rc = L.ot_kf_iekf_iter(h, _dp(w), _dp(g), R, float(rnoise[n]),
+                                               1 if it == niter - 1 else 0)
+                        if rc:
+                            break
+                elif mode == 3:
+                    T = self.eval(dims, w, xn)[0]
+                    R = float(val[n])
+                    sv = np.zeros((1, T.shape[0], self.na))
+                    for c, sl, nl in terms:
+                        if nl:
+                            R = R + c * float(T[0, 0]) * float(T[0, sl])
+                            sv[0, 0, sl] += c * float(T[0, 0])
+                            sv[0, 0, 0] += c * float(T[0, sl])
+                        else:
+                            R = R + c * float(T[0, sl])
+                            sv[0, 0, sl] += c
+                    g = self.grad(dims, w, xn, sv, nthreads=1)
+                    rc = L.ot_kf_update_grad(h, _dp(w), _dp(g), 0.0 - R, 1.0)
+                else:
+                    raise ValueError("mode")
+                if rc:
+                    raise ArithmeticError(f"filter update refused ({rc})")
+            P = np.zeros(L.ot_kf_cov_len(h))
+            lam = C.c_double(0.0)
+            ng = C.c_int(0)
+            L.ot_kf_state(h, _dp(P), C.byref(lam), C.byref(ng))
+            return w, P, lam.value, ng.value
+        finally:
+            L.ot_kf_free(h)
+
+
 class RefLib:
     """The reference's lib/ (compiled from its own sources) through dnnf90.h."""
 
@@ -294,6 +388,57 @@ class RefNet:
                               _dp(x), _dp(out))
         assert rc == 0
         return out
+
+    def _kf_lib(self):
+        L = self.L
+        if not hasattr(L, "_refx_kf_ready"):
+            L.refx_kf_seq.argtypes = [_i, _i, _i, _pi, _i, _i, _pi, _pd, _d, _d, _d, _i, _d, _d,
+                                      _i, _i, _i, _pd, _pd, _pd, _pd, _i, _pd, _pi, _pi, _pd, _pd,
+                                      _pd, _pi]
+            L.refx_kf_seq.restype = _i
+            L.refx_comm_eval.argtypes = [_i, _i, _i, _pi, _i, _i, _pi, _i, _pd, _i, _pd, _pd, _pd]
+            L.refx_comm_eval.restype = _i
+            L._refx_kf_ready = True
+        return L
+
+    def kf_seq(self, p0, lam_ini, lam0, decoupled, gate, q, mode, x, seed, val,
+               rnoise=None, niter=1, terms=()):
+        """refx_kf_seq: the same sequence as Oracle.kf_seq on the reference's
+        own kalman_module.  Returns (w, P, lam, ngated)."""
+        L = self._kf_lib()
+        x, seed, val = _da(x), _da(seed), _da(val)
+        nobs = x.shape[0]
+        rnoise = np.ones(nobs) if rnoise is None else _da(rnoise)
+        co = _da([t[0] for t in terms] or [0.0])
+        sl = _ia([t[1] for t in terms] or [0])
+        nl = _ia([1 if t[2] else 0 for t in terms] or [0])
+        dims = [int(v) for v in self.dims]
+        nw = Oracle.nweights(dims)
+        if decoupled:
+            maxb = max(dims[l - 1] + 1 for l in range(1, len(dims)))
+            plen = maxb * maxb * sum(dims[1:])
+        else:
+            plen = nw * nw
+        w, P = np.zeros(nw), np.zeros(plen)
+        lam, ng = C.c_double(0.0), C.c_int(0)
+        rc = L.refx_kf_seq(self.d0, self.k, self.nseed, _ip(self.seeds), self.act, len(dims),
+                           _ip(self.dims), _dp(self.w), p0, lam_ini, lam0, 1 if decoupled else 0,
+                           gate, q, mode, niter, nobs, _dp(x), _dp(seed), _dp(val), _dp(rnoise),
+                           len(terms), _dp(co), _ip(sl), _ip(nl), _dp(w), _dp(P), C.byref(lam),
+                           C.byref(ng))
+        assert rc == 0
+        return w, P, lam.value, ng.value
+
+    def comm_eval(self, W, x):
+        """refx_comm_eval: comm_eval of output 1 over members W[nmem][nw]."""
+        L = self._kf_lib()
+        W, x = _da(W), _da(x)
+        m, sd = np.zeros((x.shape[0], self.na)), np.zeros((x.shape[0], self.na))
+        rc = L.refx_comm_eval(self.d0, self.k, self.nseed, _ip(self.seeds), self.act,
+                                   len(self.dims), _ip(self.dims), W.shape[0], _dp(W), x.shape[0],
+                                   _dp(x), _dp(m), _dp(sd))
+        assert rc == 0
+        return m, sd
 
     def grad(self, x, seed):
         x, seed = _da(x), _da(seed)

@@ -14,6 +14,8 @@
  *   collocation seed ... app/pinn_module.f90:32-65
  *   NGD primal solve ... app/optimizer_module.f90:221-469 (primal branch),
  *                        gauss_solve :472-522
+ *   committee .......... lib/committee_module.f90:79-111
+ *   Kalman filter ...... lib/kalman_module.f90:107-198, 353-659
  * The erf-GELU activation (code 5) has no reference counterpart; its
  * sigma^(q) is the closed form x*Phi^(q) + q*Phi^(q-1) with
  * Phi^(m) = (-1)^(m-1) He_(m-1)(x) phi(x).
@@ -817,4 +819,257 @@ int ot_ngd_solve(int nrow, int nw, const double *rows, const double *b,
     int rc = gauss_solve(nw, G, u);
     free(G);
     return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* committee mean and spread: lib/committee_module.f90:79-111          */
+
+int ot_comm_eval(int h, int nlayer, const int *dims, int nmem, const double *w,
+                 int npt, const double *x, double *tmean, double *tstd)
+{
+    ot_tab *t = tab_get(h);
+    if (!t || nmem < 1 || nlayer < 2 || dims[0] != t->d0) return -1;
+    const int nw = ot_nweights(nlayer, dims), nout = dims[nlayer - 1], na = t->na;
+    const size_t len = (size_t)npt * nout * na;
+    double *tk = (double *)malloc(sizeof(double) * (len ? len : 1));
+    for (size_t i = 0; i < len; ++i) tmean[i] = 0.0, tstd[i] = 0.0;
+    for (int k = 1; k <= nmem; ++k) {
+        ot_eval(h, nlayer, dims, w + (size_t)(k - 1) * nw, npt, x, tk, 0);
+        /* Welford: delta, then M2 += delta^2 (k-1)/k, then mean += delta/k */
+        for (size_t i = 0; i < len; ++i) {
+            const double d = tk[i] - tmean[i];
+            tstd[i] = tstd[i] + d * d * (double)(k - 1) / (double)k;
+            tmean[i] = tmean[i] + d / (double)k;
+        }
+    }
+    for (size_t i = 0; i < len; ++i) tstd[i] = nmem > 1 ? sqrt(tstd[i] / (double)(nmem - 1)) : 0.0;
+    free(tk);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* extended Kalman filter: lib/kalman_module.f90                        */
+/* The weight vector is the flatten order of kf_update_grad (:378-386):
+ * l ascending, neuron j, then bias and fan-in i -- this file's layout.
+ * Dense P is kept column-major as the reference's P(nw,nw); the decoupled
+ * filter keeps one maxb x maxb column-major block per neuron, Pb(:,:,ib). */
+
+#define OT_KF_MAX 16
+
+typedef struct {
+    int used, nw, decoupled, nb, maxb, ngated;
+    double lam, lam0, gate, q;
+    int *bsz, *boff;
+    double *P, *jf, *pj, *kv, *wprior, *wcur;
+} ot_kf;
+
+static ot_kf g_kf[OT_KF_MAX];
+
+static ot_kf *kf_get(int h) { return (h >= 1 && h <= OT_KF_MAX && g_kf[h - 1].used) ? &g_kf[h - 1] : NULL; }
+
+/* kf_init :107-167 and kf_block_map :171-185 (the size guard of :133-141
+ * is the caller's business here) */
+int ot_kf_create(int nlayer, const int *dims, double p0, double lam_ini, double lam0, int decoupled)
+{
+    if (nlayer < 2 || p0 <= 0.0 || lam_ini <= 0.0 || lam_ini > 1.0 || lam0 <= 0.0 || lam0 > 1.0) return -1;
+    int h = 0;
+    for (int i = 0; i < OT_KF_MAX; ++i)
+        if (!g_kf[i].used) { h = i + 1; break; }
+    if (!h) return -1;
+    ot_kf *kf = &g_kf[h - 1];
+    memset(kf, 0, sizeof(*kf));
+    kf->used = 1;
+    kf->nw = ot_nweights(nlayer, dims);
+    kf->decoupled = decoupled != 0;
+    const int nw = kf->nw;
+    if (!kf->decoupled) {
+        kf->P = (double *)calloc((size_t)nw * nw, sizeof(double));
+        for (int i = 0; i < nw; ++i) kf->P[(size_t)i * nw + i] = p0;
+    } else {
+        for (int l = 1; l < nlayer; ++l) {
+            kf->nb += dims[l];
+            if (dims[l - 1] + 1 > kf->maxb) kf->maxb = dims[l - 1] + 1;
+        }
+        kf->bsz = (int *)malloc(sizeof(int) * kf->nb);
+        kf->boff = (int *)malloc(sizeof(int) * kf->nb);
+        int ib = 0, m = 0;
+        for (int l = 1; l < nlayer; ++l)
+            for (int j = 0; j < dims[l]; ++j) {
+                kf->bsz[ib] = dims[l - 1] + 1;
+                kf->boff[ib] = m;
+                m += kf->bsz[ib++];
+            }
+        const size_t bb = (size_t)kf->maxb * kf->maxb;
+        kf->P = (double *)calloc(bb * kf->nb, sizeof(double));
+        for (int b = 0; b < kf->nb; ++b)
+            for (int j = 0; j < kf->bsz[b]; ++j) kf->P[b * bb + (size_t)j * kf->maxb + j] = p0;
+    }
+    kf->jf = (double *)calloc(nw, sizeof(double));
+    kf->pj = (double *)calloc(nw, sizeof(double));
+    kf->kv = (double *)calloc(nw, sizeof(double));
+    kf->lam = lam_ini;
+    kf->lam0 = lam0;
+    return h;
+}
+
+/* the gate and q fields (:62, :70), set after kf_init as
+ * app/lib_net_module.f90:474-475 does */
+int ot_kf_set(int h, double gate, double q)
+{
+    ot_kf *kf = kf_get(h);
+    if (!kf) return -1;
+    kf->gate = gate;
+    kf->q = q;
+    return 0;
+}
+
+int ot_kf_free(int h)
+{
+    ot_kf *kf = kf_get(h);
+    if (!kf) return -1;
+    free(kf->bsz); free(kf->boff); free(kf->P); free(kf->jf); free(kf->pj); free(kf->kv);
+    free(kf->wprior); free(kf->wcur);
+    memset(kf, 0, sizeof(*kf));
+    return 0;
+}
+
+/* pj = P jf: per block (:389-396) or dsymv_like (:650-659) */
+static void kf_pj(ot_kf *kf)
+{
+    const int nw = kf->nw;
+    if (kf->decoupled) {
+        const size_t bb = (size_t)kf->maxb * kf->maxb;
+        for (int b = 0; b < kf->nb; ++b) {
+            const int n = kf->bsz[b], o = kf->boff[b];
+            const double *Pb = kf->P + b * bb;
+            for (int i = 0; i < n; ++i) {
+                double s = 0.0;
+                for (int r = 0; r < n; ++r) s += Pb[(size_t)i * kf->maxb + r] * kf->jf[o + r];
+                kf->pj[o + i] = s;
+            }
+        }
+    } else {
+        for (int i = 0; i < nw; ++i) {
+            double s = 0.0;
+            for (int r = 0; r < nw; ++r) s += kf->P[(size_t)i * nw + r] * kf->jf[r];
+            kf->pj[i] = s;
+        }
+    }
+}
+
+/* P = sym((P - kv pj^T)/lam) + q I, per block (:447-464) or dense (:466-487) */
+static void kf_downdate(ot_kf *kf)
+{
+    const int dec = kf->decoupled;
+    const int nblk = dec ? kf->nb : 1;
+    const size_t ld = dec ? (size_t)kf->maxb : (size_t)kf->nw;
+    for (int b = 0; b < nblk; ++b) {
+        const int n = dec ? kf->bsz[b] : kf->nw, o = dec ? kf->boff[b] : 0;
+        double *P = kf->P + (dec ? b * ld * ld : 0);
+        for (int i = 0; i < n; ++i)
+            for (int r = 0; r < n; ++r)
+                P[i * ld + r] = (P[i * ld + r] - kf->kv[o + r] * kf->pj[o + i]) / kf->lam;
+        for (int i = 0; i < n; ++i)
+            for (int j = i + 1; j < n; ++j) {
+                /* P(i,j) = 0.5 (P(i,j) + P(j,i)); P(j,i) = P(i,j) */
+                P[j * ld + i] = 0.5 * (P[j * ld + i] + P[i * ld + j]);
+                P[i * ld + j] = P[j * ld + i];
+            }
+        if (kf->q > 0.0)
+            for (int i = 0; i < n; ++i) P[i * ld + i] = P[i * ld + i] + kf->q;
+    }
+}
+
+/* kf_update_grad :353-493: w[nw] is moved in place; g is dy/dw.
+ * Returns 0, or -2 for a nonpositive innovation denominator (where the
+ * reference stops, :399-405). */
+int ot_kf_update_grad(int h, double *w, const double *g, double xi_in, double scale)
+{
+    ot_kf *kf = kf_get(h);
+    if (!kf) return -1;
+    const int nw = kf->nw;
+    double xi = scale * xi_in;
+    for (int m = 0; m < nw; ++m) kf->jf[m] = scale * g[m];
+    kf_pj(kf);
+    double dot = 0.0;
+    for (int m = 0; m < nw; ++m) dot += kf->jf[m] * kf->pj[m];
+    double denom = kf->lam + dot;
+    if (denom <= 0.0) return -2;
+    if (kf->gate > 0.0) {
+        const double jpj = denom - kf->lam;
+        if (xi * xi > kf->gate * kf->gate * denom) {
+            const double s2 = kf->gate * kf->gate * kf->lam / (xi * xi - kf->gate * kf->gate * jpj);
+            const double s = sqrt(s2);
+            xi = s * xi;
+            for (int m = 0; m < nw; ++m) kf->jf[m] = s * kf->jf[m];
+            for (int m = 0; m < nw; ++m) kf->pj[m] = s * kf->pj[m];
+            denom = kf->lam + s2 * jpj;
+            kf->ngated++;
+        }
+    }
+    for (int m = 0; m < nw; ++m) kf->kv[m] = kf->pj[m] / denom;
+    for (int m = 0; m < nw; ++m) w[m] = w[m] + kf->kv[m] * xi;
+    kf_downdate(kf);
+    kf->lam = kf->lam0 * kf->lam + 1.0 - kf->lam0;
+    return 0;
+}
+
+/* kf_iekf_begin :512-528 */
+int ot_kf_iekf_begin(int h, const double *w)
+{
+    ot_kf *kf = kf_get(h);
+    if (!kf) return -1;
+    if (!kf->wprior) {
+        kf->wprior = (double *)malloc(sizeof(double) * kf->nw);
+        kf->wcur = (double *)malloc(sizeof(double) * kf->nw);
+    }
+    memcpy(kf->wprior, w, sizeof(double) * kf->nw);
+    return 0;
+}
+
+/* kf_iekf_iter :532-648: g = dR/dw at the current weights w, resid = R there */
+int ot_kf_iekf_iter(int h, double *w, const double *g, double resid, double rnoise, int final)
+{
+    ot_kf *kf = kf_get(h);
+    if (!kf || !kf->wprior || rnoise <= 0.0) return -1;
+    const int nw = kf->nw;
+    const double scale = 1.0 / sqrt(rnoise);
+    for (int m = 0; m < nw; ++m) {
+        kf->jf[m] = scale * g[m];
+        kf->wcur[m] = w[m];
+    }
+    kf_pj(kf);
+    double dot = 0.0;
+    for (int m = 0; m < nw; ++m) dot += kf->jf[m] * kf->pj[m];
+    const double denom = kf->lam + dot;
+    if (denom <= 0.0) return -2;
+    for (int m = 0; m < nw; ++m) kf->kv[m] = kf->pj[m] / denom;
+    double c = 0.0;
+    for (int m = 0; m < nw; ++m) c += kf->jf[m] * (kf->wprior[m] - kf->wcur[m]);
+    const double xi = -scale * resid - c;
+    for (int m = 0; m < nw; ++m) w[m] = kf->wprior[m] + kf->kv[m] * xi;
+    if (final) {
+        kf_downdate(kf);
+        kf->lam = kf->lam0 * kf->lam + 1.0 - kf->lam0;
+    }
+    return 0;
+}
+
+/* covariance size in doubles: nw*nw (dense) or maxb*maxb*nb (blocks) */
+long ot_kf_cov_len(int h)
+{
+    ot_kf *kf = kf_get(h);
+    if (!kf) return -1;
+    return kf->decoupled ? (long)kf->maxb * kf->maxb * kf->nb : (long)kf->nw * kf->nw;
+}
+
+/* P (column-major, as the reference holds it), lam and the gated count */
+int ot_kf_state(int h, double *P, double *lam, int *ngated)
+{
+    ot_kf *kf = kf_get(h);
+    if (!kf) return -1;
+    if (P) memcpy(P, kf->P, sizeof(double) * (size_t)ot_kf_cov_len(h));
+    if (lam) *lam = kf->lam;
+    if (ngated) *ngated = kf->ngated;
+    return 0;
 }

@@ -62,6 +62,23 @@ int ot_pinn_rows(int h, int nlayer, const int *dims, const double *w, int npt,
 int ot_ngd_solve(int nrow, int nw, const double *rows, const double *b,
                  int nb, double mu, int mode, double *u);
 
+/* committee mean and sample standard deviation of every output slot over
+ * nmem members w[nmem][nw] (lib/committee_module.f90:79-111, Welford) */
+int ot_comm_eval(int h, int nlayer, const int *dims, int nmem, const double *w,
+                 int npt, const double *x, double *tmean, double *tstd);
+
+/* extended Kalman filter (lib/kalman_module.f90): create returns a handle
+ * (or -1 on bad arguments); update_grad / iekf_iter move w[nw] in place and
+ * return -2 on a nonpositive innovation denominator */
+int ot_kf_create(int nlayer, const int *dims, double p0, double lam_ini, double lam0, int decoupled);
+int ot_kf_set(int h, double gate, double q);
+int ot_kf_free(int h);
+int ot_kf_update_grad(int h, double *w, const double *g, double xi, double scale);
+int ot_kf_iekf_begin(int h, const double *w);
+int ot_kf_iekf_iter(int h, double *w, const double *g, double resid, double rnoise, int final);
+long ot_kf_cov_len(int h);
+int ot_kf_state(int h, double *P, double *lam, int *ngated);
+
 #ifdef __cplusplus
 }
 #endif

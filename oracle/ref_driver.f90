@@ -5,14 +5,20 @@
 ! multi-output forward, so these bind(C) entries call the module routines
 ! directly: tabset_init (multi_index_bell_module.f90:692), net_init
 ! (net_module.f90:92), net_forward_point_multi / net_backward_point_multi
-! (train_module.f90:226, :459).  Weight and gradient vectors concatenate, for
-! l = 2..nlayer, ndim(l) rows of (ndim(l-1)+1) doubles, bias first.
+! (train_module.f90:226, :459), the filter (kalman_module.f90:107, :240, :280,
+! :309, :512, :532) and the committee (committee_module.f90:79).  Weight and
+! gradient vectors concatenate, for l = 2..nlayer, ndim(l) rows of
+! (ndim(l-1)+1) doubles, bias first.
 MODULE ref_driver
   use iso_c_binding
   use multi_index_bell_module, only: tabset_t, tabset_init, tabset_free
   use net_module, only: net_t, net_init, net_free
   use train_module, only: twork_t, grad_t, twork_init, twork_free, grad_init, &
        grad_free, net_forward_point_multi, net_backward_point_multi
+  use kalman_module, only: kalman_t, kf_init, kf_free, kf_update_obs, kf_update_resid, &
+       kf_update_resid_multi, kf_iekf_begin, kf_iekf_iter
+  use committee_module, only: committee_t, comm_init, comm_free, comm_eval
+  use net_module, only: work_t, work_init, work_free
   implicit none
 CONTAINS
 
@@ -107,5 +113,149 @@ CONTAINS
     call grad_free( g );  call twork_free( tw );  call net_free( nt );  call tabset_free( ts )
     refx_grad = 0
   END FUNCTION refx_grad
+
+  !> A sequence of nobs filter updates on the reference's kalman_module.
+  !! Observation n is the point x(:,n) with seed(n, nout, na) (C order) and
+  !! the value val(n):
+  !!   mode 0  kf_update_obs( x, seed(output 1), target = val(n) )
+  !!   mode 1  kf_update_resid_multi( x, seedm, resid = val(n), rnoise(n) )
+  !!   mode 2  kf_iekf_begin, then niter kf_iekf_iter on R = seed.T - val(n)
+  !!           re-evaluated at the running weights, rnoise(n)
+  !!   mode 3  the collocation observation of app/sgd_batch_module.f90:547-550:
+  !!           R = val(n) + sum_k c_k [T_1 *] T_slot(k) on output 1, seed dR/dT
+  !!           (set_pinn_seed with R = fac = 1), kf_update_resid( x, seed, R )
+  !! slot(:) are 0-based slots.  Returns the weights, the covariance as the
+  !! filter holds it (P(nw,nw), or Pb(maxb,maxb,nb)), lambda and the gated count.
+  INTEGER(c_int) FUNCTION refx_kf_seq( d0, k, nseed, seeds, iact, nlayer, dims, w, &
+       p0, lam_ini, lam0, decoupled, gate, q, mode, niter, nobs, x, seed, val, rnoise, &
+       nterm, coeff, slot, nonlin, wout, pout, lam_out, ngated ) bind(C)
+    integer(c_int),value :: d0, k, nseed, iact, nlayer, decoupled, mode, niter, nobs, nterm
+    real(c_double),value :: p0, lam_ini, lam0, gate, q
+    integer(c_int),intent(IN) :: seeds(*), dims(*), slot(*), nonlin(*)
+    real(c_double),intent(IN) :: w(*), x(*), seed(*), val(*), rnoise(*), coeff(*)
+    real(c_double),intent(OUT) :: wout(*), pout(*), lam_out
+    integer(c_int),intent(OUT) :: ngated
+    type(tabset_t) :: ts
+    type(net_t) :: nt
+    type(twork_t) :: tw
+    type(grad_t) :: g
+    type(kalman_t) :: kf
+    real(8),allocatable :: sm(:,:), tm(:,:), sv(:), xo(:)
+    real(8) :: R
+    integer :: n, i, ia, nout, na, p, l, j, it, kk, s
+    call build_net( d0, k, nseed, seeds, iact, nlayer, dims, w, ts, nt )
+    na = nt%tab%na;  nout = dims(nlayer)
+    call twork_init( tw, nt );  call grad_init( g, nt )
+    call kf_init( kf, nt, p0, lam_ini, lam0, decoupled /= 0 )
+    kf%gate = gate;  kf%q = q
+    allocate( sm(nout,na), tm(nout,na), sv(na), xo(d0) )
+    do n = 1, nobs
+       p = (n-1)*nout*na
+       do i = 1, nout
+          do ia = 1, na
+             sm(i,ia) = seed(p + (i-1)*na + ia)
+          end do
+       end do
+       xo = x((n-1)*d0+1:n*d0)
+       select case ( mode )
+       case ( 0 )
+          sv = sm(1,:)
+          call kf_update_obs( nt, tw, g, kf, xo, sv, val(n) )
+       case ( 1 )
+          call kf_update_resid_multi( nt, tw, g, kf, xo, sm, val(n), rnoise(n) )
+       case ( 2 )
+          call kf_iekf_begin( nt, kf )
+          do it = 1, niter
+             call net_forward_point_multi( nt, tw, xo, tm )
+             R = 0.d0
+             do i = 1, nout
+                do ia = 1, na
+                   R = R + sm(i,ia)*tm(i,ia)
+                end do
+             end do
+             R = R - val(n)
+             call kf_iekf_iter( nt, tw, g, kf, xo, sm, R, rnoise(n), it == niter )
+          end do
+       case ( 3 )
+          call net_forward_point_multi( nt, tw, xo, tm )
+          R = val(n)
+          sv = 0.d0
+          do kk = 1, nterm
+             s = slot(kk) + 1
+             if ( nonlin(kk) /= 0 ) then
+                R = R + coeff(kk)*tm(1,1)*tm(1,s)
+                sv(s) = sv(s) + coeff(kk)*tm(1,1)
+                sv(1) = sv(1) + coeff(kk)*tm(1,s)
+             else
+                R = R + coeff(kk)*tm(1,s)
+                sv(s) = sv(s) + coeff(kk)
+             end if
+          end do
+          call kf_update_resid( nt, tw, g, kf, xo, sv, R )
+       case default
+          refx_kf_seq = -1
+          return
+       end select
+    end do
+    p = 0
+    do l = 2, nlayer
+       do j = 1, dims(l)
+          wout(p+1:p+dims(l-1)+1) = nt%w(l,j,0:dims(l-1))
+          p = p + dims(l-1) + 1
+       end do
+    end do
+    if ( kf%decoupled ) then
+       pout(1:size(kf%Pb)) = reshape( kf%Pb, (/ size(kf%Pb) /) )
+    else
+       pout(1:size(kf%P)) = reshape( kf%P, (/ size(kf%P) /) )
+    end if
+    lam_out = kf%lam
+    ngated = kf%ngated
+    call kf_free( kf );  call grad_free( g );  call twork_free( tw )
+    call net_free( nt );  call tabset_free( ts )
+    refx_kf_seq = 0
+  END FUNCTION refx_kf_seq
+
+  !> comm_eval over nmem members of one shape, w(nmem, nw) in C order;
+  !! tmean / tstd (npt, na) of output 1.
+  INTEGER(c_int) FUNCTION refx_comm_eval( d0, k, nseed, seeds, iact, nlayer, dims, &
+       nmem, w, npt, x, tmean, tstd ) bind(C)
+    integer(c_int),value :: d0, k, nseed, iact, nlayer, nmem, npt
+    integer(c_int),intent(IN) :: seeds(*), dims(*)
+    real(c_double),intent(IN) :: w(*), x(*)
+    real(c_double),intent(OUT) :: tmean(*), tstd(*)
+    type(tabset_t) :: ts
+    type(net_t) :: nt
+    type(committee_t) :: cm
+    type(work_t) :: wk
+    real(8),allocatable :: tm(:), tsd(:)
+    integer :: m, n, l, j, p, na, nw
+    call build_net( d0, k, nseed, seeds, iact, nlayer, dims, w, ts, nt )
+    na = nt%tab%na
+    nw = 0
+    do l = 2, nlayer
+       nw = nw + dims(l)*(dims(l-1)+1)
+    end do
+    call comm_init( cm, int(nmem), int(nlayer), int(dims(1:nlayer)), ts )
+    do m = 1, nmem
+       p = (m-1)*nw
+       do l = 2, nlayer
+          do j = 1, dims(l)
+             cm%mem(m)%w(l,j,0:dims(l-1)) = w(p+1:p+dims(l-1)+1)
+             p = p + dims(l-1) + 1
+          end do
+       end do
+    end do
+    call work_init( wk, cm%mem(1) )
+    allocate( tm(na), tsd(na) )
+    do n = 1, npt
+       call comm_eval( cm, wk, x((n-1)*d0+1:n*d0), tm, tsd )
+       tmean((n-1)*na+1:n*na) = tm
+       tstd((n-1)*na+1:n*na) = tsd
+    end do
+    call work_free( wk );  call comm_free( cm )
+    call net_free( nt );  call tabset_free( ts )
+    refx_comm_eval = 0
+  END FUNCTION refx_comm_eval
 
 END MODULE ref_driver
