@@ -132,6 +132,152 @@ def bench_ngd(args, D, torch, wl, net, xd, res, N, dev, mu=1e-3):
     return 0
 
 
+def bench_kf(args, D, torch, wl, net, w, x, res, terms, dev):
+    """The extended Kalman filter over the weights: one step presents the
+    workload's npt collocation points in order as observations -- forward
+    and adjoint of the point, R and the seed dR/dT on the device, the
+    covariance pass (pending downdate + P j), gain and weight step
+    (app/sgd_batch_module.f90:547-550, lib/kalman_module.f90:240-493) --
+    and returns the weights to the host once."""
+    p0, lam_ini, lam0, dec = wl.kf
+    nobs = min(wl.npt, args.kf_obs) if args.kf_obs else wl.npt
+    x = x[:nobs]
+    nw = net.nw
+    xd = torch.from_numpy(x).to(dev)
+
+    def fresh():
+        net.weights = w
+        return D.KalmanFilter(net, p0, lam_ini, lam0, dec)
+
+    bw = D.BatchWork(net, 1, D.PREC_FP64)
+    s = torch.cuda.Stream(device=dev)
+    bw.set_stream(s.cuda_stream)
+    kf = fresh()
+    with torch.cuda.stream(s):
+        for _ in range(args.warmup):
+            kf.update_residual(bw, xd, res)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(s)
+        for _ in range(args.steps):
+            kf.update_residual(bw, xd, res)
+        e1.record(s)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / args.steps
+        ms = e0.elapsed_time(e1) / args.steps
+        # one more step with every kernel category bracketed by HIP events
+        bw.kernel_ms()
+        bw.profile(True)
+        kf.update_residual(bw, xd, res)
+        torch.cuda.synchronize()
+        bw.profile(False)
+        kms = bw.kernel_ms()
+    st = kf.state()
+    kf.free()
+
+    # the dominant kernel: the covariance pass, one launch per observation
+    tot, cnt = kms["kf_pass"]
+    launch_ms = tot / max(cnt, 1)
+    tri = nw * (nw + 1) / 2.0 if not dec else float(sum(
+        wl.dims[l] * (wl.dims[l - 1] + 1) ** 2 for l in range(1, len(wl.dims))))
+    byts = 2.0 * 8.0 * tri          # read and write of the stored covariance
+    roof = {"bound": "hbm", "achieved": byts / (launch_ms * 1e-3) / 1e9, "peak": PEAK["hbm"], "unit": "GB/s",
+            "frac": byts / (launch_ms * 1e-3) / 1e9 / PEAK["hbm"], "traffic": traffic_of(wl.name, "fp64", "kf_pass"),
+            "kernel": "k_kf_tiles" if not dec else "k_kf_blocks", "launch_ms": launch_ms,
+            "algorithmic_bytes_per_launch": byts}
+
+    # parity on the first observations against the oracle, and the CPU
+    # baseline: the reference's own kalman_module (oracle/_ref) on a sample
+    check, cpu = None, None
+    try:
+        import oracle
+        if not os.path.exists(oracle.LIB):
+            oracle.build(ref=False)
+        o = oracle.Oracle(wl.d0, wl.k, wl.seeds, wl.act)
+        m = 40 if nw < 2000 else 3
+        kf = fresh()
+        kf.update_residual(bw, x[:m], res)
+        wg = net.weights
+        Pg = kf.covariance() if nw < 2000 else None
+        kf.free()
+        seeds0 = np.zeros((m, 1, o.na))
+        wr, Pr, _, _ = o.kf_seq(wl.dims, w, p0, lam_ini, lam0, dec, 0.0, 0.0, 3, x[:m], seeds0, np.zeros(m),
+                                terms=terms)
+        check = {"observations": m, "weights_max_rel": float(np.abs(wg - wr).max() / np.abs(wr).max())}
+        if Pg is not None:
+            check["cov_max_rel"] = float(np.abs(Pg - Pr).max() / np.abs(Pr).max())
+        if not args.no_cpu:
+            ref = oracle.RefLib.available()
+
+            def cpu_run(k):
+                if ref:
+                    # the reference's Fortran runtime writes to stdout at
+                    # exit: run it in a child process whose output is dropped
+                    import subprocess
+                    import tempfile
+                    with tempfile.TemporaryDirectory() as td:
+                        out = os.path.join(td, "t.json")
+                        subprocess.run([sys.executable, os.path.abspath(__file__), "--workload", wl.name,
+                                        "--kf-ref-timing", str(k), "--out", out], check=True,
+                                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+                        with open(out) as f:
+                            return json.load(f)["seconds"]
+                t0 = time.perf_counter()
+                o.kf_seq(wl.dims, w, p0, lam_ini, lam0, dec, 0.0, 0.0, 3, x[:k], np.zeros((k, 1, o.na)),
+                         np.zeros(k), terms=terms)
+                return time.perf_counter() - t0
+            k0 = 2
+            dt = cpu_run(k0)
+            k = int(min(nobs, max(k0, k0 * args.cpu_seconds / max(dt, 1e-6))))
+            dt = cpu_run(k)
+            cpu = {"value": k / dt, "unit": "observations/s", "cores": 1, "kind": "reference" if ref else "port",
+                   "sample": f"first {k} of the {nobs} observations, "
+                             + ("the reference's kalman_module (lib/ compiled with amdflang -O3) through "
+                                "ref_driver.f90" if ref else "the oracle's C restatement")
+                             + f", dense covariance, 1 thread, {dt:.1f} s (filter set-up included)"}
+    except Exception as exc:   # a reported baseline, never the product
+        cpu = cpu or {"error": repr(exc)}
+    bw.set_stream(None)
+    bw.free()
+    cov_bytes = st["bytes"]
+    line = {"metric": "Kalman filter observations/s (collocation residual rows, dense covariance update)",
+            "value": nobs / (ms * 1e-3), "unit": "observations/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"{wl.name}: MLP {wl.dims} tanh, D0={wl.d0} K={wl.k}, nw={nw}, {nobs} "
+                                   f"observations of {wl.note} per step, p0={p0} lam={lam_ini} lam0={lam0}, "
+                                   f"{'decoupled' if dec else 'dense'} covariance {cov_bytes / 1e9:.3f} GB"},
+            "roofline": roof, "cpu_baseline": cpu, "vs_oracle": check,
+            "wall_ms_per_step": wall * 1e3,
+            "kernels": {c: {"ms_per_step": t, "launches_per_step": n} for c, (t, n) in kms.items() if n},
+            "us_per_observation": ms * 1e3 / nobs}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def kf_ref_timing(args):
+    """CPU baseline leg of the filter workloads (a child process, no GPU):
+    the first k observations through the reference's own kalman_module,
+    compiled from its sources into oracle/_ref; the seconds go to --out."""
+    import oracle
+    from dnnf90_amd.workloads import WORKLOADS
+    wl = WORKLOADS[args.workload]
+    p0, lam_ini, lam0, dec = wl.kf
+    w, x = wl.inputs()
+    o = oracle.Oracle(wl.d0, wl.k, wl.seeds, wl.act)
+    k = args.kf_ref_timing
+    alpha = o.alpha()
+    terms = wl.terms([list(a) for a in alpha])
+    runner = oracle.RefNet(wl.d0, wl.k, wl.seeds, wl.act, wl.dims, w)
+    t0 = time.perf_counter()
+    runner.kf_seq(p0, lam_ini, lam0, dec, 0.0, 0.0, 3, x[:k], np.zeros((k, 1, o.na)), np.zeros(k), terms=terms)
+    dt = time.perf_counter() - t0
+    with open(args.out, "w") as f:
+        json.dump({"seconds": dt, "observations": k}, f)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,7 +289,12 @@ def main():
                     help="only copy a buffer of this many MiB 5 times (HBM counter calibration)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kf-obs", type=int, default=0, help="filter workloads: observations per step")
+    ap.add_argument("--kf-ref-timing", type=int, default=0, help=argparse.SUPPRESS)
+    ap.add_argument("--out", default="", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.kf_ref_timing:
+        return kf_ref_timing(args)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the gradient all-reduce across GPUs is a design note, not a measured path)")
@@ -184,6 +335,8 @@ def main():
 
     if args.workload == "ngd2_k4":
         return bench_ngd(args, D, torch, wl, net, xd, res, N, dev)
+    if wl.kf is not None:
+        return bench_kf(args, D, torch, wl, net, w, x, res, terms, dev)
 
     def run_mode(prec, name, wbytes, mma_peak, budget=0):
         bw = D.BatchWork(net, N, prec, budget=budget)

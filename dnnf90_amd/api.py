@@ -250,7 +250,8 @@ class BatchWork:
     def sync(self):
         _chk(lib().dnnf90_bwork_sync(self.id), "dnnf90_bwork_sync")
 
-    KERNEL_CATEGORIES = ("linear", "bell_fwd", "bell_bwd", "grad", "linear_t", "other", "grad_reduce")
+    KERNEL_CATEGORIES = ("linear", "bell_fwd", "bell_bwd", "grad", "linear_t", "other", "grad_reduce",
+                         "kf_pass", "kf_gain")
 
     def profile(self, on=True):
         _chk(lib().dnnf90_bwork_profile(self.id, 1 if on else 0), "dnnf90_bwork_profile")

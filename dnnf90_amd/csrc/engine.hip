@@ -237,7 +237,10 @@ struct Grad {
 };
 
 // kernel categories timed by HIP events when profiling is on
-enum { CAT_LINEAR = 0, CAT_BELL_FWD, CAT_BELL_BWD, CAT_GRAD, CAT_LINEAR_T, CAT_OTHER, CAT_GRAD_REDUCE, NCAT };
+enum {
+    CAT_LINEAR = 0, CAT_BELL_FWD, CAT_BELL_BWD, CAT_GRAD, CAT_LINEAR_T, CAT_OTHER, CAT_GRAD_REDUCE,
+    CAT_KF_PASS, CAT_KF_GAIN, NCAT
+};
 
 struct KProf {
     bool on = false;
@@ -946,10 +949,27 @@ static void kf_pass(KFilter &kf, const double *g, double scale, hipStream_t s)
     }
 }
 
-// one update from the row g (kf_update_grad :353-493, kf_iekf_iter :532-648)
+// one update from the row g (kf_update_grad :353-493, kf_iekf_iter :532-648);
+// b (optional) times the kernels when its profiling is on
 static void kf_step(Net &nt, KFilter &kf, const double *g, double scale, int xmode, double v, const double *seed,
-                    const double *T, int nsl, bool iekf, bool final, hipStream_t s)
+                    const double *T, int nsl, bool iekf, bool final, hipStream_t s, BatchWork *b = nullptr)
 {
+    if (b && b->prof.on) {
+        // the pass category brackets the one covariance kernel
+        {
+            PScope p(*b, CAT_KF_PASS);
+            if (!kf.decoupled)
+                launch_kf_tiles(kf.P, kf.nt, kf.nw, g, scale, kf.kv, kf.pj, kf.st, true, kf.part, s);
+            else
+                launch_kf_blocks(kf.P, kf.d_poff, kf.d_bsz, kf.d_boff, kf.nb, kf.maxb, g, scale, kf.kv, kf.pj,
+                                 kf.st, true, s);
+        }
+        PScope p(*b, CAT_KF_GAIN);
+        if (!kf.decoupled) launch_kf_sum_part(kf.part, kf.nt, kf.pj, s);
+        launch_kf_gain(kf.st, g, scale, kf.pj, kf.nw, xmode, v, seed, T, nsl, kf.r, kf.wprior, nt.d_w64, final, s);
+        launch_kf_apply(kf.st, kf.pj, kf.kv, nt.d_w64, kf.wprior, kf.nw, iekf, s);
+        return;
+    }
     kf_pass(kf, g, scale, s);
     launch_kf_gain(kf.st, g, scale, kf.pj, kf.nw, xmode, v, seed, T, nsl, kf.r, kf.wprior, nt.d_w64, final, s);
     launch_kf_apply(kf.st, kf.pj, kf.kv, nt.d_w64, kf.wprior, kf.nw, iekf, s);
@@ -963,16 +983,22 @@ static void kf_observe(Net &nt, BatchWork &b, KFilter &kf, const double *dx, con
 {
     hipStream_t s = b.stream;
     const int nout = nt.dims[nt.nlayer - 1], na = nt.tab.na;
-    launch_fill(kf.g, kf.nw, 0.0, s);
+    {
+        PScope p(b, CAT_OTHER);
+        launch_fill(kf.g, kf.nw, 0.0, s);
+    }
     forward_t<double>(nt, b, dx, 1);
-    if (xmode == KF_XI_LINEAR) launch_output<double>((const double *)b.tl, nout, na, 1, b.np, b.d_tout, s);
-    if (res)
-        launch_seed_residual<double>((const double *)b.tl, nout, na, 1, b.np, *res, 1.0, true, (double *)b.p0,
-                                     b.d_rsq_part, s, r0, kf.r);
-    else
-        launch_seed_user<double>(dseed, 1, nout, na, (double *)b.p0, b.np, s);
+    {
+        PScope p(b, CAT_OTHER);
+        if (xmode == KF_XI_LINEAR) launch_output<double>((const double *)b.tl, nout, na, 1, b.np, b.d_tout, s);
+        if (res)
+            launch_seed_residual<double>((const double *)b.tl, nout, na, 1, b.np, *res, 1.0, true, (double *)b.p0,
+                                         b.d_rsq_part, s, r0, kf.r);
+        else
+            launch_seed_user<double>(dseed, 1, nout, na, (double *)b.p0, b.np, s);
+    }
     backward_t<double>(nt, b, kf.g, nullptr, 0);
-    kf_step(nt, kf, kf.g, scale, xmode, v, dseed, b.d_tout, nout * na, iekf, final, s);
+    kf_step(nt, kf, kf.g, scale, xmode, v, dseed, b.d_tout, nout * na, iekf, final, s, &b);
 }
 
 // after the updates: weights back to the host copy, the stream drained and

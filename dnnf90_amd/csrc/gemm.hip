@@ -600,11 +600,13 @@ void grad_chunks(long ncol, int M, int Kd, int esz, int *nchunk, long *chunk)
     // one resident round of workgroups (one per CU: the gradient kernels
     // hold a whole output tile per wave) over tiles x chunks, so every
     // workgroup streams one long column range; chunks are whole multiples of
-    // 128 columns (every wave step stays inside its chunk)
+    // 128 columns (every wave step stays inside its chunk), and a short
+    // column range (a few points) is spread down to one 128-column step per
+    // workgroup rather than walked by a handful of them
     const int cb = grad_cb(esz, Kd - 1);
     const int tiles = ((M + 63) / 64) * ((Kd - 1 + 16 * cb - 1) / (16 * cb));
     long nc = std::max(1, GRAD_WGS / tiles);
-    nc = std::min(nc, std::max(1L, ncol / 512));
+    nc = std::min(nc, std::max(1L, ncol / 128));
     const long ch = (((ncol + nc - 1) / nc) + 127) / 128 * 128;
     *nchunk = (int)((ncol + ch - 1) / ch);
     *chunk = ch;

@@ -66,7 +66,10 @@ __global__ __launch_bounds__(256) void k_kf_init_blocks(double *__restrict__ P, 
     for (long e = threadIdx.x; e < (long)n * n; e += 256) pb[e] = (e / n == e % n) ? p0 : 0.0;
 }
 
-// pending downdate of tile (I, J), then (want) its share of P j
+// pending downdate of tile (I, J), then (want) its share of P j.  Thread t
+// owns column c = t % 64 of rows t / 64 + 4 i, i < 16: its sixteen loads are
+// issued together, kv and pj of the tile's rows come from LDS and those of
+// its column from registers.
 __global__ __launch_bounds__(256) void k_kf_tiles(double *__restrict__ P, int nt, int nw,
                                                   const double *__restrict__ g, double scale,
                                                   const double *__restrict__ kv, const double *__restrict__ pj,
@@ -75,70 +78,88 @@ __global__ __launch_bounds__(256) void k_kf_tiles(double *__restrict__ P, int nt
 {
 #pragma clang fp contract(off)
     __shared__ double S[KT][KT + 1];
-    __shared__ double jr[KT], jc[KT];
+    __shared__ double jr[KT], jc[KT], kvr[KT], pjr[KT];
     __shared__ double red[4][KT];
     int I, J;
     kf_tile_of(blockIdx.x, I, J);
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, c = t & 63, w = t >> 6;
     double *tp = P + (long)blockIdx.x * KT * KT;
+    double v[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = tp[t + 256 * i];
     const bool pend = st->pending && !st->bad;
-    const double lam = st->lam_dd, q = st->q;
-    if (want && t < KT) {
+    if (t < KT) {
         const long gr = (long)I * KT + t, gc = (long)J * KT + t;
-        jr[t] = gr < nw ? scale * g[gr] : 0.0;   // j over the rows of I
-        jc[t] = gc < nw ? scale * g[gc] : 0.0;   // j over the rows of J
-    }
-#pragma unroll 4
-    for (int i = 0; i < KT * KT / 256; ++i) {
-        const int e = t + 256 * i, r = e >> 6, c = e & 63;
-        double v = tp[e];
         if (pend) {
-            const long gr = (long)I * KT + r, gc = (long)J * KT + c;
-            // P'(gr,gc) and P'(gc,gr) from the one stored value (P symmetric)
-            const double a = (v - kv[gr] * pj[gc]) / lam;
-            if (gr != gc) {
-                const double b = (v - kv[gc] * pj[gr]) / lam;
-                v = 0.5 * (a + b);
-            } else {
-                v = a;
-                if (q > 0.0 && gr < nw) v = v + q;
-            }
-            tp[e] = v;
+            kvr[t] = kv[gr];
+            pjr[t] = pj[gr];
         }
-        S[r][c] = v;
+        if (want) {
+            jr[t] = gr < nw ? scale * g[gr] : 0.0;   // j over the rows of I
+            jc[t] = gc < nw ? scale * g[gc] : 0.0;   // j over the rows of J
+        }
+    }
+    __syncthreads();
+    if (pend) {
+        const double lam = st->lam_dd, q = st->q;
+        const long gc = (long)J * KT + c;
+        const double kvc = kv[gc], pjc = pj[gc];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int r = w + 4 * i;
+            const long gr = (long)I * KT + r;
+            // P'(gr,gc) and P'(gc,gr) from the one stored value (P symmetric)
+            const double a = (v[i] - kvr[r] * pjc) / lam;
+            if (gr != gc) {
+                const double b = (v[i] - kvc * pjr[r]) / lam;
+                v[i] = 0.5 * (a + b);
+            } else {
+                v[i] = a;
+                if (q > 0.0 && gr < nw) v[i] = v[i] + q;
+            }
+            tp[t + 256 * i] = v[i];
+        }
     }
     if (!want) return;
-    __syncthreads();
     const long ldp = (long)nt * KT;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) S[w + 4 * i][c] = v[i];
+    if (I != J) {
+        // rows of J: column sums against j over the rows of I, four waves of
+        // sixteen rows each, combined in a fixed order
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) s += v[i] * jr[w + 4 * i];
+        red[w][c] = s;
+    }
+    __syncthreads();
     {
         // rows of I: four lanes per row, sixteen columns each
         const int r = t >> 2, qd = t & 3;
         double s = 0.0;
-        for (int c = qd * 16; c < qd * 16 + 16; ++c) s += S[r][c] * jc[c];
+        for (int cc = qd * 16; cc < qd * 16 + 16; ++cc) s += S[r][cc] * jc[cc];
         s += __shfl_xor(s, 1);
         s += __shfl_xor(s, 2);
         if (qd == 0) part[(long)J * ldp + (long)I * KT + r] = s;
     }
-    if (I != J) {
-        // rows of J: the column sums, four waves of sixteen rows each
-        const int c = t & 63, qd = t >> 6;
-        double s = 0.0;
-        for (int r = qd * 16; r < qd * 16 + 16; ++r) s += S[r][c] * jr[r];
-        red[qd][c] = s;
-        __syncthreads();
-        if (t < KT) part[(long)I * ldp + (long)J * KT + t] = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
-    }
+    if (I != J && t < KT) part[(long)I * ldp + (long)J * KT + t] = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
 }
 
-// pj[row] = sum over column tiles k of part[k][row], k ascending
+// pj[row] = sum over column tiles k of part[k][row]: 32 rows a workgroup,
+// eight k-groups (k = g, g + 8, ...) per row, combined in a fixed order
 __global__ __launch_bounds__(256) void k_kf_sum_part(const double *__restrict__ part, int nt, double *__restrict__ pj)
 {
+    __shared__ double red[8][32];
     const long ldp = (long)nt * KT;
-    const long row = (long)blockIdx.x * 256 + threadIdx.x;
-    if (row >= ldp) return;
+    const int rl = threadIdx.x & 31, kg = threadIdx.x >> 5;
+    const long row = (long)blockIdx.x * 32 + rl;   // ldp is a multiple of 32
     double s = 0.0;
-    for (int k = 0; k < nt; ++k) s += part[(long)k * ldp + row];
-    pj[row] = s;
+    for (int k = kg; k < nt; k += 8) s += part[(long)k * ldp + row];
+    red[kg][rl] = s;
+    __syncthreads();
+    if (threadIdx.x < 32)
+        pj[row] = ((red[0][rl] + red[1][rl]) + (red[2][rl] + red[3][rl])) +
+                  ((red[4][rl] + red[5][rl]) + (red[6][rl] + red[7][rl]));
 }
 
 // decoupled: one workgroup per neuron block (kalman_module.f90:389-396,
@@ -296,7 +317,7 @@ void launch_kf_tiles(double *P, int nt, int nw, const double *g, double scale, c
 void launch_kf_sum_part(const double *part, int nt, double *pj, hipStream_t s)
 {
     const long rows = (long)nt * KT;
-    hipLaunchKernelGGL(k_kf_sum_part, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, s, part, nt, pj);
+    hipLaunchKernelGGL(k_kf_sum_part, dim3((unsigned)(rows / 32)), dim3(256), 0, s, part, nt, pj);
 }
 
 void launch_kf_blocks(double *P, const long *poff, const int *bsz, const int *boff, int nb, int maxb,

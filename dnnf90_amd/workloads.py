@@ -1,4 +1,4 @@
-"""The five measured workloads, with their synthetic inputs.
+"""The measured workloads, with their synthetic inputs.
 
 Each workload fixes a table family (D0, K, closure seeds), an MLP shape, an
 activation, a residual (the loss seed) and a point box.  Inputs come from
@@ -34,6 +34,7 @@ class Workload:
     seed: int
     residual: list = field(default_factory=list)   # symbolic terms, resolved by terms()
     note: str = ""
+    kf: tuple | None = None     # Kalman filter (p0, lam_ini, lam0, decoupled): npt observations in order
 
     @property
     def nweights(self):
@@ -99,4 +100,18 @@ WORKLOADS = {
                         [(-1.0, 1.0)] * 2, 5,
                         [(1.0, (4, 0), False), (1.0, (0, 2), False), (1.0, (1, 0), False)],
                         "u_xxxx + u_yy + u_x"),
+    # the extended Kalman filter of bench/opt_kalman (input_nn.dat there:
+    # [2,16,16,1] tanh, KdV residual with A = 3, GD_param 1e-3 0.995 0.9999,
+    # dense filter), one collocation observation per point
+    "kf_kdv": Workload("kf_kdv", 2, 3, [[0, 1], [1, 0], [3, 0]], [2, 16, 16, 1], ACT_TANH, 2000,
+                       [(-8.0, 8.0), (0.0, 2.0)], 6,
+                       [(1.0, (0, 1), False), (3.0, (1, 0), True), (1.0, (3, 0), False)],
+                       "u_t + 3 u u_x + u_xxx", kf=(1e-3, 0.995, 0.9999, False)),
+    # the dense filter on the closure4_k7 network (nw = 12865, covariance
+    # 0.66 GB as a lower triangle), collocation observations of its residual
+    "kf_closure4_k7": Workload("kf_closure4_k7", 4, 7, [[0, 0, 0, 1], [1, 0, 0, 0], [7, 0, 0, 0], [3, 2, 2, 0]],
+                               [4, 64, 64, 64, 64, 1], ACT_TANH, 200, [(-1.0, 1.0)] * 4, 7,
+                               [(1.0, (0, 0, 0, 1), False), (1.0, (1, 0, 0, 0), True),
+                                (1.0, (7, 0, 0, 0), False), (1.0, (3, 2, 2, 0), False)],
+                               "u_t + u u_x + d_x^7 u + d_x^3 d_y^2 d_z^2 u", kf=(1e-3, 0.995, 0.9999, False)),
 }

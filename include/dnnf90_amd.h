@@ -102,11 +102,12 @@ int dnnf90_bwork_set_stream(int bid, void *stream);
 int dnnf90_bwork_sync(int bid);
 /* kernel timing: with profiling on, every kernel category is bracketed by
  * HIP events on the work space's stream; dnnf90_bwork_kernel_ms returns and
- * resets the accumulated milliseconds and launch counts of the seven
- * categories (ms[7], count[7]): 0 linear map (W*T), 1 Faa di Bruno
+ * resets the accumulated milliseconds and launch counts of the nine
+ * categories (ms[9], count[9]): 0 linear map (W*T), 1 Faa di Bruno
  * forward, 2 adjoint pair sums, 3 weight gradient chunk partials (or
  * per-point rows), 4 transposed linear map (W^T*Sbar), 5 input/seed/output
- * kernels, 6 fixed-order reduction of the gradient partials */
+ * kernels, 6 fixed-order reduction of the gradient partials, 7 the Kalman
+ * filter's covariance pass (downdate + P j), 8 its gain and weight step */
 int dnnf90_bwork_profile(int bid, int on);
 int dnnf90_bwork_kernel_ms(int bid, double *ms, long *count);
 
