@@ -27,6 +27,10 @@ namespace dnnf90 {
 #ifndef LIN_X6_KP
 #define LIN_X6_KP 3
 #endif
+// largest row tile (in 16-row blocks) of the split linear maps
+#ifndef LIN_X6_RBT_MAX
+#define LIN_X6_RBT_MAX 8
+#endif
 
 // ---------------------------------------------------------------------------
 // K1 / K6.  The workgroup's 16*RBT rows of A (K1: the weight columns after
@@ -196,7 +200,7 @@ static void lin_x6_pick(const float *W, int ldw, const float *B, float *C, int M
                         hipStream_t s)
 {
     const size_t per_rb = (size_t)((Kd + 31) / 32) * 3 * 64 * 16;
-    int rbt = 8;
+    int rbt = LIN_X6_RBT_MAX;
     while (rbt > 1 && 16 * (rbt / 2) >= M) rbt /= 2;
     while (rbt > 1 && rbt * per_rb > (size_t)144 * 1024) rbt /= 2;
     switch (rbt) {

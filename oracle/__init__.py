@@ -378,7 +378,7 @@ class RefNet:
         self.nseed = len(seeds) if seeds else 0
         self.dims = _ia(dims)
         self.w = _da(w)
-        self.na = Oracle(d0, k, seeds, act).na
+        self.na = Oracle(d0, k, seeds).na   # the slot count does not depend on the activation
 
     def eval(self, x):
         x = _da(x)

@@ -1,0 +1,161 @@
+"""Edge cases of the batched path on the GPU, through the C ABI.
+
+  * every activation of the reference (tanh, sin, erf, J0, J1:
+    lib/multi_index_bell_module.f90:723-867) on a two-output network, forward
+    and adjoint, against the reference's own lib/ compiled by `make ref`
+    (oracle/_ref) and, for the activations the oracle restates, against the
+    oracle.  J0 and J1: the reference takes J_n from Zhang and Jin's
+    jyna/jy01b (lib/bessel_module.f90), whose sigma^(q), q <= 6, differ from
+    the Bessel function's by 2e-9 to 6e-9 relative per order (measured
+    against scipy.special.jv on [-4, 4]); this library evaluates J_n in
+    double precision, so against the reference the bound is 1e-7 (those
+    errors carried through two hidden layers), and the derivatives
+    themselves are pinned against scipy's closed form
+    d^q J_n = 2^-q sum_k (-1)^k C(q,k) J_(n-q+2k) to 1e-13;
+  * zero points, one point, and batches that shrink and grow inside one work
+    space (no state of an earlier call may leak into a later one);
+  * a point count above the work space's maximum is refused with -1;
+  * one input at the largest order the tables take (K = 15, 16 slots).
+FP64 mode throughout, 1e-12 relative per derivative order (forward) and to
+max |gradient| (adjoint).
+"""
+import numpy as np
+import pytest
+
+from helpers import rel, rel_by_order
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+SEEDS = [[4, 0], [0, 2], [1, 1]]
+
+
+@pytest.fixture(scope="module")
+def D():
+    import dnnf90_amd
+    return dnnf90_amd
+
+
+def _net(D, d0, k, seeds, act, dims, seed=1):
+    ts = D.TableSet(d0, k, seeds, act)
+    net = D.Net(ts, dims)
+    rng = np.random.default_rng(seed)
+    w = rng.uniform(-0.6, 0.6, net.nw)
+    net.weights = w
+    return ts, net, w
+
+
+@pytest.mark.parametrize("act", [0, 1, 2, 3, 4], ids=["tanh", "sin", "erf", "J0", "J1"])
+def test_activations_vs_reference(D, oracle_mod, act):
+    if not oracle_mod.RefLib.available():
+        pytest.skip("reference library not built")
+    dims = [2, 12, 10, 2]
+    ts, net, w = _net(D, 2, 4, SEEDS, act, dims)
+    npt = 37
+    rng = np.random.default_rng(2)
+    x = rng.uniform(-1, 1, (npt, 2))
+    seed = rng.standard_normal((npt, 2, ts.na))
+    bw = D.BatchWork(net, npt, D.PREC_FP64)
+    g = D.Grad(net)
+    t = bw.forward(x)
+    bw.backward(seed, g)
+    ref = oracle_mod.RefNet(2, 4, SEEDS, act, dims, w)
+    tr = ref.eval(x)
+    gr = ref.grad(x, seed)
+    deg = np.array([sum(a) for a in oracle_mod.Oracle(2, 4, SEEDS).alpha()])
+    ef = max(rel_by_order(t, tr, deg).values())
+    eg = rel(g.get(), gr)
+    print(f"activation {act}: forward {ef:.2e} adjoint {eg:.2e} vs the compiled reference")
+    tol = TOL64 if act in (0, 1, 2) else 1e-7   # the reference's own J_n error, see above
+    assert ef <= tol and eg <= tol
+    if act in (0, 1, 2):
+        o = oracle_mod.Oracle(2, 4, SEEDS, act)
+        assert max(rel_by_order(t, o.eval(dims, w, x), deg).values()) <= TOL64
+        assert rel(g.get(), o.grad(dims, w, x, seed)) <= TOL64
+    bw.free(); g.free(); net.free()
+
+
+@pytest.mark.parametrize("act,n0", [(3, 0), (4, 1)], ids=["J0", "J1"])
+def test_bessel_derivatives_vs_scipy(D, act, n0):
+    """A [1,1,1] network with unit weights and zero biases outputs sigma^(q)(x)
+    in slot q; here q = 0..8 for x in [-12, 12]."""
+    from scipy.special import comb, jv
+    ts = D.TableSet(1, 8, None, act)
+    net = D.Net(ts, [1, 1, 1])
+    net.weights = np.array([0.0, 1.0, 0.0, 1.0])
+    x = np.linspace(-12.0, 12.0, 97)[:, None]
+    bw = D.BatchWork(net, 97, D.PREC_FP64)
+    t = bw.forward(x)[:, 0, :]
+    ref = np.stack([sum((-1) ** k * comb(q, k) * jv(n0 - q + 2 * k, x[:, 0]) for k in range(q + 1)) / 2.0 ** q
+                    for q in range(9)], 1)
+    e = (np.abs(t - ref) / np.abs(ref).max(0)).max(0)
+    print(f"J{n0}^(q), q = 0..8, vs scipy: max rel per order", e)
+    assert e.max() <= 1e-13
+    bw.free(); net.free()
+
+
+def test_zero_points(D):
+    ts, net, w = _net(D, 2, 4, SEEDS, 0, [2, 8, 8, 1])
+    bw = D.BatchWork(net, 16, D.PREC_FP64)
+    g = D.Grad(net)
+    t = bw.forward(np.zeros((0, 2)))
+    assert t.shape == (0, 1, ts.na)
+    bw.backward(np.zeros((0, 1, ts.na)), g)
+    assert not g.get().any()
+    rsq = bw.residual_step(np.zeros((0, 2)), D.Residual([(1.0, 1, False), (2.0, 2, True)]), 1.0, g)
+    assert rsq == 0.0 and not g.get().any()
+    bw.free(); g.free(); net.free()
+
+
+def test_batches_shrink_and_grow_in_one_work_space(D, oracle_mod):
+    dims = [2, 16, 16, 1]
+    ts, net, w = _net(D, 2, 4, SEEDS, 0, dims)
+    o = oracle_mod.Oracle(2, 4, SEEDS)
+    deg = o.deg()
+    bw = D.BatchWork(net, 300, D.PREC_FP64)
+    g = D.Grad(net)
+    rng = np.random.default_rng(3)
+    for npt in (300, 65, 1, 64, 129, 1, 300):
+        x = rng.uniform(-1, 1, (npt, 2))
+        seed = rng.standard_normal((npt, 1, o.na))
+        g.zero()
+        t = bw.forward(x)
+        bw.backward(seed, g)
+        ef = max(rel_by_order(t, o.eval(dims, w, x), deg).values())
+        eg = rel(g.get(), o.grad(dims, w, x, seed))
+        print(f"npt {npt}: forward {ef:.2e} adjoint {eg:.2e}")
+        assert ef <= TOL64 and eg <= TOL64
+    bw.free(); g.free(); net.free()
+
+
+def test_npt_above_maxpt_refused(D):
+    ts, net, w = _net(D, 2, 4, SEEDS, 0, [2, 8, 1])
+    bw = D.BatchWork(net, 10, D.PREC_FP64)
+    with pytest.raises(D.DNNF90Error) as e:
+        bw.forward(np.zeros((11, 2)))
+    assert e.value.rc == -1
+    bw.free(); net.free()
+
+
+@pytest.mark.parametrize("codegen", [1, 0], ids=["generated", "table"])
+def test_one_input_order_15(D, oracle_mod, codegen):
+    from dnnf90_amd._lib import lib
+    prev = lib().dnnf90_set_codegen(codegen)
+    try:
+        dims = [1, 10, 10, 1]
+        ts, net, w = _net(D, 1, 15, None, 0, dims, seed=4)
+        o = oracle_mod.Oracle(1, 15, None)
+        assert ts.na == 16 == o.na
+        x = np.linspace(-0.9, 0.9, 50)[:, None]
+        seed = np.random.default_rng(5).standard_normal((50, 1, 16))
+        bw = D.BatchWork(net, 50, D.PREC_FP64)
+        g = D.Grad(net)
+        t = bw.forward(x)
+        bw.backward(seed, g)
+        ef = max(rel_by_order(t, o.eval(dims, w, x), o.deg()).values())
+        eg = rel(g.get(), o.grad(dims, w, x, seed))
+        print(f"d0 = 1, K = 15 ({'generated' if codegen else 'table'}): forward {ef:.2e} adjoint {eg:.2e}")
+        assert ef <= TOL64 and eg <= TOL64
+        bw.free(); g.free(); net.free()
+    finally:
+        lib().dnnf90_set_codegen(prev)
