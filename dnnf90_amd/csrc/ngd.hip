@@ -83,56 +83,206 @@ __device__ __forceinline__ double ldot(const double *__restrict__ x, const doubl
 // Cholesky of the diagonal block at (k, k) (nb <= PB rows), in place in G
 // (lower), then its inverse and the inverse's transpose, written to
 // Linv[PB][PB] and LinvT[PB][PB] (zero off the triangle; rows past nb hold
-// the identity).  Left-looking: column j is one dot product per row of the
-// rows above it (two barriers a column).  The inverse is forward
-// substitution of L X = I, one column of X per thread.  info > 0 marks the
-// first non-positive pivot (index + 1).
+// the identity).  Right-looking over 16-column steps: wave 0 factors the
+// 16 x 16 diagonal block in registers (one row a lane, columns exchanged by
+// lane shuffles), one thread a row solves the panel below against it, and
+// the trailing lower triangle takes the rank-16 update in 4 x 4 register
+// tiles.  The inverse X = L^-1 is blocked the same way: the eight 16 x 16
+// diagonal inverses by forward substitution (one column a thread), then
+// block row by block row X_ij = -X_ii (sum_{j <= m < i} L_im X_mj).  X (below
+// the diagonal) is kept transposed in the free upper triangle of S, its
+// diagonal in dg.  info > 0 marks the first non-positive pivot (index + 1).
+static constexpr int DB = 16;         // step width inside the diagonal block
+static constexpr int ND = PB / DB;    // steps
+
+// X[R][C] of the inverse as S holds it (R > C: S[C][R]; R == C: dg[R])
+__device__ __forceinline__ double xinv(const double (*S)[PBL], const double *dg, int R, int C)
+{
+    return R > C ? S[C][R] : (R == C ? dg[R] : 0.0);
+}
+
 __global__ __launch_bounds__(256) void k_chol_diag(double *__restrict__ A, long lda, int k, int nb,
                                                    double *__restrict__ Linv, double *__restrict__ LinvT,
                                                    int *__restrict__ info)
 {
-    __shared__ double S[PB][PBL];    // L, row-major
-    __shared__ double dg[PB];        // 1 / L[j][j]
+    __shared__ double S[PB][PBL];    // L (lower), then X transposed (upper)
+    __shared__ double dg[PB];        // 1 / L[j][j] = X[j][j]
     const int tid = threadIdx.x;
-    for (int u = tid; u < PB * PB; u += 256) {
-        const int r = u / PB, c = u % PB;
-        S[r][c] = (r < nb && c <= r) ? A[(long)(k + r) * lda + k + c] : (r == c ? 1.0 : 0.0);
+    // the block in sixteen loads a thread at a time, all in flight together
+    for (int u0 = 0; u0 < PB * PB; u0 += 256 * 16) {
+        double v[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int u = u0 + tid + 256 * i, r = u / PB, c = u % PB;
+            v[i] = (r < nb && c <= r) ? A[(long)(k + r) * lda + k + c] : (r == c ? 1.0 : 0.0);
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int u = u0 + tid + 256 * i;
+            S[u / PB][u % PB] = v[i];
+        }
     }
     __syncthreads();
-    for (int j = 0; j < nb; ++j) {
-        // thread i >= j: t_i = A[i][j] - sum_{m<j} L[i][m] L[j][m]
-        const int i = j + tid;
-        double t = 0.0;
-        if (i < nb) t = S[i][j] - ldot(&S[i][0], &S[j][0], j);
-        if (tid == 0) {
-            if (!(t > 0.0) && *info == 0) *info = k + j + 1;
-            const double l = sqrt(t > 0.0 ? t : 1.0);
-            S[j][j] = l;
-            dg[j] = 1.0 / l;
+    for (int kb = 0; kb < ND; ++kb) {
+        const int o = kb * DB;
+        // (a) the diagonal 16 x 16 block; lane r holds row o + r (lanes 16..63
+        // repeat rows 0..15 so that every shuffle source is live)
+        if (tid < 64) {
+            const int r = tid & 15;
+            double d[DB];
+#pragma unroll
+            for (int c = 0; c < DB; ++c) d[c] = S[o + r][o + c];
+#pragma unroll
+            for (int j = 0; j < DB; ++j) {
+                const double djj = __shfl(d[j], j);
+                if (tid == j && !(djj > 0.0) && *info == 0) *info = k + o + j + 1;
+                const double l = sqrt(djj > 0.0 ? djj : 1.0);
+                const double il = 1.0 / l;
+                if (r == j) d[j] = l;
+                else if (r > j) d[j] = d[j] * il;
+#pragma unroll
+                for (int c = j + 1; c < DB; ++c) {
+                    const double lcj = __shfl(d[j], c);   // L[o + c][o + j]
+                    if (c <= r) d[c] -= d[j] * lcj;
+                }
+                if (tid == 0) dg[o + j] = il;
+            }
+            if (tid < DB) {
+#pragma unroll
+                for (int c = 0; c < DB; ++c)
+                    if (c <= r) S[o + r][o + c] = d[c];
+            }
         }
         __syncthreads();
-        if (tid > 0 && i < nb) S[i][j] = t * dg[j];
+        const int t0 = o + DB, T = PB - t0;
+        if (T == 0) break;
+        // (b) the panel below: row t0 + t solves x L11^T = a
+        if (tid < T) {
+            const int row = t0 + tid;
+            double x[DB];
+#pragma unroll
+            for (int c = 0; c < DB; ++c) x[c] = S[row][o + c];
+#pragma unroll
+            for (int c = 0; c < DB; ++c) {
+                double v = x[c];
+#pragma unroll
+                for (int m = 0; m < c; ++m) v -= x[m] * S[o + c][o + m];
+                x[c] = v * dg[o + c];
+            }
+#pragma unroll
+            for (int c = 0; c < DB; ++c) S[row][o + c] = x[c];
+        }
+        __syncthreads();
+        // (c) trailing lower triangle -= L21 L21^T in 4 x 4 tiles
+        const int tq = T / 4, ntile = tq * (tq + 1) / 2;
+        for (int u = tid; u < ntile; u += 256) {
+            int ti = (int)((sqrt(8.0 * (double)u + 1.0) - 1.0) * 0.5);
+            while ((ti + 1) * (ti + 2) / 2 <= u) ++ti;
+            while (ti * (ti + 1) / 2 > u) --ti;
+            const int tj = u - ti * (ti + 1) / 2;
+            const int r0 = t0 + 4 * ti, c0 = t0 + 4 * tj;
+            double acc[4][4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+#pragma unroll 4
+            for (int m = 0; m < DB; ++m) {
+                double av[4], bv[4];
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    av[a] = S[r0 + a][o + m];
+                    bv[a] = S[c0 + a][o + m];
+                }
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) acc[a][b] += av[a] * bv[b];
+            }
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    if (c0 + b <= r0 + a) S[r0 + a][c0 + b] -= acc[a][b];
+        }
         __syncthreads();
     }
-    for (int j = nb; j < PB; ++j)
-        if (tid == 0) dg[j] = 1.0;
     for (int u = tid; u < nb * nb; u += 256) {
         const int r = u / nb, c = u % nb;
         if (c <= r) A[(long)(k + r) * lda + k + c] = S[r][c];
     }
-    __syncthreads();
-    // X = L^-1 by columns: thread c < PB solves L x = e_c.  X[i][c] (i > c)
-    // is kept at S[c][i], the free upper triangle; X[c][c] = dg[c].  Column
-    // c of X is row c of S, which only thread c writes:
-    //   X[i][c] = -(L[i][c] dg[c] + sum_{c < m < i} L[i][m] X[m][c]) dg[i]
+    // (d) the diagonal blocks of X: thread (jb, b) solves column b of block jb
+    // (the sums over m unrolled, so their LDS loads are in flight together)
     if (tid < PB) {
-        const int c = tid;
-        for (int i = c + 1; i < PB; ++i) {
-            const double sum = S[i][c] * dg[c] + ldot(&S[i][c + 1], &S[c][c + 1], i - c - 1);
-            S[c][i] = -sum * dg[i];
+        const int base = (tid / DB) * DB, b = tid % DB;
+        for (int a = b + 1; a < DB; ++a) {
+            double s0 = S[base + a][base + b] * dg[base + b], s1 = 0.0;
+#pragma unroll
+            for (int m = 1; m < DB; m += 2) {
+                if (m > b && m < a) s0 += S[base + a][base + m] * S[base + b][base + m];
+                if (m + 1 > b && m + 1 < a && m + 1 < DB) s1 += S[base + a][base + m + 1] * S[base + b][base + m + 1];
+            }
+            S[base + b][base + a] = -(s0 + s1) * dg[base + a];
         }
     }
     __syncthreads();
+    // (e) block row i: Y = sum_{j <= m < i} L_im X_mj into the slot of X_ij,
+    // then X_ij = -X_ii Y (each thread keeps its entries through a barrier).
+    // Within block j, X_mj (m >= C) is dg at m = C and S[C][m] above it; in
+    // the blocks past j every X_mj sits at S[C][m].
+    for (int ib = 1; ib < ND; ++ib) {
+        const int ne = ib * DB * DB;
+        for (int e = tid; e < ne; e += 256) {
+            const int jb = e / (DB * DB), a = (e / DB) % DB, b = e % DB;
+            const int R = ib * DB + a, C = jb * DB + b;
+            const double *Lr = &S[R][0], *Xc = &S[C][0];
+            double y0 = Lr[C] * dg[C], y1 = 0.0, y2 = 0.0, y3 = 0.0;
+#pragma unroll
+            for (int mm = 1; mm < DB; ++mm)
+                if (mm > b) y1 += Lr[jb * DB + mm] * Xc[jb * DB + mm];
+            for (int kb2 = jb + 1; kb2 < ib; ++kb2) {
+                const int mb = kb2 * DB;
+#pragma unroll
+                for (int mm = 0; mm < DB; mm += 4) {
+                    y0 += Lr[mb + mm] * Xc[mb + mm];
+                    y1 += Lr[mb + mm + 1] * Xc[mb + mm + 1];
+                    y2 += Lr[mb + mm + 2] * Xc[mb + mm + 2];
+                    y3 += Lr[mb + mm + 3] * Xc[mb + mm + 3];
+                }
+            }
+            S[C][R] = (y0 + y1) + (y2 + y3);
+        }
+        __syncthreads();
+        constexpr int NX = (ND - 1) * DB * DB / 256;   // entries a thread owns, at most
+        double xr[NX];
+#pragma unroll
+        for (int q = 0; q < NX; ++q) {
+            const int e = tid + 256 * q;
+            xr[q] = 0.0;
+            if (e < ne) {
+                const int jb = e / (DB * DB), a = (e / DB) % DB, b = e % DB;
+                const int R = ib * DB + a, C = jb * DB + b;
+                // X_ii[a][m] = S[ib DB + m][R] below the diagonal, dg[R] on it
+                double x0 = dg[R] * S[C][R], x1 = 0.0;
+#pragma unroll
+                for (int m = 0; m < DB; m += 2) {
+                    if (m < a) x0 += S[ib * DB + m][R] * S[C][ib * DB + m];
+                    if (m + 1 < a) x1 += S[ib * DB + m + 1][R] * S[C][ib * DB + m + 1];
+                }
+                xr[q] = -(x0 + x1);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NX; ++q) {
+            const int e = tid + 256 * q;
+            if (e < ne) {
+                const int jb = e / (DB * DB), a = (e / DB) % DB, b = e % DB;
+                S[jb * DB + b][ib * DB + a] = xr[q];
+            }
+        }
+        __syncthreads();
+    }
     for (int u = tid; u < PB * PB; u += 256) {
         const int r = u / PB, c = u % PB;
         const double x = r == c ? dg[r] : (r > c ? S[c][r] : 0.0);   // X[r][c]
