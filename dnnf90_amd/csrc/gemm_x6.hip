@@ -31,6 +31,10 @@ namespace dnnf90 {
 #ifndef LIN_X6_RBT_MAX
 #define LIN_X6_RBT_MAX 8
 #endif
+// adjacent 16-column blocks a wave streams per step
+#ifndef LIN_X6_NCB
+#define LIN_X6_NCB 2
+#endif
 
 // ---------------------------------------------------------------------------
 // K1 / K6.  The workgroup's 16*RBT rows of A (K1: the weight columns after
@@ -173,7 +177,7 @@ template <bool kTrans, bool kAcc64, int RBT>
 static void lin_x6_go(const float *W, int ldw, const float *B, float *C, int M, int Kd, long ncol, long Np,
                       hipStream_t s)
 {
-    constexpr int NCB = 2, KP = LIN_X6_KP;
+    constexpr int NCB = LIN_X6_NCB, KP = LIN_X6_KP;
     const int nkc = (Kd + 31) / 32;
     const size_t lds = (size_t)RBT * nkc * 3 * 64 * 16 + 16 * RBT * sizeof(float);
     static size_t lds_set = 64 * 1024;   // dynamic LDS above 64 KiB must be requested
