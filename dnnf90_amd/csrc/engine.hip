@@ -504,6 +504,11 @@ static void backward_t(const Net &nt, BatchWork &b, double *grad, float *rows_t,
         if (rows_t) {
             PScope p(b, CAT_GRAD);
             launch_point_rows<T>((const T *)b.p0, X, nd, ndm + 1, na, Np, rows_t + nt.woff[l] * ldr, ldr, s);
+        } else if (b.npt == 1) {
+            // one point (the per-point ABI, the Kalman filter): one thread a
+            // weight instead of the batched chunked product and its reduction
+            PScope p(b, CAT_GRAD);
+            launch_grad_point1<T>((const T *)b.p0, X, nd, ndm + 1, na, Np, grad + nt.woff[l], s);
         } else {
             int nc;
             long ch;

@@ -701,7 +701,49 @@ void launch_point_rows(const T *A, const T *B, int M, int Kd, int na, long Np, f
     hipLaunchKernelGGL((k_point_rows<T>), grid, dim3(256), 0, s, A, B, M, Kd, na, Np, rowsT, ldr);
 }
 
+// weight gradient of one point (column 0 of every slot), added to grad[M][Kd]:
+// the bias column takes slot 0 of Sbar, weight column k the sum over slots
+// in slot order, in FP64 -- one thread per weight, no partials, no reduction
+template <typename T>
+__global__ __launch_bounds__(256) void k_grad_point1(const T *__restrict__ A, const T *__restrict__ B, int M, int Kd,
+                                                     int na, long Np, double *__restrict__ grad)
+{
+    const int u = blockIdx.x * 256 + threadIdx.x;
+    if (u >= M * Kd) return;
+    const int j = u / Kd, k = u - j * Kd;
+    const long ldc = (long)na * Np;
+    double s;
+    if (k == 0) {
+        s = (double)A[(long)j * ldc];
+    } else {
+        // eight slots' loads in flight at a time, then their terms in order
+        const T *pa = A + (long)j * ldc, *pb = B + (long)(k - 1) * ldc;
+        s = 0.0;
+        int a = 0;
+        for (; a + 8 <= na; a += 8) {
+            double x[8], y[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                x[e] = (double)pa[(long)(a + e) * Np];
+                y[e] = (double)pb[(long)(a + e) * Np];
+            }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) s += x[e] * y[e];
+        }
+        for (; a < na; ++a) s += (double)pa[(long)a * Np] * (double)pb[(long)a * Np];
+    }
+    grad[u] += s;
+}
+
+template <typename T>
+void launch_grad_point1(const T *A, const T *B, int M, int Kd, int na, long Np, double *grad, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_grad_point1<T>), dim3((unsigned)((M * Kd + 255) / 256)), dim3(256), 0, s, A, B, M, Kd, na,
+                       Np, grad);
+}
+
 #define INST(T)                                                                                             \
+    template void launch_grad_point1<T>(const T *, const T *, int, int, int, long, double *, hipStream_t);   \
     template void launch_gemm_wx<T>(int, bool, const T *, int, const T *, T *, int, int, long, long,         \
                                     hipStream_t);                                                           \
     template void launch_gemm_grad<T>(int, const T *, const T *, int, int, long, long, double *, int, long,  \

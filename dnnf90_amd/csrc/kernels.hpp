@@ -133,6 +133,9 @@ void launch_gram(const float *R, int nw, long ld, double *partial, int nchunk, l
                  hipStream_t s);
 
 // per-point rows: rows[(j*Kd + k)][n] = sum_a A[j][a*Np+n] * Bext[k][a*Np+n]
+// K5 of a single point (column 0 of every slot), added to grad[M][Kd] in FP64
+template <typename T>
+void launch_grad_point1(const T *A, const T *B, int M, int Kd, int na, long Np, double *grad, hipStream_t s);
 template <typename T>
 void launch_point_rows(const T *A, const T *B, int M, int Kd, int na, long Np, float *rowsT,
                        long ldr, hipStream_t s);
