@@ -158,3 +158,34 @@ def test_tabset_free_returns_handles(L):
         assert t.id > 0
         t.free()
     assert L.dnnf90_net_free(nid) == 0
+
+
+def test_filter_committee_value_argument_checks(L):
+    """The extension entries refuse bad arguments before any device work
+    (kf_init's own checks, lib/kalman_module.f90:118-128)."""
+    import dnnf90_amd as D
+    ts = D.TableSet(2, 3, [[0, 1], [1, 0], [3, 0]])
+    dims = np.array([2, 4, 1], np.int32)
+    nid = L.dnnf90_net_create_ts(ts.id, 3, dims.ctypes.data)
+    assert nid > 0
+    for p0, lam, lam0 in ((0.0, 1.0, 1.0), (-1.0, 1.0, 1.0), (float("nan"), 1.0, 1.0), (1.0, 0.0, 1.0),
+                          (1.0, 1.5, 1.0), (1.0, 1.0, 0.0), (1.0, 1.0, 1.01)):
+        assert L.dnnf90_kf_create(nid, p0, lam, lam0, 0) == -1, (p0, lam, lam0)
+    assert L.dnnf90_kf_create(nid + 50, 1.0, 1.0, 1.0, 0) == -1
+    assert L.dnnf90_kf_set(60, 0.0, 0.0) == -1
+    assert L.dnnf90_kf_free(60) == -1
+    assert L.dnnf90_kf_get(60, None, None) == -1
+    assert L.dnnf90_kf_update_grad(nid, 60, 60, 0.0, 1.0) == -1
+    assert L.dnnf90_kf_iekf_begin(nid, 60) == -1
+    x = np.zeros(2)
+    assert L.dnnf90_kf_update_obs(nid, 60, 60, 1, x.ctypes.data, x.ctypes.data, x.ctypes.data, None, 0) == -1
+    ids = np.array([nid], np.int32)
+    out = np.zeros(16)
+    assert L.dnnf90_comm_eval_batch(0, ids.ctypes.data, 60, 1, x.ctypes.data, out.ctypes.data,
+                                    out.ctypes.data, 0) == -1
+    assert L.dnnf90_comm_eval_batch(1, ids.ctypes.data, 60, 1, x.ctypes.data, out.ctypes.data,
+                                    out.ctypes.data, 0) == -1
+    assert L.dnnf90_grad_batch(nid, 60, 1, x.ctypes.data, x.ctypes.data, 1.0, 60, 0) == -1
+    assert L.dnnf90_tabset_write_order(60, b"/dev/null") == -1
+    assert L.dnnf90_net_free(nid) == 0
+    ts.free()
