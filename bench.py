@@ -44,15 +44,18 @@ def traffic_of(workload, mode, category):
         return None
 
 
-def algorithmic(wl, na, F, Q, P, N, wbytes):
+def algorithmic(wl, na, F, Q, P, N, wbytes, keep_s):
     """Per-step algorithmic work of each kernel category: (flops, bytes).
 
     K1 and K5 are 2 n_l (n_{l-1}+1) |A| flops per point and layer, K6
     2 n_l n_{l-1} |A| for l > 2; K1 reads T of the layer below and writes S,
     K5 reads S-bar and T of the layer below, K6 reads S-bar and writes T-bar
-    (|A| words per row and point).  The Faa di Bruno forward is
+    (|A| words per row and point).  Keeping h, the Faa di Bruno forward is
     n (3F + 4Q + 3(|A|-1)) flops and reads S, writes T and h; the adjoint is
-    n (3P + 2|A|) flops and reads T-bar and h, writes S-bar."""
+    n (3P + 2|A|) flops and reads T-bar and h, writes S-bar.  Keeping S
+    (keep_s), the forward is n (3F + 2Q + (|A|-1)) flops and reads S, writes
+    T; the adjoint recomputes h (n (3F + 2Q + (|A|-1)) flops more) and reads
+    T-bar and S, writes S-bar."""
     d = wl.dims
     L = len(d)
     w = float(wbytes) * na * N
@@ -61,10 +64,15 @@ def algorithmic(wl, na, F, Q, P, N, wbytes):
     lin_t = sum(2.0 * d[l] * d[l - 1] * na for l in range(2, L)) * N
     lin_t_b = sum(d[l] + d[l - 1] for l in range(2, L)) * w
     hid = sum(d[l] for l in range(1, L - 1))
-    bf_flops = hid * (3.0 * F + 4.0 * Q + 3.0 * (na - 1)) * N
     bb_flops = hid * (3.0 * P + 2.0 * na) * N
+    if keep_s:
+        h_flops = hid * (3.0 * F + 2.0 * Q + (na - 1)) * N
+        bell_fwd, bell_bwd = (h_flops, 2.0 * hid * w), (bb_flops + h_flops, 3.0 * hid * w)
+    else:
+        bell_fwd = (hid * (3.0 * F + 4.0 * Q + 3.0 * (na - 1)) * N, 3.0 * hid * w)
+        bell_bwd = (bb_flops, 3.0 * hid * w)
     return {"linear": (lin, lin_b), "linear_t": (lin_t, lin_t_b), "grad": (lin, lin_b),
-            "bell_fwd": (bf_flops, 3.0 * hid * w), "bell_bwd": (bb_flops, 3.0 * hid * w)}
+            "bell_fwd": bell_fwd, "bell_bwd": bell_bwd}
 
 
 def rel_by_order(x, ref, deg):
@@ -341,6 +349,9 @@ def main():
     def run_mode(prec, name, wbytes, mma_peak, budget=0):
         bw = D.BatchWork(net, N, prec, budget=budget)
         info = bw.info()
+        kinfo = np.zeros(6, np.int32)
+        lib().dnnf90_net_kernel_info(net.id, prec, kinfo.ctypes.data)
+        keep_s = int(kinfo[0]) == 2
         g = D.Grad(net)
         s = torch.cuda.Stream(device=dev)
         bw.set_stream(s.cuda_stream)
@@ -371,7 +382,7 @@ def main():
         bw.set_stream(None)
         bw.free()
         g.free()
-        work = algorithmic(wl, na, F, Q, P, N, wbytes)
+        work = algorithmic(wl, na, F, Q, P, N, wbytes, keep_s)
         kern = {}
         for c, (tot, cnt) in kms.items():
             ent = {"ms_per_step": tot / args.steps, "launches_per_step": cnt / args.steps}
@@ -390,6 +401,7 @@ def main():
                 ent["algorithmic_flops_per_step"] = flops
                 ent["algorithmic_bytes_per_step"] = byts
             kern[c] = ent
+        info["keep_s"] = keep_s
         return {"name": name, "ms_per_step": ms, "wall_ms_per_step": wall * 1e3 / args.steps, "info": info,
                 "points_per_s": N / (ms * 1e-3), "kernels": kern, "T": t_host, "grad": grad, "rsq": rsq}
 
@@ -466,6 +478,8 @@ def main():
         "speedup_vs_fp64": mixed["points_per_s"] / fp64["points_per_s"],
         "err_vs_fp64": {"forward_by_order": err_T, "grad_max_rel": err_g},
         "kernels": mixed["kernels"],
+        "kept_plane": {"fp64": "S" if fp64["info"]["keep_s"] else "h",
+                       args.mixed: "S" if mixed["info"]["keep_s"] else "h"},
         "recompute": recompute,
         "modes": {k: {"points_per_s": r["points_per_s"], "ms_per_step": r["ms_per_step"],
                       "speedup_vs_fp64": r["points_per_s"] / fp64["points_per_s"],

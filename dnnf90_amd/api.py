@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -18,6 +19,18 @@ PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_BF16X6 = 0, 1, 2, 3
 DEVICE_PTRS = 1
 NGD_DUAL = 2
 ACT_TANH, ACT_SIN, ACT_ERF, ACT_J0, ACT_J1, ACT_GELU = 0, 1, 2, 3, 4, 5
+
+
+def _free_on_del(self):
+    """__del__ of every handle class: the library holds 64 handles of each
+    kind, so the slot goes back when the Python object goes.  An object built
+    on a network holds a reference to it, so a network outlives its
+    dependents.  Nothing is released at interpreter exit."""
+    try:
+        if not sys.is_finalizing():
+            self.free()
+    except Exception:
+        pass
 
 
 class DNNF90Error(RuntimeError):
@@ -75,13 +88,7 @@ class TableSet:
             lib().dnnf90_tabset_free(self.id)
             self.id = 0
 
-    def __del__(self):
-        # the library holds 64 table sets at most; hand the slot back when the
-        # Python object goes (host memory only, no GPU call)
-        try:
-            self.free()
-        except Exception:
-            pass
+    __del__ = _free_on_del
 
 
 def current_alpha():
@@ -145,9 +152,11 @@ class Net:
         _chk(lib().dnnf90_net_set_weights(self.id, p), "dnnf90_net_set_weights")
 
     def free(self):
-        if self.id:
+        if getattr(self, "id", 0):
             lib().dnnf90_net_free(self.id)
             self.id = 0
+
+    __del__ = _free_on_del
 
 
 class Work:
@@ -164,7 +173,11 @@ class Work:
         return t
 
     def free(self):
-        lib().dnnf90_work_free(self.id)
+        if getattr(self, "id", 0):
+            lib().dnnf90_work_free(self.id)
+            self.id = 0
+
+    __del__ = _free_on_del
 
 
 class Grad:
@@ -187,7 +200,11 @@ class Grad:
         _chk(lib().dnnf90_grad_set(self.id, p), "dnnf90_grad_set")
 
     def free(self):
-        lib().dnnf90_grad_free(self.id)
+        if getattr(self, "id", 0):
+            lib().dnnf90_grad_free(self.id)
+            self.id = 0
+
+    __del__ = _free_on_del
 
 
 class TWork:
@@ -203,7 +220,11 @@ class TWork:
         _chk(lib().dnnf90_grad_point(self.net.id, self.id, px, ps, grad.id), "dnnf90_grad_point")
 
     def free(self):
-        lib().dnnf90_twork_free(self.id)
+        if getattr(self, "id", 0):
+            lib().dnnf90_twork_free(self.id)
+            self.id = 0
+
+    __del__ = _free_on_del
 
 
 def adam_step(net: Net, grad: Grad, eta, beta1, beta2, eps, nbatch, istep):
@@ -224,8 +245,8 @@ class Residual:
 class BatchWork:
     """Batched work space of up to maxpt points (dnnf90_bwork_create).
 
-    budget: device bytes; when keeping every hidden layer's h would pass it,
-    the reverse pass recomputes each hidden layer's S and h instead
+    budget: device bytes; when keeping a plane (S or h) of every hidden
+    layer would pass it, the reverse pass recomputes that plane instead
     (dnnf90_bwork_create_budget).  0 keeps everything."""
 
     def __init__(self, net: Net, maxpt, prec=PREC_FP64, budget=0):
@@ -341,9 +362,11 @@ class BatchWork:
         return u
 
     def free(self):
-        if self.id:
+        if getattr(self, "id", 0):
             lib().dnnf90_bwork_free(self.id)
             self.id = 0
+
+    __del__ = _free_on_del
 
 
 class KalmanFilter:
@@ -423,6 +446,8 @@ class KalmanFilter:
         if getattr(self, "id", 0):
             lib().dnnf90_kf_free(self.id)
             self.id = 0
+
+    __del__ = _free_on_del
 
 
 class Committee:

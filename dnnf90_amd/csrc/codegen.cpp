@@ -42,38 +42,21 @@ static int kq_of(int k)
     return k <= 3 ? 5 : k <= 4 ? 6 : k <= 7 ? 9 : 17;
 }
 
-std::string gen_bell_source(const TableSet &t, const FwdSchedule &s, bool fp64)
-{
-    std::ostringstream o;
-    const int na = t.na, kq = kq_of(t.k);
-    const bool bes = t.iact == ACT_BESSEL_J0 || t.iact == ACT_BESSEL_J1;
-    // hipRTC supplies the HIP device environment itself (no runtime header)
-    o << "struct ActParams { int iact, nq; const double *tpc; int ldt; };\n"
-      << kActSrc << "\n"
-      << "typedef " << (fp64 ? "double" : "float") << " real;\n";
+// The recurrence over slots 1..na-1 of one lane: for every slot the partial
+// Bell sums v of each q (terms in the reference's order, stored entries kept
+// as registers), T_a = d1 S_a + sum_q d_q v and h_a = d2 S_a + sum_q d_(q+1) v.
+// REC_T stores T_a, REC_H stores h_a, REC_HREG assigns h_a to register h<a>.
+enum { REC_T = 1, REC_H = 2, REC_HREG = 4 };
 
-    // ---- forward: T_a and h_a of every slot from the S_a of one lane
-    const char *sfx = fp64 ? "_f64" : "_f32";
-    o << "extern \"C\" __global__ __launch_bounds__(256) void dnnf90_bell_fwd" << sfx << "(const real *__restrict__ S, "
-         "real *__restrict__ Tout, real *__restrict__ H, int nrow, long Np, ActParams ap)\n{\n"
-      << "  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;\n"
-      << "  const int j = blockIdx.y * (blockDim.x >> 6) + wave;\n"
-      << "  if (j >= nrow) return;\n"
-      << "  const long n = (long)blockIdx.x * 64 + lane;\n"
-      << "  const long ldr = " << na << "L * Np;\n"
-      << "  const real *Sj = S + (long)j * ldr + n;\n"
-      << "  real *Tj = Tout + (long)j * ldr + n;\n"
-      << "  real *Hj = H + (long)j * ldr + n;\n";
-    for (int a = 0; a < na; ++a) o << "  const real e" << a << " = Sj[" << a << "L * Np];\n";
-    o << "  double d[" << kq << "];\n"
-      << "  act_derivs<" << kq << ", " << (bes ? "true" : "false") << ">(ap, (double)e0, d);\n";
-    for (int q = 0; q < kq; ++q) o << "  const real d" << q << " = (real)d[" << q << "];\n";
-    o << "  Tj[0] = d0;\n  Hj[0] = d1;\n";
-    // stored entries are function-scope registers
-    for (int e = na; e < s.nentry; ++e) o << "  real e" << e << ";\n";
-    for (int a = 1; a < na; ++a) {
+static void emit_recurrence(std::ostringstream &o, const TableSet &t, const FwdSchedule &s, bool fp64, int m)
+{
+    const bool wt = (m & REC_T) != 0, wh = (m & (REC_H | REC_HREG)) != 0;
+    for (int e = t.na; e < s.nentry; ++e) o << "  real e" << e << ";\n";
+    for (int a = 1; a < t.na; ++a) {
         const int g0 = s.slot_group[a], g1 = s.slot_group[a + 1];
-        o << "  {\n    real t = d1 * e" << a << ", h = d2 * e" << a << ";\n";
+        o << "  {\n";
+        if (wt) o << "    real t = d1 * e" << a << ";\n";
+        if (wh) o << "    real h = d2 * e" << a << ";\n";
         for (int g = g0; g < g1; ++g) {
             const int q = s.grp_q[g];
             o << "    {\n      real v = (real)0;\n";
@@ -81,35 +64,103 @@ std::string gen_bell_source(const TableSet &t, const FwdSchedule &s, bool fp64)
                 o << "      v += " << lit(s.term_c[it], fp64) << " * e" << s.term_s[it] << " * e" << s.term_b[it]
                   << ";\n";
             if (s.grp_store[g] >= 0) o << "      e" << s.grp_store[g] << " = v;\n";
-            o << "      t += d" << q << " * v;\n      h += d" << q + 1 << " * v;\n    }\n";
+            if (wt) o << "      t += d" << q << " * v;\n";
+            if (wh) o << "      h += d" << q + 1 << " * v;\n";
+            o << "    }\n";
         }
-        o << "    Tj[" << a << "L * Np] = t;\n    Hj[" << a << "L * Np] = h;\n  }\n";
+        if (m & REC_T) o << "    Tj[" << a << "L * Np] = t;\n";
+        if (m & REC_H) o << "    Hj[" << a << "L * Np] = h;\n";
+        if (m & REC_HREG) o << "    h" << a << " = h;\n";
+        o << "  }\n";
     }
-    o << "}\n";
+}
 
-    // ---- adjoint: Sbar from Tbar and h of one lane
-    o << "extern \"C\" __global__ __launch_bounds__(256) void dnnf90_bell_bwd" << sfx << "(const real *__restrict__ Tb, "
-         "const real *__restrict__ H, real *__restrict__ Sb, int nrow, long Np)\n{\n"
-      << "  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;\n"
-      << "  const int j = blockIdx.y * (blockDim.x >> 6) + wave;\n"
-      << "  if (j >= nrow) return;\n"
-      << "  const long n = (long)blockIdx.x * 64 + lane;\n"
-      << "  const long ldr = " << na << "L * Np;\n"
-      << "  const real *tb = Tb + (long)j * ldr + n;\n"
-      << "  const real *hh = H + (long)j * ldr + n;\n"
-      << "  real *sb = Sb + (long)j * ldr + n;\n";
-    for (int a = 0; a < na; ++a)
-        o << "  const real t" << a << " = tb[" << a << "L * Np], h" << a << " = hh[" << a << "L * Np];\n";
+// Sbar_0 = sum_a Tbar_a h_a and the pair sums Sbar_b = sum binom Tbar_ia h_id
+// (lib/multi_index_bell_module.f90:267-297) from registers t<a>, h<a>
+static void emit_pair_sums(std::ostringstream &o, const TableSet &t, bool fp64)
+{
     o << "  {\n    real v = t0 * h0;\n";
-    for (int a = 1; a < na; ++a) o << "    v += t" << a << " * h" << a << ";\n";
+    for (int a = 1; a < t.na; ++a) o << "    v += t" << a << " * h" << a << ";\n";
     o << "    sb[0] = v;\n  }\n";
-    for (int b = 1; b < na; ++b) {
+    for (int b = 1; b < t.na; ++b) {
         o << "  {\n    real v = (real)0;\n";
         for (int ip = t.pair_start[b]; ip < t.pair_start[b] + t.pair_num[b]; ++ip)
             o << "    v += " << lit(t.pair_binom[ip], fp64) << " * t" << t.pair_ia[ip] << " * h" << t.pair_id[ip]
               << ";\n";
         o << "    sb[" << b << "L * Np] = v;\n  }\n";
     }
+}
+
+std::string gen_bell_source(const TableSet &t, const FwdSchedule &s, bool fp64)
+{
+    std::ostringstream o;
+    const int na = t.na, kq = kq_of(t.k);
+    const bool bes = t.iact == ACT_BESSEL_J0 || t.iact == ACT_BESSEL_J1;
+    const char *sfx = fp64 ? "_f64" : "_f32";
+    // hipRTC supplies the HIP device environment itself (no runtime header)
+    o << "struct ActParams { int iact, nq; const double *tpc; int ldt; };\n"
+      << kActSrc << "\n"
+      << "typedef " << (fp64 ? "double" : "float") << " real;\n";
+    auto head = [&](const char *name, const char *args) {
+        o << "extern \"C\" __global__ __launch_bounds__(256) void " << name << sfx << "(" << args << ")\n{\n"
+          << "  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;\n"
+          << "  const int j = blockIdx.y * (blockDim.x >> 6) + wave;\n"
+          << "  if (j >= nrow) return;\n"
+          << "  const long n = (long)blockIdx.x * 64 + lane;\n"
+          << "  const long ldr = " << na << "L * Np;\n";
+    };
+    // S of one lane and the activation derivatives sigma^(q)(S_0), q < kq, in FP64
+    auto load_s = [&]() {
+        for (int a = 0; a < na; ++a) o << "  const real e" << a << " = Sj[" << a << "L * Np];\n";
+        o << "  double d[" << kq << "];\n"
+          << "  act_derivs<" << kq << ", " << (bes ? "true" : "false") << ">(ap, (double)e0, d);\n";
+        for (int q = 0; q < kq; ++q) o << "  const real d" << q << " = (real)d[" << q << "];\n";
+    };
+    // ---- forward: T_a and h_a of every slot from the S_a of one lane
+    head("dnnf90_bell_fwd", "const real *__restrict__ S, real *__restrict__ Tout, real *__restrict__ H, int nrow, "
+                            "long Np, ActParams ap");
+    o << "  const real *Sj = S + (long)j * ldr + n;\n"
+      << "  real *Tj = Tout + (long)j * ldr + n;\n"
+      << "  real *Hj = H + (long)j * ldr + n;\n";
+    load_s();
+    o << "  Tj[0] = d0;\n  Hj[0] = d1;\n";
+    emit_recurrence(o, t, s, fp64, REC_T | REC_H);
+    o << "}\n";
+
+    // ---- adjoint: Sbar from Tbar and h of one lane
+    head("dnnf90_bell_bwd", "const real *__restrict__ Tb, const real *__restrict__ H, real *__restrict__ Sb, int nrow, "
+                            "long Np");
+    o << "  const real *tb = Tb + (long)j * ldr + n;\n"
+      << "  const real *hh = H + (long)j * ldr + n;\n"
+      << "  real *sb = Sb + (long)j * ldr + n;\n";
+    for (int a = 0; a < na; ++a)
+        o << "  const real t" << a << " = tb[" << a << "L * Np], h" << a << " = hh[" << a << "L * Np];\n";
+    emit_pair_sums(o, t, fp64);
+    o << "}\n";
+
+    // ---- forward, T only: the layer keeps S and the adjoint recomputes h
+    head("dnnf90_bell_fwdt", "const real *__restrict__ S, real *__restrict__ Tout, int nrow, long Np, ActParams ap");
+    o << "  const real *Sj = S + (long)j * ldr + n;\n"
+      << "  real *Tj = Tout + (long)j * ldr + n;\n";
+    load_s();
+    o << "  Tj[0] = d0;\n";
+    emit_recurrence(o, t, s, fp64, REC_T);
+    o << "}\n";
+
+    // ---- adjoint from Tbar and S: h of every slot recomputed in registers
+    // (the same expressions as the forward above, so bit-identical), then
+    // the pair sums
+    head("dnnf90_bell_bwds", "const real *__restrict__ Tb, const real *__restrict__ S, real *__restrict__ Sb, int nrow, "
+                             "long Np, ActParams ap");
+    o << "  const real *Sj = S + (long)j * ldr + n;\n"
+      << "  const real *tb = Tb + (long)j * ldr + n;\n"
+      << "  real *sb = Sb + (long)j * ldr + n;\n";
+    load_s();
+    o << "  const real h0 = d1;\n";
+    for (int a = 1; a < na; ++a) o << "  real h" << a << ";\n";
+    emit_recurrence(o, t, s, fp64, REC_HREG);
+    for (int a = 0; a < na; ++a) o << "  const real t" << a << " = tb[" << a << "L * Np];\n";
+    emit_pair_sums(o, t, fp64);
     o << "}\n";
     return o.str();
 }
@@ -151,9 +202,12 @@ int gen_bell_load(const std::vector<char> &code, bool fp64, GenKernels *g)
 {
     *g = GenKernels();
     g->code = code;
-    if (hipModuleLoadData(&g->mod, g->code.data()) != hipSuccess ||
-        hipModuleGetFunction(&g->fwd, g->mod, fp64 ? "dnnf90_bell_fwd_f64" : "dnnf90_bell_fwd_f32") != hipSuccess ||
-        hipModuleGetFunction(&g->bwd, g->mod, fp64 ? "dnnf90_bell_bwd_f64" : "dnnf90_bell_bwd_f32") != hipSuccess) {
+    const std::string sfx = fp64 ? "_f64" : "_f32";
+    auto get = [&](hipFunction_t *f, const char *name) {
+        return hipModuleGetFunction(f, g->mod, (name + sfx).c_str()) == hipSuccess;
+    };
+    if (hipModuleLoadData(&g->mod, g->code.data()) != hipSuccess || !get(&g->fwd, "dnnf90_bell_fwd") ||
+        !get(&g->bwd, "dnnf90_bell_bwd") || !get(&g->fwdt, "dnnf90_bell_fwdt") || !get(&g->bwds, "dnnf90_bell_bwds")) {
         // the table-driven kernels take over; leave no error behind for
         // the next launch check
         (void)hipGetLastError();
@@ -164,6 +218,10 @@ int gen_bell_load(const std::vector<char> &code, bool fp64, GenKernels *g)
     (void)hipFuncGetAttribute(&g->bwd_scratch, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, g->bwd);
     (void)hipFuncGetAttribute(&g->fwd_regs, HIP_FUNC_ATTRIBUTE_NUM_REGS, g->fwd);
     (void)hipFuncGetAttribute(&g->bwd_regs, HIP_FUNC_ATTRIBUTE_NUM_REGS, g->bwd);
+    (void)hipFuncGetAttribute(&g->fwdt_scratch, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, g->fwdt);
+    (void)hipFuncGetAttribute(&g->bwds_scratch, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, g->bwds);
+    (void)hipFuncGetAttribute(&g->fwdt_regs, HIP_FUNC_ATTRIBUTE_NUM_REGS, g->fwdt);
+    (void)hipFuncGetAttribute(&g->bwds_regs, HIP_FUNC_ATTRIBUTE_NUM_REGS, g->bwds);
     g->ok = true;
     return 0;
 }
@@ -172,7 +230,7 @@ void GenKernels::release()
 {
     if (mod) (void)hipModuleUnload(mod);
     mod = nullptr;
-    fwd = bwd = nullptr;
+    fwd = bwd = fwdt = bwds = nullptr;
     ok = false;
 }
 
@@ -190,6 +248,24 @@ hipError_t gen_launch_bwd(const GenKernels &g, const void *Tb, const void *H, vo
 {
     void *params[] = {&Tb, &H, &Sb, &nrow, &Np};
     return hipModuleLaunchKernel(g.bwd, (unsigned)(Np / 64), (unsigned)((nrow + 3) / 4), 1, 256, 1, 1, 0, s, params,
+                                 nullptr);
+}
+
+hipError_t gen_launch_fwdt(const GenKernels &g, const void *S, void *Tout, int nrow, long Np, const ActParams &ap,
+                           hipStream_t s)
+{
+    ActParams a = ap;
+    void *params[] = {&S, &Tout, &nrow, &Np, &a};
+    return hipModuleLaunchKernel(g.fwdt, (unsigned)(Np / 64), (unsigned)((nrow + 3) / 4), 1, 256, 1, 1, 0, s, params,
+                                 nullptr);
+}
+
+hipError_t gen_launch_bwds(const GenKernels &g, const void *Tb, const void *S, void *Sb, int nrow, long Np,
+                           const ActParams &ap, hipStream_t s)
+{
+    ActParams a = ap;
+    void *params[] = {&Tb, &S, &Sb, &nrow, &Np, &a};
+    return hipModuleLaunchKernel(g.bwds, (unsigned)(Np / 64), (unsigned)((nrow + 3) / 4), 1, 256, 1, 1, 0, s, params,
                                  nullptr);
 }
 

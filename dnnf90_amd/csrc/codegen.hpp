@@ -11,12 +11,18 @@
 
 namespace dnnf90 {
 
+// fwd/bwd: the forward writes T and h, the adjoint reads Tbar and h.
+// fwdt/bwds: the forward writes T only, the layer keeps S, and the adjoint
+// reads Tbar and S and recomputes h in registers (one plane less per hidden
+// layer); used when neither spills.
 struct GenKernels {
     hipModule_t mod = nullptr;
-    hipFunction_t fwd = nullptr, bwd = nullptr;
+    hipFunction_t fwd = nullptr, bwd = nullptr, fwdt = nullptr, bwds = nullptr;
     std::vector<char> code;
     int fwd_regs = 0, bwd_regs = 0, fwd_scratch = 0, bwd_scratch = 0;
+    int fwdt_regs = 0, bwds_regs = 0, fwdt_scratch = 0, bwds_scratch = 0;
     bool ok = false;
+    bool s_ok() const { return ok && fwdt_scratch == 0 && bwds_scratch == 0; }
     void release();
 };
 
@@ -34,5 +40,9 @@ hipError_t gen_launch_fwd(const GenKernels &g, const void *S, void *Tout, void *
                           const ActParams &ap, hipStream_t s);
 hipError_t gen_launch_bwd(const GenKernels &g, const void *Tb, const void *H, void *Sb, int nrow, long Np,
                           hipStream_t s);
+hipError_t gen_launch_fwdt(const GenKernels &g, const void *S, void *Tout, int nrow, long Np, const ActParams &ap,
+                           hipStream_t s);
+hipError_t gen_launch_bwds(const GenKernels &g, const void *Tb, const void *S, void *Sb, int nrow, long Np,
+                           const ActParams &ap, hipStream_t s);
 
 }  // namespace dnnf90

@@ -43,6 +43,11 @@ static bool codegen_on()
     return g_codegen != 0;
 }
 
+// with the generated kernels: each hidden layer keeps S and the adjoint
+// recomputes h from it (on unless switched off by dnnf90_set_store_s), or
+// the layer keeps h as the table-driven kernels need
+static int g_store_s = 1;
+
 static thread_local std::string g_err;
 
 static int fail(int rc, const std::string &m)
@@ -259,10 +264,12 @@ struct BatchWork {
     hipStream_t own = nullptr, stream = nullptr;
     void *t0 = nullptr, *tl = nullptr, *p0 = nullptr, *p1 = nullptr, *scratch = nullptr;
     std::vector<void *> tpl, hpl;  // per hidden layer (hpl unused when recomputing)
-    // recompute: h of a hidden layer is not kept from the forward pass; the
-    // reverse pass recomputes S and h of that layer (K1 + Faa di Bruno
-    // forward) from the stored T of the layer below, into the plane hs
-    bool recompute = false;
+    // hpl[l] holds S of layer l when the last forward pass kept S (s_kept:
+    // generated kernels, the adjoint recomputes h in registers), else h.
+    // recompute: that plane is not kept from the forward pass; the reverse
+    // pass recomputes it from the stored T of the layer below (K1, and the
+    // Faa di Bruno forward when it holds h) into the plane hs
+    bool recompute = false, s_kept = false;
     void *hs = nullptr;
     double bytes = 0.0, bytes_stored = 0.0;   // device bytes held / needed with every h kept
     void *hplane(int l) const { return recompute ? hs : hpl[l]; }
@@ -436,24 +443,32 @@ template <typename T> static const T *weights(const Net &nt);
 template <> const double *weights<double>(const Net &nt) { return nt.d_w64; }
 template <> const float *weights<float>(const Net &nt) { return nt.d_w32; }
 
-// K1 of layer l from the stored T of the layer below, then (hidden layers)
-// the Faa di Bruno step writing T and h of layer l; S lives in p0
+// K1 of layer l from the stored T of the layer below, into the layer's
+// kept plane (s_kept) or p0
+template <typename T>
+static void layer_linear(const Net &nt, BatchWork &b, int l, long ncol, long Np)
+{
+    const int L = nt.nlayer, nd = nt.dims[l], ndm = nt.dims[l - 1];
+    const T *X = (const T *)(l == 1 ? b.t0 : b.tpl[l - 1]);
+    T *out = (T *)(l == L - 1 ? b.tl : b.s_kept ? b.hplane(l) : b.p0);
+    PScope p(b, CAT_LINEAR);
+    launch_gemm_wx<T>(b.prec, false, weights<T>(nt) + nt.woff[l], ndm + 1, X, out, nd, ndm + 1, ncol, Np, b.stream);
+}
+
+// K1 of layer l, then (hidden layers) the Faa di Bruno step writing T of
+// layer l and, unless the layer keeps S, h
 template <typename T>
 static void layer_forward(const Net &nt, BatchWork &b, int l, long ncol, long Np)
 {
-    const int L = nt.nlayer, nd = nt.dims[l], ndm = nt.dims[l - 1];
+    const int L = nt.nlayer, nd = nt.dims[l];
     hipStream_t s = b.stream;
-    const T *W = weights<T>(nt);
-    const T *X = (const T *)(l == 1 ? b.t0 : b.tpl[l - 1]);
-    T *out = (T *)(l < L - 1 ? b.p0 : b.tl);
-    {
-        PScope p(b, CAT_LINEAR);
-        launch_gemm_wx<T>(b.prec, false, W + nt.woff[l], ndm + 1, X, out, nd, ndm + 1, ncol, Np, s);
-    }
+    layer_linear<T>(nt, b, l, ncol, Np);
     if (l < L - 1) {
         PScope p(b, CAT_BELL_FWD);
         const GenKernels *gk = gen_for<T>(nt);
-        if (gk)
+        if (b.s_kept)
+            (void)gen_launch_fwdt(nt.gen[sizeof(T) == 8 ? 1 : 0], b.hplane(l), b.tpl[l], nd, Np, nt.dt.act(), s);
+        else if (gk)
             (void)gen_launch_fwd(*gk, b.p0, b.tpl[l], b.hplane(l), nd, Np, nt.dt.act(), s);
         else
             launch_bell_fwd<T>((const T *)b.p0, (T *)b.tpl[l], (T *)b.hplane(l), nd, Np, nt.dt.fwd(), nt.dt.act(),
@@ -469,6 +484,8 @@ static void forward_t(const Net &nt, BatchWork &b, const double *d_x, int npt)
     const long ncol = (long)na * Np;
     b.np = Np;
     b.npt = npt;
+    const GenKernels *gk = gen_for<T>(nt);
+    b.s_kept = g_store_s && gk && gk->s_ok();
     hipStream_t s = b.stream;
     {
         PScope p(b, CAT_OTHER);
@@ -491,11 +508,20 @@ static void backward_t(const Net &nt, BatchWork &b, double *grad, float *rows_t,
         const int nd = nt.dims[l], ndm = nt.dims[l - 1];
         if (l < L - 1) {
             // p0 is free here (its S-bar was consumed by layer l+1); the
-            // recomputed T of layer l equals the stored one bit for bit
-            if (b.recompute) layer_forward<T>(nt, b, l, ncol, Np);
+            // recomputed T of layer l equals the stored one bit for bit.
+            // A layer that keeps S needs K1 alone.
+            if (b.recompute) {
+                if (b.s_kept)
+                    layer_linear<T>(nt, b, l, ncol, Np);
+                else
+                    layer_forward<T>(nt, b, l, ncol, Np);
+            }
             PScope p(b, CAT_BELL_BWD);
             const GenKernels *gk = gen_for<T>(nt);
-            if (gk)
+            if (b.s_kept)
+                (void)gen_launch_bwds(nt.gen[sizeof(T) == 8 ? 1 : 0], b.p1, b.hplane(l), b.p0, nd, Np, nt.dt.act(),
+                                      s);
+            else if (gk)
                 (void)gen_launch_bwd(*gk, b.p1, b.hplane(l), b.p0, nd, Np, s);
             else
                 launch_bell_bwd<T>((const T *)b.p1, (const T *)b.hplane(l), (T *)b.p0, nd, Np, pr, s);
@@ -1618,6 +1644,13 @@ int dnnf90_set_codegen(int on)
     return prev;
 }
 
+int dnnf90_set_store_s(int on)
+{
+    const int prev = g_store_s;
+    g_store_s = on ? 1 : 0;
+    return prev;
+}
+
 int dnnf90_net_kernel_info(int nid, int prec, int *info)
 {
     Net *nt = g_net.get(nid);
@@ -1625,12 +1658,12 @@ int dnnf90_net_kernel_info(int nid, int prec, int *info)
     int rc = nt->ensure_device(prec != PREC_FP64);
     if (rc) return rc;
     const GenKernels &g = nt->gen[prec == PREC_FP64 ? 1 : 0];
-    const bool on = codegen_on() && g.ok;
-    info[0] = on ? 1 : 0;
-    info[1] = on ? g.fwd_regs : 0;
-    info[2] = on ? g.bwd_regs : 0;
-    info[3] = on ? g.fwd_scratch : 0;
-    info[4] = on ? g.bwd_scratch : 0;
+    const bool on = codegen_on() && g.ok, ks = on && g_store_s && g.s_ok();
+    info[0] = on ? (ks ? 2 : 1) : 0;
+    info[1] = on ? (ks ? g.fwdt_regs : g.fwd_regs) : 0;
+    info[2] = on ? (ks ? g.bwds_regs : g.bwd_regs) : 0;
+    info[3] = on ? (ks ? g.fwdt_scratch : g.fwd_scratch) : 0;
+    info[4] = on ? (ks ? g.bwds_scratch : g.bwd_scratch) : 0;
     info[5] = nt->sched.nentry;
     return 0;
 }

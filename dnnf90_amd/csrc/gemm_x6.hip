@@ -31,20 +31,38 @@ namespace dnnf90 {
 #ifndef LIN_X6_RBT_MAX
 #define LIN_X6_RBT_MAX 8
 #endif
-// adjacent 16-column blocks a wave streams per step
+// 16-column blocks a wave streams per step; LIN_X6_NCB_T8 for the
+// transposed map at the 128-row tile, which holds one workgroup per CU (its
+// LDS) and so one wave per SIMD whatever its registers: four blocks keep
+// more loads in flight there (measured: 14 % faster on a 128-wide layer),
+// while elsewhere they cost occupancy, and the forward map's FP64
+// accumulators would spill
 #ifndef LIN_X6_NCB
 #define LIN_X6_NCB 2
+#endif
+#ifndef LIN_X6_NCB_T8
+#define LIN_X6_NCB_T8 4
+#endif
+// column order of a wave step: 0 -- block j is columns 16 j .. 16 j + 15
+// (dword loads and stores, sixteen lanes cover 64 bytes of a row); 1 --
+// lane n owns the NCB consecutive columns NCB n .. NCB n + NCB - 1, one per
+// block (NCB-wide vector loads and stores, sixteen lanes cover 64 NCB bytes)
+#ifndef LIN_X6_VEC
+#define LIN_X6_VEC 1
 #endif
 
 // ---------------------------------------------------------------------------
 // K1 / K6.  The workgroup's 16*RBT rows of A (K1: the weight columns after
 // the bias; K6: the transposed weights) are split once into LDS in operand
 // order, Ap[rb][kc][piece][lane] = pieces of A[16 rb + lane%16][32 kc +
-// 8 (lane/16) .. +7], zero past Kd.  A wave streams NCB adjacent 16-column
-// blocks of B: lane (col = lane%16, q = lane/16) reads rows 32 kc + 8 q ..
-// +7 of its column (sixteen lanes read 64 contiguous bytes of a row), splits
+// 8 (lane/16) .. +7], zero past Kd.  A wave streams units of 16 NCB
+// columns of B as NCB MFMA column blocks: lane (n = lane%16, q = lane/16)
+// reads rows 32 kc + 8 q .. +7 of its NCB columns (LIN_X6_VEC: the
+// consecutive columns NCB n .., one NCB-wide vector load a row, so sixteen
+// lanes read 64 NCB contiguous bytes; block j takes column NCB n + j), splits
 // them and multiplies by every row block; the next chunk's rows are loaded
-// before the current chunk's MFMAs.  K1's bias is added in the epilogue to
+// before the current chunk's MFMAs.  The results go out the same way, one
+// NCB-wide vector store per row.  K1's bias is added in the epilogue to
 // the first Np columns (slot 0; NCB*16 columns never straddle Np).
 template <bool kTrans, bool kAcc64, int RBT, int NCB, int KP>
 __global__ __launch_bounds__(256) void k_lin_x6(const float *__restrict__ W, int ldw, const float *__restrict__ B,
@@ -90,16 +108,29 @@ __global__ __launch_bounds__(256) void k_lin_x6(const float *__restrict__ W, int
     // statically by unrolling the loop KP times)
     const long nu = (cu1 - cu0 - wave + 3) / 4;
     const long nit = nu * nkc;
-    const float *bl = B + (lane & 15);
+    // column of block j held by this lane, relative to the unit's first
+    const auto col_of = [&](int j) { return LIN_X6_VEC ? NCB * (lane & 15) + j : 16 * j + (lane & 15); };
+    const float *bl = B + col_of(0);
     float rb_[KP][NCB][8];
     auto load = [&](float (&dst)[NCB][8], long it) {
         const long cul = cu0 + wave + 4 * (it / nkc);
         const int kc = (int)(it % nkc);
+        if constexpr (LIN_X6_VEC && NCB > 1) {
+            typedef float fv __attribute__((ext_vector_type(NCB)));
+            const float *src = bl + cul * 16 * NCB;
 #pragma unroll
-        for (int j = 0; j < NCB; ++j) {
-            const float *src = bl + cul * 16 * NCB + 16 * j;
+            for (int e = 0; e < 8; ++e) {
+                const fv v = *reinterpret_cast<const fv *>(src + (long)min(32 * kc + 8 * q + e, Kd - 1) * ncol);
 #pragma unroll
-            for (int e = 0; e < 8; ++e) dst[j][e] = src[(long)min(32 * kc + 8 * q + e, Kd - 1) * ncol];
+                for (int j = 0; j < NCB; ++j) dst[j][e] = v[j];
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < NCB; ++j) {
+                const float *src = bl + cul * 16 * NCB + 16 * j;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) dst[j][e] = src[(long)min(32 * kc + 8 * q + e, Kd - 1) * ncol];
+            }
         }
     };
     flt4 cacc[RBT][NCB];
@@ -148,15 +179,25 @@ __global__ __launch_bounds__(256) void k_lin_x6(const float *__restrict__ W, int
                 for (int r = 0; r < 4; ++r) {
                     const int mr = 16 * rb + 4 * q + r, m = m0 + mr;
                     const float add = bias ? Bias[mr] : 0.f;
+                    float val[NCB];
 #pragma unroll
                     for (int j = 0; j < NCB; ++j) {
-                        const long c = cu * 16 * NCB + 16 * j + (lane & 15);
-                        float val;
                         if (kAcc64)
-                            val = (float)(acc64[kAcc64 ? rb : 0][kAcc64 ? j : 0][r] + (double)add);
+                            val[j] = (float)(acc64[kAcc64 ? rb : 0][kAcc64 ? j : 0][r] + (double)add);
                         else
-                            val = cacc[rb][j][r] + add;
-                        if (m < M) C[(long)m * ncol + c] = val;
+                            val[j] = cacc[rb][j][r] + add;
+                    }
+                    if (m >= M) continue;
+                    float *dst = C + (long)m * ncol + cu * 16 * NCB;
+                    if constexpr (LIN_X6_VEC && NCB > 1) {
+                        typedef float fv __attribute__((ext_vector_type(NCB)));
+                        fv v;
+#pragma unroll
+                        for (int j = 0; j < NCB; ++j) v[j] = val[j];
+                        *reinterpret_cast<fv *>(dst + col_of(0)) = v;
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < NCB; ++j) dst[col_of(j)] = val[j];
                     }
                 }
         }
@@ -177,7 +218,7 @@ template <bool kTrans, bool kAcc64, int RBT>
 static void lin_x6_go(const float *W, int ldw, const float *B, float *C, int M, int Kd, long ncol, long Np,
                       hipStream_t s)
 {
-    constexpr int NCB = LIN_X6_NCB, KP = LIN_X6_KP;
+    constexpr int NCB = (kTrans && RBT == 8) ? LIN_X6_NCB_T8 : LIN_X6_NCB, KP = LIN_X6_KP;
     const int nkc = (Kd + 31) / 32;
     const size_t lds = (size_t)RBT * nkc * 3 * 64 * 16 + 16 * RBT * sizeof(float);
     static size_t lds_set = 64 * 1024;   // dynamic LDS above 64 KiB must be requested

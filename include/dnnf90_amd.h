@@ -59,10 +59,18 @@ int dnnf90_tabset_write_order(int tid, const char *filename);
  * straight-line, operands in registers) are used where the family is small
  * enough; otherwise, or with codegen off, the table-driven kernels run.
  * dnnf90_set_codegen switches this process-wide (also DNNF90_CODEGEN=0)
- * and returns the previous setting.  dnnf90_net_kernel_info fills info[6]:
- * generated kernels active (0/1), their forward and adjoint VGPR counts and
- * scratch bytes, and the operand entries per lane of the family. */
+ * and returns the previous setting.  With the generated kernels each hidden
+ * layer keeps its pre-activation planes S and the adjoint recomputes the
+ * activation derivative planes h from them in registers, so the forward pass
+ * writes one plane less per hidden layer (results identical bit for bit);
+ * dnnf90_set_store_s(0) keeps h instead (the forward writes T and h), and
+ * returns the previous setting.  A family whose kept-S kernels would spill
+ * keeps h.  dnnf90_net_kernel_info fills info[6]: generated kernels active
+ * (0 no, 1 keeping h, 2 keeping S), the VGPR counts and scratch bytes of
+ * the forward and adjoint kernels in use, and the operand entries per lane
+ * of the family. */
 int dnnf90_set_codegen(int on);
+int dnnf90_set_store_s(int on);
 int dnnf90_net_kernel_info(int nid, int prec, int *info);
 
 int dnnf90_net_nweights(int nid);
@@ -83,15 +91,16 @@ int dnnf90_grad_set(int gid, const double *g);
 
 /* batched work space for up to maxpt points in arithmetic prec */
 int dnnf90_bwork_create(int nid, int maxpt, int prec);
-/* the same with a device-memory budget in bytes: when keeping every hidden
- * layer's h (the second Faa di Bruno output, needed only by the adjoint)
- * would pass the budget, the work space keeps the layers' T only and the
- * reverse pass recomputes S and h of each hidden layer from the stored T of
- * the layer below, into one scratch plane -- one more linear map and Faa di
- * Bruno step per hidden layer, results identical bit for bit.
- * budget_bytes <= 0: keep everything (dnnf90_bwork_create). */
+/* the same with a device-memory budget in bytes: when keeping the plane the
+ * adjoint needs of every hidden layer (S, or h where the layers keep h; see
+ * dnnf90_set_store_s) would pass the budget, the work space keeps the
+ * layers' T only and the reverse pass recomputes that plane of each hidden
+ * layer from the stored T of the layer below, into one scratch plane -- one
+ * more linear map (and, for h, Faa di Bruno step) per hidden layer, results
+ * identical bit for bit.  budget_bytes <= 0: keep everything
+ * (dnnf90_bwork_create). */
 int dnnf90_bwork_create_budget(int nid, int maxpt, int prec, double budget_bytes);
-/* info[5]: recompute (0/1), device bytes held, bytes needed with every h
+/* info[5]: recompute (0/1), device bytes held, bytes needed with every plane
  * kept, maxpt, prec */
 int dnnf90_bwork_info(int bid, double *info);
 int dnnf90_bwork_free(int bid);

@@ -22,13 +22,18 @@ def D():
     return dnnf90_amd
 
 
-@pytest.fixture(params=[1, 0], ids=["generated", "table"])
+@pytest.fixture(params=[(1, 1), (1, 0), (0, 1)], ids=["generated", "generated_keep_h", "table"])
 def codegen(request, D):
-    """Run with the per-family generated kernels, then with the table-driven ones."""
+    """Run with the per-family generated kernels (each hidden layer keeping
+    S, the adjoint recomputing h; then keeping h), then with the
+    table-driven ones."""
     from dnnf90_amd._lib import lib
-    prev = lib().dnnf90_set_codegen(request.param)
-    yield request.param
+    on, store_s = request.param
+    prev = lib().dnnf90_set_codegen(on)
+    prev_s = lib().dnnf90_set_store_s(store_s)
+    yield on
     lib().dnnf90_set_codegen(prev)
+    lib().dnnf90_set_store_s(prev_s)
 
 
 def _setup(D, oracle_mod, wl, npt):
@@ -106,7 +111,8 @@ def test_residual_step_fp64(D, oracle_mod, name, npt):
 
 @pytest.mark.parametrize("name", ["kdv", "dense4_k3", "closure4_k7", "gelu3_k4", "ngd2_k4"])
 def test_generated_kernels_active(D, name):
-    """The measured families run the generated kernels, without scratch spills."""
+    """The measured families run the generated kernels, keeping S in every
+    hidden layer, without scratch spills."""
     from dnnf90_amd._lib import lib
     from dnnf90_amd.workloads import WORKLOADS
     wl = WORKLOADS[name]
@@ -115,8 +121,40 @@ def test_generated_kernels_active(D, name):
         info = np.zeros(6, np.int32)
         assert lib().dnnf90_net_kernel_info(net.id, prec, info.ctypes.data) == 0
         print(name, "prec", prec, "generated", info[0], "vgprs fwd/bwd", info[1], info[2], "scratch", info[3], info[4])
-        assert info[0] == 1
+        assert info[0] == 2
         assert info[3] == 0 and info[4] == 0
+
+
+@pytest.mark.parametrize("prec", [0, 1, 3], ids=["fp64", "fp32", "bf16x6"])
+@pytest.mark.parametrize("name,npt", [("dense4_k3", 197), ("closure4_k7", 130), ("kdv", 100)])
+def test_store_s_identical(D, oracle_mod, name, npt, prec):
+    """Keeping S and recomputing h in the adjoint gives the same forward
+    planes and gradient, bit for bit, as keeping h."""
+    from dnnf90_amd._lib import lib
+    from dnnf90_amd.workloads import WORKLOADS
+    wl = WORKLOADS[name]
+    ts, net, w, x, o = _setup(D, oracle_mod, wl, npt)
+    seed = np.random.default_rng(7).standard_normal((npt, wl.dims[-1], o.na))
+    out = []
+    prev = lib().dnnf90_set_store_s(1)
+    try:
+        for store_s in (1, 0):
+            lib().dnnf90_set_store_s(store_s)
+            info = np.zeros(6, np.int32)
+            assert lib().dnnf90_net_kernel_info(net.id, prec, info.ctypes.data) == 0
+            bw = D.BatchWork(net, npt, prec)
+            g = D.Grad(net)
+            t = bw.forward(x)
+            bw.backward(seed, g)
+            out.append((info[0], t, g.get()))
+            bw.free()
+            g.free()
+    finally:
+        lib().dnnf90_set_store_s(prev)
+    (k1, t1, g1), (k0, t0, g0) = out
+    print(name, "prec", prec, "kernels", k1, k0, "max |dT|", np.abs(t1 - t0).max(), "max |dg|", np.abs(g1 - g0).max())
+    assert (k1, k0) == (2, 1)
+    assert np.array_equal(t1, t0) and np.array_equal(g1, g0)
 
 
 def test_per_point_abi_and_adam(D, oracle_mod):

@@ -3,8 +3,9 @@ torch.autograd.Function (the differentiation workload: MLP [4,64x4,1], tanh,
 closure{t, x, x^7, x^3 y^2 z^2}, loss mean square of u_t + u u_x + d_x^7 u +
 d_x^3 d_y^2 d_z^2 u).
 
-Recompute keeps each hidden layer's T and rebuilds S and h in the reverse
-pass; the kernels are deterministic, so the results must be identical bit
+Recompute keeps each hidden layer's T and rebuilds S (and h, where the
+layers keep h rather than S) in the reverse pass; the kernels are
+deterministic, so the results must be identical bit
 for bit to the keep-everything work space.  The autograd gradient is checked
 against the oracle (the C restatement of lib/train_module.f90:241-450 and
 app/pinn_module.f90:32-65) and against central differences with h = 1e-4 of
@@ -35,21 +36,27 @@ def _setup(D, oracle_mod, name, npt):
     return wl, ts, net, w, x, o
 
 
+@pytest.mark.parametrize("store_s", [1, 0], ids=["keep_s", "keep_h"])
 @pytest.mark.parametrize("prec", [0, 3], ids=["fp64", "bf16x6"])
 @pytest.mark.parametrize("name,npt", [("closure4_k7", 300), ("dense4_k3", 150)])
-def test_recompute_identical(D, oracle_mod, name, npt, prec):
+def test_recompute_identical(D, oracle_mod, name, npt, prec, store_s):
+    from dnnf90_amd._lib import lib
     wl, ts, net, w, x, o = _setup(D, oracle_mod, name, npt)
     res = D.Residual(wl.terms(o.alpha()))
     out = []
-    for budget in (0, 1):
-        bw = D.BatchWork(net, npt, prec, budget=budget)
-        info = bw.info()
-        g = D.Grad(net)
-        rsq = bw.residual_step(x, res, 2.0 / npt, g)
-        out.append((info, rsq, g.get()))
-        print(name, "prec", prec, "budget", budget, info)
-        bw.free()
-        g.free()
+    prev = lib().dnnf90_set_store_s(store_s)
+    try:
+        for budget in (0, 1):
+            bw = D.BatchWork(net, npt, prec, budget=budget)
+            info = bw.info()
+            g = D.Grad(net)
+            rsq = bw.residual_step(x, res, 2.0 / npt, g)
+            out.append((info, rsq, g.get()))
+            print(name, "prec", prec, "store S", store_s, "budget", budget, info)
+            bw.free()
+            g.free()
+    finally:
+        lib().dnnf90_set_store_s(prev)
     (i0, r0, g0), (i1, r1, g1) = out
     assert not i0["recompute"] and i1["recompute"]
     assert i1["bytes"] < i0["bytes"] == i0["bytes_stored"]
