@@ -22,26 +22,43 @@
 
 namespace dnnf90 {
 
-// register prefetch depth (chunks of 32 k) of the split linear maps and
-// workgroups per CU of the split weight gradient
+// register prefetch depth (chunks of 32 k) of the split linear maps: the
+// forward map K1 and the transposed map K6
 #ifndef LIN_X6_KP
 #define LIN_X6_KP 3
 #endif
-// largest row tile (in 16-row blocks) of the split linear maps
-#ifndef LIN_X6_RBT_MAX
-#define LIN_X6_RBT_MAX 8
+#ifndef LIN_X6_KP_T
+#define LIN_X6_KP_T 3
 #endif
-// 16-column blocks a wave streams per step; LIN_X6_NCB_T8 for the
-// transposed map at the 128-row tile, which holds one workgroup per CU (its
-// LDS) and so one wave per SIMD whatever its registers: four blocks keep
-// more loads in flight there (measured: 14 % faster on a 128-wide layer),
-// while elsewhere they cost occupancy, and the forward map's FP64
-// accumulators would spill
+// largest row tile (in 16-row blocks) of K1 and of K6.  K1 at 64 rows: two
+// workgroups share a column range (its rows come from L2 the second time)
+// and two fit a CU, against one at 128 rows (11 % faster on a 128-wide
+// layer); K6 at 128 rows with four column blocks a step
+#ifndef LIN_X6_RBT_MAX
+#define LIN_X6_RBT_MAX 4
+#endif
+#ifndef LIN_X6_RBT_MAX_T
+#define LIN_X6_RBT_MAX_T 8
+#endif
+// 16-column blocks a wave streams per step.  Where the split weights' LDS
+// leaves room for one workgroup per CU, occupancy is one wave per SIMD
+// whatever the registers, and the _DEEP settings (more column blocks, a
+// longer prefetch ring) keep more loads in flight; elsewhere those cost
+// occupancy.  Measured: four blocks make K6 14 % faster on a 128-wide layer
+// (128-row tile) and 20 % on a 256-wide one (64-row tile), K1 17 % faster on
+// a 256-wide layer; a ring of 4 or 6 chunks changes < 2 % (6 spills K6 at
+// 128 rows).
 #ifndef LIN_X6_NCB
 #define LIN_X6_NCB 2
 #endif
-#ifndef LIN_X6_NCB_T8
-#define LIN_X6_NCB_T8 4
+#ifndef LIN_X6_NCB_DEEP
+#define LIN_X6_NCB_DEEP 4
+#endif
+#ifndef LIN_X6_NCB_DEEP_T
+#define LIN_X6_NCB_DEEP_T 4
+#endif
+#ifndef LIN_X6_KP_DEEP
+#define LIN_X6_KP_DEEP 3
 #endif
 // column order of a wave step: 0 -- block j is columns 16 j .. 16 j + 15
 // (dword loads and stores, sixteen lanes cover 64 bytes of a row); 1 --
@@ -214,13 +231,10 @@ __global__ __launch_bounds__(256) void k_lin_x6(const float *__restrict__ W, int
     }
 }
 
-template <bool kTrans, bool kAcc64, int RBT>
-static void lin_x6_go(const float *W, int ldw, const float *B, float *C, int M, int Kd, long ncol, long Np,
-                      hipStream_t s)
+template <bool kTrans, bool kAcc64, int RBT, int NCB, int KP>
+static void lin_x6_launch(const float *W, int ldw, const float *B, float *C, int M, int Kd, long ncol, long Np,
+                          size_t lds, int per_cu, hipStream_t s)
 {
-    constexpr int NCB = (kTrans && RBT == 8) ? LIN_X6_NCB_T8 : LIN_X6_NCB, KP = LIN_X6_KP;
-    const int nkc = (Kd + 31) / 32;
-    const size_t lds = (size_t)RBT * nkc * 3 * 64 * 16 + 16 * RBT * sizeof(float);
     static size_t lds_set = 64 * 1024;   // dynamic LDS above 64 KiB must be requested
     if (lds > lds_set) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_lin_x6<kTrans, kAcc64, RBT, NCB, KP>),
@@ -229,7 +243,6 @@ static void lin_x6_go(const float *W, int ldw, const float *B, float *C, int M, 
     }
     const int nrt = (M + 16 * RBT - 1) / (16 * RBT);
     const long ncu = ncol / (16 * NCB);
-    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / lds));
     const long want = std::max(1L, (long)256 * per_cu / nrt);
     long cupw = (ncu + want - 1) / want;
     cupw = (cupw + 3) / 4 * 4;
@@ -240,12 +253,27 @@ static void lin_x6_go(const float *W, int ldw, const float *B, float *C, int M, 
                        Kd, ncol, Np, nrt, (int)ncw, cupw);
 }
 
+template <bool kTrans, bool kAcc64, int RBT>
+static void lin_x6_go(const float *W, int ldw, const float *B, float *C, int M, int Kd, long ncol, long Np,
+                      hipStream_t s)
+{
+    const int nkc = (Kd + 31) / 32;
+    const size_t lds = (size_t)RBT * nkc * 3 * 64 * 16 + 16 * RBT * sizeof(float);
+    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / lds));
+    if (per_cu == 1)
+        lin_x6_launch<kTrans, kAcc64, RBT, kTrans ? LIN_X6_NCB_DEEP_T : LIN_X6_NCB_DEEP, LIN_X6_KP_DEEP>(
+            W, ldw, B, C, M, Kd, ncol, Np, lds, per_cu, s);
+    else
+        lin_x6_launch<kTrans, kAcc64, RBT, LIN_X6_NCB, kTrans ? LIN_X6_KP_T : LIN_X6_KP>(W, ldw, B, C, M, Kd, ncol,
+                                                                                          Np, lds, per_cu, s);
+}
+
 template <bool kTrans, bool kAcc64>
 static void lin_x6_pick(const float *W, int ldw, const float *B, float *C, int M, int Kd, long ncol, long Np,
                         hipStream_t s)
 {
     const size_t per_rb = (size_t)((Kd + 31) / 32) * 3 * 64 * 16;
-    int rbt = LIN_X6_RBT_MAX;
+    int rbt = kTrans ? LIN_X6_RBT_MAX_T : LIN_X6_RBT_MAX;
     while (rbt > 1 && 16 * (rbt / 2) >= M) rbt /= 2;
     while (rbt > 1 && rbt * per_rb > (size_t)144 * 1024) rbt /= 2;
     switch (rbt) {
