@@ -588,6 +588,33 @@ __global__ void k_reduce_pass2(const double *__restrict__ part2, int nsplit, lon
 
 int grad_reduce_splits(int nchunk) { return nchunk >= 64 ? 16 : (nchunk >= 16 ? 4 : 1); }
 
+// the same two-level sum in one launch: thread (slice, weight) sums its
+// chunk range in order (k_reduce_pass1's loop), the slice sums meet in LDS
+// and are added in slice order (k_reduce_pass2's loop) -- the same
+// operations in the same order, so the same bits.  256 / nsplit weights a
+// workgroup.
+__global__ __launch_bounds__(256) void k_reduce_fused(const double *__restrict__ partial, int nchunk, int nsplit,
+                                                      long len, double *__restrict__ grad)
+{
+    __shared__ double red[256];
+    const int wpb = 256 / nsplit, t = threadIdx.x;
+    const int wi = t % wpb, sidx = t / wpb;
+    const long i = (long)blockIdx.x * wpb + wi;
+    if (i < len) {
+        const int per = (nchunk + nsplit - 1) / nsplit;
+        const int c0 = sidx * per, c1 = min(nchunk, c0 + per);
+        double s = 0.0;
+        for (int c = c0; c < c1; ++c) s += partial[(long)c * len + i];
+        red[t] = s;
+    }
+    __syncthreads();
+    if (sidx == 0 && i < len) {
+        double s = 0.0;
+        for (int k = 0; k < nsplit; ++k) s += red[k * wpb + wi];
+        grad[i] += s;
+    }
+}
+
 #ifndef GRAD_WGS
 #define GRAD_WGS 256
 #endif
@@ -639,10 +666,9 @@ void launch_gemm_grad(int prec, const T *A, const T *B, int M, int Kd, long ncol
 void launch_grad_reduce(double *partial, int nchunk, long len, double *grad, hipStream_t s)
 {
     const int ns = grad_reduce_splits(nchunk);
-    double *part2 = partial + (long)nchunk * len;
-    const unsigned gx = (unsigned)((len + 255) / 256);
-    hipLaunchKernelGGL(k_reduce_pass1, dim3(gx, (unsigned)ns), dim3(256), 0, s, partial, nchunk, ns, len, part2);
-    hipLaunchKernelGGL(k_reduce_pass2, dim3(gx), dim3(256), 0, s, part2, ns, len, grad);
+    const long wpb = 256 / ns;
+    hipLaunchKernelGGL(k_reduce_fused, dim3((unsigned)((len + wpb - 1) / wpb)), dim3(256), 0, s, partial, nchunk, ns,
+                       len, grad);
 }
 
 // the Gram matrix of Gauss-Newton rows: G[i][j] = sum_n R[i][n] R[j][n]

@@ -300,7 +300,7 @@ void launch_lin_x6(bool trans_a, const float *W, int ldw, const float *B, float 
 // before the current step's MFMAs.  A rows are split once per step, B rows
 // one block at a time.  The bias column (sum of A over the first Np columns)
 // is added up from the loaded A rows by the first k tile.
-template <int RB, int CB>
+template <int RB, int CB, int ST>
 __global__ __launch_bounds__(256) void k_grad_x6(const float *__restrict__ A, const float *__restrict__ B, int M,
                                                  int Kd, long ncol, long Np, long chunk, int ntm, int ntk,
                                                  int nchunk, double *__restrict__ partial)
@@ -339,7 +339,10 @@ __global__ __launch_bounds__(256) void k_grad_x6(const float *__restrict__ A, co
     // one register stage of raw rows: a step splits them into bf16 pieces,
     // then reuses the registers for the next step's loads, which are in
     // flight during the step's MFMAs
-    float ar[RB][8], br[CB][8];
+    // ST stages: the rows of the wave's step k + ST are loaded into the
+    // stage step k has just split (the loop unrolled ST times so that every
+    // stage is indexed statically)
+    float ar[ST][RB][8], br[ST][CB][8];
     auto ld8 = [](float (&d)[8], const float *src) {
         const f4 x = *reinterpret_cast<const f4 *>(src), y = *reinterpret_cast<const f4 *>(src + 4);
 #pragma unroll
@@ -348,31 +351,38 @@ __global__ __launch_bounds__(256) void k_grad_x6(const float *__restrict__ A, co
             d[4 + e] = y[e];
         }
     };
-    auto load = [&](long c) {
+    auto load = [&](float (&a)[RB][8], float (&b)[CB][8], long c) {
 #pragma unroll
-        for (int i = 0; i < RB; ++i) ld8(ar[i], pa[i] + c);
+        for (int i = 0; i < RB; ++i) ld8(a[i], pa[i] + c);
 #pragma unroll
-        for (int j = 0; j < CB; ++j) ld8(br[j], pb[j] + c);
+        for (int j = 0; j < CB; ++j) ld8(b[j], pb[j] + c);
     };
-    long c = cb + (long)wave * SC;
-    if (c < ce) load(c);
-    for (; c < ce; c += 4 * SC) {
+    auto step = [&](float (&a)[RB][8], float (&b)[CB][8], long c) {
         if (do_bias && c < Np) {   // a step never straddles Np (multiples of 32)
 #pragma unroll
             for (int i = 0; i < RB; ++i)
 #pragma unroll
-                for (int e = 0; e < 8; ++e) bacc[i] += ar[i][e];
+                for (int e = 0; e < 8; ++e) bacc[i] += a[i][e];
         }
         bf16x8 ah[RB], am[RB], al[RB], bh[CB], bm[CB], bl[CB];
 #pragma unroll
-        for (int i = 0; i < RB; ++i) split3(ar[i], ah[i], am[i], al[i]);
+        for (int i = 0; i < RB; ++i) split3(a[i], ah[i], am[i], al[i]);
 #pragma unroll
-        for (int j = 0; j < CB; ++j) split3(br[j], bh[j], bm[j], bl[j]);
-        if (c + 4 * SC < ce) load(c + 4 * SC);
+        for (int j = 0; j < CB; ++j) split3(b[j], bh[j], bm[j], bl[j]);
+        if (c + ST * 4 * SC < ce) load(a, b, c + ST * 4 * SC);
 #pragma unroll
         for (int j = 0; j < CB; ++j)
 #pragma unroll
             for (int i = 0; i < RB; ++i) acc[i][j] = mma_x6(ah[i], am[i], al[i], bh[j], bm[j], bl[j], acc[i][j]);
+    };
+    const long c0 = cb + (long)wave * SC;
+#pragma unroll
+    for (int st = 0; st < ST; ++st)
+        if (c0 + st * 4 * SC < ce) load(ar[st], br[st], c0 + st * 4 * SC);
+    for (long c = c0; c < ce; c += ST * 4 * SC) {
+#pragma unroll
+        for (int st = 0; st < ST; ++st)
+            if (c + st * 4 * SC < ce) step(ar[st], br[st], c + st * 4 * SC);
     }
     for (int wv = 0; wv < 4; ++wv) {
         if (wave == wv) {
@@ -415,6 +425,15 @@ __global__ __launch_bounds__(256) void k_grad_x6(const float *__restrict__ A, co
     }
 }
 
+// register stages of raw rows of the split K5 at 128 and at 64 rows (two
+// stages spill at 128 rows, and gain nothing measurable at 64)
+#ifndef GRAD_X6_ST8
+#define GRAD_X6_ST8 1
+#endif
+#ifndef GRAD_X6_ST4
+#define GRAD_X6_ST4 1
+#endif
+
 // tile of the split K5: (16 RB) x 64 of the weight block, RB = 8 where the
 // block has more than 64 rows (each split element then feeds more MFMAs)
 int grad_x6_rb(int M) { return M > 64 ? 8 : 4; }
@@ -427,11 +446,11 @@ void launch_grad_x6(const float *A, const float *B, int M, int Kd, long ncol, lo
     long g = (long)ntm * ntk * nchunk;
     g = (g + 7) / 8 * 8;
     if (rb == 8)
-        hipLaunchKernelGGL((k_grad_x6<8, 4>), dim3((unsigned)g), dim3(256), 0, s, A, B, M, Kd, ncol, Np, chunk, ntm,
-                           ntk, nchunk, partial);
+        hipLaunchKernelGGL((k_grad_x6<8, 4, GRAD_X6_ST8>), dim3((unsigned)g), dim3(256), 0, s, A, B, M, Kd, ncol, Np,
+                           chunk, ntm, ntk, nchunk, partial);
     else
-        hipLaunchKernelGGL((k_grad_x6<4, 4>), dim3((unsigned)g), dim3(256), 0, s, A, B, M, Kd, ncol, Np, chunk, ntm,
-                           ntk, nchunk, partial);
+        hipLaunchKernelGGL((k_grad_x6<4, 4, GRAD_X6_ST4>), dim3((unsigned)g), dim3(256), 0, s, A, B, M, Kd, ncol, Np,
+                           chunk, ntm, ntk, nchunk, partial);
 }
 
 }  // namespace dnnf90
