@@ -390,18 +390,32 @@ __global__ __launch_bounds__(256) void k_chol_syrk(double *__restrict__ A, long 
 
 // y_blk <- M y_blk for the block at k (nb rows), M = Linv or Linv^T given
 // row-major (Mr = Linv, or LinvT for the transposed solve): one wave per
-// row, lanes along the row, 256 threads
+// row, lanes along the row, SW waves.  A wave takes its rows eight at a time
+// with all their loads issued together, then reduces each (the same
+// products and order as a row at a time).
+static constexpr int SW = 16;   // waves of the solve kernels
+
 __device__ void blk_solve(const double *__restrict__ Mr, int k, int nb, double *__restrict__ y,
                           double *__restrict__ v)
 {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
     if (tid < PB) v[tid] = tid < nb ? y[k + tid] : 0.0;
     __syncthreads();
     const double v0 = v[lane], v1 = v[lane + 64];
-    for (int r = wave; r < nb; r += 4) {
-        double s = Mr[r * PB + lane] * v0 + Mr[r * PB + lane + 64] * v1;
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-        if (lane == 0) y[k + r] = s;
+    for (int r0 = wave; r0 < nb; r0 += 8 * nw) {   // rows r0, r0 + nw, ..., r0 + 7 nw
+        double a0[8], a1[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int r = min(r0 + nw * q, PB - 1);   // M has PB rows
+            a0[q] = Mr[r * PB + lane];
+            a1[q] = Mr[r * PB + lane + 64];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            double s = a0[q] * v0 + a1[q] * v1;
+            for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+            if (lane == 0 && r0 + nw * q < nb) y[k + r0 + nw * q] = s;
+        }
     }
 }
 
@@ -413,23 +427,34 @@ __global__ __launch_bounds__(256) void k_chol_solve_first(const double *__restri
 }
 
 // forward step of panel k (y_k already solved): rows i >= k+nb get
-// y_i -= sum_m L[i][k+m] y[k+m], one wave per row; workgroup 0 owns the next
-// diagonal block [k+nb, k+nb+nb2) and then solves it with its Linv
-__global__ __launch_bounds__(256) void k_chol_fwd(const double *__restrict__ A, long lda, int k, int nb, int n,
-                                                  const double *__restrict__ Linv_next, int nb2,
-                                                  double *__restrict__ y)
+// y_i -= sum_m L[i][k+m] y[k+m], one wave per row, lanes along the row, a
+// wave's rows eight at a time with their loads in flight together (SW
+// waves: PB rows in one round).  Workgroup 0 owns the next diagonal block
+// [k+nb, k+nb+nb2) and then solves it with its Linv; workgroup b > 0 takes
+// the PB rows after those of workgroup b - 1.
+__global__ __launch_bounds__(64 * SW) void k_chol_fwd(const double *__restrict__ A, long lda, int k, int nb, int n,
+                                                      const double *__restrict__ Linv_next, int nb2,
+                                                      double *__restrict__ y)
 {
     __shared__ double v[PB];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int r0 = k + nb + blockIdx.x * PB;
+    const int r0 = k + nb + (int)blockIdx.x * PB;
     const double y0 = lane < nb ? y[k + lane] : 0.0, y1 = lane + 64 < nb ? y[k + 64 + lane] : 0.0;
-    for (int rr = wave; rr < PB; rr += 4) {
-        const int i = r0 + rr;
-        if (i >= n) break;
-        const double *Li = A + (long)i * lda + k;
-        double s = (lane < nb ? Li[lane] * y0 : 0.0) + (lane + 64 < nb ? Li[lane + 64] * y1 : 0.0);
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-        if (lane == 0) y[i] -= s;
+    for (int rb = wave; rb < PB; rb += 8 * SW) {   // rows rb, rb + SW, ..., rb + 7 SW of the range
+        double a0[8], a1[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const double *Li = A + (long)min(r0 + rb + SW * q, n - 1) * lda + k;
+            a0[q] = lane < nb ? Li[lane] : 0.0;
+            a1[q] = lane + 64 < nb ? Li[lane + 64] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            double s = a0[q] * y0 + a1[q] * y1;
+            for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+            const int i = r0 + rb + SW * q;
+            if (lane == 0 && i < n) y[i] -= s;
+        }
     }
     if (blockIdx.x == 0 && nb2 > 0) {
         __threadfence_block();
@@ -439,34 +464,45 @@ __global__ __launch_bounds__(256) void k_chol_fwd(const double *__restrict__ A, 
 }
 
 // backward step of panel k (u_k already solved): rows i < k get
-// y_i -= sum_m L[k+m][i] u[k+m] (lanes along i: row reads coalesce);
-// workgroup 0 owns the previous block [k-PB, k) and solves it with Linv^T
-// (LinvT_prev, row-major); the m range is split over the two halves of the
-// workgroup
-__global__ __launch_bounds__(256) void k_chol_bwd(const double *__restrict__ A, long lda, int k, int nb,
-                                                  const double *__restrict__ LinvT_prev, double *__restrict__ y)
+// y_i -= sum_m L[k+m][i] u[k+m] (lanes along i: row reads coalesce).  A
+// workgroup takes PB rows; the m range is split in eight parts (two waves
+// a part), each summed with eight loads in flight, and the parts meet in LDS
+// in a fixed order.  Workgroup 0 owns the previous block [k-PB, k) and then
+// solves it with Linv^T (LinvT_prev, row-major); workgroup b > 0 takes the
+// PB rows below those of workgroup b - 1.
+__global__ __launch_bounds__(64 * SW) void k_chol_bwd(const double *__restrict__ A, long lda, int k, int nb,
+                                                      const double *__restrict__ LinvT_prev, double *__restrict__ y)
 {
+    constexpr int NPART = 64 * SW / PB;
     __shared__ double v[PB];
     __shared__ double uk[PB];
-    const int tid = threadIdx.x;
+    __shared__ double red[NPART][PB];
+    const int tid = threadIdx.x, t = tid % PB, part = tid / PB;
     if (tid < PB) uk[tid] = tid < nb ? y[k + tid] : 0.0;
     __syncthreads();
-    const int hi = k - blockIdx.x * PB;   // rows [hi - PB, hi)
-    const int t = tid & (PB - 1), half = tid >> 7;
-    const int i = hi - PB + t;
-    const int m0 = half * (nb / 2), m1 = half ? nb : nb / 2;
+    const int per = (nb + NPART - 1) / NPART, m0 = min(nb, part * per), m1 = min(nb, m0 + per);
+    const int i = k - PB * (1 + (int)blockIdx.x) + t;
     double s0 = 0.0, s1 = 0.0;
     if (i >= 0) {
-        int m = m0;
-        for (; m + 2 <= m1; m += 2) {
-            s0 += A[(long)(k + m) * lda + i] * uk[m];
-            s1 += A[(long)(k + m + 1) * lda + i] * uk[m + 1];
+        for (int m = m0; m < m1; m += 8) {
+            double a[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) a[j] = m + j < m1 ? A[(long)(k + m + j) * lda + i] : 0.0;
+#pragma unroll
+            for (int j = 0; j < 8; j += 2) {
+                s0 += a[j] * uk[m + j < m1 ? m + j : 0];
+                s1 += a[j + 1] * uk[m + j + 1 < m1 ? m + j + 1 : 0];
+            }
         }
-        for (; m < m1; ++m) s0 += A[(long)(k + m) * lda + i] * uk[m];
     }
-    if (half) v[t] = s0 + s1;
+    red[part][t] = s0 + s1;
     __syncthreads();
-    if (!half && i >= 0) y[i] -= (s0 + s1) + v[t];
+    if (part == 0 && i >= 0) {
+        double s = red[0][t];
+#pragma unroll
+        for (int q = 1; q < NPART; ++q) s += red[q][t];
+        y[i] -= s;
+    }
     if (blockIdx.x == 0) {
         __threadfence_block();
         __syncthreads();
@@ -510,16 +546,16 @@ int ngd_factor_solve(double *G, int n, long lda, double nb, double mu, int mode,
     for (int p = 0; p + 1 < np; ++p) {
         const int k = p * PB, nb_ = PB, rest = n - k - nb_;
         const int nb2 = std::min(PB, rest);
-        hipLaunchKernelGGL(k_chol_fwd, dim3((unsigned)((rest + PB - 1) / PB)), dim3(256), 0, s, (const double *)G, lda,
-                           k, nb_, n, (const double *)(Linv + (long)(p + 1) * PB * PB), nb2, b);
+        hipLaunchKernelGGL(k_chol_fwd, dim3((unsigned)((rest + PB - 1) / PB)), dim3(64 * SW), 0, s, (const double *)G,
+                           lda, k, nb_, n, (const double *)(Linv + (long)(p + 1) * PB * PB), nb2, b);
     }
     const int kl = (np - 1) * PB;
     hipLaunchKernelGGL(k_chol_solve_first, dim3(1), dim3(256), 0, s, (const double *)(LinvT + (long)(np - 1) * PB * PB),
                        kl, n - kl, b);
     for (int p = np - 1; p >= 1; --p) {
         const int k = p * PB, nb_ = std::min(PB, n - k);
-        hipLaunchKernelGGL(k_chol_bwd, dim3((unsigned)((k + PB - 1) / PB)), dim3(256), 0, s, (const double *)G, lda, k,
-                           nb_, (const double *)(LinvT + (long)(p - 1) * PB * PB), b);
+        hipLaunchKernelGGL(k_chol_bwd, dim3((unsigned)((k + PB - 1) / PB)), dim3(64 * SW), 0, s, (const double *)G,
+                           lda, k, nb_, (const double *)(LinvT + (long)(p - 1) * PB * PB), b);
     }
     return 0;
 }
