@@ -48,11 +48,6 @@ static bool codegen_on()
 // the layer keeps h as the table-driven kernels need
 static int g_store_s = 1;
 
-// batched weight gradient on a second stream beside the adjoint chain
-#ifndef DNNF90_GRAD_AUX
-#define DNNF90_GRAD_AUX 1
-#endif
-
 static thread_local std::string g_err;
 
 static int fail(int rc, const std::string &m)
@@ -276,12 +271,6 @@ struct BatchWork {
     // Faa di Bruno forward when it holds h) into the plane hs
     bool recompute = false, s_kept = false;
     void *hs = nullptr;
-    // the batched weight gradient of a layer (K5 and its reduction) runs on
-    // aux beside the rest of the adjoint chain; S-bar of every other layer
-    // then lives in p2 (work spaces of more than one point)
-    hipStream_t aux = nullptr;
-    hipEvent_t ev_sbar = nullptr, ev_grad[2] = {nullptr, nullptr}, ev_done = nullptr;
-    void *p2 = nullptr;
     double bytes = 0.0, bytes_stored = 0.0;   // device bytes held / needed with every h kept
     void *hplane(int l) const { return recompute ? hs : hpl[l]; }
     double *partial = nullptr;
@@ -335,13 +324,7 @@ struct BatchWork {
         for (auto ev : prof.ev) (void)hipEventDestroy(ev);
         prof.ev.clear();
         auto f = [](void *&p) { if (p) (void)hipFree(p); p = nullptr; };
-        f(t0); f(tl); f(p0); f(p1); f(p2); f(scratch); f(hs);
-        for (hipEvent_t *e : {&ev_sbar, &ev_grad[0], &ev_grad[1], &ev_done}) {
-            if (*e) (void)hipEventDestroy(*e);
-            *e = nullptr;
-        }
-        if (aux) (void)hipStreamDestroy(aux);
-        aux = nullptr;
+        f(t0); f(tl); f(p0); f(p1); f(scratch); f(hs);
         for (auto &p : tpl) f(p);
         for (auto &p : hpl) f(p);
         tpl.clear(); hpl.clear();
@@ -413,8 +396,7 @@ static int bwork_init(BatchWork &b, const Net &nt, int maxpt, int prec, double b
         pc = std::max(pc, (long)(nc + grad_reduce_splits(nc)) * nt.dims[l] * (nt.dims[l - 1] + 1));
     }
     const int nfwd = bell_fwd_threads(nt.sched.nentry, (int)b.esz);
-    const bool two = maxpt > 1 && DNNF90_GRAD_AUX;
-    const double fixed = plane(nt.dims[0]) + plane(nt.dims[L - 1]) + (two ? 3 : 2) * plane(nmax) + 8.0 * pc +
+    const double fixed = plane(nt.dims[0]) + plane(nt.dims[L - 1]) + 2 * plane(nmax) + 8.0 * pc +
                          (nfwd == 0 ? (double)nt.sched.nentry * nmax * b.np_cap * b.esz : 0.0) +
                          8.0 * ((double)b.np_cap * (nt.dims[0] + 2.0 * nt.dims[L - 1] * na) + b.np_cap / 256 + 2);
     b.bytes_stored = fixed + 2 * plane(hsum);
@@ -426,15 +408,6 @@ static int bwork_init(BatchWork &b, const Net &nt, int maxpt, int prec, double b
     if ((rc = valloc(&b.tl, (size_t)nt.dims[L - 1] * col * b.esz))) return rc;
     if ((rc = valloc(&b.p0, (size_t)nmax * col * b.esz))) return rc;
     if ((rc = valloc(&b.p1, (size_t)nmax * col * b.esz))) return rc;
-    if (two) {
-        if ((rc = valloc(&b.p2, (size_t)nmax * col * b.esz))) return rc;
-        HIPCHK(hipStreamCreateWithFlags(&b.aux, hipStreamNonBlocking));
-        for (hipEvent_t *e : {&b.ev_sbar, &b.ev_grad[0], &b.ev_grad[1], &b.ev_done})
-            HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
-        // the first adjoint of each plane waits on these: complete from the start
-        HIPCHK(hipEventRecord(b.ev_grad[0], b.aux));
-        HIPCHK(hipEventRecord(b.ev_grad[1], b.aux));
-    }
     b.tpl.assign(L, nullptr);
     b.hpl.assign(L, nullptr);
     for (int l = 1; l < L - 1; ++l) {
@@ -523,16 +496,6 @@ static void forward_t(const Net &nt, BatchWork &b, const double *d_x, int npt)
 
 // adjoint from the output seed already in p0 (rows < nout).  rows_t == null:
 // gradient added to grad; else the per-point rows are written to rows_t.
-//
-// The batched weight gradient of layer l (K5 and its reduction) reads S-bar
-// of layer l and T of layer l-1 and writes only grad, and nothing later in
-// the chain reads grad: it runs on the second stream b.aux while K6 and the
-// next layer's Faa di Bruno adjoint go on.  S-bar of consecutive layers
-// alternates between p0 and p2, and the adjoint that writes one of them
-// waits until the gradient that read it two layers earlier is done; the
-// caller's stream waits for the last gradient before returning.  Kernel
-// profiling (every launch bracketed on one stream) and recompute keeping h
-// (which rewrites T of a layer whose gradient may still run) stay serial.
 template <typename T>
 static void backward_t(const Net &nt, BatchWork &b, double *grad, float *rows_t, long ldr)
 {
@@ -541,13 +504,11 @@ static void backward_t(const Net &nt, BatchWork &b, double *grad, float *rows_t,
     hipStream_t s = b.stream;
     const T *W = weights<T>(nt);
     const DevPairs pr = nt.dt.pairs();
-    const bool two = !rows_t && b.npt > 1 && b.aux && !b.prof.on && (!b.recompute || b.s_kept);
     for (int l = L - 1; l >= 1; --l) {
         const int nd = nt.dims[l], ndm = nt.dims[l - 1];
-        const int par = two ? (L - 1 - l) & 1 : 0;
-        T *sb = (T *)(par ? b.p2 : b.p0);
         if (l < L - 1) {
-            // the recomputed T of layer l equals the stored one bit for bit.
+            // p0 is free here (its S-bar was consumed by layer l+1); the
+            // recomputed T of layer l equals the stored one bit for bit.
             // A layer that keeps S needs K1 alone.
             if (b.recompute) {
                 if (b.s_kept)
@@ -555,54 +516,41 @@ static void backward_t(const Net &nt, BatchWork &b, double *grad, float *rows_t,
                 else
                     layer_forward<T>(nt, b, l, ncol, Np);
             }
-            if (two) (void)hipStreamWaitEvent(s, b.ev_grad[par], 0);
             PScope p(b, CAT_BELL_BWD);
             const GenKernels *gk = gen_for<T>(nt);
             if (b.s_kept)
-                (void)gen_launch_bwds(nt.gen[sizeof(T) == 8 ? 1 : 0], b.p1, b.hplane(l), sb, nd, Np, nt.dt.act(),
+                (void)gen_launch_bwds(nt.gen[sizeof(T) == 8 ? 1 : 0], b.p1, b.hplane(l), b.p0, nd, Np, nt.dt.act(),
                                       s);
             else if (gk)
-                (void)gen_launch_bwd(*gk, b.p1, b.hplane(l), sb, nd, Np, s);
+                (void)gen_launch_bwd(*gk, b.p1, b.hplane(l), b.p0, nd, Np, s);
             else
-                launch_bell_bwd<T>((const T *)b.p1, (const T *)b.hplane(l), sb, nd, Np, pr, s);
+                launch_bell_bwd<T>((const T *)b.p1, (const T *)b.hplane(l), (T *)b.p0, nd, Np, pr, s);
         }
         const T *X = (const T *)(l == 1 ? b.t0 : b.tpl[l - 1]);
         if (rows_t) {
             PScope p(b, CAT_GRAD);
-            launch_point_rows<T>(sb, X, nd, ndm + 1, na, Np, rows_t + nt.woff[l] * ldr, ldr, s);
+            launch_point_rows<T>((const T *)b.p0, X, nd, ndm + 1, na, Np, rows_t + nt.woff[l] * ldr, ldr, s);
         } else if (b.npt == 1) {
             // one point (the per-point ABI, the Kalman filter): one thread a
             // weight instead of the batched chunked product and its reduction
             PScope p(b, CAT_GRAD);
-            launch_grad_point1<T>(sb, X, nd, ndm + 1, na, Np, grad + nt.woff[l], s);
+            launch_grad_point1<T>((const T *)b.p0, X, nd, ndm + 1, na, Np, grad + nt.woff[l], s);
         } else {
-            hipStream_t gs = s;
-            if (two) {
-                (void)hipEventRecord(b.ev_sbar, s);
-                (void)hipStreamWaitEvent(b.aux, b.ev_sbar, 0);
-                gs = b.aux;
-            }
             int nc;
             long ch;
             grad_split(ncol, nd, ndm + 1, (int)b.esz, &nc, &ch);
             {
                 PScope p(b, CAT_GRAD);
-                launch_gemm_grad<T>(b.prec, sb, X, nd, ndm + 1, ncol, Np, b.partial, nc, ch, gs);
+                launch_gemm_grad<T>(b.prec, (const T *)b.p0, X, nd, ndm + 1, ncol, Np, b.partial, nc, ch, s);
             }
-            {
-                PScope p(b, CAT_GRAD_REDUCE);
-                launch_grad_reduce(b.partial, nc, (long)nd * (ndm + 1), grad + nt.woff[l], gs);
-            }
-            if (two) (void)hipEventRecord(b.ev_grad[par], b.aux);
+            PScope p(b, CAT_GRAD_REDUCE);
+            launch_grad_reduce(b.partial, nc, (long)nd * (ndm + 1), grad + nt.woff[l], s);
         }
         if (l > 1) {
             PScope p(b, CAT_LINEAR_T);
-            launch_gemm_wx<T>(b.prec, true, W + nt.woff[l], ndm + 1, sb, (T *)b.p1, ndm, nd, ncol, Np, s);
+            launch_gemm_wx<T>(b.prec, true, W + nt.woff[l], ndm + 1, (const T *)b.p0, (T *)b.p1, ndm, nd,
+                              ncol, Np, s);
         }
-    }
-    if (two) {
-        (void)hipEventRecord(b.ev_done, b.aux);
-        (void)hipStreamWaitEvent(s, b.ev_done, 0);
     }
 }
 
