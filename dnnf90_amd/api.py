@@ -274,8 +274,10 @@ class BatchWork:
     KERNEL_CATEGORIES = ("linear", "bell_fwd", "bell_bwd", "grad", "linear_t", "other", "grad_reduce",
                          "kf_pass", "kf_gain")
 
-    def profile(self, on=True):
-        _chk(lib().dnnf90_bwork_profile(self.id, 1 if on else 0), "dnnf90_bwork_profile")
+    def profile(self, on=True, only=None):
+        """Bracket every kernel category with HIP events (only: one category name)."""
+        v = (2 + self.KERNEL_CATEGORIES.index(only) if only else 1) if on else 0
+        _chk(lib().dnnf90_bwork_profile(self.id, v), "dnnf90_bwork_profile")
 
     def kernel_ms(self):
         """{category: (total ms, launches)} since the last call (resets)."""

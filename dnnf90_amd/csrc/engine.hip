@@ -249,6 +249,7 @@ enum {
 
 struct KProf {
     bool on = false;
+    int only = -1;               // bracket this category alone (-1: every one)
     std::vector<hipEvent_t> ev;  // begin/end pairs
     std::vector<int> cat;        // one per pair
     size_t used = 0;             // events recorded since the last resolve
@@ -340,16 +341,17 @@ struct BatchWork {
 // brackets the launches of one kernel category with events when profiling
 struct PScope {
     BatchWork &b;
-    PScope(BatchWork &bb, int c) : b(bb)
+    bool act;
+    PScope(BatchWork &bb, int c) : b(bb), act(bb.prof.on && (bb.prof.only < 0 || bb.prof.only == c))
     {
-        if (b.prof.on) {
+        if (act) {
             if (b.prof.used >= 8192) (void)b.resolve();
             b.mark(c);
         }
     }
     ~PScope()
     {
-        if (b.prof.on) b.mark(-1);
+        if (act) b.mark(-1);
     }
 };
 
@@ -1674,7 +1676,9 @@ int dnnf90_bwork_profile(int bid, int on)
     if (!b) return fail(-1, "bad work-space handle");
     int rc = b->resolve();
     if (rc) return rc;
+    if (on < 0 || on >= 2 + NCAT) return fail(-1, "profile: 0, 1 or 2 + category");
     b->prof.on = on != 0;
+    b->prof.only = on >= 2 ? on - 2 : -1;
     return 0;
 }
 

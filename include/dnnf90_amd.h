@@ -109,8 +109,10 @@ int dnnf90_bwork_free(int bid);
 int dnnf90_bwork_set_stream(int bid, void *stream);
 /* wait for everything launched on the work space's stream */
 int dnnf90_bwork_sync(int bid);
-/* kernel timing: with profiling on, every kernel category is bracketed by
- * HIP events on the work space's stream; dnnf90_bwork_kernel_ms returns and
+/* kernel timing: with profiling on (on = 1), every kernel category is
+ * bracketed by HIP events on the work space's stream; on = 2 + c brackets
+ * category c alone (so that one kernel can be timed inside otherwise
+ * unmarked steps); on = 0 stops.  dnnf90_bwork_kernel_ms returns and
  * resets the accumulated milliseconds and launch counts of the nine
  * categories (ms[9], count[9]): 0 linear map (W*T), 1 Faa di Bruno
  * forward, 2 adjoint pair sums, 3 weight gradient chunk partials (or
