@@ -189,3 +189,24 @@ def test_filter_committee_value_argument_checks(L):
     assert L.dnnf90_tabset_write_order(60, b"/dev/null") == -1
     assert L.dnnf90_net_free(nid) == 0
     ts.free()
+
+
+def test_switches_return_previous_setting(L):
+    """dnnf90_set_codegen / dnnf90_set_store_s return the setting they
+    replace (no device work); a bad work-space handle is refused by the
+    kernel-timing entries."""
+    prev = L.dnnf90_set_store_s(0)
+    try:
+        assert L.dnnf90_set_store_s(1) == 0
+        assert L.dnnf90_set_store_s(1) == 1
+    finally:
+        L.dnnf90_set_store_s(prev)
+    prev = L.dnnf90_set_codegen(0)
+    try:
+        assert L.dnnf90_set_codegen(1) == 0
+    finally:
+        L.dnnf90_set_codegen(prev)
+    assert L.dnnf90_bwork_profile(63, 1) == -1
+    ms = np.zeros(16)
+    cnt = np.zeros(16, np.int64)
+    assert L.dnnf90_bwork_kernel_ms(63, ms.ctypes.data, cnt.ctypes.data) == -1

@@ -159,3 +159,32 @@ def test_one_input_order_15(D, oracle_mod, codegen):
         bw.free(); g.free(); net.free()
     finally:
         lib().dnnf90_set_codegen(prev)
+
+
+def test_profile_one_category(D):
+    """dnnf90_bwork_profile(bid, 2 + c) brackets category c alone; values
+    outside 0 .. 2 + ncategories - 1 are refused."""
+    from dnnf90_amd._lib import lib
+    ts, net, w = _net(D, 2, 3, None, 0, [2, 8, 8, 1])
+    bw = D.BatchWork(net, 64, D.PREC_FP64)
+    g = D.Grad(net)
+    x = np.random.default_rng(9).uniform(-1, 1, (64, 2))
+    seed = np.ones((64, 1, ts.na))
+    bw.kernel_ms()
+    bw.profile(True, only="bell_bwd")
+    bw.forward(x, want=False)
+    bw.backward(seed, g)
+    bw.profile(False)
+    k = bw.kernel_ms()
+    print("one category:", {c: v for c, v in k.items() if v[1]})
+    assert k["bell_bwd"][1] == 2 and k["bell_bwd"][0] > 0.0   # two hidden layers
+    assert all(v[1] == 0 for c, v in k.items() if c != "bell_bwd")
+    bw.profile(True)
+    bw.forward(x, want=False)
+    bw.profile(False)
+    k = bw.kernel_ms()
+    assert k["linear"][1] == 3 and k["bell_fwd"][1] == 2
+    n = len(D.BatchWork.KERNEL_CATEGORIES)
+    assert lib().dnnf90_bwork_profile(bw.id, 2 + n) == -1
+    assert lib().dnnf90_bwork_profile(bw.id, -1) == -1
+    bw.free(); g.free(); net.free()
