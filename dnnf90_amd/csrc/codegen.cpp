@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <sstream>
 #include <string>
@@ -91,6 +92,114 @@ static void emit_pair_sums(std::ostringstream &o, const TableSet &t, bool fp64)
     }
 }
 
+bool gen_point_family(const TableSet &t, const FwdSchedule &s)
+{
+    return t.na <= PT_MAXA && s.nentry <= PT_MAXE;
+}
+
+// One point through the whole network in one workgroup of 256 threads (FP64;
+// the per-point ABI and the Kalman filter).  The planes are the work
+// space's, column 0 (point 0) of each slot: T[0] the input, S[l] and T[l]
+// of the hidden layers, T[L-1] the output.  Per layer: every (neuron, slot)
+// dot product of K1 by one thread (bias first, then the inputs in order),
+// a barrier, the layer's Faa di Bruno forward one lane a neuron (the
+// generated T-only body), a barrier.  The adjoint, from the seed in Sb:
+// the Faa di Bruno adjoint one lane a neuron (h recomputed from S, then the
+// pair sums, into Sb), the weight gradient one thread a weight (the slots
+// in order, as the one-point gradient kernel sums them) added into grad,
+// and the transposed map one thread a (neuron, slot) of the layer below
+// (into Tb), a barrier between phases.  Used for networks whose widths fit
+// the workgroup; the batched kernels take the rest.
+template <typename LoadS>
+void emit_point_kernels(std::ostringstream &o, const TableSet &t, const FwdSchedule &s, LoadS load_s)
+{
+    const int na = t.na;
+    o << "#define PT_MAXL " << PT_MAXL << "\n"
+      << "struct PtNet { int L, pad; long Np, ldr; int dims[PT_MAXL]; long woff[PT_MAXL]; real *T[PT_MAXL]; "
+         "real *S[PT_MAXL]; };\n";
+    o << "extern \"C\" __global__ __launch_bounds__(256) void dnnf90_pt_fwd_f64(const real *__restrict__ W, "
+         "const double *__restrict__ x, const int *__restrict__ ind_e1, PtNet p, ActParams ap)\n{\n"
+      << "  const int tid = threadIdx.x;\n"
+      << "  const long Np = p.Np, ldr = p.ldr;\n"
+      << "  for (int u = tid; u < p.dims[0] * " << na << "; u += 256) {\n"
+      << "    const int i = u / " << na << ", a = u % " << na << ";\n"
+      << "    p.T[0][i * ldr + a * Np] = a == 0 ? (real)x[i] : (a == ind_e1[i] ? (real)1 : (real)0);\n"
+      << "  }\n"
+      << "  __syncthreads();\n"
+      << "  for (int l = 1; l < p.L; ++l) {\n"
+      << "    const int nn = p.dims[l], m = p.dims[l - 1];\n"
+      << "    const real *Wl = W + p.woff[l];\n"
+      << "    const real *X = p.T[l - 1];\n"
+      << "    real *out = l < p.L - 1 ? p.S[l] : p.T[l];\n"
+      << "    for (int u = tid; u < nn * " << na << "; u += 256) {\n"
+      << "      const int jj = u / " << na << ", a = u % " << na << ";\n"
+      << "      const real *wr = Wl + (long)jj * (m + 1);\n"
+      << "      real v = a == 0 ? wr[0] : (real)0;\n"
+      << "      for (int i = 0; i < m; ++i) v += wr[1 + i] * X[i * ldr + a * Np];\n"
+      << "      out[jj * ldr + a * Np] = v;\n"
+      << "    }\n"
+      << "    __syncthreads();\n"
+      << "    if (l < p.L - 1) {\n"
+      << "      if (tid < nn) {\n"
+      << "        const real *Sj = p.S[l] + (long)tid * ldr;\n"
+      << "        real *Tj = p.T[l] + (long)tid * ldr;\n";
+    load_s();
+    o << "        Tj[0] = d0;\n";
+    emit_recurrence(o, t, s, true, REC_T);
+    o << "      }\n"
+      << "      __syncthreads();\n"
+      << "    }\n"
+      << "  }\n"
+      << "}\n";
+
+    o << "extern \"C\" __global__ __launch_bounds__(256) void dnnf90_pt_bwd_f64(const real *__restrict__ W, "
+         "double *__restrict__ grad, real *__restrict__ Sb, real *__restrict__ Tb, PtNet p, ActParams ap)\n{\n"
+      << "  const int tid = threadIdx.x;\n"
+      << "  const long Np = p.Np, ldr = p.ldr;\n"
+      << "  for (int l = p.L - 1; l >= 1; --l) {\n"
+      << "    const int nn = p.dims[l], m = p.dims[l - 1];\n"
+      << "    const real *Wl = W + p.woff[l];\n"
+      << "    const real *X = p.T[l - 1];\n"
+      << "    if (l < p.L - 1) {\n"
+      << "      if (tid < nn) {\n"
+      << "        const real *Sj = p.S[l] + (long)tid * ldr;\n"
+      << "        const real *tb = Tb + (long)tid * ldr;\n"
+      << "        real *sb = Sb + (long)tid * ldr;\n";
+    load_s();
+    o << "        const real h0 = d1;\n";
+    for (int a = 1; a < na; ++a) o << "        real h" << a << ";\n";
+    emit_recurrence(o, t, s, true, REC_HREG);
+    for (int a = 0; a < na; ++a) o << "        const real t" << a << " = tb[" << a << "L * Np];\n";
+    emit_pair_sums(o, t, true);
+    o << "      }\n"
+      << "      __syncthreads();\n"
+      << "    }\n"
+      << "    for (int u = tid; u < nn * (m + 1); u += 256) {\n"
+      << "      const int jj = u / (m + 1), kk = u % (m + 1);\n"
+      << "      const real *sj = Sb + (long)jj * ldr;\n"
+      << "      double g;\n"
+      << "      if (kk == 0) {\n"
+      << "        g = (double)sj[0];\n"
+      << "      } else {\n"
+      << "        const real *xk = X + (long)(kk - 1) * ldr;\n"
+      << "        g = 0.0;\n"
+      << "        for (int a = 0; a < " << na << "; ++a) g += (double)sj[a * Np] * (double)xk[a * Np];\n"
+      << "      }\n"
+      << "      grad[p.woff[l] + u] += g;\n"
+      << "    }\n"
+      << "    if (l > 1) {\n"
+      << "      for (int u = tid; u < m * " << na << "; u += 256) {\n"
+      << "        const int i = u / " << na << ", a = u % " << na << ";\n"
+      << "        real v = (real)0;\n"
+      << "        for (int jj = 0; jj < nn; ++jj) v += Wl[(long)jj * (m + 1) + 1 + i] * Sb[jj * ldr + a * Np];\n"
+      << "        Tb[i * ldr + a * Np] = v;\n"
+      << "      }\n"
+      << "    }\n"
+      << "    __syncthreads();\n"
+      << "  }\n"
+      << "}\n";
+}
+
 std::string gen_bell_source(const TableSet &t, const FwdSchedule &s, bool fp64)
 {
     std::ostringstream o;
@@ -162,6 +271,7 @@ std::string gen_bell_source(const TableSet &t, const FwdSchedule &s, bool fp64)
     for (int a = 0; a < na; ++a) o << "  const real t" << a << " = tb[" << a << "L * Np];\n";
     emit_pair_sums(o, t, fp64);
     o << "}\n";
+    if (fp64 && gen_point_family(t, s)) emit_point_kernels(o, t, s, load_s);
     return o.str();
 }
 
@@ -222,6 +332,17 @@ int gen_bell_load(const std::vector<char> &code, bool fp64, GenKernels *g)
     (void)hipFuncGetAttribute(&g->bwds_scratch, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, g->bwds);
     (void)hipFuncGetAttribute(&g->fwdt_regs, HIP_FUNC_ATTRIBUTE_NUM_REGS, g->fwdt);
     (void)hipFuncGetAttribute(&g->bwds_regs, HIP_FUNC_ATTRIBUTE_NUM_REGS, g->bwds);
+    // the one-point kernels are in the FP64 modules of small families only
+    if (fp64 && hipModuleGetFunction(&g->pt_fwd, g->mod, "dnnf90_pt_fwd_f64") == hipSuccess &&
+        hipModuleGetFunction(&g->pt_bwd, g->mod, "dnnf90_pt_bwd_f64") == hipSuccess) {
+        int a = 0, b = 0;
+        (void)hipFuncGetAttribute(&a, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, g->pt_fwd);
+        (void)hipFuncGetAttribute(&b, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, g->pt_bwd);
+        g->pt_scratch = std::max(a, b);
+    } else {
+        g->pt_fwd = g->pt_bwd = nullptr;
+        (void)hipGetLastError();
+    }
     g->ok = true;
     return 0;
 }
@@ -230,7 +351,7 @@ void GenKernels::release()
 {
     if (mod) (void)hipModuleUnload(mod);
     mod = nullptr;
-    fwd = bwd = fwdt = bwds = nullptr;
+    fwd = bwd = fwdt = bwds = pt_fwd = pt_bwd = nullptr;
     ok = false;
 }
 
@@ -267,6 +388,24 @@ hipError_t gen_launch_bwds(const GenKernels &g, const void *Tb, const void *S, v
     void *params[] = {&Tb, &S, &Sb, &nrow, &Np, &a};
     return hipModuleLaunchKernel(g.bwds, (unsigned)(Np / 64), (unsigned)((nrow + 3) / 4), 1, 256, 1, 1, 0, s, params,
                                  nullptr);
+}
+
+hipError_t gen_launch_pt_fwd(const GenKernels &g, const double *W, const double *x, const int *ind_e1, const PtNet &p,
+                             const ActParams &ap, hipStream_t s)
+{
+    ActParams a = ap;
+    PtNet q = p;
+    void *params[] = {&W, &x, &ind_e1, &q, &a};
+    return hipModuleLaunchKernel(g.pt_fwd, 1, 1, 1, 256, 1, 1, 0, s, params, nullptr);
+}
+
+hipError_t gen_launch_pt_bwd(const GenKernels &g, const double *W, double *grad, double *Sb, double *Tb,
+                             const PtNet &p, const ActParams &ap, hipStream_t s)
+{
+    ActParams a = ap;
+    PtNet q = p;
+    void *params[] = {&W, &grad, &Sb, &Tb, &q, &a};
+    return hipModuleLaunchKernel(g.pt_bwd, 1, 1, 1, 256, 1, 1, 0, s, params, nullptr);
 }
 
 }  // namespace dnnf90

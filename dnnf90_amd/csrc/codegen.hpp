@@ -15,12 +15,28 @@ namespace dnnf90 {
 // fwdt/bwds: the forward writes T only, the layer keeps S, and the adjoint
 // reads Tbar and S and recomputes h in registers (one plane less per hidden
 // layer); used when neither spills.
+// one-point kernels (FP64 families with at most PT_MAXA slots and PT_MAXE
+// operand entries; networks of at most PT_MAXL layers whose widths fit one
+// workgroup): the whole forward, and the whole adjoint with the weight
+// gradient, each one launch
+constexpr int PT_MAXL = 16, PT_MAXA = 48, PT_MAXE = 96, PT_MAXW = 256;
+struct PtNet {                    // mirrors the generated kernels' argument
+    int L, pad;
+    long Np, ldr;
+    int dims[PT_MAXL];
+    long woff[PT_MAXL];
+    double *T[PT_MAXL];
+    double *S[PT_MAXL];
+};
+
 struct GenKernels {
     hipModule_t mod = nullptr;
     hipFunction_t fwd = nullptr, bwd = nullptr, fwdt = nullptr, bwds = nullptr;
+    hipFunction_t pt_fwd = nullptr, pt_bwd = nullptr;   // FP64 small families only
     std::vector<char> code;
     int fwd_regs = 0, bwd_regs = 0, fwd_scratch = 0, bwd_scratch = 0;
     int fwdt_regs = 0, bwds_regs = 0, fwdt_scratch = 0, bwds_scratch = 0;
+    int pt_scratch = 0;
     bool ok = false;
     bool s_ok() const { return ok && fwdt_scratch == 0 && bwds_scratch == 0; }
     void release();
@@ -30,6 +46,8 @@ struct GenKernels {
 std::string gen_bell_source(const TableSet &t, const FwdSchedule &s, bool fp64);
 // whether the family is small enough to unroll
 bool gen_bell_usable(const TableSet &t, const FwdSchedule &s);
+// whether the FP64 module carries the one-point kernels for this family
+bool gen_point_family(const TableSet &t, const FwdSchedule &s);
 // compile with hipRTC and load on the current device; 0 on success
 int gen_bell_compile(const TableSet &t, const FwdSchedule &s, bool fp64, GenKernels *g, std::string *log);
 // load compiled code (from gen_bell_compile or a cache of it) into g;
@@ -44,5 +62,9 @@ hipError_t gen_launch_fwdt(const GenKernels &g, const void *S, void *Tout, int n
                            hipStream_t s);
 hipError_t gen_launch_bwds(const GenKernels &g, const void *Tb, const void *S, void *Sb, int nrow, long Np,
                            const ActParams &ap, hipStream_t s);
+hipError_t gen_launch_pt_fwd(const GenKernels &g, const double *W, const double *x, const int *ind_e1, const PtNet &p,
+                             const ActParams &ap, hipStream_t s);
+hipError_t gen_launch_pt_bwd(const GenKernels &g, const double *W, double *grad, double *Sb, double *Tb,
+                             const PtNet &p, const ActParams &ap, hipStream_t s);
 
 }  // namespace dnnf90

@@ -48,6 +48,10 @@ static bool codegen_on()
 // the layer keeps h as the table-driven kernels need
 static int g_store_s = 1;
 
+// one point through small networks in one launch each way (the generated
+// one-point kernels); on unless switched off by dnnf90_set_point_kernels
+static int g_point = 1;
+
 static thread_local std::string g_err;
 
 static int fail(int rc, const std::string &m)
@@ -271,6 +275,8 @@ struct BatchWork {
     // pass recomputes it from the stored T of the layer below (K1, and the
     // Faa di Bruno forward when it holds h) into the plane hs
     bool recompute = false, s_kept = false;
+    bool pt_used = false;   // the last forward went through the one-point kernel
+    bool rows_next = false; // the adjoint of this forward writes per-point rows (batched kernels)
     void *hs = nullptr;
     double bytes = 0.0, bytes_stored = 0.0;   // device bytes held / needed with every h kept
     void *hplane(int l) const { return recompute ? hs : hpl[l]; }
@@ -478,6 +484,39 @@ static void layer_forward(const Net &nt, BatchWork &b, int l, long ncol, long Np
     }
 }
 
+// the one-point kernels take the work space when it holds one FP64 point,
+// the layers keep S (the adjoint recomputes h), nothing is recomputed, and
+// the network fits: at most PT_MAXL layers, every width at most PT_MAXW
+// lanes and width x slots at most 2048 (wider layers go faster through the
+// batched kernels, which spread a layer over many workgroups)
+template <typename T>
+static const GenKernels *point_kernels(const Net &nt, const BatchWork &b)
+{
+    if (sizeof(T) != 8 || !g_point || b.npt != 1 || !b.s_kept || b.recompute || b.rows_next || nt.nlayer > PT_MAXL)
+        return nullptr;
+    const GenKernels &g = nt.gen[1];
+    if (!g.ok || !g.pt_fwd || !g.pt_bwd || g.pt_scratch != 0) return nullptr;
+    for (int l = 0; l < nt.nlayer; ++l)
+        if (nt.dims[l] > PT_MAXW || (long)nt.dims[l] * nt.tab.na > 2048) return nullptr;
+    return &g;
+}
+
+static PtNet point_net(const Net &nt, const BatchWork &b)
+{
+    PtNet p{};
+    const int L = nt.nlayer;
+    p.L = L;
+    p.Np = b.np;
+    p.ldr = (long)nt.tab.na * b.np;
+    for (int l = 0; l < L; ++l) {
+        p.dims[l] = nt.dims[l];
+        p.woff[l] = nt.woff[l];
+        p.T[l] = (double *)(l == 0 ? b.t0 : (l == L - 1 ? b.tl : b.tpl[l]));
+        p.S[l] = (double *)(l > 0 && l < L - 1 ? b.hplane(l) : nullptr);
+    }
+    return p;
+}
+
 template <typename T>
 static void forward_t(const Net &nt, BatchWork &b, const double *d_x, int npt)
 {
@@ -489,6 +528,14 @@ static void forward_t(const Net &nt, BatchWork &b, const double *d_x, int npt)
     const GenKernels *gk = gen_for<T>(nt);
     b.s_kept = g_store_s && gk && gk->s_ok();
     hipStream_t s = b.stream;
+    if (const GenKernels *pk = point_kernels<T>(nt, b)) {
+        b.pt_used = true;
+        PScope p(b, CAT_BELL_FWD);
+        (void)gen_launch_pt_fwd(*pk, (const double *)weights<T>(nt), d_x, nt.dt.ind_e1, point_net(nt, b), nt.dt.act(),
+                                s);
+        return;
+    }
+    b.pt_used = false;
     {
         PScope p(b, CAT_OTHER);
         launch_input<T>(d_x, npt, nt.dims[0], na, nt.dt.ind_e1, (T *)b.t0, Np, s);
@@ -506,6 +553,13 @@ static void backward_t(const Net &nt, BatchWork &b, double *grad, float *rows_t,
     hipStream_t s = b.stream;
     const T *W = weights<T>(nt);
     const DevPairs pr = nt.dt.pairs();
+    if (!rows_t && b.pt_used) {
+        // the forward went through the one-point kernel: so does the adjoint
+        PScope p(b, CAT_BELL_BWD);
+        (void)gen_launch_pt_bwd(nt.gen[1], (const double *)W, grad, (double *)b.p0, (double *)b.p1, point_net(nt, b),
+                                nt.dt.act(), s);
+        return;
+    }
     for (int l = L - 1; l >= 1; --l) {
         const int nd = nt.dims[l], ndm = nt.dims[l - 1];
         if (l < L - 1) {
@@ -843,7 +897,9 @@ static int run_residual(Net &nt, BatchWork &b, int npt, const double *x, int nte
     DevResidual r;
     int rc = set_residual(b, nterm, coeff, slot, nonlin, na, &r);
     if (rc) return rc;
+    b.rows_next = rows_t != nullptr;
     rc = run_forward<T>(nt, b, npt, x, nullptr, dev);
+    b.rows_next = false;
     if (rc) return rc;
     const int nblk = (int)((b.np + 255) / 256);
     // rows: the seed is dR/dT, set_pinn_seed(t, R=1, fac=1) (sgd_batch_module.f90:742)
@@ -1643,6 +1699,13 @@ int dnnf90_set_codegen(int on)
 {
     const int prev = codegen_on() ? 1 : 0;
     g_codegen = on ? 1 : 0;
+    return prev;
+}
+
+int dnnf90_set_point_kernels(int on)
+{
+    const int prev = g_point;
+    g_point = on ? 1 : 0;
     return prev;
 }
 

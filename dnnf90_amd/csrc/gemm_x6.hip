@@ -436,7 +436,10 @@ __global__ __launch_bounds__(256) void k_grad_x6(const float *__restrict__ A, co
 
 // tile of the split K5: (16 RB) x 64 of the weight block, RB = 8 where the
 // block has more than 64 rows (each split element then feeds more MFMAs)
-int grad_x6_rb(int M) { return M > 64 ? 8 : 4; }
+#ifndef GRAD_X6_RB8_ABOVE
+#define GRAD_X6_RB8_ABOVE 64
+#endif
+int grad_x6_rb(int M) { return M > GRAD_X6_RB8_ABOVE ? 8 : 4; }
 
 void launch_grad_x6(const float *A, const float *B, int M, int Kd, long ncol, long Np, double *partial,
                     int nchunk, long chunk, hipStream_t s)

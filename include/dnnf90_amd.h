@@ -71,6 +71,14 @@ int dnnf90_tabset_write_order(int tid, const char *filename);
  * of the family. */
 int dnnf90_set_codegen(int on);
 int dnnf90_set_store_s(int on);
+/* a work space holding one point (the per-point ABI, the Kalman filter)
+ * runs the whole forward, and the whole adjoint with the weight gradient,
+ * as one generated kernel each when the network is small (FP64, at most 16
+ * layers, every width at most 256 and width x nderiv at most 2048, a table
+ * family of at most 48 slots); dnnf90_set_point_kernels(0) uses the layer
+ * kernels instead.  Returns the previous setting.  No counterpart in the
+ * reference (its per-point path is the only one it has). */
+int dnnf90_set_point_kernels(int on);
 int dnnf90_net_kernel_info(int nid, int prec, int *info);
 
 int dnnf90_net_nweights(int nid);

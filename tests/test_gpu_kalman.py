@@ -73,8 +73,18 @@ CASES = [(mode, dec, gate, q) for mode in (0, 1, 2, 3) for dec in (False, True)
          for gate, q in ((0.0, 0.0), (0.2, 1e-3))]
 
 
+@pytest.fixture(params=[1, 0], ids=["point_kernels", "layer_kernels"])
+def point(request, D):
+    """One observation's forward and adjoint as the one-point kernels (one
+    launch each way), then as the layer kernels."""
+    from dnnf90_amd._lib import lib
+    prev = lib().dnnf90_set_point_kernels(request.param)
+    yield request.param
+    lib().dnnf90_set_point_kernels(prev)
+
+
 @pytest.mark.parametrize("mode,dec,gate,q", CASES)
-def test_kf_sequence_vs_oracle(D, oracle_mod, mode, dec, gate, q):
+def test_kf_sequence_vs_oracle(D, oracle_mod, point, mode, dec, gate, q):
     ts = D.TableSet(2, 3, SEEDS)
     o = oracle_mod.Oracle(2, 3, SEEDS)
     dims = [2, 6, 5, 1] if mode in (0, 3) else [2, 6, 5, 2]

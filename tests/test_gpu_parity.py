@@ -157,6 +157,51 @@ def test_store_s_identical(D, oracle_mod, name, npt, prec):
     assert np.array_equal(t1, t0) and np.array_equal(g1, g0)
 
 
+@pytest.mark.parametrize("name,dims", [("kdv", None), ("ngd2_k4", None), ("dense4_k3", [4, 24, 24, 1]),
+                                       ("gelu3_k4", [3, 16, 16, 16, 1])])
+def test_point_kernels_vs_layer_kernels(D, oracle_mod, name, dims):
+    """The per-point ABI (dnnf90_eval, dnnf90_grad_point) with the one-point
+    kernels and with the layer kernels, both against the oracle (1e-12) and
+    against each other."""
+    from dnnf90_amd._lib import lib
+    from dnnf90_amd.workloads import WORKLOADS
+    wl = WORKLOADS[name]
+    dims = dims or wl.dims
+    ts = D.TableSet(wl.d0, wl.k, wl.seeds, wl.act)
+    net = D.Net(ts, dims)
+    o = oracle_mod.Oracle(wl.d0, wl.k, wl.seeds, wl.act)
+    rng = np.random.default_rng(11)
+    w = rng.uniform(-0.5, 0.5, net.nw)
+    net.weights = w
+    x = rng.uniform(-1, 1, (5, wl.d0))
+    seed = rng.standard_normal((5, 1, o.na))
+    tr = o.eval(dims, w, x)
+    gr = o.grad(dims, w, x, seed)
+    out = []
+    prev = lib().dnnf90_set_point_kernels(1)
+    try:
+        for on in (1, 0):
+            lib().dnnf90_set_point_kernels(on)
+            wk = D.Work(net)
+            tw = D.TWork(net)
+            g = D.Grad(net)
+            t = np.array([wk.eval(xn) for xn in x])
+            for n in range(5):
+                tw.grad_point(x[n], seed[n, 0], g)
+            out.append((t, g.get()))
+            et = max(rel_by_order(t[:, None, :], tr[:, :1, :], o.deg()).values())
+            eg = rel(g.get(), gr)
+            print(name, dims, "point kernels" if on else "layer kernels", "forward", et, "gradient", eg)
+            assert et <= TOL64 and eg <= TOL64
+            wk.free(); tw.free(); g.free()
+    finally:
+        lib().dnnf90_set_point_kernels(prev)
+    (t1, g1), (t0, g0) = out
+    print("point vs layer kernels: forward", np.abs(t1 - t0).max(), "gradient", rel(g1, g0))
+    assert rel(g1, g0) <= 1e-13
+    net.free()
+
+
 def test_per_point_abi_and_adam(D, oracle_mod):
     """dnnf90_grad_point and dnnf90_adam_step (per-point reference ABI)."""
     from dnnf90_amd.workloads import WORKLOADS
