@@ -98,73 +98,93 @@ bool gen_point_family(const TableSet &t, const FwdSchedule &s)
 }
 
 // One point through the whole network in one workgroup of 256 threads (FP64;
-// the per-point ABI and the Kalman filter).  The planes are the work
-// space's, column 0 (point 0) of each slot: T[0] the input, S[l] and T[l]
-// of the hidden layers, T[L-1] the output.  Per layer: every (neuron, slot)
-// dot product of K1 by one thread (bias first, then the inputs in order),
-// a barrier, the layer's Faa di Bruno forward one lane a neuron (the
-// generated T-only body), a barrier.  The adjoint, from the seed in Sb:
-// the Faa di Bruno adjoint one lane a neuron (h recomputed from S, then the
-// pair sums, into Sb), the weight gradient one thread a weight (the slots
-// in order, as the one-point gradient kernel sums them) added into grad,
-// and the transposed map one thread a (neuron, slot) of the layer below
-// (into Tb), a barrier between phases.  Used for networks whose widths fit
-// the workgroup; the batched kernels take the rest.
+// the per-point ABI and the Kalman filter).  Every plane a phase reads is
+// staged in LDS as [row][slot]; the work space's planes (column 0 of each
+// slot) are written where later kernels read them (the forward's S and T)
+// and read where the forward left them (the adjoint's S and T).  Forward,
+// per layer: every (neuron, slot) dot product of K1 by one thread (bias
+// first, then the inputs in order), a barrier, the Faa di Bruno forward one
+// lane a neuron (the generated T-only body), a barrier.  Adjoint, from the
+// seed in Sb, per layer: the Faa di Bruno adjoint one lane a neuron (h
+// recomputed from S, then the pair sums), the weight gradient one thread a
+// weight (the slots in order, as the one-point gradient kernel sums them)
+// added into grad, the transposed map one thread a (neuron, slot) of the
+// layer below, barriers between phases.  Networks whose width x slots stay
+// within PT_MAXWA; the batched kernels take the rest.
 template <typename LoadS>
 void emit_point_kernels(std::ostringstream &o, const TableSet &t, const FwdSchedule &s, LoadS load_s)
 {
     const int na = t.na;
     o << "#define PT_MAXL " << PT_MAXL << "\n"
+      << "#define PT_LDS " << PT_MAXWA << "\n"
       << "struct PtNet { int L, pad; long Np, ldr; int dims[PT_MAXL]; long woff[PT_MAXL]; real *T[PT_MAXL]; "
          "real *S[PT_MAXL]; };\n";
     o << "extern \"C\" __global__ __launch_bounds__(256) void dnnf90_pt_fwd_f64(const real *__restrict__ W, "
          "const double *__restrict__ x, const int *__restrict__ ind_e1, PtNet p, ActParams ap)\n{\n"
+      << "  __shared__ real xs[PT_LDS];   // T of the layer below\n"
+      << "  __shared__ real ss[PT_LDS];   // S of this layer\n"
       << "  const int tid = threadIdx.x;\n"
-      << "  const long Np = p.Np, ldr = p.ldr;\n"
+      << "  const long GNp = p.Np, ldr = p.ldr;\n"
       << "  for (int u = tid; u < p.dims[0] * " << na << "; u += 256) {\n"
       << "    const int i = u / " << na << ", a = u % " << na << ";\n"
-      << "    p.T[0][i * ldr + a * Np] = a == 0 ? (real)x[i] : (a == ind_e1[i] ? (real)1 : (real)0);\n"
+      << "    const real v = a == 0 ? (real)x[i] : (a == ind_e1[i] ? (real)1 : (real)0);\n"
+      << "    xs[u] = v;\n"
+      << "    p.T[0][i * ldr + a * GNp] = v;\n"
       << "  }\n"
       << "  __syncthreads();\n"
       << "  for (int l = 1; l < p.L; ++l) {\n"
       << "    const int nn = p.dims[l], m = p.dims[l - 1];\n"
       << "    const real *Wl = W + p.woff[l];\n"
-      << "    const real *X = p.T[l - 1];\n"
-      << "    real *out = l < p.L - 1 ? p.S[l] : p.T[l];\n"
+      << "    real *outg = l < p.L - 1 ? p.S[l] : p.T[l];\n"
       << "    for (int u = tid; u < nn * " << na << "; u += 256) {\n"
       << "      const int jj = u / " << na << ", a = u % " << na << ";\n"
       << "      const real *wr = Wl + (long)jj * (m + 1);\n"
       << "      real v = a == 0 ? wr[0] : (real)0;\n"
-      << "      for (int i = 0; i < m; ++i) v += wr[1 + i] * X[i * ldr + a * Np];\n"
-      << "      out[jj * ldr + a * Np] = v;\n"
+      << "      for (int i = 0; i < m; ++i) v += wr[1 + i] * xs[i * " << na << " + a];\n"
+      << "      ss[u] = v;\n"
+      << "      outg[jj * ldr + a * GNp] = v;\n"
       << "    }\n"
       << "    __syncthreads();\n"
       << "    if (l < p.L - 1) {\n"
       << "      if (tid < nn) {\n"
-      << "        const real *Sj = p.S[l] + (long)tid * ldr;\n"
-      << "        real *Tj = p.T[l] + (long)tid * ldr;\n";
+      << "        const long Np = 1;   // LDS rows: slot stride 1\n"
+      << "        const real *Sj = ss + tid * " << na << ";\n"
+      << "        real *Tj = xs + tid * " << na << ";   // the next layer's input; K1 is done with xs\n";
     load_s();
     o << "        Tj[0] = d0;\n";
     emit_recurrence(o, t, s, true, REC_T);
-    o << "      }\n"
+    o << "        for (int a = 0; a < " << na << "; ++a) p.T[l][tid * ldr + a * GNp] = Tj[a];\n"
+      << "      }\n"
       << "      __syncthreads();\n"
       << "    }\n"
       << "  }\n"
       << "}\n";
 
     o << "extern \"C\" __global__ __launch_bounds__(256) void dnnf90_pt_bwd_f64(const real *__restrict__ W, "
-         "double *__restrict__ grad, real *__restrict__ Sb, real *__restrict__ Tb, PtNet p, ActParams ap)\n{\n"
+         "double *__restrict__ grad, const real *__restrict__ Sb, PtNet p, ActParams ap)\n{\n"
+      << "  __shared__ real ss[PT_LDS];    // S of this layer\n"
+      << "  __shared__ real xs[PT_LDS];    // T of the layer below\n"
+      << "  __shared__ real sbs[PT_LDS];   // Sbar of this layer\n"
+      << "  __shared__ real tbs[PT_LDS];   // Tbar of this layer\n"
       << "  const int tid = threadIdx.x;\n"
-      << "  const long Np = p.Np, ldr = p.ldr;\n"
+      << "  const long GNp = p.Np, ldr = p.ldr;\n"
+      << "  for (int u = tid; u < p.dims[p.L - 1] * " << na << "; u += 256)\n"
+      << "    sbs[u] = Sb[(u / " << na << ") * ldr + (u % " << na << ") * GNp];\n"
       << "  for (int l = p.L - 1; l >= 1; --l) {\n"
       << "    const int nn = p.dims[l], m = p.dims[l - 1];\n"
       << "    const real *Wl = W + p.woff[l];\n"
-      << "    const real *X = p.T[l - 1];\n"
+      << "    for (int u = tid; u < m * " << na << "; u += 256)\n"
+      << "      xs[u] = p.T[l - 1][(u / " << na << ") * ldr + (u % " << na << ") * GNp];\n"
+      << "    if (l < p.L - 1)\n"
+      << "      for (int u = tid; u < nn * " << na << "; u += 256)\n"
+      << "        ss[u] = p.S[l][(u / " << na << ") * ldr + (u % " << na << ") * GNp];\n"
+      << "    __syncthreads();\n"
       << "    if (l < p.L - 1) {\n"
       << "      if (tid < nn) {\n"
-      << "        const real *Sj = p.S[l] + (long)tid * ldr;\n"
-      << "        const real *tb = Tb + (long)tid * ldr;\n"
-      << "        real *sb = Sb + (long)tid * ldr;\n";
+      << "        const long Np = 1;   // LDS rows: slot stride 1\n"
+      << "        const real *Sj = ss + tid * " << na << ";\n"
+      << "        const real *tb = tbs + tid * " << na << ";\n"
+      << "        real *sb = sbs + tid * " << na << ";\n";
     load_s();
     o << "        const real h0 = d1;\n";
     for (int a = 1; a < na; ++a) o << "        real h" << a << ";\n";
@@ -176,14 +196,14 @@ void emit_point_kernels(std::ostringstream &o, const TableSet &t, const FwdSched
       << "    }\n"
       << "    for (int u = tid; u < nn * (m + 1); u += 256) {\n"
       << "      const int jj = u / (m + 1), kk = u % (m + 1);\n"
-      << "      const real *sj = Sb + (long)jj * ldr;\n"
+      << "      const real *sj = sbs + jj * " << na << ";\n"
       << "      double g;\n"
       << "      if (kk == 0) {\n"
       << "        g = (double)sj[0];\n"
       << "      } else {\n"
-      << "        const real *xk = X + (long)(kk - 1) * ldr;\n"
+      << "        const real *xk = xs + (kk - 1) * " << na << ";\n"
       << "        g = 0.0;\n"
-      << "        for (int a = 0; a < " << na << "; ++a) g += (double)sj[a * Np] * (double)xk[a * Np];\n"
+      << "        for (int a = 0; a < " << na << "; ++a) g += (double)sj[a] * (double)xk[a];\n"
       << "      }\n"
       << "      grad[p.woff[l] + u] += g;\n"
       << "    }\n"
@@ -191,8 +211,8 @@ void emit_point_kernels(std::ostringstream &o, const TableSet &t, const FwdSched
       << "      for (int u = tid; u < m * " << na << "; u += 256) {\n"
       << "        const int i = u / " << na << ", a = u % " << na << ";\n"
       << "        real v = (real)0;\n"
-      << "        for (int jj = 0; jj < nn; ++jj) v += Wl[(long)jj * (m + 1) + 1 + i] * Sb[jj * ldr + a * Np];\n"
-      << "        Tb[i * ldr + a * Np] = v;\n"
+      << "        for (int jj = 0; jj < nn; ++jj) v += Wl[(long)jj * (m + 1) + 1 + i] * sbs[jj * " << na << " + a];\n"
+      << "        tbs[u] = v;\n"
       << "      }\n"
       << "    }\n"
       << "    __syncthreads();\n"
@@ -399,12 +419,12 @@ hipError_t gen_launch_pt_fwd(const GenKernels &g, const double *W, const double 
     return hipModuleLaunchKernel(g.pt_fwd, 1, 1, 1, 256, 1, 1, 0, s, params, nullptr);
 }
 
-hipError_t gen_launch_pt_bwd(const GenKernels &g, const double *W, double *grad, double *Sb, double *Tb,
-                             const PtNet &p, const ActParams &ap, hipStream_t s)
+hipError_t gen_launch_pt_bwd(const GenKernels &g, const double *W, double *grad, const double *Sb, const PtNet &p,
+                             const ActParams &ap, hipStream_t s)
 {
     ActParams a = ap;
     PtNet q = p;
-    void *params[] = {&W, &grad, &Sb, &Tb, &q, &a};
+    void *params[] = {&W, &grad, &Sb, &q, &a};
     return hipModuleLaunchKernel(g.pt_bwd, 1, 1, 1, 256, 1, 1, 0, s, params, nullptr);
 }
 

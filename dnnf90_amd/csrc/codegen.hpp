@@ -17,9 +17,10 @@ namespace dnnf90 {
 // layer); used when neither spills.
 // one-point kernels (FP64 families with at most PT_MAXA slots and PT_MAXE
 // operand entries; networks of at most PT_MAXL layers whose widths fit one
-// workgroup): the whole forward, and the whole adjoint with the weight
-// gradient, each one launch
-constexpr int PT_MAXL = 16, PT_MAXA = 48, PT_MAXE = 96, PT_MAXW = 256;
+// workgroup, width x slots at most PT_MAXWA, the LDS rows of a plane): the
+// whole forward, and the whole adjoint with the weight gradient, each one
+// launch
+constexpr int PT_MAXL = 16, PT_MAXA = 48, PT_MAXE = 96, PT_MAXW = 256, PT_MAXWA = 1536;
 struct PtNet {                    // mirrors the generated kernels' argument
     int L, pad;
     long Np, ldr;
@@ -64,7 +65,7 @@ hipError_t gen_launch_bwds(const GenKernels &g, const void *Tb, const void *S, v
                            const ActParams &ap, hipStream_t s);
 hipError_t gen_launch_pt_fwd(const GenKernels &g, const double *W, const double *x, const int *ind_e1, const PtNet &p,
                              const ActParams &ap, hipStream_t s);
-hipError_t gen_launch_pt_bwd(const GenKernels &g, const double *W, double *grad, double *Sb, double *Tb,
-                             const PtNet &p, const ActParams &ap, hipStream_t s);
+hipError_t gen_launch_pt_bwd(const GenKernels &g, const double *W, double *grad, const double *Sb, const PtNet &p,
+                             const ActParams &ap, hipStream_t s);
 
 }  // namespace dnnf90

@@ -487,8 +487,9 @@ static void layer_forward(const Net &nt, BatchWork &b, int l, long ncol, long Np
 // the one-point kernels take the work space when it holds one FP64 point,
 // the layers keep S (the adjoint recomputes h), nothing is recomputed, and
 // the network fits: at most PT_MAXL layers, every width at most PT_MAXW
-// lanes and width x slots at most 2048 (wider layers go faster through the
-// batched kernels, which spread a layer over many workgroups)
+// lanes and width x slots at most PT_MAXWA (a plane's LDS rows; wider
+// layers go faster through the batched kernels anyway, which spread a layer
+// over many workgroups)
 template <typename T>
 static const GenKernels *point_kernels(const Net &nt, const BatchWork &b)
 {
@@ -497,7 +498,7 @@ static const GenKernels *point_kernels(const Net &nt, const BatchWork &b)
     const GenKernels &g = nt.gen[1];
     if (!g.ok || !g.pt_fwd || !g.pt_bwd || g.pt_scratch != 0) return nullptr;
     for (int l = 0; l < nt.nlayer; ++l)
-        if (nt.dims[l] > PT_MAXW || (long)nt.dims[l] * nt.tab.na > 2048) return nullptr;
+        if (nt.dims[l] > PT_MAXW || (long)nt.dims[l] * nt.tab.na > PT_MAXWA) return nullptr;
     return &g;
 }
 
@@ -556,7 +557,7 @@ static void backward_t(const Net &nt, BatchWork &b, double *grad, float *rows_t,
     if (!rows_t && b.pt_used) {
         // the forward went through the one-point kernel: so does the adjoint
         PScope p(b, CAT_BELL_BWD);
-        (void)gen_launch_pt_bwd(nt.gen[1], (const double *)W, grad, (double *)b.p0, (double *)b.p1, point_net(nt, b),
+        (void)gen_launch_pt_bwd(nt.gen[1], (const double *)W, grad, (const double *)b.p0, point_net(nt, b),
                                 nt.dt.act(), s);
         return;
     }

@@ -74,7 +74,7 @@ int dnnf90_set_store_s(int on);
 /* a work space holding one point (the per-point ABI, the Kalman filter)
  * runs the whole forward, and the whole adjoint with the weight gradient,
  * as one generated kernel each when the network is small (FP64, at most 16
- * layers, every width at most 256 and width x nderiv at most 2048, a table
+ * layers, every width at most 256 and width x nderiv at most 1536, a table
  * family of at most 48 slots); dnnf90_set_point_kernels(0) uses the layer
  * kernels instead.  Returns the previous setting.  No counterpart in the
  * reference (its per-point path is the only one it has). */
