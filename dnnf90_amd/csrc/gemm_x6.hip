@@ -436,6 +436,8 @@ __global__ __launch_bounds__(256) void k_grad_x6(const float *__restrict__ A, co
 
 // tile of the split K5: (16 RB) x 64 of the weight block, RB = 8 where the
 // block has more than 64 rows (each split element then feeds more MFMAs)
+// weight blocks taller than this take the 128-row tile (64-row tiles with
+// two register stages measured 6-13 % slower on 128- and 256-wide layers)
 #ifndef GRAD_X6_RB8_ABOVE
 #define GRAD_X6_RB8_ABOVE 64
 #endif
