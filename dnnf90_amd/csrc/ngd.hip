@@ -333,12 +333,16 @@ __global__ __launch_bounds__(256) void k_chol_panel(double *__restrict__ A, long
 }
 
 // A22 -= L21 L21^T over lower 64 x 64 tiles of the trailing matrix (rows
-// and columns >= base); L21 packed in P[row][PB], nb columns used
+// and columns >= base); L21 packed in P[row][PB], nb columns used.  The
+// panel goes through LDS in 32-column chunks, stored transposed with rows
+// padded to 65 doubles (the transposing stores of one wave then fall in
+// distinct banks); the next chunk's values are loaded into registers while
+// the current chunk is multiplied.
 __global__ __launch_bounds__(256) void k_chol_syrk(double *__restrict__ A, long lda, const double *__restrict__ P,
                                                    int base, int nb, int n)
 {
-    __shared__ double Pi[32][64];
-    __shared__ double Pj[32][64];
+    __shared__ double Pi[32][65];
+    __shared__ double Pj[32][65];
     const long t = blockIdx.x;
     int ti = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) / 2.0);
     while ((long)(ti + 1) * (ti + 2) / 2 <= t) ++ti;
@@ -352,14 +356,27 @@ __global__ __launch_bounds__(256) void k_chol_syrk(double *__restrict__ A, long 
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int jj = 0; jj < 2; ++jj) acc[i][jj] = dbl4{0, 0, 0, 0};
+    // thread tid stages (row tid / 32 + 8 q, column tid % 32) of a chunk, q < 8
+    const int kk_t = tid & 31, r_t = tid >> 5;
+    double vi[8], vj[8];
+    auto fetch = [&](int k0) {
+        const bool ok = k0 + kk_t < nb;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int r = r_t + 8 * q;
+            vi[q] = (ok && r0 + r < n) ? P[(long)(r0 + r) * PB + k0 + kk_t] : 0.0;
+            vj[q] = (ok && c0 + r < n) ? P[(long)(c0 + r) * PB + k0 + kk_t] : 0.0;
+        }
+    };
+    fetch(0);
     for (int k0 = 0; k0 < nb; k0 += 32) {
-        for (int u = tid; u < 64 * 32; u += 256) {
-            const int r = u >> 5, kk = u & 31;
-            const bool ok = k0 + kk < nb;
-            Pi[kk][r] = (ok && r0 + r < n) ? P[(long)(r0 + r) * PB + k0 + kk] : 0.0;
-            Pj[kk][r] = (ok && c0 + r < n) ? P[(long)(c0 + r) * PB + k0 + kk] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            Pi[kk_t][r_t + 8 * q] = vi[q];
+            Pj[kk_t][r_t + 8 * q] = vj[q];
         }
         __syncthreads();
+        if (k0 + 32 < nb) fetch(k0 + 32);
 #pragma unroll
         for (int kk = 0; kk < 32; kk += 4) {
             const int kr = kk + (lane >> 4);
