@@ -291,40 +291,71 @@ __global__ __launch_bounds__(256) void k_chol_diag(double *__restrict__ A, long 
     }
 }
 
-// L21 = A21 Linv^T for 64 rows per workgroup (rows k+nb.. of the panel's
-// columns k..k+nb-1); the rows are staged in LDS, Linv is read through L2;
-// the result is written back into G and packed into P[row][PB]
+// L21 = A21 Linv^T for 32 rows per workgroup (rows k+nb.. of the panel's
+// columns k..k+nb-1); the rows are staged in LDS (all of a thread's loads
+// in flight together), Linv is read through L2 four k-steps ahead of its
+// products; wave (b, h) forms rows 16 b .. +15 by columns 64 h .. +63.  The
+// result is written back into G and packed into P[row][PB].
+static constexpr int PR = 32;   // rows a workgroup
+
 __global__ __launch_bounds__(256) void k_chol_panel(double *__restrict__ A, long lda, int k, int nb, int n,
                                                     const double *__restrict__ Linv, double *__restrict__ P)
 {
-    __shared__ double Ar[64][PBL];
+    __shared__ double Ar[PR][PBL];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int r0 = k + nb + blockIdx.x * 64;
-    for (int u = tid; u < 64 * PB; u += 256) {
-        const int r = u / PB, c = u % PB;
-        Ar[r][c] = (r0 + r < n && c < nb) ? A[(long)(r0 + r) * lda + k + c] : 0.0;
+    const int rb = wave >> 1, hc = wave & 1;
+    const int r0 = k + nb + blockIdx.x * PR;
+    {
+        constexpr int NL = PR * PB / 256;
+        double v[NL];
+#pragma unroll
+        for (int i = 0; i < NL; ++i) {
+            const int u = tid + 256 * i, r = u / PB, c = u % PB;
+            v[i] = (r0 + r < n && c < nb) ? A[(long)(r0 + r) * lda + k + c] : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < NL; ++i) {
+            const int u = tid + 256 * i;
+            Ar[u / PB][u % PB] = v[i];
+        }
     }
     __syncthreads();
-    // wave: rows 16 wave .. +15, all PB columns (8 blocks of 16)
-    dbl4 acc[PB / 16];
+    constexpr int NJ = PB / 32;   // 16-column blocks a wave
+    dbl4 acc[NJ];
 #pragma unroll
-    for (int jb = 0; jb < PB / 16; ++jb) acc[jb] = dbl4{0, 0, 0, 0};
+    for (int jb = 0; jb < NJ; ++jb) acc[jb] = dbl4{0, 0, 0, 0};
     const int q = lane >> 4, p = lane & 15;
-    for (int kk = 0; kk < nb; kk += 4) {
-        const double a = Ar[16 * wave + p][kk + q];
+    // the Linv^T operands of k-step kk + 4 R are loaded while step kk is
+    // multiplied (a ring of R register sets, the loop unrolled R times)
+    constexpr int R = 4;
+    double bv[R][NJ];
+    auto fetch = [&](double (&d)[NJ], int kk) {
 #pragma unroll
-        for (int jb = 0; jb < PB / 16; ++jb) {
-            const double bv = Linv[(long)(16 * jb + p) * PB + kk + q];   // Linv^T[kk+q][col]
-            acc[jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[jb], 0, 0, 0);
+        for (int jb = 0; jb < NJ; ++jb)
+            d[jb] = kk < nb ? Linv[(long)(16 * (NJ * hc + jb) + p) * PB + kk + q] : 0.0;   // Linv^T[kk+q][col]
+    };
+#pragma unroll
+    for (int st = 0; st < R; ++st) fetch(bv[st], 4 * st);
+    for (int kk0 = 0; kk0 < nb; kk0 += 4 * R) {
+#pragma unroll
+        for (int st = 0; st < R; ++st) {
+            const int kk = kk0 + 4 * st;
+            if (kk < nb) {
+                const double a = Ar[16 * rb + p][kk + q];
+#pragma unroll
+                for (int jb = 0; jb < NJ; ++jb)
+                    acc[jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv[st][jb], acc[jb], 0, 0, 0);
+                fetch(bv[st], kk + 4 * R);
+            }
         }
     }
 #pragma unroll
-    for (int jb = 0; jb < PB / 16; ++jb)
+    for (int jb = 0; jb < NJ; ++jb)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int gr = r0 + 16 * wave + q + 4 * r;   // f64 MFMA result rows
-            const int c = 16 * jb + p;
+            const int gr = r0 + 16 * rb + q + 4 * r;   // f64 MFMA result rows
+            const int c = 16 * (NJ * hc + jb) + p;
             if (gr < n && c < nb) {
                 A[(long)gr * lda + k + c] = acc[jb][r];
                 P[(long)gr * PB + c] = acc[jb][r];
@@ -551,8 +582,8 @@ int ngd_factor_solve(double *G, int n, long lda, double nb, double mu, int mode,
                            info);
         const int rest = n - k - nb_;
         if (rest > 0) {
-            hipLaunchKernelGGL(k_chol_panel, dim3((unsigned)((rest + 63) / 64)), dim3(256), 0, s, G, lda, k, nb_, n,
-                               (const double *)Li, P);
+            hipLaunchKernelGGL(k_chol_panel, dim3((unsigned)((rest + PR - 1) / PR)), dim3(256), 0, s, G, lda, k, nb_,
+                               n, (const double *)Li, P);
             const long T = (rest + 63) / 64;
             hipLaunchKernelGGL(k_chol_syrk, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, s, G, lda,
                                (const double *)P, k + nb_, nb_, n);
