@@ -89,7 +89,8 @@ __device__ __forceinline__ double ldot(const double *__restrict__ x, const doubl
 // the trailing lower triangle takes the rank-16 update in 4 x 4 register
 // tiles.  The inverse X = L^-1 is blocked the same way: the eight 16 x 16
 // diagonal inverses by forward substitution (one column a thread), then
-// block row by block row X_ij = -X_ii (sum_{j <= m < i} L_im X_mj).  X (below
+// block row by block row X_ij = -X_ii (sum_{j <= m < i} L_im X_mj) on FP64
+// MFMA, a wave a block.  X (below
 // the diagonal) is kept transposed in the free upper triangle of S, its
 // diagonal in dg.  info > 0 marks the first non-positive pivot (index + 1).
 static constexpr int DB = 16;         // step width inside the diagonal block
@@ -226,68 +227,51 @@ __global__ __launch_bounds__(256) void k_chol_diag(double *__restrict__ A, long 
         }
     }
     __syncthreads();
-    // (e) block row i: Y = sum_{j <= m < i} L_im X_mj into the slot of X_ij,
-    // then X_ij = -X_ii Y (each thread keeps its entries through a barrier).
-    // Within block j, X_mj (m >= C) is dg at m = C and S[C][m] above it; in
-    // the blocks past j every X_mj sits at S[C][m].
-    for (int ib = 1; ib < ND; ++ib) {
-        const int ne = ib * DB * DB;
-        for (int e = tid; e < ne; e += 256) {
-            const int jb = e / (DB * DB), a = (e / DB) % DB, b = e % DB;
-            const int R = ib * DB + a, C = jb * DB + b;
-            const double *Lr = &S[R][0], *Xc = &S[C][0];
-            double y0 = Lr[C] * dg[C], y1 = 0.0, y2 = 0.0, y3 = 0.0;
+    // (e) block row ib = 1 .. 7: each wave takes blocks jb < ib in turn and
+    // forms Y = sum_{jb <= mb < ib} L_(ib,mb) X_(mb,jb) on
+    // v_mfma_f64_16x16x4f64 (A operand: lane (i = lane % 16, k = lane / 16);
+    // B: (k = lane / 16, j = lane % 16); accumulator element r: row
+    // lane / 16 + 4 r, column lane % 16 -- so Y's element c is the B operand
+    // of k-chunk c of the next product), then X_(ib,jb) = -X_(ib,ib) Y into
+    // its transposed slot S[C][R].  Those slots (rows in block jb, columns
+    // in block ib) are read by nothing in the same block row; the blocks of
+    // row ib are complete at the barrier that ends it.
+    {
+        const int li = tid & 15, lk = (tid & 63) >> 4;
+        const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+        for (int ib = 1; ib < ND; ++ib) {
+            for (int jb = wave; jb < ib; jb += 4) {
+                dbl4 y = {0.0, 0.0, 0.0, 0.0};
+                for (int mb = jb; mb < ib; ++mb) {
 #pragma unroll
-            for (int mm = 1; mm < DB; ++mm)
-                if (mm > b) y1 += Lr[jb * DB + mm] * Xc[jb * DB + mm];
-            for (int kb2 = jb + 1; kb2 < ib; ++kb2) {
-                const int mb = kb2 * DB;
-#pragma unroll
-                for (int mm = 0; mm < DB; mm += 4) {
-                    y0 += Lr[mb + mm] * Xc[mb + mm];
-                    y1 += Lr[mb + mm + 1] * Xc[mb + mm + 1];
-                    y2 += Lr[mb + mm + 2] * Xc[mb + mm + 2];
-                    y3 += Lr[mb + mm + 3] * Xc[mb + mm + 3];
+                    for (int c = 0; c < 4; ++c) {
+                        const int kk = 4 * c + lk;
+                        const double a = S[DB * ib + li][DB * mb + kk];   // L[16 ib + li][16 mb + kk]
+                        const int Rb = DB * mb + kk, Cb = DB * jb + li;   // X[Rb][Cb]
+                        const double b = mb == jb ? xinv(S, dg, Rb, Cb) : S[Cb][Rb];
+                        y = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, y, 0, 0, 0);
+                    }
                 }
-            }
-            S[C][R] = (y0 + y1) + (y2 + y3);
-        }
-        __syncthreads();
-        constexpr int NX = (ND - 1) * DB * DB / 256;   // entries a thread owns, at most
-        double xr[NX];
+                dbl4 z = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int q = 0; q < NX; ++q) {
-            const int e = tid + 256 * q;
-            xr[q] = 0.0;
-            if (e < ne) {
-                const int jb = e / (DB * DB), a = (e / DB) % DB, b = e % DB;
-                const int R = ib * DB + a, C = jb * DB + b;
-                // X_ii[a][m] = S[ib DB + m][R] below the diagonal, dg[R] on it
-                double x0 = dg[R] * S[C][R], x1 = 0.0;
-#pragma unroll
-                for (int m = 0; m < DB; m += 2) {
-                    if (m < a) x0 += S[ib * DB + m][R] * S[C][ib * DB + m];
-                    if (m + 1 < a) x1 += S[ib * DB + m + 1][R] * S[C][ib * DB + m + 1];
+                for (int c = 0; c < 4; ++c) {
+                    const double a = xinv(S, dg, DB * ib + li, DB * ib + 4 * c + lk);   // X_(ib,ib)[li][4 c + lk]
+                    z = __builtin_amdgcn_mfma_f64_16x16x4f64(a, y[c], z, 0, 0, 0);
                 }
-                xr[q] = -(x0 + x1);
-            }
-        }
-        __syncthreads();
 #pragma unroll
-        for (int q = 0; q < NX; ++q) {
-            const int e = tid + 256 * q;
-            if (e < ne) {
-                const int jb = e / (DB * DB), a = (e / DB) % DB, b = e % DB;
-                S[jb * DB + b][ib * DB + a] = xr[q];
+                for (int r = 0; r < 4; ++r) S[DB * jb + li][DB * ib + lk + 4 * r] = -z[r];
             }
+            __syncthreads();
         }
-        __syncthreads();
     }
+    // X and its transpose, each written in its own row order (coalesced)
     for (int u = tid; u < PB * PB; u += 256) {
         const int r = u / PB, c = u % PB;
-        const double x = r == c ? dg[r] : (r > c ? S[c][r] : 0.0);   // X[r][c]
-        Linv[u] = x;
-        LinvT[c * PB + r] = x;
+        Linv[u] = r == c ? dg[r] : (r > c ? S[c][r] : 0.0);    // X[r][c]
+    }
+    for (int u = tid; u < PB * PB; u += 256) {
+        const int c = u / PB, r = u % PB;
+        LinvT[u] = r == c ? dg[r] : (r > c ? S[c][r] : 0.0);   // X[r][c] at LinvT[c][r]
     }
 }
 
