@@ -137,8 +137,14 @@ __global__ __launch_bounds__(256) void k_chol_diag(double *__restrict__ A, long 
             for (int j = 0; j < DB; ++j) {
                 const double djj = __shfl(d[j], j);
                 if (tid == j && !(djj > 0.0) && *info == 0) *info = k + o + j + 1;
-                const double l = sqrt(djj > 0.0 ? djj : 1.0);
-                const double il = 1.0 / l;
+                // 1/sqrt from v_rsq_f64 and two Newton steps (each doubles
+                // the correct bits), l = d * (1/sqrt d): no IEEE sqrt and
+                // division sequences on the column's dependent path
+                const double dd = djj > 0.0 ? djj : 1.0;
+                double il = __builtin_amdgcn_rsq(dd);
+                il = il * (1.5 - 0.5 * dd * il * il);
+                il = il * (1.5 - 0.5 * dd * il * il);
+                const double l = dd * il;
                 if (r == j) d[j] = l;
                 else if (r > j) d[j] = d[j] * il;
 #pragma unroll
