@@ -1710,6 +1710,8 @@ int dnnf90_set_point_kernels(int on)
     return prev;
 }
 
+int dnnf90_set_split_kernels(int on) { return set_lin_x6c(on); }
+
 int dnnf90_set_store_s(int on)
 {
     const int prev = g_store_s;

@@ -231,6 +231,232 @@ __global__ __launch_bounds__(256) void k_lin_x6(const float *__restrict__ W, int
     }
 }
 
+// K1 / K6 where the reduction is whole 32-row chunks (Kd = 32 NKC, NKC
+// fixed at compile time).  Same operand layout, products, summation order
+// and results (bit for bit) as k_lin_x6, with less VALU work a step: the
+// chunk loop is unrolled, a load or store address is a wave-uniform row base
+// plus one 32-bit lane offset (no per-lane 64-bit row arithmetic), and the
+// NKC chunks of the wave's next unit are loaded into the registers of the
+// chunk just split, so a whole unit of loads is in flight during the
+// products (PD chunks ahead: the chunk g of the wave's sequence is held in
+// register buffer g mod PD, refilled with chunk g + PD right after it is
+// split; PD divides NKC).  The lane offsets need 24 ncol + 16 NCB < 2^30
+// (host-checked).  WPE: the waves per SIMD the registers are held to.
+template <bool kTrans, bool kAcc64, int RBT, int NCB, int NKC, int PD, int WPE>
+__global__ __launch_bounds__(256, WPE) void k_lin_x6c(const float *__restrict__ W, int ldw, const float *__restrict__ B,
+                                                 float *__restrict__ C, int M, long ncol, long Np, int nrt, int ncw,
+                                                 long cupw)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char x6_lds[];
+    bf16x8 *Ap = reinterpret_cast<bf16x8 *>(x6_lds);
+    const int G = gridDim.x, bid = blockIdx.x;
+    const int w = (bid & 7) * (G >> 3) + (bid >> 3);
+    const int rt = w % nrt, cw = w / nrt;
+    if (cw >= ncw) return;
+    const long ncu = ncol / (16 * NCB);
+    const long cu0 = (long)cw * cupw, cu1 = min(ncu, cu0 + cupw);
+    if (cu0 >= cu1) return;
+    const int tid = threadIdx.x, lane = tid & 63, q = lane >> 4;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int m0 = rt * 16 * RBT;
+    for (int u = tid; u < RBT * NKC * 64; u += 256) {
+        const int rb = u / (NKC * 64), rem = u - rb * NKC * 64, kc = rem >> 6, ln = rem & 63;
+        const int m = m0 + 16 * rb + (ln & 15), k = 32 * kc + 8 * (ln >> 4);
+        float v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int kk = k + e;
+            v[e] = m < M ? (kTrans ? W[(long)kk * ldw + 1 + m] : W[(long)m * ldw + 1 + kk]) : 0.f;
+        }
+        bf16x8 h, mm, l;
+        split3(v, h, mm, l);
+        bf16x8 *dst = Ap + (long)(rb * NKC + kc) * 3 * 64 + ln;
+        dst[0] = h;
+        dst[64] = mm;
+        dst[128] = l;
+    }
+    float *Bias = reinterpret_cast<float *>(Ap + (long)RBT * NKC * 3 * 64);
+    if (!kTrans && tid < 16 * RBT) Bias[tid] = m0 + tid < M ? W[(long)(m0 + tid) * ldw] : 0.f;
+    __syncthreads();
+    if (cu0 + wave >= cu1) return;
+    const int nu = (int)((cu1 - cu0 - wave + 3) / 4);
+    const int col0 = NCB * (lane & 15);   // lane n: columns NCB n .. NCB n + NCB - 1
+    // byte offsets of this lane: loads, rows 8 q + e of a chunk; stores,
+    // rows 4 q + r of a row block.  Loads and stores are raw buffer accesses:
+    // a scalar resource at the chunk's (row block's) first row and unit, the
+    // row within it as a scalar offset, the lane's offset in one VGPR
+    const unsigned ld_off = ((unsigned)(8 * q) * (unsigned)ncol + (unsigned)col0) * 4u;
+    const unsigned st_off = ((unsigned)(4 * q) * (unsigned)ncol + (unsigned)col0) * 4u;
+    const long rowb = ncol * 4;   // bytes per row
+    const int rowb32 = (int)rowb; // e rowb for e < 8 stays below 2^31 (host-checked)
+    typedef float fv __attribute__((ext_vector_type(NCB)));
+    const auto rsrc = [](const void *p) {
+        return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, -1, 0x00020000);
+    };
+    const auto ldv = [](__amdgpu_buffer_rsrc_t r, unsigned vo, int so) -> fv {
+        if constexpr (NCB == 4) return __builtin_bit_cast(fv, __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, 0));
+        else if constexpr (NCB == 2) return __builtin_bit_cast(fv, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+        else return __builtin_bit_cast(fv, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+    };
+    const auto stv = [](fv v, __amdgpu_buffer_rsrc_t r, unsigned vo, int so) {
+        if constexpr (NCB == 4) {
+            typedef unsigned u4 __attribute__((ext_vector_type(4)));
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, v), r, vo, so, 0);
+        } else if constexpr (NCB == 2) {
+            typedef unsigned u2 __attribute__((ext_vector_type(2)));
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, vo, so, 0);
+        } else {
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vo, so, 0);
+        }
+    };
+    static_assert(NKC % PD == 0, "PD divides NKC");
+    float buf[PD][NCB][8];
+    auto load = [&](float (&dst)[NCB][8], int kc, long cu) {
+        const __amdgpu_buffer_rsrc_t r =
+            rsrc(reinterpret_cast<const char *>(B) + (long)(32 * kc) * rowb + cu * (16 * NCB * 4));
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const fv v = ldv(r, ld_off, e * rowb32);
+#pragma unroll
+            for (int j = 0; j < NCB; ++j) dst[j][e] = v[j];
+        }
+    };
+#pragma unroll
+    for (int g = 0; g < PD; ++g)
+        if (g / NKC < nu) load(buf[g], g % NKC, cu0 + wave + 4 * (g / NKC));
+    for (int i = 0; i < nu; ++i) {
+        const long cu = cu0 + wave + 4L * i;
+        // the weight pieces are read from LDS again for every unit: hidden
+        // from the compiler, which would otherwise hold all of them in
+        // registers across the loop (RBT NKC 12 VGPRs)
+        int la = lane;
+        asm volatile("" : "+v"(la));
+        flt4 cacc[RBT][NCB];
+        double acc64[kAcc64 ? RBT : 1][kAcc64 ? NCB : 1][4];
+#pragma unroll
+        for (int kc = 0; kc < NKC; ++kc) {
+            bf16x8 bh[NCB], bm[NCB], blo[NCB];
+#pragma unroll
+            for (int j = 0; j < NCB; ++j) split3(buf[kc % PD][j], bh[j], bm[j], blo[j]);
+            const int di = (kc + PD) / NKC;
+            if (i + di < nu) load(buf[kc % PD], (kc + PD) % NKC, cu + 4 * di);
+            const bf16x8 *Ak = Ap + kc * 3 * 64 + la;
+#pragma unroll
+            for (int rb = 0; rb < RBT; ++rb) {
+                const bf16x8 *a = Ak + rb * NKC * 3 * 64;
+                const bf16x8 ah = a[0], am = a[64], al = a[128];
+#pragma unroll
+                for (int j = 0; j < NCB; ++j)
+                    cacc[rb][j] = mma_x6(ah, am, al, bh[j], bm[j], blo[j],
+                                         (kc == 0 || kAcc64) ? flt4{0, 0, 0, 0} : cacc[rb][j]);
+            }
+            if constexpr (kAcc64) {
+#pragma unroll
+                for (int rb = 0; rb < RBT; ++rb)
+#pragma unroll
+                    for (int j = 0; j < NCB; ++j)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            acc64[rb][j][r] = kc == 0 ? 0.0 + (double)cacc[rb][j][r]
+                                                      : acc64[rb][j][r] + (double)cacc[rb][j][r];
+            }
+        }
+        const bool bias = !kTrans && cu * 16 * NCB < Np;
+        char *cb = reinterpret_cast<char *>(C) + cu * (16 * NCB * 4);
+#pragma unroll
+        for (int rb = 0; rb < RBT; ++rb) {
+            const __amdgpu_buffer_rsrc_t r = rsrc(cb + (long)(m0 + 16 * rb) * rowb);
+#pragma unroll
+            for (int r4 = 0; r4 < 4; ++r4) {
+                const int mr = 16 * rb + 4 * q + r4;
+                const float add = bias ? Bias[mr] : 0.f;
+                fv v;
+#pragma unroll
+                for (int j = 0; j < NCB; ++j) {
+                    if constexpr (kAcc64)
+                        v[j] = (float)(acc64[rb][j][r4] + (double)add);
+                    else
+                        v[j] = cacc[rb][j][r4] + add;
+                }
+                if (m0 + mr < M) stv(v, r, st_off, r4 * rowb32);
+            }
+        }
+    }
+}
+
+// prefetch distance (chunks) of the fixed-chunk maps where a CU holds
+// several workgroups, and where it holds one
+#ifndef LIN_X6C_PD
+#define LIN_X6C_PD 2
+#endif
+// 1: the general kernel's grid is one round at the resident workgroups per
+// CU that the runtime reports (registers and LDS), not at the LDS bound
+// alone -- the 64-row K1 of a 128-wide layer holds 240 registers, so two
+// workgroups a CU, not three (its grid was 1.5 rounds; one round measured
+// 5 % faster)
+#ifndef LIN_X6_OCC
+#define LIN_X6_OCC 1
+#endif
+#ifndef LIN_X6C_PD_DEEP
+#define LIN_X6C_PD_DEEP 4
+#endif
+
+// 0: every split linear map takes k_lin_x6 (the fixed-chunk kernels off)
+static int g_lin_x6c = 1;
+int set_lin_x6c(int on)
+{
+    const int prev = g_lin_x6c;
+    g_lin_x6c = on ? 1 : 0;
+    return prev;
+}
+
+template <bool kTrans, bool kAcc64, int RBT, int NCB, int NKC, int PD, int WPE>
+static void lin_x6c_launch(const float *W, int ldw, const float *B, float *C, int M, long ncol, long Np, size_t lds,
+                           hipStream_t s)
+{
+    const void *fn = reinterpret_cast<const void *>(&k_lin_x6c<kTrans, kAcc64, RBT, NCB, NKC, PD, WPE>);
+    // resident workgroups per CU, from the registers and the LDS (queried
+    // once per instantiation and LDS size), so the grid is one full round
+    static size_t lds_q = 0;
+    static int per_cu = 0;
+    if (lds != lds_q) {
+        if (lds > 64 * 1024) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 256, lds) != hipSuccess || n < 1) n = 1;
+        per_cu = std::min(n, 4);
+        lds_q = lds;
+    }
+    const int nrt = (M + 16 * RBT - 1) / (16 * RBT);
+    const long ncu = ncol / (16 * NCB);
+    const long want = std::max(1L, (long)256 * per_cu / nrt);
+    long cupw = (ncu + want - 1) / want;
+    cupw = (cupw + 3) / 4 * 4;
+    const long ncw = (ncu + cupw - 1) / cupw;
+    long g = (long)nrt * ncw;
+    g = (g + 7) / 8 * 8;
+    hipLaunchKernelGGL((k_lin_x6c<kTrans, kAcc64, RBT, NCB, NKC, PD, WPE>), dim3((unsigned)g), dim3(256), lds, s, W, ldw, B, C,
+                       M, ncol, Np, nrt, (int)ncw, cupw);
+}
+
+// PDMAX: the prefetch distance in chunks, at most (a whole unit where one
+// workgroup fills a CU anyway; two chunks elsewhere, which keeps two waves a
+// SIMD, held there by the register bound WPE)
+template <bool kTrans, bool kAcc64, int RBT, int NCB, int PDMAX, bool DEEP>
+static bool lin_x6c_try(const float *W, int ldw, const float *B, float *C, int M, int Kd, long ncol, long Np,
+                        size_t lds, hipStream_t s)
+{
+    if (!g_lin_x6c || !LIN_X6_VEC || Kd % 32 != 0 || ncol % (16 * NCB) != 0 || 24 * ncol + 16 * NCB >= (1L << 28))
+        return false;
+    constexpr int WPE = DEEP ? 1 : 2;
+    switch (Kd / 32) {
+    case 1: lin_x6c_launch<kTrans, kAcc64, RBT, NCB, 1, 1, WPE>(W, ldw, B, C, M, ncol, Np, lds, s); return true;
+    case 2:
+        lin_x6c_launch<kTrans, kAcc64, RBT, NCB, 2, std::min(2, PDMAX), WPE>(W, ldw, B, C, M, ncol, Np, lds, s);
+        return true;
+    default: return false;   // four chunks and more: k_lin_x6 measured faster (128-wide layers)
+    }
+}
+
 template <bool kTrans, bool kAcc64, int RBT, int NCB, int KP>
 static void lin_x6_launch(const float *W, int ldw, const float *B, float *C, int M, int Kd, long ncol, long Np,
                           size_t lds, int per_cu, hipStream_t s)
@@ -241,6 +467,19 @@ static void lin_x6_launch(const float *W, int ldw, const float *B, float *C, int
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         lds_set = lds;
     }
+#if LIN_X6_OCC
+    {
+        static int occ = 0;
+        if (!occ) {
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &occ, reinterpret_cast<const void *>(&k_lin_x6<kTrans, kAcc64, RBT, NCB, KP>), 256, lds) !=
+                    hipSuccess ||
+                occ < 1)
+                occ = 1;
+        }
+        per_cu = std::min(per_cu, occ);
+    }
+#endif
     const int nrt = (M + 16 * RBT - 1) / (16 * RBT);
     const long ncu = ncol / (16 * NCB);
     const long want = std::max(1L, (long)256 * per_cu / nrt);
@@ -260,6 +499,14 @@ static void lin_x6_go(const float *W, int ldw, const float *B, float *C, int M, 
     const int nkc = (Kd + 31) / 32;
     const size_t lds = (size_t)RBT * nkc * 3 * 64 * 16 + 16 * RBT * sizeof(float);
     const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / lds));
+    if (per_cu == 1) {
+        if (lin_x6c_try<kTrans, kAcc64, RBT, kTrans ? LIN_X6_NCB_DEEP_T : LIN_X6_NCB_DEEP, LIN_X6C_PD_DEEP, true>(
+                W, ldw, B, C, M, Kd, ncol, Np, lds, s))
+            return;
+    } else if (lin_x6c_try<kTrans, kAcc64, RBT, LIN_X6_NCB, LIN_X6C_PD, false>(W, ldw, B, C, M, Kd, ncol, Np, lds,
+                                                                                s)) {
+        return;
+    }
     if (per_cu == 1)
         lin_x6_launch<kTrans, kAcc64, RBT, kTrans ? LIN_X6_NCB_DEEP_T : LIN_X6_NCB_DEEP, LIN_X6_KP_DEEP>(
             W, ldw, B, C, M, Kd, ncol, Np, lds, per_cu, s);

@@ -92,6 +92,9 @@ void launch_grad_reduce(double *partial, int nchunk, long len, double *grad, hip
 // arguments as launch_gemm_wx
 void launch_lin_x6(bool trans_a, const float *W, int ldw, const float *B, float *C, int M, int Kd, long ncol,
                    long Np, hipStream_t s);
+// fixed-chunk split linear maps on (1) or off (0); returns the previous
+// setting
+int set_lin_x6c(int on);
 // K5 chunk partials on BF16 three-way split products (gemm_x6.hip);
 // partial[chunk][M][Kd] as launch_gemm_grad, without the final reduction
 void launch_grad_x6(const float *A, const float *B, int M, int Kd, long ncol, long Np, double *partial,

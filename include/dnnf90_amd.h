@@ -79,6 +79,12 @@ int dnnf90_set_store_s(int on);
  * kernels instead.  Returns the previous setting.  No counterpart in the
  * reference (its per-point path is the only one it has). */
 int dnnf90_set_point_kernels(int on);
+/* the BF16x6 linear maps whose reduction is whole 32-row chunks (K1 and K6
+ * of layers 32, 64 or 128 wide) take kernels with the chunk count fixed at
+ * compile time; dnnf90_set_split_kernels(0) takes the general kernel for
+ * every map (results identical bit for bit).  Returns the previous setting.
+ * No counterpart in the reference. */
+int dnnf90_set_split_kernels(int on);
 int dnnf90_net_kernel_info(int nid, int prec, int *info);
 
 int dnnf90_net_nweights(int nid);

@@ -192,9 +192,9 @@ def test_filter_committee_value_argument_checks(L):
 
 
 def test_switches_return_previous_setting(L):
-    """dnnf90_set_codegen / dnnf90_set_store_s return the setting they
-    replace (no device work); a bad work-space handle is refused by the
-    kernel-timing entries."""
+    """dnnf90_set_codegen / dnnf90_set_store_s / dnnf90_set_split_kernels
+    return the setting they replace (no device work); a bad work-space
+    handle is refused by the kernel-timing entries."""
     prev = L.dnnf90_set_store_s(0)
     try:
         assert L.dnnf90_set_store_s(1) == 0
@@ -206,6 +206,12 @@ def test_switches_return_previous_setting(L):
         assert L.dnnf90_set_codegen(1) == 0
     finally:
         L.dnnf90_set_codegen(prev)
+    prev = L.dnnf90_set_split_kernels(0)
+    try:
+        assert L.dnnf90_set_split_kernels(1) == 0
+        assert L.dnnf90_set_split_kernels(1) == 1
+    finally:
+        L.dnnf90_set_split_kernels(prev)
     assert L.dnnf90_bwork_profile(63, 1) == -1
     ms = np.zeros(16)
     cnt = np.zeros(16, np.int64)

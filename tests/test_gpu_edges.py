@@ -15,8 +15,10 @@
   * zero points, one point, and batches that shrink and grow inside one work
     space (no state of an earlier call may leak into a later one);
   * a point count above the work space's maximum is refused with -1;
-  * one input at the largest order the tables take (K = 15, 16 slots).
-FP64 mode throughout, 1e-12 relative per derivative order (forward) and to
+  * one input at the largest order the tables take (K = 15, 16 slots);
+  * the BF16x6 linear maps of 32- and 64-wide layers: the fixed-chunk kernels
+    and the general kernel give the same bits (forward planes and gradient).
+FP64 mode elsewhere, 1e-12 relative per derivative order (forward) and to
 max |gradient| (adjoint).
 """
 import numpy as np
@@ -188,3 +190,32 @@ def test_profile_one_category(D):
     assert lib().dnnf90_bwork_profile(bw.id, 2 + n) == -1
     assert lib().dnnf90_bwork_profile(bw.id, -1) == -1
     bw.free(); g.free(); net.free()
+
+
+@pytest.mark.parametrize("dims", [[2, 64, 64, 64, 1], [3, 32, 32, 2]], ids=["w64", "w32"])
+def test_split_kernels_identical(D, dims):
+    """dnnf90_set_split_kernels(0) routes every BF16x6 linear map through the
+    general kernel; the fixed-chunk kernels (whole 32-row chunks) must give
+    the same bits: the same products in the same order."""
+    from dnnf90_amd._lib import lib
+    ts, net, w = _net(D, dims[0], 4, None, 0, dims, seed=7)
+    rng = np.random.default_rng(8)
+    npt = 300
+    x = rng.uniform(-1, 1, (npt, dims[0]))
+    seed = rng.standard_normal((npt, dims[-1], ts.na))
+    out = []
+    for on in (1, 0):
+        prev = lib().dnnf90_set_split_kernels(on)
+        try:
+            bw = D.BatchWork(net, npt, D.PREC_BF16X6)
+            g = D.Grad(net)
+            t = np.array(bw.forward(x), copy=True)
+            bw.backward(seed, g)
+            out.append((t, np.array(g.get(), copy=True)))
+            bw.free(); g.free()
+        finally:
+            lib().dnnf90_set_split_kernels(prev)
+    (t1, g1), (t0, g0) = out
+    assert np.all(np.isfinite(t1)) and np.all(np.isfinite(g1))
+    assert np.array_equal(t1, t0) and np.array_equal(g1, g0)
+    net.free()
