@@ -376,10 +376,25 @@ static int valloc(void **p, size_t bytes)
     return 0;
 }
 
-// gradient GEMM split over the column axis (chunks of the K5 kernel)
-static void grad_split(long ncol, int M, int Kd, int esz, int *nchunk, long *chunk)
+// gradient GEMM split over the column axis (chunks of the K5 kernel): the
+// one-block split K5 takes one chunk per workgroup it keeps resident, the
+// tiled kernels theirs (grad_chunks).  any: the larger of the two, whatever
+// the split-kernel switch (a work space's partials are sized once)
+static void grad_split(long ncol, int M, int Kd, int esz, int prec, int *nchunk, long *chunk, bool any = false)
 {
     grad_chunks(ncol, M, Kd, esz, nchunk, chunk);
+    if (prec != PREC_BF16X6) return;
+    const int prev = any ? set_lin_x6c(1) : -1;
+    const int wgs = grad_x6s_wgs(M, Kd);
+    if (any) set_lin_x6c(prev);
+    if (wgs <= 0) return;
+    const long nc = std::min((long)wgs, std::max(1L, ncol / 128));
+    const long ch = (((ncol + nc - 1) / nc) + 127) / 128 * 128;
+    if (ch <= 0) return;
+    const int n = (int)((ncol + ch - 1) / ch);
+    if (any && n <= *nchunk) return;
+    *nchunk = n;
+    *chunk = ch;
 }
 
 static int bwork_init(BatchWork &b, const Net &nt, int maxpt, int prec, double budget)
@@ -400,7 +415,7 @@ static int bwork_init(BatchWork &b, const Net &nt, int maxpt, int prec, double b
     for (int l = 1; l < L; ++l) {
         int nc;
         long ch;
-        grad_split((long)col, nt.dims[l], nt.dims[l - 1] + 1, (int)b.esz, &nc, &ch);
+        grad_split((long)col, nt.dims[l], nt.dims[l - 1] + 1, (int)b.esz, prec, &nc, &ch, true);
         pc = std::max(pc, (long)(nc + grad_reduce_splits(nc)) * nt.dims[l] * (nt.dims[l - 1] + 1));
     }
     const int nfwd = bell_fwd_threads(nt.sched.nentry, (int)b.esz);
@@ -595,7 +610,7 @@ static void backward_t(const Net &nt, BatchWork &b, double *grad, float *rows_t,
         } else {
             int nc;
             long ch;
-            grad_split(ncol, nd, ndm + 1, (int)b.esz, &nc, &ch);
+            grad_split(ncol, nd, ndm + 1, (int)b.esz, b.prec, &nc, &ch);
             {
                 PScope p(b, CAT_GRAD);
                 launch_gemm_grad<T>(b.prec, (const T *)b.p0, X, nd, ndm + 1, ncol, Np, b.partial, nc, ch, s);
@@ -830,7 +845,7 @@ static int run_grad_value(Net &nt, BatchWork &b, int npt, const double *x, const
         }
         int nc;
         long ch;
-        grad_split(Np, nd, ndm + 1, (int)b.esz, &nc, &ch);
+        grad_split(Np, nd, ndm + 1, (int)b.esz, b.prec, &nc, &ch);
         {
             PScope p(b, CAT_GRAD);
             launch_gemm_grad<T>(b.prec, (const T *)b.p0, Z(l - 1), nd, ndm + 1, Np, Np, b.partial, nc, ch, s);
@@ -1710,7 +1725,7 @@ int dnnf90_set_point_kernels(int on)
     return prev;
 }
 
-int dnnf90_set_split_kernels(int on) { return set_lin_x6c(on); }
+int dnnf90_set_split_kernels(int on) { return set_lin_x6c(on ? 1 : 0); }
 
 int dnnf90_set_store_s(int on)
 {

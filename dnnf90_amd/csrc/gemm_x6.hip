@@ -406,7 +406,7 @@ static int g_lin_x6c = 1;
 int set_lin_x6c(int on)
 {
     const int prev = g_lin_x6c;
-    g_lin_x6c = on ? 1 : 0;
+    if (on >= 0) g_lin_x6c = on ? 1 : 0;
     return prev;
 }
 
@@ -672,6 +672,213 @@ __global__ __launch_bounds__(256) void k_grad_x6(const float *__restrict__ A, co
     }
 }
 
+// K5 with the whole weight block (at most 128 x 128) in one workgroup, for
+// one column chunk.  A step of 32 columns: every A (Sbar) and B (T) row
+// fragment of the step -- row p of a 16-row block, columns 8 q .. 8 q + 7,
+// exactly one lane's operand of a bf16 MFMA -- is loaded by one thread,
+// split once and written to LDS in operand order; after a barrier, wave
+// (wr, wk) multiplies the pieces of its RW row blocks by its CW column blocks
+// (the four waves own disjoint quarters of the block, so no cross-wave sum),
+// and a second barrier frees the LDS for the next step, whose loads were
+// issued before the products.  Against k_grad_x6 each element is split once
+// (there the two 64-column tiles of a 128-wide block split Sbar twice), and
+// two workgroups share a CU, one splitting while the other multiplies.  The
+// bias column (sum of Sbar over the first Np columns) is summed in FP64 by
+// the threads holding the A fragments and met in LDS in a fixed order.  The
+// chunk partials and their reduction are those of k_grad_x6.
+template <int MB, int KB, int WR>
+__global__ __launch_bounds__(256, 2) void k_grad_x6s(const float *__restrict__ A, const float *__restrict__ B, int M,
+                                                      int Kd, long ncol, long Np, long chunk, int nchunk,
+                                                      double *__restrict__ partial)
+{
+    constexpr int WK = 4 / WR, RW = MB / WR, CW = KB / WK;
+    static_assert(RW * WR == MB && CW * WK == KB, "wave grid divides the block grid");
+    constexpr int NF = (MB + KB) * 64;   // fragments a step
+    constexpr int NJ = (NF + 255) / 256; // fragments a thread
+    __shared__ bf16x8 pc[(MB + KB) * 3 * 64];
+    const int chk = blockIdx.x;
+    if (chk >= nchunk) return;
+    const int Kg = Kd - 1;
+    const long cb = (long)chk * chunk, ce = min(ncol, cb + chunk);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wr = wave % WR, wk = wave / WR;
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const float *src[NJ];
+    bool live[NJ], isA[NJ];
+    double bsum[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+        const int f = tid + 256 * j, blk = f >> 6, ln = f & 63, r16 = ln & 15, qq = ln >> 4;
+        bsum[j] = 0.0;
+        isA[j] = f < NF && blk < MB;
+        if (f >= NF) {
+            live[j] = false;
+            src[j] = A;
+        } else if (blk < MB) {
+            const int m = 16 * blk + r16;
+            live[j] = m < M;
+            src[j] = A + (long)min(m, M - 1) * ncol + cb + 8 * qq;
+        } else {
+            const int k = 16 * (blk - MB) + r16;
+            live[j] = k < Kg;
+            src[j] = B + (long)min(k, Kg - 1) * ncol + cb + 8 * qq;
+        }
+    }
+    float raw[NJ][8];
+    auto load = [&](long off) {
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            if (!live[j]) continue;
+            const f4 x = *reinterpret_cast<const f4 *>(src[j] + off), y = *reinterpret_cast<const f4 *>(src[j] + off + 4);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                raw[j][e] = x[e];
+                raw[j][4 + e] = y[e];
+            }
+        }
+    };
+    flt4 acc[RW][CW];
+#pragma unroll
+    for (int i = 0; i < RW; ++i)
+#pragma unroll
+        for (int j = 0; j < CW; ++j) acc[i][j] = flt4{0, 0, 0, 0};
+    if (cb < ce) load(0);
+    for (long c = cb; c < ce; c += 32) {
+        const bool bias = c < Np;   // a 32-column step never straddles Np
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const int f = tid + 256 * j;
+            if (f >= NF) continue;
+            bf16x8 h, m, l;
+            if (live[j]) {
+                if (isA[j] && bias) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) t += (double)raw[j][e];
+                    bsum[j] += t;
+                }
+                split3(raw[j], h, m, l);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) h[e] = m[e] = l[e] = (__bf16)0.f;
+            }
+            bf16x8 *d = pc + (f >> 6) * 3 * 64 + (f & 63);
+            d[0] = h;
+            d[64] = m;
+            d[128] = l;
+        }
+        if (c + 32 < ce) load(c + 32 - cb);
+        __syncthreads();
+        bf16x8 bh[CW], bm[CW], bl[CW];
+#pragma unroll
+        for (int j = 0; j < CW; ++j) {
+            const bf16x8 *s = pc + (MB + wk * CW + j) * 3 * 64 + lane;
+            bh[j] = s[0];
+            bm[j] = s[64];
+            bl[j] = s[128];
+        }
+#pragma unroll
+        for (int i = 0; i < RW; ++i) {
+            const bf16x8 *s = pc + (wr * RW + i) * 3 * 64 + lane;
+            const bf16x8 ah = s[0], am = s[64], al = s[128];
+#pragma unroll
+            for (int j = 0; j < CW; ++j) acc[i][j] = mma_x6(ah, am, al, bh[j], bm[j], bl[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+    double *P = partial + (long)chk * M * Kd;
+    const int q = lane >> 4;
+#pragma unroll
+    for (int i = 0; i < RW; ++i)
+#pragma unroll
+        for (int j = 0; j < CW; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = 16 * (wr * RW + i) + 4 * q + r, k = 16 * (wk * CW + j) + (lane & 15);
+                if (m < M && k < Kg) P[(long)m * Kd + 1 + k] = (double)acc[i][j][r];
+            }
+    // bias column: the four q partials of row m met in LDS, summed in order
+    double *red = reinterpret_cast<double *>(pc);   // 16 MB x 4 doubles, free after the last barrier
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+        const int f = tid + 256 * j;
+        if (isA[j]) red[((f >> 6) * 16 + (f & 15)) * 4 + ((f & 63) >> 4)] = bsum[j];
+    }
+    __syncthreads();
+    for (int m = tid; m < M; m += 256) {
+        const double *rr = red + m * 4;
+        P[(long)m * Kd] = ((rr[0] + rr[1]) + rr[2]) + rr[3];
+    }
+}
+
+// the wave grid of k_grad_x6s for an MB x KB block grid (0: not handled)
+static int grad_x6s_wr(int MB, int KB)
+{
+    if (MB >= 2 && KB >= 2) return 2;
+    if (MB >= 4) return 4;
+    if (KB >= 4) return 1;
+    return 0;
+}
+
+static int pow2_blocks(int n)
+{
+    const int b = (n + 15) / 16;
+    return b <= 1 ? 1 : b <= 2 ? 2 : b <= 4 ? 4 : b <= 8 ? 8 : 0;
+}
+
+template <int MB, int KB>
+static const void *grad_x6s_fn()
+{
+    constexpr int WR = (MB >= 2 && KB >= 2) ? 2 : (MB >= 4 ? 4 : 1);
+    return reinterpret_cast<const void *>(&k_grad_x6s<MB, KB, WR>);
+}
+
+template <int MB, int KB>
+static void grad_x6s_go(const float *A, const float *B, int M, int Kd, long ncol, long Np, double *partial,
+                        int nchunk, long chunk, hipStream_t s)
+{
+    constexpr int WR = (MB >= 2 && KB >= 2) ? 2 : (MB >= 4 ? 4 : 1);
+    hipLaunchKernelGGL((k_grad_x6s<MB, KB, WR>), dim3((unsigned)nchunk), dim3(256), 0, s, A, B, M, Kd, ncol, Np,
+                       chunk, nchunk, partial);
+}
+
+// dispatch over the power-of-two block counts: fn(MB, KB) instantiated
+#define GRAD_X6S_CASES(X)                                                                                            \
+    X(8, 8) X(8, 4) X(8, 2) X(8, 1) X(4, 8) X(4, 4) X(4, 2) X(4, 1) X(2, 8) X(2, 4) X(2, 2) X(1, 8) X(1, 4)
+
+int grad_x6s_wgs(int M, int Kd)
+{
+    if (!set_lin_x6c(-1)) return 0;
+    const int MB = pow2_blocks(M), KB = pow2_blocks(Kd - 1);
+    if (!MB || !KB || !grad_x6s_wr(MB, KB)) return 0;
+    const void *fn = nullptr;
+#define X(a, b)                                                                                                      \
+    if (MB == a && KB == b) fn = grad_x6s_fn<a, b>();
+    GRAD_X6S_CASES(X)
+#undef X
+    if (!fn) return 0;
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 256, 0) != hipSuccess || n < 1) n = 1;
+    return 256 * std::min(n, 2);
+}
+
+static bool launch_grad_x6s(const float *A, const float *B, int M, int Kd, long ncol, long Np, double *partial,
+                            int nchunk, long chunk, hipStream_t s)
+{
+    if (!set_lin_x6c(-1)) return false;
+    const int MB = pow2_blocks(M), KB = pow2_blocks(Kd - 1);
+    if (!MB || !KB || !grad_x6s_wr(MB, KB) || chunk % 32 != 0) return false;
+#define X(a, b)                                                                                                      \
+    if (MB == a && KB == b) {                                                                                        \
+        grad_x6s_go<a, b>(A, B, M, Kd, ncol, Np, partial, nchunk, chunk, s);                                        \
+        return true;                                                                                                 \
+    }
+    GRAD_X6S_CASES(X)
+#undef X
+    return false;
+}
+
 // register stages of raw rows of the split K5 at 128 and at 64 rows (two
 // stages spill at 128 rows, and gain nothing measurable at 64)
 #ifndef GRAD_X6_ST8
@@ -693,6 +900,7 @@ int grad_x6_rb(int M) { return M > GRAD_X6_RB8_ABOVE ? 8 : 4; }
 void launch_grad_x6(const float *A, const float *B, int M, int Kd, long ncol, long Np, double *partial,
                     int nchunk, long chunk, hipStream_t s)
 {
+    if (launch_grad_x6s(A, B, M, Kd, ncol, Np, partial, nchunk, chunk, s)) return;
     const int rb = grad_x6_rb(M);
     const int ntm = (M + 16 * rb - 1) / (16 * rb), ntk = (Kd - 1 + 63) / 64;
     long g = (long)ntm * ntk * nchunk;
