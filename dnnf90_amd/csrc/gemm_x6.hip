@@ -686,18 +686,25 @@ __global__ __launch_bounds__(256) void k_grad_x6(const float *__restrict__ A, co
 // bias column (sum of Sbar over the first Np columns) is summed in FP64 by
 // the threads holding the A fragments and met in LDS in a fixed order.  The
 // chunk partials and their reduction are those of k_grad_x6.
+// Blocks larger than 128 x 128 are cut into ntm x ntk tiles of 128 x 128,
+// one workgroup each; the tiles of one chunk are adjacent work items (one
+// XCD, so the second read of an operand comes from L2).
 template <int MB, int KB, int WR>
 __global__ __launch_bounds__(256, 2) void k_grad_x6s(const float *__restrict__ A, const float *__restrict__ B, int M,
-                                                      int Kd, long ncol, long Np, long chunk, int nchunk,
-                                                      double *__restrict__ partial)
+                                                      int Kd, long ncol, long Np, long chunk, int nchunk, int ntm,
+                                                      int ntk, double *__restrict__ partial)
 {
     constexpr int WK = 4 / WR, RW = MB / WR, CW = KB / WK;
     static_assert(RW * WR == MB && CW * WK == KB, "wave grid divides the block grid");
     constexpr int NF = (MB + KB) * 64;   // fragments a step
     constexpr int NJ = (NF + 255) / 256; // fragments a thread
     __shared__ bf16x8 pc[(MB + KB) * 3 * 64];
-    const int chk = blockIdx.x;
+    const int G = gridDim.x, bid = blockIdx.x;
+    const int w = (bid & 7) * (G >> 3) + (bid >> 3);
+    const int ntile = ntm * ntk, tile = w % ntile, chk = w / ntile;
     if (chk >= nchunk) return;
+    const int m0 = (tile % ntm) * 16 * MB, k0 = (tile / ntm) * 16 * KB;
+    const bool do_bias = k0 == 0;
     const int Kg = Kd - 1;
     const long cb = (long)chk * chunk, ce = min(ncol, cb + chunk);
     const int tid = threadIdx.x, lane = tid & 63;
@@ -716,11 +723,11 @@ __global__ __launch_bounds__(256, 2) void k_grad_x6s(const float *__restrict__ A
             live[j] = false;
             src[j] = A;
         } else if (blk < MB) {
-            const int m = 16 * blk + r16;
+            const int m = m0 + 16 * blk + r16;
             live[j] = m < M;
             src[j] = A + (long)min(m, M - 1) * ncol + cb + 8 * qq;
         } else {
-            const int k = 16 * (blk - MB) + r16;
+            const int k = k0 + 16 * (blk - MB) + r16;
             live[j] = k < Kg;
             src[j] = B + (long)min(k, Kg - 1) * ncol + cb + 8 * qq;
         }
@@ -752,7 +759,7 @@ __global__ __launch_bounds__(256, 2) void k_grad_x6s(const float *__restrict__ A
             if (f >= NF) continue;
             bf16x8 h, m, l;
             if (live[j]) {
-                if (isA[j] && bias) {
+                if (isA[j] && bias && do_bias) {
                     double t = 0.0;
 #pragma unroll
                     for (int e = 0; e < 8; ++e) t += (double)raw[j][e];
@@ -795,10 +802,11 @@ __global__ __launch_bounds__(256, 2) void k_grad_x6s(const float *__restrict__ A
         for (int j = 0; j < CW; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int m = 16 * (wr * RW + i) + 4 * q + r, k = 16 * (wk * CW + j) + (lane & 15);
+                const int m = m0 + 16 * (wr * RW + i) + 4 * q + r, k = k0 + 16 * (wk * CW + j) + (lane & 15);
                 if (m < M && k < Kg) P[(long)m * Kd + 1 + k] = (double)acc[i][j][r];
             }
     // bias column: the four q partials of row m met in LDS, summed in order
+    if (!do_bias) return;
     double *red = reinterpret_cast<double *>(pc);   // 16 MB x 4 doubles, free after the last barrier
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
@@ -806,9 +814,9 @@ __global__ __launch_bounds__(256, 2) void k_grad_x6s(const float *__restrict__ A
         if (isA[j]) red[((f >> 6) * 16 + (f & 15)) * 4 + ((f & 63) >> 4)] = bsum[j];
     }
     __syncthreads();
-    for (int m = tid; m < M; m += 256) {
+    for (int m = tid; m < 16 * MB && m0 + m < M; m += 256) {
         const double *rr = red + m * 4;
-        P[(long)m * Kd] = ((rr[0] + rr[1]) + rr[2]) + rr[3];
+        P[(long)(m0 + m) * Kd] = ((rr[0] + rr[1]) + rr[2]) + rr[3];
     }
 }
 
@@ -821,11 +829,13 @@ static int grad_x6s_wr(int MB, int KB)
     return 0;
 }
 
+// 16-row blocks of a tile (a power of two, at most 8) and tiles along n
 static int pow2_blocks(int n)
 {
     const int b = (n + 15) / 16;
-    return b <= 1 ? 1 : b <= 2 ? 2 : b <= 4 ? 4 : b <= 8 ? 8 : 0;
+    return b <= 1 ? 1 : b <= 2 ? 2 : b <= 4 ? 4 : 8;
 }
+static int tiles_of(int n) { return (n + 127) / 128; }
 
 template <int MB, int KB>
 static const void *grad_x6s_fn()
@@ -839,8 +849,11 @@ static void grad_x6s_go(const float *A, const float *B, int M, int Kd, long ncol
                         int nchunk, long chunk, hipStream_t s)
 {
     constexpr int WR = (MB >= 2 && KB >= 2) ? 2 : (MB >= 4 ? 4 : 1);
-    hipLaunchKernelGGL((k_grad_x6s<MB, KB, WR>), dim3((unsigned)nchunk), dim3(256), 0, s, A, B, M, Kd, ncol, Np,
-                       chunk, nchunk, partial);
+    const int ntm = tiles_of(M), ntk = tiles_of(Kd - 1);
+    long g = (long)ntm * ntk * nchunk;
+    g = (g + 7) / 8 * 8;
+    hipLaunchKernelGGL((k_grad_x6s<MB, KB, WR>), dim3((unsigned)g), dim3(256), 0, s, A, B, M, Kd, ncol, Np, chunk,
+                       nchunk, ntm, ntk, partial);
 }
 
 // dispatch over the power-of-two block counts: fn(MB, KB) instantiated
@@ -860,7 +873,7 @@ int grad_x6s_wgs(int M, int Kd)
     if (!fn) return 0;
     int n = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 256, 0) != hipSuccess || n < 1) n = 1;
-    return 256 * std::min(n, 2);
+    return std::max(1, 256 * std::min(n, 2) / (tiles_of(M) * tiles_of(Kd - 1)));
 }
 
 static bool launch_grad_x6s(const float *A, const float *B, int M, int Kd, long ncol, long Np, double *partial,

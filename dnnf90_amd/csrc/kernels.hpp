@@ -95,9 +95,10 @@ void launch_lin_x6(bool trans_a, const float *W, int ldw, const float *B, float 
 // fixed-chunk split linear maps and the one-block split K5 on (1) or off
 // (0), or the setting queried (-1); returns the previous setting
 int set_lin_x6c(int on);
-// workgroups (= column chunks) the one-block split K5 takes for an M x Kd
-// weight block, 0 where it does not apply (launch_grad_x6 then takes the
-// tiled kernel, whose chunks grad_chunks gives)
+// column chunks the one-block split K5 takes for an M x Kd weight block
+// (one workgroup per chunk and 128 x 128 tile), 0 where it does not apply
+// (launch_grad_x6 then takes the tiled kernel, whose chunks grad_chunks
+// gives)
 int grad_x6s_wgs(int M, int Kd);
 // K5 chunk partials on BF16 three-way split products (gemm_x6.hip);
 // partial[chunk][M][Kd] as launch_gemm_grad, without the final reduction
