@@ -394,6 +394,11 @@ __global__ __launch_bounds__(256, WPE) void k_lin_x6c(const float *__restrict__ 
 // alone -- the 64-row K1 of a 128-wide layer holds 240 registers, so two
 // workgroups a CU, not three (its grid was 1.5 rounds; one round measured
 // 5 % faster)
+// 1: the 128 x 128 forward map K1 takes k_lin_x6b (one workgroup a CU, B
+// split once; 6 % faster than k_lin_x6 on dense4_k3)
+#ifndef LIN_X6B
+#define LIN_X6B 1
+#endif
 #ifndef LIN_X6_OCC
 #define LIN_X6_OCC 1
 #endif
@@ -408,6 +413,163 @@ int set_lin_x6c(int on)
     const int prev = g_lin_x6c;
     if (on >= 0) g_lin_x6c = on ? 1 : 0;
     return prev;
+}
+
+// K1 (and, by the same code, K6) of a 128 x 128 map (M = 128 rows, Kd = 128
+// = four 32-row chunks) in one workgroup per CU: the split weights (96 KB) and one 64-column
+// step of split B (48 KB) in LDS.  A step: wave w loads chunk w of B --
+// lane (n, q) the rows 32 w + 8 q .. +7 of the four consecutive columns
+// 4 n .. 4 n + 3, one 16-byte load a row -- splits it and writes the pieces
+// to LDS where that same lane's operand sits (block j = column 4 n + j), so
+// B is split once (k_lin_x6 at 64-row tiles splits it twice); after a
+// barrier wave w multiplies row blocks 2 w, 2 w + 1 by all four column
+// blocks over the four chunks, flushes each chunk's sum to FP64 (K1) as
+// k_lin_x6 does, and stores its rows as 16-byte vectors; the next step's
+// loads are in flight meanwhile.  Same products in the same order as
+// k_lin_x6: the same bits.
+template <bool kTrans, bool kAcc64>
+__global__ __launch_bounds__(256, 1) void k_lin_x6b(const float *__restrict__ W, int ldw, const float *__restrict__ B,
+                                                    float *__restrict__ C, long ncol, long Np, long cupw)
+{
+    constexpr int RBT = 8, NKC = 4, NCB = 4, Kd = 128;
+    extern __shared__ __attribute__((aligned(16))) unsigned char x6_lds[];
+    bf16x8 *Ap = reinterpret_cast<bf16x8 *>(x6_lds);
+    bf16x8 *Bq = Ap + RBT * NKC * 3 * 64;                                   // [kc][j][piece][lane]
+    float *Bias = reinterpret_cast<float *>(Bq + NKC * NCB * 3 * 64);
+    const long ncu = ncol / 64;
+    const long cu0 = (long)blockIdx.x * cupw, cu1 = min(ncu, cu0 + cupw);
+    if (cu0 >= cu1) return;
+    const int tid = threadIdx.x, lane = tid & 63, q = lane >> 4, n = lane & 15;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    for (int u = tid; u < RBT * NKC * 64; u += 256) {
+        const int rb = u / (NKC * 64), rem = u - rb * NKC * 64, kc = rem >> 6, ln = rem & 63;
+        const int m = 16 * rb + (ln & 15), k = 32 * kc + 8 * (ln >> 4);
+        float v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+            v[e] = kTrans ? W[(long)(k + e) * ldw + 1 + m] : W[(long)m * ldw + 1 + k + e];
+        bf16x8 h, mm, l;
+        split3(v, h, mm, l);
+        bf16x8 *dst = Ap + (long)(rb * NKC + kc) * 3 * 64 + ln;
+        dst[0] = h;
+        dst[64] = mm;
+        dst[128] = l;
+    }
+    if (!kTrans && tid < 128) Bias[tid] = W[(long)tid * ldw];
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    const long rowb = ncol * 4;
+    const int rowb32 = (int)rowb;   // 7 rowb < 2^31 (host-checked)
+    const auto rsrc = [](const void *p) {
+        return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, -1, 0x00020000);
+    };
+    const unsigned ld_off = ((unsigned)(8 * q) * (unsigned)ncol + 4u * n) * 4u;
+    const unsigned st_off = ((unsigned)(4 * q) * (unsigned)ncol + 4u * n) * 4u;
+    float raw[8][4];   // rows 32 wave + 8 q + e, columns 4 n .. 4 n + 3
+    auto load = [&](long cu) {
+        const __amdgpu_buffer_rsrc_t r =
+            rsrc(reinterpret_cast<const char *>(B) + (long)(32 * wave) * rowb + cu * 256);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const f4 v = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(r, ld_off, e * rowb32, 0));
+#pragma unroll
+            for (int j = 0; j < 4; ++j) raw[e][j] = v[j];
+        }
+    };
+    load(cu0);
+    for (long cu = cu0; cu < cu1; ++cu) {
+        // this step's chunk: split into LDS (the previous step's reads ended
+        // at its closing barrier)
+#pragma unroll
+        for (int j = 0; j < NCB; ++j) {
+            float v[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] = raw[e][j];
+            bf16x8 h, mm, l;
+            split3(v, h, mm, l);
+            bf16x8 *dst = Bq + (wave * NCB + j) * 3 * 64 + lane;
+            dst[0] = h;
+            dst[64] = mm;
+            dst[128] = l;
+        }
+        if (cu + 1 < cu1) load(cu + 1);
+        __syncthreads();
+        flt4 cacc[2][NCB];
+        double acc64[kAcc64 ? 2 : 1][kAcc64 ? NCB : 1][4];
+#pragma unroll
+        for (int kc = 0; kc < NKC; ++kc) {
+            bf16x8 bh[NCB], bm[NCB], bl[NCB];
+#pragma unroll
+            for (int j = 0; j < NCB; ++j) {
+                const bf16x8 *sb = Bq + (kc * NCB + j) * 3 * 64 + lane;
+                bh[j] = sb[0];
+                bm[j] = sb[64];
+                bl[j] = sb[128];
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const bf16x8 *a = Ap + ((2 * wave + i) * NKC + kc) * 3 * 64 + lane;
+                const bf16x8 ah = a[0], am = a[64], al = a[128];
+#pragma unroll
+                for (int j = 0; j < NCB; ++j)
+                    cacc[i][j] =
+                        mma_x6(ah, am, al, bh[j], bm[j], bl[j], (kc == 0 || kAcc64) ? flt4{0, 0, 0, 0} : cacc[i][j]);
+            }
+            if constexpr (kAcc64) {
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < NCB; ++j)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            acc64[i][j][r] = kc == 0 ? 0.0 + (double)cacc[i][j][r]
+                                                     : acc64[i][j][r] + (double)cacc[i][j][r];
+            }
+        }
+        const bool bias = !kTrans && cu * 64 < Np;
+        char *cb = reinterpret_cast<char *>(C) + cu * 256;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int rb = 2 * wave + i;
+            const __amdgpu_buffer_rsrc_t r = rsrc(cb + (long)(16 * rb) * rowb);
+#pragma unroll
+            for (int r4 = 0; r4 < 4; ++r4) {
+                const float add = bias ? Bias[16 * rb + 4 * q + r4] : 0.f;
+                f4 v;
+#pragma unroll
+                for (int j = 0; j < NCB; ++j) {
+                    if constexpr (kAcc64)
+                        v[j] = (float)(acc64[i][j][r4] + (double)add);
+                    else
+                        v[j] = cacc[i][j][r4] + add;
+                }
+                __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, v), r, st_off, r4 * rowb32, 0);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <bool kTrans, bool kAcc64>
+static bool lin_x6b_try(const float *W, int ldw, const float *B, float *C, int M, int Kd, long ncol, long Np,
+                        hipStream_t s)
+{
+    // K1 only: K6 at its 128-row tile already splits B once (measured equal)
+    if (kTrans || !LIN_X6B || !g_lin_x6c || M != 128 || Kd != 128 || ncol % 64 != 0 || 24 * ncol + 64 >= (1L << 28))
+        return false;
+    constexpr size_t lds = (size_t)(8 * 4 + 4 * 4) * 3 * 64 * 16 + 128 * sizeof(float);
+    static bool set = false;
+    if (!set) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_lin_x6b<kTrans, kAcc64>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        set = true;
+    }
+    const long ncu = ncol / 64;
+    const long cupw = (ncu + 255) / 256;
+    const long g = (ncu + cupw - 1) / cupw;
+    hipLaunchKernelGGL((k_lin_x6b<kTrans, kAcc64>), dim3((unsigned)g), dim3(256), lds, s, W, ldw, B, C, ncol, Np,
+                       cupw);
+    return true;
 }
 
 template <bool kTrans, bool kAcc64, int RBT, int NCB, int NKC, int PD, int WPE>
@@ -519,6 +681,7 @@ template <bool kTrans, bool kAcc64>
 static void lin_x6_pick(const float *W, int ldw, const float *B, float *C, int M, int Kd, long ncol, long Np,
                         hipStream_t s)
 {
+    if (lin_x6b_try<kTrans, kAcc64>(W, ldw, B, C, M, Kd, ncol, Np, s)) return;
     const size_t per_rb = (size_t)((Kd + 31) / 32) * 3 * 64 * 16;
     int rbt = kTrans ? LIN_X6_RBT_MAX_T : LIN_X6_RBT_MAX;
     while (rbt > 1 && 16 * (rbt / 2) >= M) rbt /= 2;

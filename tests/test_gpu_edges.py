@@ -17,8 +17,8 @@
   * a point count above the work space's maximum is refused with -1;
   * one input at the largest order the tables take (K = 15, 16 slots);
   * the BF16x6 kernels selected by dnnf90_set_split_kernels: the fixed-chunk
-    linear maps of 32- and 64-wide layers give the general kernel's bits
-    (forward planes), the one-block weight gradient (128 x 128 tiles of
+    linear maps of 32- and 64-wide layers and the one-block forward map of
+    128 x 128 layers give the general kernel's bits (forward planes), the one-block weight gradient (128 x 128 tiles of
     wider blocks) the tiled kernel's
     gradient to 1e-6 of max |gradient| (the same products summed in another
     FP32 order).
@@ -196,7 +196,8 @@ def test_profile_one_category(D):
     bw.free(); g.free(); net.free()
 
 
-@pytest.mark.parametrize("dims", [[2, 64, 64, 64, 1], [3, 32, 32, 2], [3, 256, 200, 1]], ids=["w64", "w32", "w256"])
+@pytest.mark.parametrize("dims", [[2, 64, 64, 64, 1], [3, 32, 32, 2], [2, 128, 128, 1], [3, 256, 200, 1]],
+                         ids=["w64", "w32", "w128", "w256"])
 def test_split_kernels_vs_general(D, dims):
     """dnnf90_set_split_kernels(0) routes every BF16x6 linear map through the
     general kernel and the weight gradient through the tiled kernel.  The
