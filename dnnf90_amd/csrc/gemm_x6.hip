@@ -431,7 +431,7 @@ template <bool kTrans, bool kAcc64>
 __global__ __launch_bounds__(256, 1) void k_lin_x6b(const float *__restrict__ W, int ldw, const float *__restrict__ B,
                                                     float *__restrict__ C, long ncol, long Np, long cupw)
 {
-    constexpr int RBT = 8, NKC = 4, NCB = 4, Kd = 128;
+    constexpr int RBT = 8, NKC = 4, NCB = 4;
     extern __shared__ __attribute__((aligned(16))) unsigned char x6_lds[];
     bf16x8 *Ap = reinterpret_cast<bf16x8 *>(x6_lds);
     bf16x8 *Bq = Ap + RBT * NKC * 3 * 64;                                   // [kc][j][piece][lane]
@@ -494,36 +494,37 @@ __global__ __launch_bounds__(256, 1) void k_lin_x6b(const float *__restrict__ W,
         }
         if (cu + 1 < cu1) load(cu + 1);
         __syncthreads();
-        flt4 cacc[2][NCB];
+        flt4 cacc[kAcc64 ? 1 : 2][NCB];
         double acc64[kAcc64 ? 2 : 1][kAcc64 ? NCB : 1][4];
 #pragma unroll
         for (int kc = 0; kc < NKC; ++kc) {
-            bf16x8 bh[NCB], bm[NCB], bl[NCB];
-#pragma unroll
-            for (int j = 0; j < NCB; ++j) {
-                const bf16x8 *sb = Bq + (kc * NCB + j) * 3 * 64 + lane;
-                bh[j] = sb[0];
-                bm[j] = sb[64];
-                bl[j] = sb[128];
-            }
+            bf16x8 ah[2], am[2], al[2];
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
                 const bf16x8 *a = Ap + ((2 * wave + i) * NKC + kc) * 3 * 64 + lane;
-                const bf16x8 ah = a[0], am = a[64], al = a[128];
-#pragma unroll
-                for (int j = 0; j < NCB; ++j)
-                    cacc[i][j] =
-                        mma_x6(ah, am, al, bh[j], bm[j], bl[j], (kc == 0 || kAcc64) ? flt4{0, 0, 0, 0} : cacc[i][j]);
+                ah[i] = a[0];
+                am[i] = a[64];
+                al[i] = a[128];
             }
-            if constexpr (kAcc64) {
+            // one column block's pieces live at a time (each accumulator's
+            // own sequence over kc is unchanged)
 #pragma unroll
-                for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < NCB; ++j) {
+                const bf16x8 *sb = Bq + (kc * NCB + j) * 3 * 64 + lane;
+                const bf16x8 bh = sb[0], bm = sb[64], bl = sb[128];
 #pragma unroll
-                    for (int j = 0; j < NCB; ++j)
+                for (int i = 0; i < 2; ++i) {
+                    if constexpr (kAcc64) {
+                        // the chunk's sum goes to FP64 at once (no FP32
+                        // accumulator held across column blocks)
+                        const flt4 c = mma_x6(ah[i], am[i], al[i], bh, bm, bl, flt4{0, 0, 0, 0});
 #pragma unroll
                         for (int r = 0; r < 4; ++r)
-                            acc64[i][j][r] = kc == 0 ? 0.0 + (double)cacc[i][j][r]
-                                                     : acc64[i][j][r] + (double)cacc[i][j][r];
+                            acc64[i][j][r] = kc == 0 ? 0.0 + (double)c[r] : acc64[i][j][r] + (double)c[r];
+                    } else {
+                        cacc[i][j] = mma_x6(ah[i], am[i], al[i], bh, bm, bl, kc == 0 ? flt4{0, 0, 0, 0} : cacc[i][j]);
+                    }
+                }
             }
         }
         const bool bias = !kTrans && cu * 64 < Np;
@@ -565,7 +566,7 @@ static bool lin_x6b_try(const float *W, int ldw, const float *B, float *C, int M
         set = true;
     }
     const long ncu = ncol / 64;
-    const long cupw = (ncu + 255) / 256;
+    const long cupw = (ncu + 255) / 256;   // one workgroup a CU (its LDS)
     const long g = (ncu + cupw - 1) / cupw;
     hipLaunchKernelGGL((k_lin_x6b<kTrans, kAcc64>), dim3((unsigned)g), dim3(256), lds, s, W, ldw, B, C, ncol, Np,
                        cupw);
