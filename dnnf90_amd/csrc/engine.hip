@@ -384,9 +384,7 @@ static void grad_split(long ncol, int M, int Kd, int esz, int prec, int *nchunk,
 {
     grad_chunks(ncol, M, Kd, esz, nchunk, chunk);
     if (prec != PREC_BF16X6) return;
-    const int prev = any ? set_lin_x6c(1) : -1;
-    const int wgs = grad_x6s_wgs(M, Kd);
-    if (any) set_lin_x6c(prev);
+    const int wgs = grad_x6s_wgs(M, Kd, any);
     if (wgs <= 0) return;
     const long nc = std::min((long)wgs, std::max(1L, ncol / 128));
     const long ch = (((ncol + nc - 1) / nc) + 127) / 128 * 128;

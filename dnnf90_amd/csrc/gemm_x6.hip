@@ -1024,9 +1024,9 @@ static void grad_x6s_go(const float *A, const float *B, int M, int Kd, long ncol
 #define GRAD_X6S_CASES(X)                                                                                            \
     X(8, 8) X(8, 4) X(8, 2) X(8, 1) X(4, 8) X(4, 4) X(4, 2) X(4, 1) X(2, 8) X(2, 4) X(2, 2) X(1, 8) X(1, 4)
 
-int grad_x6s_wgs(int M, int Kd)
+int grad_x6s_wgs(int M, int Kd, bool any)
 {
-    if (!set_lin_x6c(-1)) return 0;
+    if (!any && !set_lin_x6c(-1)) return 0;
     const int MB = pow2_blocks(M), KB = pow2_blocks(Kd - 1);
     if (!MB || !KB || !grad_x6s_wr(MB, KB)) return 0;
     const void *fn = nullptr;
@@ -1035,8 +1035,10 @@ int grad_x6s_wgs(int M, int Kd)
     GRAD_X6S_CASES(X)
 #undef X
     if (!fn) return 0;
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 256, 0) != hipSuccess || n < 1) n = 1;
+    // resident workgroups a CU, queried once per instantiation
+    static int occ[4][4] = {};
+    int &n = occ[__builtin_ctz(MB)][__builtin_ctz(KB)];
+    if (n == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 256, 0) != hipSuccess || n < 1)) n = 1;
     return std::max(1, 256 * std::min(n, 2) / (tiles_of(M) * tiles_of(Kd - 1)));
 }
 

@@ -98,8 +98,8 @@ int set_lin_x6c(int on);
 // column chunks the one-block split K5 takes for an M x Kd weight block
 // (one workgroup per chunk and 128 x 128 tile), 0 where it does not apply
 // (launch_grad_x6 then takes the tiled kernel, whose chunks grad_chunks
-// gives)
-int grad_x6s_wgs(int M, int Kd);
+// gives); any: whatever the split-kernel switch says
+int grad_x6s_wgs(int M, int Kd, bool any = false);
 // K5 chunk partials on BF16 three-way split products (gemm_x6.hip);
 // partial[chunk][M][Kd] as launch_gemm_grad, without the final reduction
 void launch_grad_x6(const float *A, const float *B, int M, int Kd, long ncol, long Np, double *partial,
