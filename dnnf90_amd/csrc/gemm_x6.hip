@@ -616,7 +616,7 @@ static bool lin_x6c_try(const float *W, int ldw, const float *B, float *C, int M
     case 2:
         lin_x6c_launch<kTrans, kAcc64, RBT, NCB, 2, std::min(2, PDMAX), WPE>(W, ldw, B, C, M, ncol, Np, lds, s);
         return true;
-    default: return false;   // four chunks and more: k_lin_x6 measured faster (128-wide layers)
+    default: return false;   // four and eight chunks: k_lin_x6 measured faster (128- and 256-wide layers)
     }
 }
 
