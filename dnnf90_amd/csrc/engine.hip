@@ -9,6 +9,7 @@
 // reference ABI are batches of one.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -899,6 +900,10 @@ static int set_residual(BatchWork &b, int nterm, const double *coeff, const int 
     r->coeff = b.d_coeff;
     r->slot = b.d_slot;
     r->nonlin = b.d_nonlin;
+    std::vector<int> sorted(s);
+    std::sort(sorted.begin(), sorted.end());
+    r->distinct = (std::adjacent_find(sorted.begin(), sorted.end()) == sorted.end() &&
+                   (sorted.empty() || sorted[0] > 0)) ? 1 : 0;
     return 0;
 }
 

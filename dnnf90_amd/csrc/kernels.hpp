@@ -165,6 +165,7 @@ struct DevResidual {
     const double *coeff;
     const int *slot;
     const int *nonlin;
+    int distinct = 0;   // 1: the terms' slots are pairwise distinct and none is slot 0
 };
 
 // R_n = [r0_n +] sum_k c_k [T_0 *] T_slot (output 0); seed = fac*R*dR/dT on

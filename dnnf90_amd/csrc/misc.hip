@@ -69,6 +69,7 @@ __global__ __launch_bounds__(256) void k_seed_residual(const T *__restrict__ TL,
         if (n < npt) {
             const double t0 = (double)TL[n];
             double R = r0 ? r0[n] : 0.0;
+#pragma unroll 4
             for (int k = 0; k < rs.nterm; ++k) {
                 const double tv = (double)TL[(long)rs.slot[k] * Np + n];
                 R += rs.nonlin[k] ? rs.coeff[k] * t0 * tv : rs.coeff[k] * tv;
@@ -77,18 +78,36 @@ __global__ __launch_bounds__(256) void k_seed_residual(const T *__restrict__ TL,
             if (r_out) r_out[n] = R;
             if (unit_r) R = 1.0;  // Gauss-Newton row seed dR/dT
             double s0 = 0.0;
-            for (int k = 0; k < rs.nterm; ++k) {
-                const long o = (long)rs.slot[k] * Np + n;
-                double add;
-                if (rs.nonlin[k]) {
-                    add = fac * R * rs.coeff[k] * t0;
-                    s0 += fac * R * rs.coeff[k] * (double)TL[o];
-                } else {
-                    add = fac * R * rs.coeff[k];
+            if (rs.distinct) {
+                // each seed slot is written once, from the zero it starts at
+                // (no read-back of the plane just cleared; the same values)
+#pragma unroll 4
+                for (int k = 0; k < rs.nterm; ++k) {
+                    const long o = (long)rs.slot[k] * Np + n;
+                    double add;
+                    if (rs.nonlin[k]) {
+                        add = fac * R * rs.coeff[k] * t0;
+                        s0 += fac * R * rs.coeff[k] * (double)TL[o];
+                    } else {
+                        add = fac * R * rs.coeff[k];
+                    }
+                    Tb[o] = (T)(0.0 + add);
                 }
-                Tb[o] = (T)((double)Tb[o] + add);
+                Tb[n] = (T)(0.0 + s0);
+            } else {
+                for (int k = 0; k < rs.nterm; ++k) {
+                    const long o = (long)rs.slot[k] * Np + n;
+                    double add;
+                    if (rs.nonlin[k]) {
+                        add = fac * R * rs.coeff[k] * t0;
+                        s0 += fac * R * rs.coeff[k] * (double)TL[o];
+                    } else {
+                        add = fac * R * rs.coeff[k];
+                    }
+                    Tb[o] = (T)((double)Tb[o] + add);
+                }
+                Tb[n] = (T)((double)Tb[n] + s0);
             }
-            Tb[n] = (T)((double)Tb[n] + s0);
         }
     }
     red[threadIdx.x] = r2;
