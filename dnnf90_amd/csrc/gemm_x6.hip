@@ -984,6 +984,14 @@ __global__ __launch_bounds__(256, 2) void k_grad_x6s(const float *__restrict__ A
     }
 }
 
+// resident workgroups a CU the one-block K5 is given at most: each is a
+// column chunk with its own FP64 partial, so more of them cost reduction
+// traffic (four measured slower than two: dense4_k3 reduction 0.078 ->
+// 0.100 ms, K5 no faster)
+#ifndef GRAD_X6S_MAXWG
+#define GRAD_X6S_MAXWG 2
+#endif
+
 // the wave grid of k_grad_x6s for an MB x KB block grid (0: not handled)
 static int grad_x6s_wr(int MB, int KB)
 {
@@ -1039,7 +1047,7 @@ int grad_x6s_wgs(int M, int Kd, bool any)
     static int occ[4][4] = {};
     int &n = occ[__builtin_ctz(MB)][__builtin_ctz(KB)];
     if (n == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 256, 0) != hipSuccess || n < 1)) n = 1;
-    return std::max(1, 256 * std::min(n, 2) / (tiles_of(M) * tiles_of(Kd - 1)));
+    return std::max(1, 256 * std::min(n, GRAD_X6S_MAXWG) / (tiles_of(M) * tiles_of(Kd - 1)));
 }
 
 static bool launch_grad_x6s(const float *A, const float *B, int M, int Kd, long ncol, long Np, double *partial,
