@@ -604,6 +604,8 @@ __global__ __launch_bounds__(256) void k_reduce_fused(const double *__restrict__
         const int per = (nchunk + nsplit - 1) / nsplit;
         const int c0 = sidx * per, c1 = min(nchunk, c0 + per);
         double s = 0.0;
+        // loads unrolled so several are in flight; the sum stays in chunk order
+#pragma unroll 8
         for (int c = c0; c < c1; ++c) s += partial[(long)c * len + i];
         red[t] = s;
     }
