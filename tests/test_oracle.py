@@ -238,3 +238,95 @@ def test_vectors_from_reference(oracle_mod):
         idx = z[f"{name}_gidx"]
         g = o.grad(wl.dims, w, x, z[f"{name}_seed"])
         assert rel(g[idx], z[f"{name}_g"]) <= 1e-14, name
+
+
+# ---- Gauss-Newton rows: pins of Oracle.pinn_rows / Oracle.grad_rows ----
+# The reference forms row n as the gradient of R_n: set_pinn_seed(t, 1, 1)
+# then lnet_seed_row, i.e. net_backward_point on a zeroed nabla
+# (app/sgd_batch_module.f90:740-745, app/lib_net_module.f90:314-330).
+
+def _row_case(oracle_mod):
+    o = oracle_mod.Oracle(2, 4, [[4, 0], [0, 2], [1, 0]])
+    al = [tuple(a) for a in o.alpha()]
+    # linear terms and two product terms, one of them on slot 0 itself (u*u)
+    terms = [(1.0, al.index((4, 0)), False), (-0.7, al.index((0, 2)), False),
+             (2.5, al.index((1, 0)), True), (0.3, 0, True)]
+    dims = [2, 7, 6, 1]
+    rng = np.random.default_rng(11)
+    w = rng.normal(0, 0.6, o.nweights(dims))
+    x = rng.uniform(-1, 1, (9, 2))
+    return o, terms, dims, w, x
+
+
+def _residual(tv, terms):
+    """R = sum_k c_k [T_0 *] T_{a_k} (app/pinn_module.f90:32-47), tv[slot]."""
+    return sum(c * (tv[0] * tv[s] if nl else tv[s]) for c, s, nl in terms)
+
+
+def _dR_dT(tv, terms, na):
+    """dR/dT written out: c T_a gives c at a; c T_0 T_a gives c T_0 at a and
+    c T_a at 0 (2 c T_0 at 0 when a = 0)."""
+    d = np.zeros(na)
+    for c, s, nl in terms:
+        if nl:
+            d[s] += c * tv[0]
+            d[0] += c * tv[s]
+        else:
+            d[s] += c
+    return d
+
+
+def test_pinn_rows_are_residual_jacobian(oracle_mod):
+    """Row n is dR_n/dw: central differences of R_n computed here from the
+    forward's carried derivatives."""
+    o, terms, dims, w, x = _row_case(oracle_mod)
+    rows = o.pinn_rows(dims, w, x, terms)
+    h = 1e-6
+    for p in range(len(w)):
+        wp, wm = w.copy(), w.copy()
+        wp[p] += h
+        wm[p] -= h
+        tp = o.eval(dims, wp, x)[:, 0, :]
+        tm = o.eval(dims, wm, x)[:, 0, :]
+        num = np.array([(_residual(a, terms) - _residual(b, terms)) / (2 * h) for a, b in zip(tp, tm)])
+        assert np.abs(num - rows[:, p]).max() <= 1e-7 * max(np.abs(rows).max(), 1.0), p
+
+
+def test_pinn_rows_weighted_sum_is_step_gradient(oracle_mod):
+    """sum_n R_n row_n is the gradient of (1/2) sum_n R_n^2, i.e. the
+    collocation step at fac = 1."""
+    o, terms, dims, w, x = _row_case(oracle_mod)
+    rows = o.pinn_rows(dims, w, x, terms)
+    g, rsq, tout = o.pinn_step(dims, w, x, terms, 1.0, want_t=True)
+    R = np.array([_residual(tv, terms) for tv in tout[:, 0, :]])
+    assert abs(float((R * R).sum()) - rsq) <= 1e-14 * rsq
+    assert rel(R @ rows, g) <= 1e-13
+
+
+def test_pinn_rows_one_point_seeded_gradient(oracle_mod):
+    """Row n equals the one-point gradient under the seed dR/dT written out
+    here, on the oracle and, where built, on the compiled reference's
+    net_backward_point (what lnet_seed_row calls)."""
+    o, terms, dims, w, x = _row_case(oracle_mod)
+    rows = o.pinn_rows(dims, w, x, terms)
+    t = o.eval(dims, w, x)[:, 0, :]
+    ref = None
+    if oracle_mod.RefLib.available():
+        ref = oracle_mod.RefNet(2, 4, [[4, 0], [0, 2], [1, 0]], 0, dims, w)
+    for n in range(x.shape[0]):
+        seed = _dR_dT(t[n], terms, o.na)[None, None, :]
+        gn = o.grad(dims, w, x[n:n + 1], seed, 1)
+        assert np.array_equal(gn, rows[n]), n
+        if ref is not None:
+            assert np.array_equal(ref.grad(x[n:n + 1], seed), rows[n]), n
+
+
+def test_grad_rows_sum_and_single_points(oracle_mod):
+    """grad_rows: the rows sum to the batch gradient, and each row is the
+    gradient of its point alone."""
+    o, terms, dims, w, x = _row_case(oracle_mod)
+    seed = np.random.default_rng(12).standard_normal((x.shape[0], 1, o.na))
+    rows = o.grad_rows(dims, w, x, seed)
+    assert rel(rows.sum(axis=0), o.grad(dims, w, x, seed)) <= 1e-14
+    for n in range(x.shape[0]):
+        assert np.array_equal(rows[n], o.grad(dims, w, x[n:n + 1], seed[n:n + 1], 1)), n
