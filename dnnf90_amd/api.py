@@ -307,6 +307,11 @@ class BatchWork:
         return tout
 
     def backward(self, seed, grad: Grad):
+        """Adjoint of seed [npt][nout][na] after a forward of npt points on
+        this work space; the gradient is added to grad."""
+        shape = tuple(int(v) for v in seed.shape)
+        if shape != (self.npt, self.net.dims[-1], self.net.na):
+            raise ValueError(f"seed of shape {shape} after a forward of {self.npt} points")
         sk, ps, dev = _ptr(seed, np.float64)
         _chk(lib().dnnf90_backward_batch(self.net.id, self.id, ps, grad.id, DEVICE_PTRS if dev else 0),
              "dnnf90_backward_batch")
@@ -320,6 +325,7 @@ class BatchWork:
             raise TypeError("x and y must both be host or both be device")
         _chk(lib().dnnf90_grad_batch(self.net.id, self.id, int(x.shape[0]), px, py, float(coef), grad.id,
                                      DEVICE_PTRS if xdev else 0), "dnnf90_grad_batch")
+        self.npt = 0   # the planes now hold the one-slot layout: a backward needs a new forward
 
     def residual_step(self, x, res: Residual, fac, grad: Grad, want_rsq=True):
         """Fused forward + collocation seed + adjoint; returns sum R^2."""
@@ -331,6 +337,7 @@ class BatchWork:
                                           float(fac), grad.id, C.addressof(rsq) if want_rsq else None,
                                           DEVICE_PTRS if dev else 0),
              "dnnf90_residual_step_batch")
+        self.npt = int(x.shape[0])
         return rsq.value if want_rsq else None
 
     def residual_rows(self, x, res: Residual, rows_t, ld):
@@ -343,6 +350,7 @@ class BatchWork:
         _chk(lib().dnnf90_residual_rows_batch(self.net.id, self.id, int(x.shape[0]), px, res.nterm, *args,
                                               pr, int(ld), DEVICE_PTRS if dev else 0),
              "dnnf90_residual_rows_batch")
+        self.npt = int(x.shape[0])
 
     def ngd_solve(self, rows_t, ld, nrow, b, nb, mu, mode=1, dual=False):
         """u = (R R^T/nb + cI)^{-1} (b/nb); c = mu (mode 1) or mu tr(G) (mode 0).

@@ -9,6 +9,14 @@ first hidden layer, so no gradient flows to x (lib/train_module.f90:430).
 This is what the Python comparison scripts (bench/post/compare_frameworks.py,
 bench/compare/grad_cost.py) consume: any loss written in torch on T seeds the
 adjoint through autograd.
+
+One work space serves every call of a DerivMLP.  The reference's contract is
+that a backward follows the matching forward on the same work space
+(lib/train_module.f90:341-355); a loss built from several forwards (interior
+and boundary points summed into one loss) breaks it, so every forward is
+stamped, and a backward whose forward is no longer the one held in the work
+space first runs that forward again on its saved inputs.  The kernels are
+deterministic, so the re-run reproduces the planes bit for bit.
 """
 from __future__ import annotations
 
@@ -24,19 +32,23 @@ class _DerivForward(torch.autograd.Function):
     # default stream cannot be handed over directly
     @staticmethod
     def forward(ctx, x, w, layer):
-        layer._sync_weights(w)
-        cur = torch.cuda.current_stream(x.device)
         x = x.contiguous()
-        T = torch.empty((x.shape[0], layer.net.dims[-1], layer.net.na), dtype=torch.float64, device=x.device)
-        layer.stream.wait_stream(cur)
-        layer.bwork.forward(x, T)
-        cur.wait_stream(layer.stream)
+        T = layer._run_forward(x, w, want=True)
         ctx.layer = layer
+        ctx.stamp = layer._held
+        ctx.save_for_backward(x, w)
         return T
 
     @staticmethod
     def backward(ctx, gT):
         layer = ctx.layer
+        x, w = ctx.saved_tensors
+        if layer._held != ctx.stamp:
+            # another forward ran on the work space since this one: restore it
+            layer._run_forward(x, w, want=False)
+            ctx.stamp = layer._held
+        if gT.shape != (x.shape[0], layer.net.dims[-1], layer.net.na):
+            raise ValueError(f"seed of shape {tuple(gT.shape)} for a forward of {x.shape[0]} points")
         layer.grad.zero()
         cur = torch.cuda.current_stream(gT.device)
         seed = gT.contiguous().to(torch.float64)
@@ -65,12 +77,28 @@ class DerivMLP(torch.nn.Module):
         self.bwork.set_stream(self.stream.cuda_stream)
         self.grad = Grad(net)
         self._wver = None
+        self._stamps = 0     # forwards run so far
+        self._held = 0       # stamp of the forward whose planes the work space holds
 
     def _sync_weights(self, w):
         ver = (w.data_ptr(), w._version)
         if ver != self._wver:
             self.net.weights = w.detach().cpu().numpy()
             self._wver = ver
+
+    def _run_forward(self, x, w, want):
+        self._sync_weights(w)
+        cur = torch.cuda.current_stream(x.device)
+        T = None
+        if want:
+            T = torch.empty((x.shape[0], self.net.dims[-1], self.net.na), dtype=torch.float64,
+                            device=x.device)
+        self.stream.wait_stream(cur)
+        self.bwork.forward(x, T, want=want)
+        cur.wait_stream(self.stream)
+        self._stamps += 1
+        self._held = self._stamps
+        return T
 
     def forward(self, x):
         if x.shape[0] > self.bwork.maxpt:

@@ -35,16 +35,25 @@ def build(verbose: bool = False) -> str:
               "-I", os.path.join(ROOT, "include"), "-Wall", "-Wno-unused-function"]
     bdir = os.path.join(HERE, "_build")
     os.makedirs(bdir, exist_ok=True)
-    for src in SOURCES:
+
+    def compile_one(src):
         obj = os.path.join(bdir, os.path.splitext(src)[0] + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
         cmd = [hipcc()] + common + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
         r = subprocess.run(cmd, capture_output=True, text=True)
         msg = "\n".join(l for l in (r.stdout + r.stderr).splitlines()
                         if "packed-fp32-ops" not in l and l.strip())
-        if r.returncode != 0:
+        return src, obj, cmd, r.returncode, msg
+
+    # one hipcc per source, a few at a time
+    from concurrent.futures import ThreadPoolExecutor
+    jobs = max(1, min(len(SOURCES), int(os.environ.get("MAX_JOBS", "0")) or (os.cpu_count() or 1), 8))
+    with ThreadPoolExecutor(jobs) as ex:
+        results = list(ex.map(compile_one, SOURCES))
+    for src, obj, cmd, rc, msg in results:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        if rc != 0:
             raise RuntimeError(f"hipcc failed on {src}:\n{msg}")
         if verbose and msg:
             print(msg)

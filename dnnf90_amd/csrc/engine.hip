@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -31,18 +32,16 @@ namespace dnnf90 {
 
 // generated Faa di Bruno kernels: on unless DNNF90_CODEGEN=0 or switched off
 // through dnnf90_set_codegen; code objects are cached by source text
-static int g_codegen = -1;
+static int codegen_env()
+{
+    const char *e = std::getenv("DNNF90_CODEGEN");
+    return (e && e[0] == '0') ? 0 : 1;
+}
+static std::atomic<int> g_codegen{codegen_env()};
 static std::mutex g_gen_mu;
 static std::map<std::string, std::vector<char>> g_gen_cache;
 
-static bool codegen_on()
-{
-    if (g_codegen < 0) {
-        const char *e = std::getenv("DNNF90_CODEGEN");
-        g_codegen = (e && e[0] == '0') ? 0 : 1;
-    }
-    return g_codegen != 0;
-}
+static bool codegen_on() { return g_codegen.load(std::memory_order_relaxed) != 0; }
 
 // with the generated kernels: each hidden layer keeps S and the adjoint
 // recomputes h from it (on unless switched off by dnnf90_set_store_s), or
@@ -82,6 +81,13 @@ static int dalloc(T **p, size_t n)
         return e == hipErrorOutOfMemory ? -4 : -3;
     }
     return 0;
+}
+
+// a failed launch or stream call: -4 when the device is out of memory, else -3
+static int hip_fail(hipError_t e, const char *where)
+{
+    g_err = std::string(where) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? -4 : -3;
 }
 
 template <typename T>
@@ -177,6 +183,44 @@ struct Net {
     GenKernels gen[2];       // [0] FP32, [1] FP64
     bool gen_tried[2] = {false, false};
     std::mutex mu;
+    // weight uploads and device-side weight steps run on the network's own
+    // stream after the last launch of every stream that read the weights:
+    // one event per such stream, recorded after its launches (used_on)
+    hipStream_t ns = nullptr;
+    mutable std::mutex use_mu;
+    mutable std::vector<std::pair<hipStream_t, hipEvent_t>> uses;
+
+    // kernels just enqueued on s read (or, for the filter, wrote) the device weights
+    void used_on(hipStream_t s) const
+    {
+        std::lock_guard<std::mutex> lk(use_mu);
+        for (auto &u : uses)
+            if (u.first == s) { (void)hipEventRecord(u.second, s); return; }
+        hipEvent_t e;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return;
+        (void)hipEventRecord(e, s);
+        uses.emplace_back(s, e);
+    }
+    // a work space's own stream is going away: drop its entry
+    void forget(hipStream_t s) const
+    {
+        std::lock_guard<std::mutex> lk(use_mu);
+        for (size_t i = 0; i < uses.size(); ++i)
+            if (uses[i].first == s) {
+                (void)hipEventSynchronize(uses[i].second);
+                (void)hipEventDestroy(uses[i].second);
+                uses.erase(uses.begin() + (long)i);
+                return;
+            }
+    }
+    // the network's stream, waiting for every recorded reader
+    int own_stream_after_readers()
+    {
+        if (!ns) HIPCHK(hipStreamCreateWithFlags(&ns, hipStreamNonBlocking));
+        std::lock_guard<std::mutex> lk(use_mu);
+        for (auto &u : uses) HIPCHK(hipStreamWaitEvent(ns, u.second, 0));
+        return 0;
+    }
 
     // the generated kernels of one precision (compiled on first use)
     const GenKernels *generated(bool fp64)
@@ -213,25 +257,34 @@ struct Net {
             int rc = dalloc(&d_w64, (size_t)nw);
             if (rc) return rc;
         }
-        if (dev_ver != host_ver) {
-            // kernels of any stream may still read the previous weights
-            HIPCHK(hipDeviceSynchronize());
-            HIPCHK(hipMemcpy(d_w64, w.data(), sizeof(double) * nw, hipMemcpyHostToDevice));
-            dev_ver = host_ver;
-        }
-        if (need32 && dev32_ver != dev_ver) {
-            if (!d_w32) {
-                int rc = dalloc(&d_w32, (size_t)nw);
-                if (rc) return rc;
+        if (dev_ver != host_ver || (need32 && dev32_ver != host_ver)) {
+            // kernels of other streams may still read the previous weights:
+            // the copy waits for this network's readers only
+            int rc = own_stream_after_readers();
+            if (rc) return rc;
+            if (dev_ver != host_ver) {
+                HIPCHK(hipMemcpyAsync(d_w64, w.data(), sizeof(double) * nw, hipMemcpyHostToDevice, ns));
+                dev_ver = host_ver;
             }
-            launch_convert_f64_f32(d_w64, d_w32, nw, 0);
-            HIPCHK(hipDeviceSynchronize());
-            dev32_ver = dev_ver;
+            if (need32 && dev32_ver != dev_ver) {
+                if (!d_w32 && (rc = dalloc(&d_w32, (size_t)nw))) return rc;
+                launch_convert_f64_f32(d_w64, d_w32, nw, ns);
+                dev32_ver = dev_ver;
+            }
+            HIPCHK(hipStreamSynchronize(ns));
         }
         return 0;
     }
     void release()
     {
+        if (ns) (void)hipStreamSynchronize(ns);
+        for (auto &u : uses) {
+            (void)hipEventSynchronize(u.second);
+            (void)hipEventDestroy(u.second);
+        }
+        uses.clear();
+        if (ns) (void)hipStreamDestroy(ns);
+        ns = nullptr;
         dt.release();
         dfree(d_w64);
         dfree(d_w32);
@@ -240,10 +293,42 @@ struct Net {
     }
 };
 
+// gradient accumulator: kernels of any work space's stream add into d_g
+// (one thread at a time per accumulator, the reference's contract); the
+// event marks the last of them, and the accumulator's own stream does the
+// clears and read-outs after it
 struct Grad {
     int nw = 0;
     double *d_g = nullptr, *d_m = nullptr, *d_v = nullptr;
-    void release() { dfree(d_g); dfree(d_m); dfree(d_v); }
+    hipStream_t s = nullptr;
+    hipEvent_t ev = nullptr;
+    int init()
+    {
+        HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        return 0;
+    }
+    // before kernels on st touch d_g: after the accumulator's last writer
+    int before(hipStream_t st) const
+    {
+        HIPCHK(hipStreamWaitEvent(st, ev, 0));
+        return 0;
+    }
+    // after kernels on st wrote d_g
+    int after(hipStream_t st) const
+    {
+        HIPCHK(hipEventRecord(ev, st));
+        return 0;
+    }
+    void release()
+    {
+        if (ev) (void)hipEventSynchronize(ev);
+        dfree(d_g); dfree(d_m); dfree(d_v);
+        if (ev) (void)hipEventDestroy(ev);
+        if (s) (void)hipStreamDestroy(s);
+        ev = nullptr;
+        s = nullptr;
+    }
 };
 
 // kernel categories timed by HIP events when profiling is on
@@ -480,7 +565,7 @@ static void layer_linear(const Net &nt, BatchWork &b, int l, long ncol, long Np)
 // K1 of layer l, then (hidden layers) the Faa di Bruno step writing T of
 // layer l and, unless the layer keeps S, h
 template <typename T>
-static void layer_forward(const Net &nt, BatchWork &b, int l, long ncol, long Np)
+static hipError_t layer_forward(const Net &nt, BatchWork &b, int l, long ncol, long Np)
 {
     const int L = nt.nlayer, nd = nt.dims[l];
     hipStream_t s = b.stream;
@@ -489,13 +574,14 @@ static void layer_forward(const Net &nt, BatchWork &b, int l, long ncol, long Np
         PScope p(b, CAT_BELL_FWD);
         const GenKernels *gk = gen_for<T>(nt);
         if (b.s_kept)
-            (void)gen_launch_fwdt(nt.gen[sizeof(T) == 8 ? 1 : 0], b.hplane(l), b.tpl[l], nd, Np, nt.dt.act(), s);
+            return gen_launch_fwdt(nt.gen[sizeof(T) == 8 ? 1 : 0], b.hplane(l), b.tpl[l], nd, Np, nt.dt.act(), s);
         else if (gk)
-            (void)gen_launch_fwd(*gk, b.p0, b.tpl[l], b.hplane(l), nd, Np, nt.dt.act(), s);
+            return gen_launch_fwd(*gk, b.p0, b.tpl[l], b.hplane(l), nd, Np, nt.dt.act(), s);
         else
             launch_bell_fwd<T>((const T *)b.p0, (T *)b.tpl[l], (T *)b.hplane(l), nd, Np, nt.dt.fwd(), nt.dt.act(),
                                (T *)b.scratch, s);
     }
+    return hipSuccess;
 }
 
 // the one-point kernels take the work space when it holds one FP64 point,
@@ -532,8 +618,11 @@ static PtNet point_net(const Net &nt, const BatchWork &b)
     return p;
 }
 
+// the forward's launches on the work space's stream; the first failed
+// launch of a generated kernel is returned (the others surface through
+// hipGetLastError)
 template <typename T>
-static void forward_t(const Net &nt, BatchWork &b, const double *d_x, int npt)
+static hipError_t forward_t(const Net &nt, BatchWork &b, const double *d_x, int npt)
 {
     const int na = nt.tab.na, L = nt.nlayer;
     const long Np = round_up(std::max(npt, 1), 64);
@@ -545,23 +634,30 @@ static void forward_t(const Net &nt, BatchWork &b, const double *d_x, int npt)
     hipStream_t s = b.stream;
     if (const GenKernels *pk = point_kernels<T>(nt, b)) {
         b.pt_used = true;
-        PScope p(b, CAT_BELL_FWD);
-        (void)gen_launch_pt_fwd(*pk, (const double *)weights<T>(nt), d_x, nt.dt.ind_e1, point_net(nt, b), nt.dt.act(),
-                                s);
-        return;
+        hipError_t e;
+        {
+            PScope p(b, CAT_BELL_FWD);
+            e = gen_launch_pt_fwd(*pk, (const double *)weights<T>(nt), d_x, nt.dt.ind_e1, point_net(nt, b),
+                                  nt.dt.act(), s);
+        }
+        nt.used_on(s);
+        return e;
     }
     b.pt_used = false;
     {
         PScope p(b, CAT_OTHER);
         launch_input<T>(d_x, npt, nt.dims[0], na, nt.dt.ind_e1, (T *)b.t0, Np, s);
     }
-    for (int l = 1; l < L; ++l) layer_forward<T>(nt, b, l, ncol, Np);
+    hipError_t e = hipSuccess;
+    for (int l = 1; l < L && e == hipSuccess; ++l) e = layer_forward<T>(nt, b, l, ncol, Np);
+    nt.used_on(s);
+    return e;
 }
 
 // adjoint from the output seed already in p0 (rows < nout).  rows_t == null:
 // gradient added to grad; else the per-point rows are written to rows_t.
 template <typename T>
-static void backward_t(const Net &nt, BatchWork &b, double *grad, float *rows_t, long ldr)
+static hipError_t backward_t(const Net &nt, BatchWork &b, double *grad, float *rows_t, long ldr)
 {
     const int na = nt.tab.na, L = nt.nlayer;
     const long Np = b.np, ncol = (long)na * Np;
@@ -570,12 +666,17 @@ static void backward_t(const Net &nt, BatchWork &b, double *grad, float *rows_t,
     const DevPairs pr = nt.dt.pairs();
     if (!rows_t && b.pt_used) {
         // the forward went through the one-point kernel: so does the adjoint
-        PScope p(b, CAT_BELL_BWD);
-        (void)gen_launch_pt_bwd(nt.gen[1], (const double *)W, grad, (const double *)b.p0, point_net(nt, b),
-                                nt.dt.act(), s);
-        return;
+        hipError_t e;
+        {
+            PScope p(b, CAT_BELL_BWD);
+            e = gen_launch_pt_bwd(nt.gen[1], (const double *)W, grad, (const double *)b.p0, point_net(nt, b),
+                                  nt.dt.act(), s);
+        }
+        nt.used_on(s);
+        return e;
     }
-    for (int l = L - 1; l >= 1; --l) {
+    hipError_t e = hipSuccess;
+    for (int l = L - 1; l >= 1 && e == hipSuccess; --l) {
         const int nd = nt.dims[l], ndm = nt.dims[l - 1];
         if (l < L - 1) {
             // p0 is free here (its S-bar was consumed by layer l+1); the
@@ -584,18 +685,18 @@ static void backward_t(const Net &nt, BatchWork &b, double *grad, float *rows_t,
             if (b.recompute) {
                 if (b.s_kept)
                     layer_linear<T>(nt, b, l, ncol, Np);
-                else
-                    layer_forward<T>(nt, b, l, ncol, Np);
+                else if ((e = layer_forward<T>(nt, b, l, ncol, Np)) != hipSuccess)
+                    break;
             }
             PScope p(b, CAT_BELL_BWD);
             const GenKernels *gk = gen_for<T>(nt);
             if (b.s_kept)
-                (void)gen_launch_bwds(nt.gen[sizeof(T) == 8 ? 1 : 0], b.p1, b.hplane(l), b.p0, nd, Np, nt.dt.act(),
-                                      s);
+                e = gen_launch_bwds(nt.gen[sizeof(T) == 8 ? 1 : 0], b.p1, b.hplane(l), b.p0, nd, Np, nt.dt.act(), s);
             else if (gk)
-                (void)gen_launch_bwd(*gk, b.p1, b.hplane(l), b.p0, nd, Np, s);
+                e = gen_launch_bwd(*gk, b.p1, b.hplane(l), b.p0, nd, Np, s);
             else
                 launch_bell_bwd<T>((const T *)b.p1, (const T *)b.hplane(l), (T *)b.p0, nd, Np, pr, s);
+            if (e != hipSuccess) break;
         }
         const T *X = (const T *)(l == 1 ? b.t0 : b.tpl[l - 1]);
         if (rows_t) {
@@ -623,6 +724,8 @@ static void backward_t(const Net &nt, BatchWork &b, double *grad, float *rows_t,
                               ncol, Np, s);
         }
     }
+    nt.used_on(s);
+    return e;
 }
 
 // ---------------------------------------------------------------------------
@@ -754,7 +857,7 @@ static int run_forward(Net &nt, BatchWork &b, int npt, const double *x, double *
             HIPCHK(hipMemcpyAsync(b.d_x, x, sizeof(double) * npt * nt.dims[0], hipMemcpyHostToDevice, b.stream));
         dx = b.d_x;
     }
-    forward_t<T>(nt, b, dx, npt);
+    if (hipError_t e = forward_t<T>(nt, b, dx, npt)) return hip_fail(e, "forward launch");
     if (tout) {
         const int nout = nt.dims[nt.nlayer - 1], na = nt.tab.na;
         double *dt = dev ? tout : b.d_tout;
@@ -786,7 +889,11 @@ static int run_backward_seed(Net &nt, BatchWork &b, const double *seed, Grad &g,
         PScope p(b, CAT_OTHER);
         launch_seed_user<T>(ds, b.npt, nout, na, (T *)b.p0, b.np, b.stream);
     }
-    backward_t<T>(nt, b, g.d_g, nullptr, 0);
+    int rc = g.before(b.stream);
+    if (rc) return rc;
+    hipError_t e = backward_t<T>(nt, b, g.d_g, nullptr, 0);
+    if ((rc = g.after(b.stream))) return rc;
+    if (e) return hip_fail(e, "adjoint launch");
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -836,6 +943,7 @@ static int run_grad_value(Net &nt, BatchWork &b, int npt, const double *x, const
         PScope p(b, CAT_OTHER);
         launch_seed_value<T>(Z(L - 1), dy, coef, npt, Np, (T *)b.p0, s);
     }
+    if ((rc = g.before(s))) return rc;
     for (int l = L - 1; l >= 1; --l) {
         const int nd = nt.dims[l], ndm = nt.dims[l - 1];
         if (l < L - 1) {
@@ -858,6 +966,8 @@ static int run_grad_value(Net &nt, BatchWork &b, int npt, const double *x, const
             launch_gemm_wx<T>(b.prec, true, W + nt.woff[l], ndm + 1, (const T *)b.p0, (T *)b.p1, ndm, nd, Np, Np, s);
         }
     }
+    nt.used_on(s);
+    if ((rc = g.after(s))) return rc;
     b.fwd_valid = false;   // the planes now hold the one-slot layout
     HIPCHK(hipGetLastError());
     return 0;
@@ -927,11 +1037,15 @@ static int run_residual(Net &nt, BatchWork &b, int npt, const double *x, int nte
         launch_seed_residual<T>((const T *)b.tl, nout, na, npt, b.np, r, rows_t ? 1.0 : fac, rows_t != nullptr,
                                 (T *)b.p0, b.d_rsq_part, b.stream);
     }
+    hipError_t e;
     if (rows_t) {
-        backward_t<T>(nt, b, nullptr, rows_t, ldr);
+        e = backward_t<T>(nt, b, nullptr, rows_t, ldr);
     } else {
-        backward_t<T>(nt, b, g->d_g, nullptr, 0);
+        if ((rc = g->before(b.stream))) return rc;
+        e = backward_t<T>(nt, b, g->d_g, nullptr, 0);
+        if ((rc = g->after(b.stream))) return rc;
     }
+    if (e) return hip_fail(e, "adjoint launch");
     if (rsq) {
         launch_reduce_sum(b.d_rsq_part, nblk, b.d_rsq, b.stream);
         HIPCHK(hipMemcpyAsync(rsq, b.d_rsq, sizeof(double), hipMemcpyDeviceToHost, b.stream));
@@ -974,6 +1088,7 @@ struct KFilter {
     long x_cap = 0, seed_cap = 0, r0_cap = 0;
     KfScalars *st = nullptr;
     bool prior = false;
+    hipError_t lerr = hipSuccess;     // first failed launch of the current call
     void release()
     {
         dfree(P); dfree(d_poff); dfree(d_bsz); dfree(d_boff); dfree(pj); dfree(kv); dfree(part); dfree(g);
@@ -1041,7 +1156,7 @@ static int kf_build(const Net &nt, KFilter &kf, double p0, double lam_ini, doubl
     h.lam0 = lam0;
     h.gs = 1.0;
     HIPCHK(hipMemcpy(kf.st, &h, sizeof(h), hipMemcpyHostToDevice));
-    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipStreamSynchronize(nullptr));   // the fills above, on the null stream
     return 0;
 }
 
@@ -1095,7 +1210,8 @@ static void kf_observe(Net &nt, BatchWork &b, KFilter &kf, const double *dx, con
         PScope p(b, CAT_OTHER);
         launch_fill(kf.g, kf.nw, 0.0, s);
     }
-    forward_t<double>(nt, b, dx, 1);
+    hipError_t e = forward_t<double>(nt, b, dx, 1);
+    if (e && !kf.lerr) kf.lerr = e;
     {
         PScope p(b, CAT_OTHER);
         if (xmode == KF_XI_LINEAR) launch_output<double>((const double *)b.tl, nout, na, 1, b.np, b.d_tout, s);
@@ -1105,7 +1221,8 @@ static void kf_observe(Net &nt, BatchWork &b, KFilter &kf, const double *dx, con
         else
             launch_seed_user<double>(dseed, 1, nout, na, (double *)b.p0, b.np, s);
     }
-    backward_t<double>(nt, b, kf.g, nullptr, 0);
+    e = backward_t<double>(nt, b, kf.g, nullptr, 0);
+    if (e && !kf.lerr) kf.lerr = e;
     kf_step(nt, kf, kf.g, scale, xmode, v, dseed, b.d_tout, nout * na, iekf, final, s, &b);
 }
 
@@ -1123,6 +1240,11 @@ static int kf_finish(Net &nt, KFilter &kf, hipStream_t s)
         nt.dev_ver = nt.host_ver;   // the device copy is the updated one
     }
     HIPCHK(hipGetLastError());
+    if (kf.lerr) {
+        const hipError_t e = kf.lerr;
+        kf.lerr = hipSuccess;
+        return hip_fail(e, "filter launch");
+    }
     if (h.bad)
         return fail(-5, "nonpositive innovation denominator: the covariance has lost positive definiteness");
     return 0;
@@ -1328,6 +1450,8 @@ static int free_work(Registry<BatchWork> &reg, int id)
     BatchWork *b = reg.get(id);
     if (!b) return fail(-1, "bad work-space handle");
     (void)hipStreamSynchronize(b->stream);
+    if (const Net *nt = g_net.get(reg.owner[id]))
+        if (reg.ogen[id] == g_net.gen[reg.owner[id]]) nt->forget(b->own);
     b->release();
     delete b;
     reg.obj[id] = nullptr;
@@ -1362,10 +1486,11 @@ int dnnf90_grad_create(int nid)
     if (id < 0) return fail(-1, "no free accumulator handle");
     Grad *g = new Grad();
     g->nw = nt->nw;
-    int rc = dalloc(&g->d_g, (size_t)std::max(1, nt->nw));
-    if (rc) { delete g; return rc; }
-    launch_fill(g->d_g, nt->nw, 0.0, 0);
-    if (hipDeviceSynchronize() != hipSuccess) { g->release(); delete g; return fail(-3, "device failure"); }
+    int rc = g->init();
+    if (!rc) rc = dalloc(&g->d_g, (size_t)std::max(1, nt->nw));
+    if (rc) { g->release(); delete g; return rc; }
+    launch_fill(g->d_g, nt->nw, 0.0, g->s);
+    if (hipStreamSynchronize(g->s) != hipSuccess) { g->release(); delete g; return fail(-3, "device failure"); }
     g_grad.obj[id] = g;
     g_grad.owner[id] = nid;
     g_grad.ogen[id] = g_net.gen[nid];
@@ -1376,8 +1501,7 @@ int dnnf90_grad_free(int gid)
 {
     Grad *g = g_grad.get(gid);
     if (!g) return fail(-1, "bad accumulator handle");
-    (void)hipDeviceSynchronize();
-    g->release();
+    g->release();   // after the accumulator's last writer
     delete g;
     g_grad.obj[gid] = nullptr;
     return 0;
@@ -1387,9 +1511,11 @@ int dnnf90_grad_zero(int gid)
 {
     Grad *g = g_grad.get(gid);
     if (!g) return fail(-1, "bad accumulator handle");
-    HIPCHK(hipDeviceSynchronize());
-    launch_fill(g->d_g, g->nw, 0.0, 0);
-    HIPCHK(hipDeviceSynchronize());
+    int rc = g->before(g->s);
+    if (rc) return rc;
+    launch_fill(g->d_g, g->nw, 0.0, g->s);
+    if ((rc = g->after(g->s))) return rc;
+    HIPCHK(hipStreamSynchronize(g->s));
     return 0;
 }
 
@@ -1418,19 +1544,23 @@ int dnnf90_adam_step(int nid, int gid, double eta, double beta1, double beta2, d
     Grad &g = *g_grad.obj[gid];
     int rc = nt->ensure_device(false);
     if (rc) return rc;
-    HIPCHK(hipDeviceSynchronize());
+    // the step writes the weights: on the network's stream, after the
+    // weights' readers and the accumulator's last writer
+    if ((rc = nt->own_stream_after_readers()) || (rc = g.before(nt->ns))) return rc;
+    hipStream_t s = nt->ns;
     if (!g.d_m) {
         if ((rc = dalloc(&g.d_m, (size_t)std::max(1, g.nw))) || (rc = dalloc(&g.d_v, (size_t)std::max(1, g.nw))))
             return rc;
-        launch_fill(g.d_m, g.nw, 0.0, 0);
-        launch_fill(g.d_v, g.nw, 0.0, 0);
+        launch_fill(g.d_m, g.nw, 0.0, s);
+        launch_fill(g.d_v, g.nw, 0.0, s);
     }
     const int is = std::max(istep, 1);
     const double bc1 = 1.0 - std::pow(beta1, is), bc2 = 1.0 - std::pow(beta2, is);
-    launch_adam(nt->d_w64, g.d_m, g.d_v, g.d_g, g.nw, eta, beta1, beta2, eps, (double)nbatch, bc1, bc2, 0);
-    HIPCHK(hipDeviceSynchronize());
+    launch_adam(nt->d_w64, g.d_m, g.d_v, g.d_g, g.nw, eta, beta1, beta2, eps, (double)nbatch, bc1, bc2, s);
+    if ((rc = g.after(s))) return rc;
     std::lock_guard<std::mutex> lk(nt->mu);
-    HIPCHK(hipMemcpy(nt->w.data(), nt->d_w64, sizeof(double) * nt->nw, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(nt->w.data(), nt->d_w64, sizeof(double) * nt->nw, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
     nt->host_ver++;
     nt->dev_ver = nt->host_ver;  // the device copy is the updated one
     return 0;
@@ -1576,8 +1706,10 @@ int dnnf90_grad_get(int gid, double *g)
 {
     Grad *gr = g_grad.get(gid);
     if (!gr || !g) return fail(-1, "bad accumulator handle");
-    HIPCHK(hipDeviceSynchronize());
-    HIPCHK(hipMemcpy(g, gr->d_g, sizeof(double) * gr->nw, hipMemcpyDeviceToHost));
+    int rc = gr->before(gr->s);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(g, gr->d_g, sizeof(double) * gr->nw, hipMemcpyDeviceToHost, gr->s));
+    HIPCHK(hipStreamSynchronize(gr->s));
     return 0;
 }
 
@@ -1585,8 +1717,11 @@ int dnnf90_grad_set(int gid, const double *g)
 {
     Grad *gr = g_grad.get(gid);
     if (!gr || !g) return fail(-1, "bad accumulator handle");
-    HIPCHK(hipDeviceSynchronize());
-    HIPCHK(hipMemcpy(gr->d_g, g, sizeof(double) * gr->nw, hipMemcpyHostToDevice));
+    int rc = gr->before(gr->s);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(gr->d_g, g, sizeof(double) * gr->nw, hipMemcpyHostToDevice, gr->s));
+    if ((rc = gr->after(gr->s))) return rc;
+    HIPCHK(hipStreamSynchronize(gr->s));
     return 0;
 }
 
@@ -1901,7 +2036,7 @@ int dnnf90_kf_set(int kid, double gate, double q)
     KFilter *kf = g_kf.get(kid);
     if (!kf) return fail(-1, "bad filter handle");
     if (gate < 0.0 || q < 0.0) return fail(-1, "gate and q must be >= 0");
-    HIPCHK(hipDeviceSynchronize());
+    // every update call drains its stream before it returns: the state is settled
     KfScalars h;
     HIPCHK(hipMemcpy(&h, kf->st, sizeof(h), hipMemcpyDeviceToHost));
     h.gate = gate;
@@ -1914,8 +2049,7 @@ int dnnf90_kf_free(int kid)
 {
     KFilter *kf = g_kf.get(kid);
     if (!kf) return fail(-1, "bad filter handle");
-    (void)hipDeviceSynchronize();
-    kf->release();
+    kf->release();   // every update call drained its stream before it returned
     delete kf;
     g_kf.obj[kid] = nullptr;
     return 0;
@@ -1928,9 +2062,13 @@ int dnnf90_kf_update_grad(int nid, int kid, int gid, double xi, double scale)
     KFilter &kf = *g_kf.obj[kid];
     int rc = nt->ensure_device(false);
     if (rc) return rc;
-    HIPCHK(hipDeviceSynchronize());
-    kf_step(*nt, kf, g_grad.obj[gid]->d_g, scale, KF_XI_GIVEN, xi, nullptr, nullptr, 0, false, true, 0);
-    return kf_finish(*nt, kf, 0);
+    // on the network's stream (it writes the weights), after the weights'
+    // readers and the accumulator's last writer
+    if ((rc = nt->own_stream_after_readers())) return rc;
+    const Grad &g = *g_grad.obj[gid];
+    if ((rc = g.before(nt->ns))) return rc;
+    kf_step(*nt, kf, g.d_g, scale, KF_XI_GIVEN, xi, nullptr, nullptr, 0, false, true, nt->ns);
+    return kf_finish(*nt, kf, nt->ns);
 }
 
 int dnnf90_kf_update_obs(int nid, int kid, int bid, int nobs, const double *x, const double *seed,
@@ -2016,7 +2154,6 @@ int dnnf90_kf_iekf_begin(int nid, int kid)
     int rc = nt->ensure_device(false);
     if (rc) return rc;
     if (!kf.wprior && (rc = dalloc(&kf.wprior, (size_t)std::max(1, kf.nw)))) return rc;
-    HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(kf.wprior, nt->d_w64, sizeof(double) * kf.nw, hipMemcpyDeviceToDevice));
     kf.prior = true;
     return 0;
@@ -2046,14 +2183,13 @@ int dnnf90_kf_get(int kid, double *P, double *info)
     KFilter *kfp = g_kf.get(kid);
     if (!kfp) return fail(-1, "bad filter handle");
     KFilter &kf = *kfp;
-    HIPCHK(hipDeviceSynchronize());
     KfScalars h;
     HIPCHK(hipMemcpy(&h, kf.st, sizeof(h), hipMemcpyDeviceToHost));
     if (P) {
         if (h.pending && !h.bad) {   // settle the downdate the last update left owing
             kf_pass(kf, nullptr, 1.0, 0);
             launch_kf_clear(kf.st, 0);
-            HIPCHK(hipDeviceSynchronize());
+            HIPCHK(hipStreamSynchronize(nullptr));
         }
         std::vector<double> hp((size_t)kf.plen);
         HIPCHK(hipMemcpy(hp.data(), kf.P, sizeof(double) * kf.plen, hipMemcpyDeviceToHost));

@@ -255,12 +255,7 @@ static void lin_go(const T *W, int ldw, const T *B, T *C, int M, int Kd, long nc
 {
     const int ks = (Kd + 3) / 4, ksp = (ks + KC - 1) / KC * KC;
     const size_t lds = ((size_t)RBT * ksp * 64 + 16 * RBT) * sizeof(T);
-    static size_t lds_set = 64 * 1024;   // dynamic LDS above 64 KiB must be requested
-    if (lds > lds_set) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_lin<T, kTrans, kAcc64, RBT, KC>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        lds_set = lds;
-    }
+    lds_allow(reinterpret_cast<const void *>(&k_lin<T, kTrans, kAcc64, RBT, KC>), lds);
     const int nrt = (M + 16 * RBT - 1) / (16 * RBT);
     const long ncb = ncol / 16;
     const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / std::max<size_t>(lds, 1)));

@@ -559,12 +559,7 @@ static bool lin_x6b_try(const float *W, int ldw, const float *B, float *C, int M
     if (kTrans || !LIN_X6B || !g_lin_x6c || M != 128 || Kd != 128 || ncol % 64 != 0 || 24 * ncol + 64 >= (1L << 28))
         return false;
     constexpr size_t lds = (size_t)(8 * 4 + 4 * 4) * 3 * 64 * 16 + 128 * sizeof(float);
-    static bool set = false;
-    if (!set) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_lin_x6b<kTrans, kAcc64>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        set = true;
-    }
+    lds_allow(reinterpret_cast<const void *>(&k_lin_x6b<kTrans, kAcc64>), lds);
     const long ncu = ncol / 64;
     const long cupw = (ncu + 255) / 256;   // one workgroup a CU (its LDS)
     const long g = (ncu + cupw - 1) / cupw;
@@ -580,15 +575,7 @@ static void lin_x6c_launch(const float *W, int ldw, const float *B, float *C, in
     const void *fn = reinterpret_cast<const void *>(&k_lin_x6c<kTrans, kAcc64, RBT, NCB, NKC, PD, WPE>);
     // resident workgroups per CU, from the registers and the LDS (queried
     // once per instantiation and LDS size), so the grid is one full round
-    static size_t lds_q = 0;
-    static int per_cu = 0;
-    if (lds != lds_q) {
-        if (lds > 64 * 1024) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        int n = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 256, lds) != hipSuccess || n < 1) n = 1;
-        per_cu = std::min(n, 4);
-        lds_q = lds;
-    }
+    const int per_cu = std::min(occupancy_cached(fn, lds), 4);
     const int nrt = (M + 16 * RBT - 1) / (16 * RBT);
     const long ncu = ncol / (16 * NCB);
     const long want = std::max(1L, (long)256 * per_cu / nrt);
@@ -624,24 +611,11 @@ template <bool kTrans, bool kAcc64, int RBT, int NCB, int KP>
 static void lin_x6_launch(const float *W, int ldw, const float *B, float *C, int M, int Kd, long ncol, long Np,
                           size_t lds, int per_cu, hipStream_t s)
 {
-    static size_t lds_set = 64 * 1024;   // dynamic LDS above 64 KiB must be requested
-    if (lds > lds_set) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_lin_x6<kTrans, kAcc64, RBT, NCB, KP>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        lds_set = lds;
-    }
+    const void *fn = reinterpret_cast<const void *>(&k_lin_x6<kTrans, kAcc64, RBT, NCB, KP>);
 #if LIN_X6_OCC
-    {
-        static int occ = 0;
-        if (!occ) {
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                    &occ, reinterpret_cast<const void *>(&k_lin_x6<kTrans, kAcc64, RBT, NCB, KP>), 256, lds) !=
-                    hipSuccess ||
-                occ < 1)
-                occ = 1;
-        }
-        per_cu = std::min(per_cu, occ);
-    }
+    per_cu = std::min(per_cu, occupancy_cached(fn, lds));
+#else
+    lds_allow(fn, lds);
 #endif
     const int nrt = (M + 16 * RBT - 1) / (16 * RBT);
     const long ncu = ncol / (16 * NCB);
@@ -1044,9 +1018,7 @@ int grad_x6s_wgs(int M, int Kd, bool any)
 #undef X
     if (!fn) return 0;
     // resident workgroups a CU, queried once per instantiation
-    static int occ[4][4] = {};
-    int &n = occ[__builtin_ctz(MB)][__builtin_ctz(KB)];
-    if (n == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 256, 0) != hipSuccess || n < 1)) n = 1;
+    const int n = occupancy_cached(fn, 0);
     return std::max(1, 256 * std::min(n, GRAD_X6S_MAXWG) / (tiles_of(M) * tiles_of(Kd - 1)));
 }
 

@@ -234,4 +234,13 @@ void launch_kf_clear(KfScalars *st, hipStream_t s);
 void launch_comm_welford(const double *t, double *mean, double *m2, long n, int k, hipStream_t s);
 void launch_comm_finish(double *m2, long n, int nmem, hipStream_t s);
 
+// launch-time kernel attributes, shared by every host thread (thread-safe):
+// dynamic LDS above 64 KiB must be requested per kernel; the limit of fn is
+// raised to lds bytes and never lowered
+void lds_allow(const void *fn, size_t lds);
+// resident workgroups of `threads` threads a CU for fn with lds bytes of
+// dynamic LDS (at least 1), queried once per (kernel, lds) and cached; raises
+// the LDS limit first
+int occupancy_cached(const void *fn, size_t lds, int threads = 256);
+
 }  // namespace dnnf90

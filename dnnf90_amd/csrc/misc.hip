@@ -2,6 +2,10 @@
 // elementwise kernels (Adam, fills) of the derivative-carrying pass.
 #include "kernels.hpp"
 
+#include <map>
+#include <mutex>
+#include <utility>
+
 namespace dnnf90 {
 
 // input layer: T(i, 0) = x_i, T(i, e_i) = 1, all else 0
@@ -256,5 +260,38 @@ void launch_seed_value(const T *Z, const double *y, double coef, int npt, long N
 INST(double)
 INST(float)
 #undef INST
+
+static std::mutex g_attr_mu;
+static std::map<const void *, size_t> g_lds_limit;
+static std::map<std::pair<const void *, size_t>, int> g_occ;
+
+static void lds_allow_locked(const void *fn, size_t lds)
+{
+    if (lds <= 64 * 1024) return;
+    size_t &cur = g_lds_limit[fn];
+    if (lds > cur) {
+        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        cur = lds;
+    }
+}
+
+void lds_allow(const void *fn, size_t lds)
+{
+    std::lock_guard<std::mutex> lk(g_attr_mu);
+    lds_allow_locked(fn, lds);
+}
+
+int occupancy_cached(const void *fn, size_t lds, int threads)
+{
+    std::lock_guard<std::mutex> lk(g_attr_mu);
+    lds_allow_locked(fn, lds);
+    const auto key = std::make_pair(fn, lds);
+    auto it = g_occ.find(key);
+    if (it != g_occ.end()) return it->second;
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, threads, lds) != hipSuccess || n < 1) n = 1;
+    g_occ[key] = n;
+    return n;
+}
 
 }  // namespace dnnf90

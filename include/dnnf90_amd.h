@@ -141,7 +141,8 @@ int dnnf90_bwork_kernel_ms(int bid, double *ms, long *count);
 int dnnf90_forward_batch(int nid, int bid, int npt, const double *x,
                          double *tout, int flags);
 /* adjoint of seed [npt][nout][nderiv] (dL/dT) after a forward on the same
- * work space; the weight gradient is ADDED to gid */
+ * work space, npt being that forward's point count (the seed must hold that
+ * many rows); the weight gradient is ADDED to gid */
 int dnnf90_backward_batch(int nid, int bid, const double *seed, int gid,
                           int flags);
 /* value-only batched step of a single-output network (net_grad_batch,

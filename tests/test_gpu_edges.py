@@ -228,3 +228,36 @@ def test_split_kernels_vs_general(D, dims):
     assert np.array_equal(t1, t0)
     assert rel(g1, g0) <= 1e-6
     net.free()
+
+
+def test_stale_dependents_refused(D):
+    """Handles of work spaces and accumulators built on a network that was
+    freed are refused (-1) after the network's slot is reused: the
+    generation counters of lib/c_api_module.f90:44-62."""
+    import ctypes as C
+    from dnnf90_amd._lib import lib
+    L = lib()
+    ts = D.TableSet(2, 3, [[0, 1], [1, 0], [3, 0]], 0)
+    dims = (C.c_int * 3)(2, 8, 1)
+    nid = L.dnnf90_net_create_ts(ts.id, 3, dims)
+    assert nid > 0
+    wid, tid, gid = L.dnnf90_work_create(nid), L.dnnf90_twork_create(nid), L.dnnf90_grad_create(nid)
+    bid = L.dnnf90_bwork_create(nid, 4, 0)
+    assert min(wid, tid, gid, bid) > 0
+    x = np.array([0.3, 0.2])
+    t = np.zeros(5)
+    seed = np.ones(5)
+    assert L.dnnf90_eval(nid, wid, x.ctypes.data, t.ctypes.data) == 0
+    assert L.dnnf90_net_free(nid) == 0
+    nid2 = L.dnnf90_net_create_ts(ts.id, 3, dims)
+    assert nid2 == nid                     # the slot is reused
+    assert L.dnnf90_eval(nid2, wid, x.ctypes.data, t.ctypes.data) == -1
+    assert L.dnnf90_grad_point(nid2, tid, x.ctypes.data, seed.ctypes.data, gid) == -1
+    assert L.dnnf90_forward_batch(nid2, bid, 1, x.ctypes.data, None, 0) == -1
+    # fresh dependents of the new network work
+    wid2 = L.dnnf90_work_create(nid2)
+    assert wid2 > 0 and L.dnnf90_eval(nid2, wid2, x.ctypes.data, t.ctypes.data) == 0
+    for f, h in ((L.dnnf90_work_free, wid), (L.dnnf90_work_free, wid2), (L.dnnf90_twork_free, tid),
+                 (L.dnnf90_grad_free, gid), (L.dnnf90_bwork_free, bid)):
+        assert f(h) == 0
+    assert L.dnnf90_net_free(nid2) == 0

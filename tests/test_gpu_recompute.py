@@ -110,3 +110,36 @@ def test_autograd_recompute_closure4_k7(D, oracle_mod):
             worst = max(worst, abs(fd - g[i]) / gmax)
     print("central FD (h=1e-4) vs autograd: max |fd - g| / max|g| =", worst)
     assert worst <= 1e-5
+
+
+@pytest.mark.parametrize("n1,n2", [(48, 80), (80, 48)])
+def test_autograd_two_forwards_one_loss(D, oracle_mod, n1, n2):
+    """Interior and boundary batches of different sizes through one DerivMLP,
+    summed into one loss before a single backward: each node's adjoint must
+    run on its own forward's planes (the work space is re-filled when a later
+    forward replaced them)."""
+    import torch
+    from dnnf90_amd.autograd import DerivMLP
+    wl, ts, net, w, x, o = _setup(D, oracle_mod, "kdv", n1 + n2)
+    terms = wl.terms(o.alpha())
+    x1, x2 = x[:n1], x[n1:]
+    model = DerivMLP(net, max(n1, n2), D.PREC_FP64)
+    co = [t[0] for t in terms]
+
+    def loss_of(T):
+        u = T[:, 0, 0]
+        R = torch.zeros_like(u)
+        for k, (c, s, nl) in enumerate(terms):
+            R = R + c * (u if nl else 1.0) * T[:, 0, s]
+        return (R * R).mean()
+
+    print(f"two forwards n1={n1} n2={n2}", flush=True)
+    loss = loss_of(model(torch.from_numpy(x1).cuda())) + loss_of(model(torch.from_numpy(x2).cuda()))
+    loss.backward()
+    g = model.w.grad.detach().cpu().numpy()
+    g1, r1, _ = o.pinn_step(wl.dims, w, x1, terms, 2.0 / n1)
+    g2, r2, _ = o.pinn_step(wl.dims, w, x2, terms, 2.0 / n2)
+    e = rel(g, g1 + g2)
+    print("two-forward loss gradient vs oracle: rel", e, co)
+    assert e <= 1e-12
+    assert abs(float(loss) - (r1 / n1 + r2 / n2)) <= 1e-12 * (r1 / n1 + r2 / n2)
