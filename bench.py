@@ -1,6 +1,6 @@
 """Throughput of the derivative-carrying forward pass + exact adjoint on one GPU.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-configs]
 
 A step is one pass of the hot path over the workload's whole batch, inputs
 resident in HBM: input layer, forward (linear map + Faa di Bruno step per
@@ -8,21 +8,33 @@ layer), collocation seed of the mean-square residual loss, adjoint
 (pair sums, weight-gradient GEMM, transposed linear map per layer) and
 accumulation of the batch gradient.  It runs once in FP64 (the drop-in
 parity mode) and once per mixed-precision mode; the headline value is the
-mixed-precision (BF16 hi+lo split MFMA) points/s, reported with its error
-against the FP64 run per derivative order.  Prints one JSON line.
+mixed-precision (BF16 three-way split MFMA) points/s of closure4_k7 (MLP
+[4,64x4,1], tanh, closure{t, x, x^7, x^3 y^2 z^2}, N = 16384), reported with
+its error against the FP64 run per derivative order.  Every step is timed
+by its own pair of HIP events on the work space's stream; the reported time
+is the median over the K timed steps.  Under "configs" the same line carries
+FP64 and mixed points/s, speed-up and per-order error of the other
+workloads (kdv, dense4_k3, gelu3_k4) and the NGD step of ngd2_k4.  Prints
+one JSON line.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import time
+from types import SimpleNamespace
 
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+
+HEADLINE = "closure4_k7"
+CONFIGS = ("kdv", "dense4_k3", "gelu3_k4")
 
 METRIC = "fwd+adjoint points/s, mixed-prec vs FP64 on GPU, at stated err vs FP64 oracle"
 PEAK = {"hbm": 8000.0, "fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
@@ -85,19 +97,53 @@ def rel_by_order(x, ref, deg):
     return out
 
 
-def bench_ngd(args, D, torch, wl, net, xd, res, N, dev, mu=1e-3):
+
+
+def host_cores():
+    """Host threads this run may use (the OpenMP setting where one is given)."""
+    try:
+        return max(1, int(os.environ.get("OMP_NUM_THREADS", "")))
+    except ValueError:
+        return os.cpu_count() or 1
+
+
+def prepare(D, lib, torch, wl, dev, npt=None):
+    """Table set, network, inputs and residual of one workload."""
+    N = wl.npt if npt is None else npt
+    ts = D.TableSet(wl.d0, wl.k, wl.seeds, wl.act)
+    alpha = [a[:wl.d0] for a in D.current_alpha()]
+    net = D.Net(ts, wl.dims)
+    w, x = wl.inputs(N)
+    net.weights = w
+    terms = wl.terms(alpha)
+    deg = np.array([sum(a) for a in alpha])
+    counts = np.zeros(4, np.int32)
+    lib().dnnf90_tabset_counts(ts.id, counts.ctypes.data)
+    return SimpleNamespace(wl=wl, N=N, ts=ts, net=net, w=w, x=x, xd=torch.from_numpy(x).to(dev), terms=terms,
+                           res=D.Residual(terms), deg=deg, na=len(alpha), F=int(counts[2]), P=int(counts[1]),
+                           Q=int(sum(max(int(p) - 1, 0) for p in deg)), fac=2.0 / N)
+
+
+def release(c):
+    c.net.free()
+    c.ts.free()
+
+
+def bench_ngd(args, D, torch, c, dev, mu=1e-3):
     """One natural-gradient step of the NGD workload: Gauss-Newton rows dR/dw
     of every point (FP64 adjoint, rows stored FP32), the loss gradient b,
     G = J^T J on FP32 MFMA and the FP64 Cholesky solve of (G/N + mu I) u = b/N
-    (app/optimizer_module.f90:221-469 with ABS damping)."""
+    (app/optimizer_module.f90:221-469 with ABS damping); the dual (Gram)
+    route's metric and solve timed beside it.  Median over the steps."""
+    net, xd, res, N, wl = c.net, c.xd, c.res, c.N, c.wl
     bw = D.BatchWork(net, N, D.PREC_FP64)
     g = D.Grad(net)
     s = torch.cuda.Stream(device=dev)
     bw.set_stream(s.cuda_stream)
     ld = ((N + 63) // 64) * 64
     rows = torch.zeros((net.nw, ld), dtype=torch.float32, device=dev)
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-    acc = np.zeros(4)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    acc = []
     with torch.cuda.stream(s):
         for it in range(args.warmup + args.steps):
             ev[0].record(s)
@@ -116,29 +162,244 @@ def bench_ngd(args, D, torch, wl, net, xd, res, N, dev, mu=1e-3):
             ud = bw.ngd_solve(rows, ld, N, b, N, mu, 1, dual=True)
             t_dual = time.perf_counter() - t0
             if it >= args.warmup:
-                acc += [ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]), t_solve * 1e3, t_dual * 1e3]
-    acc /= args.steps
-    ms = float(acc[:3].sum())
+                acc.append([ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]), t_solve * 1e3, t_dual * 1e3])
+    acc = np.array(acc)
+    med = np.median(acc, axis=0)
+    step = np.median(acc[:, :3].sum(axis=1))
+    step_dual = np.median(acc[:, [0, 1, 3]].sum(axis=1))
     nw = net.nw
-    gram_flop = 2.0 * nw * nw * N
-    chol_flop = nw ** 3 / 3.0
-    line = {"metric": "NGD step rows/s (Gauss-Newton rows + gradient + J^T J + FP64 Cholesky solve)",
-            "value": N / (ms * 1e-3), "unit": "rows/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "dtype": "f64 rows/solve, f32 metric", "data": "synthetic",
+    bw.set_stream(None)
+    bw.free()
+    g.free()
+    return {"metric": "NGD step rows/s (Gauss-Newton rows + gradient + J^T J + FP64 Cholesky solve)",
+            "value": N / (step * 1e-3), "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": float(step), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64 rows/solve, f32 metric", "data": "synthetic",
             "config": {"workload": f"{wl.name}: MLP {wl.dims} tanh, D0={wl.d0} K={wl.k}, N={N}, nw={nw}, "
                                    f"rows of {wl.note}, ABS damping {mu}"},
-            "breakdown_ms": {"rows": acc[0], "gradient": acc[1], "metric_and_solve": acc[2]},
-            "dual_route": {"metric_and_solve_ms": acc[3], "ms_per_step": float(acc[0] + acc[1] + acc[3]),
-                           "rows_per_s": N / (float(acc[0] + acc[1] + acc[3]) * 1e-3),
+            "breakdown_ms": {"rows": float(med[0]), "gradient": float(med[1]), "metric_and_solve": float(med[2])},
+            "dual_route": {"metric_and_solve_ms": float(med[3]), "ms_per_step": float(step_dual),
+                           "rows_per_s": N / (float(step_dual) * 1e-3),
                            "rel_diff_to_primal": float((torch.as_tensor(ud) - torch.as_tensor(u)).norm() /
                                                        torch.as_tensor(u).norm())},
-            "metric_solve_flops": {"gram": gram_flop, "cholesky": chol_flop},
+            "metric_solve_flops": {"gram": 2.0 * nw * nw * N, "cholesky": nw ** 3 / 3.0},
             "u_norm": float(torch.as_tensor(u).norm())}
-    print(json.dumps(line), flush=True)
+
+
+def run_mode(args, D, torch, lib, c, dev, prec, name, wbytes, mma_peak, budget=0, breakdown=True):
+    """K timed residual steps of one precision mode (each step between its
+    own HIP events, median reported); the Faa di Bruno adjoint is bracketed
+    by HIP events in the last timed step (the roofline kernel); with
+    breakdown, K more untimed steps with every kernel category bracketed."""
+    net, xd, res, N = c.net, c.xd, c.res, c.N
+    bw = D.BatchWork(net, N, prec, budget=budget)
+    info = bw.info()
+    kinfo = np.zeros(6, np.int32)
+    lib().dnnf90_net_kernel_info(net.id, prec, kinfo.ctypes.data)
+    keep_s = int(kinfo[0]) == 2
+    g = D.Grad(net)
+    s = torch.cuda.Stream(device=dev)
+    bw.set_stream(s.cuda_stream)
+    with torch.cuda.stream(s):
+        for _ in range(args.warmup):
+            bw.residual_step(xd, res, c.fac, g, want_rsq=False)
+        torch.cuda.synchronize()
+        bw.kernel_ms()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+        t0 = time.perf_counter()
+        ev[0].record(s)
+        for i in range(args.steps):
+            if i == args.steps - 1:
+                bw.profile(True, only="bell_bwd")
+            bw.residual_step(xd, res, c.fac, g, want_rsq=False)
+            ev[i + 1].record(s)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        step_ms = [ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)]
+        bw.profile(False)
+        kms_timed = bw.kernel_ms()
+        if breakdown:
+            # every category bracketed (the markers add launch gaps, so the
+            # timed steps carry only the roofline kernel's)
+            bw.profile(True)
+            for _ in range(args.steps):
+                bw.residual_step(xd, res, c.fac, g, want_rsq=False)
+            torch.cuda.synchronize()
+            bw.profile(False)
+            kms = bw.kernel_ms()
+            kms["bell_bwd"] = tuple(v * args.steps for v in kms_timed["bell_bwd"])   # one step -> K
+        else:
+            kms = {"bell_bwd": tuple(v * args.steps for v in kms_timed["bell_bwd"])}
+        tout = bw.forward(xd)
+        torch.cuda.synchronize()
+    g.zero()
+    rsq = bw.residual_step(xd, res, c.fac, g)
+    grad = g.get()
+    t_host = tout.cpu().numpy()
     bw.set_stream(None)
+    bw.free()
+    g.free()
+    work = algorithmic(c.wl, c.na, c.F, c.Q, c.P, N, wbytes, keep_s)
+    kern = {}
+    for cat, (tot, cnt) in kms.items():
+        ent = {"ms_per_step": tot / args.steps, "launches_per_step": cnt / args.steps}
+        if cat in work and tot > 0:
+            # judged against the roof that binds: MFMA for the linear maps
+            # unless their bytes take longer at 8 TB/s than their flops at the
+            # mode's MFMA peak; HBM for the Faa di Bruno kernels
+            flops, byts = work[cat]
+            sec = tot / args.steps * 1e-3
+            mfma_bound = cat in ("linear", "linear_t", "grad") and \
+                flops / (mma_peak * 1e12) > byts / (PEAK["hbm"] * 1e9)
+            if mfma_bound:
+                ent.update(bound="mfma", achieved=flops / sec / 1e12, unit="TFLOP/s", peak=mma_peak)
+            else:
+                ent.update(bound="hbm", achieved=byts / sec / 1e9, unit="GB/s", peak=PEAK["hbm"])
+            ent["frac"] = ent["achieved"] / ent["peak"]
+            ent["algorithmic_flops_per_step"] = flops
+            ent["algorithmic_bytes_per_step"] = byts
+        kern[cat] = ent
+    info["keep_s"] = keep_s
+    ms = float(np.median(step_ms))
+    return {"name": name, "ms_per_step": ms, "step_ms": {"median": ms, "min": float(min(step_ms)),
+                                                          "max": float(max(step_ms)),
+                                                          "mean": float(np.mean(step_ms))},
+            "wall_ms_per_step": wall * 1e3 / args.steps, "info": info, "points_per_s": N / (ms * 1e-3),
+            "kernels": kern, "T": t_host, "grad": grad, "rsq": rsq}
+
+
+def roofline_of(kk, workload, mode):
+    """The roofline entry names one kernel: the dominant Faa di Bruno
+    category (all its launches in a step are one generated kernel; the
+    linear-map and gradient categories mix instantiations for the input,
+    hidden and output layers -- every category is in "kernels")."""
+    dom = max((k for k in ("bell_fwd", "bell_bwd") if "frac" in kk.get(k, {})), key=lambda k: kk[k]["ms_per_step"])
+    dk = kk[dom]
+    return {"bound": dk["bound"], "achieved": dk["achieved"], "peak": dk["peak"], "unit": dk["unit"],
+            "frac": dk["frac"], "traffic": traffic_of(workload, mode, dom), "kernel": dom,
+            "launch_ms": dk["ms_per_step"] / max(dk["launches_per_step"], 1e-9),
+            "algorithmic_bytes_per_launch": dk["algorithmic_bytes_per_step"] / max(dk["launches_per_step"], 1e-9)}
+
+
+def cpu_port(c, o, seconds):
+    """The oracle's C restatement (OpenMP over points, one work space and
+    accumulator per thread, fixed-order reduction) on a bounded sample."""
+    cores = host_cores()
+    n0 = min(c.N, 64)
+    t0 = time.perf_counter()
+    o.pinn_step(c.wl.dims, c.w, c.x[:n0], c.terms, c.fac, nthreads=cores)
+    dt = time.perf_counter() - t0
+    ns = int(min(c.N, max(n0, n0 * seconds / max(dt, 1e-6))))
+    t0 = time.perf_counter()
+    o.pinn_step(c.wl.dims, c.w, c.x[:ns], c.terms, c.fac, nthreads=cores)
+    dt = time.perf_counter() - t0
+    return {"value": ns / dt, "unit": "points/s", "cores": cores, "kind": "port",
+            "sample": f"first {ns} of the {c.N} points, the oracle's FP64 fwd+seed+adjoint+accumulate "
+                      f"(bitwise the reference's on one thread), OpenMP over points, {cores} threads, {dt:.1f} s"}
+
+
+def cpu_reference(c, seconds):
+    """The reference's own lib/ (oracle/_ref, compiled from its sources with
+    amdflang -O3) through ref_driver.f90's refx_pinn_step: one work space
+    and accumulator per worker over a disjoint slice of the points -- the
+    reference's threading model (tools/example_train.f90:128-170) with one
+    process a host core, since its modules are compiled without OpenMP.
+    Rate = points / the slowest worker's loop time."""
+    import oracle
+    if not oracle.RefLib.available():
+        return None
+    cores = min(host_cores(), max(1, c.N // 8))
+    wl = c.wl
+
+    def run(parts):
+        procs = []
+        tds = []
+        for start, count in parts:
+            td = tempfile.mkdtemp()
+            out = os.path.join(td, "t.json")
+            p = subprocess.Popen([sys.executable, os.path.abspath(__file__), "--workload", wl.name,
+                                  "--ref-timing", f"{start}:{count}", "--out", out],
+                                 stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+            procs.append((p, out))
+            tds.append(td)
+        secs = []
+        for p, out in procs:
+            if p.wait() != 0:
+                raise RuntimeError("reference timing child failed")
+            with open(out) as f:
+                secs.append(json.load(f)["seconds"])
+        for td in tds:
+            for fn in os.listdir(td):
+                os.remove(os.path.join(td, fn))
+            os.rmdir(td)
+        return secs
+
+    n0 = min(c.N, 16)
+    per_pt = max(run([(0, n0)])[0] / n0, 1e-7)
+    share = int(min(c.N // cores, max(4, seconds / per_pt)))
+    secs = run([(i * share, share) for i in range(cores)])
+    dt = max(secs)
+    return {"value": cores * share / dt, "unit": "points/s", "cores": cores, "kind": "reference",
+            "sample": f"{cores} disjoint slices of {share} of the {c.N} points, the reference's lib/ (amdflang "
+                      f"-O3, loop build) through net_forward_point_multi / net_backward_point_multi, one worker "
+                      f"process a core, slowest worker {dt:.1f} s (point loops only)"}
+
+
+def ref_timing(args):
+    """CPU baseline leg (a child process, no GPU): refx_pinn_step over the
+    points start .. start+count-1 of the workload; the loop's seconds go to --out."""
+    import oracle
+    from dnnf90_amd.workloads import WORKLOADS
+    wl = WORKLOADS[args.workload]
+    start, count = (int(v) for v in args.ref_timing.split(":"))
+    w, x = wl.inputs()
+    o = oracle.Oracle(wl.d0, wl.k, wl.seeds, wl.act)
+    terms = wl.terms([list(a) for a in o.alpha()])
+    r = oracle.RefNet(wl.d0, wl.k, wl.seeds, wl.act, wl.dims, w)
+    _, _, secs = r.pinn_step(x[start:start + count], terms, 2.0 / wl.npt)
+    with open(args.out, "w") as f:
+        json.dump({"seconds": secs, "points": count}, f)
     return 0
 
+
+def mixed_vs_fp64(r, fp64, deg):
+    r["err_T"] = rel_by_order(r["T"], fp64["T"], deg)
+    r["err_g"] = float(np.abs(r["grad"] - fp64["grad"]).max() / np.abs(fp64["grad"]).max())
+
+
+def oracle_sample(c, t_fp64, ns):
+    """FP64 GPU forward against the oracle on the first ns points."""
+    import oracle
+    if not os.path.exists(oracle.LIB):
+        oracle.build(ref=False)
+    o = oracle.Oracle(c.wl.d0, c.wl.k, c.wl.seeds, c.wl.act)
+    ns = min(c.N, ns)
+    to = o.eval(c.wl.dims, c.w, c.x[:ns])
+    return o, {"points": ns, "forward_max_rel": max(v["max"] for v in rel_by_order(t_fp64[:ns], to, c.deg).values())}
+
+
+def config_entry(args, D, torch, lib, wl, dev):
+    """FP64 and BF16x6 steps of one more workload, for the "configs" field."""
+    c = prepare(D, lib, torch, wl, dev)
+    print(f"bench: config {wl.name} N={c.N}", file=sys.stderr, flush=True)
+    fp = run_mode(args, D, torch, lib, c, dev, D.PREC_FP64, "fp64", 8, PEAK["fp64"])
+    mx = run_mode(args, D, torch, lib, c, dev, D.PREC_BF16X6, "bf16x6", 4, PEAK["bf16"] / 6.0)
+    mixed_vs_fp64(mx, fp, c.deg)
+    chk = None
+    try:
+        _, chk = oracle_sample(c, fp["T"], 64)
+    except Exception as exc:   # the checker, never the product
+        chk = {"error": repr(exc)}
+    release(c)
+    return {"workload": f"{wl.name}: MLP {wl.dims} {'tanh' if wl.act == 0 else 'erf-GELU'}, D0={wl.d0} K={wl.k} "
+                        f"|A|={c.na}, N={c.N}, loss mean square of {wl.note}",
+            "fp64": {"points_per_s": fp["points_per_s"], "ms_per_step": fp["ms_per_step"],
+                     "vs_oracle": chk, "kernel_ms": {k: v["ms_per_step"] for k, v in fp["kernels"].items()}},
+            "bf16x6": {"points_per_s": mx["points_per_s"], "ms_per_step": mx["ms_per_step"],
+                       "kernel_ms": {k: v["ms_per_step"] for k, v in mx["kernels"].items()},
+                       "roofline": roofline_of(mx["kernels"], wl.name, "bf16x6")},
+            "speedup_vs_fp64": mx["points_per_s"] / fp["points_per_s"],
+            "err_vs_fp64": {"forward_by_order": mx["err_T"], "grad_max_rel": mx["err_g"]}}
 
 def bench_kf(args, D, torch, wl, net, w, x, res, terms, dev):
     """The extended Kalman filter over the weights: one step presents the
@@ -299,18 +560,22 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="dense4_k3")
+    ap.add_argument("--workload", default=HEADLINE)
     ap.add_argument("--mixed", default="bf16x6", choices=["bf16x6", "fp32", "bf16x3"])
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only copy a buffer of this many MiB 5 times (HBM counter calibration)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="the headline workload only")
     ap.add_argument("--kf-obs", type=int, default=0, help="filter workloads: observations per step")
     ap.add_argument("--kf-ref-timing", type=int, default=0, help=argparse.SUPPRESS)
+    ap.add_argument("--ref-timing", default="", help=argparse.SUPPRESS)
     ap.add_argument("--out", default="", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.kf_ref_timing:
         return kf_ref_timing(args)
+    if args.ref_timing:
+        return ref_timing(args)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the gradient all-reduce across GPUs is a design note, not a measured path)")
@@ -331,160 +596,43 @@ def main():
     from dnnf90_amd._lib import lib
 
     wl = WORKLOADS[args.workload]
-    N = wl.npt
-    ts = D.TableSet(wl.d0, wl.k, wl.seeds, wl.act)
-    net = D.Net(ts, wl.dims)
-    w, x = wl.inputs()
-    net.weights = w
-    alpha = [a[:wl.d0] for a in D.current_alpha()]
-    terms = wl.terms(alpha)
-    res = D.Residual(terms)
-    deg = np.array([sum(a) for a in alpha])
-    na = len(alpha)
-    counts = np.zeros(4, np.int32)
-    lib().dnnf90_tabset_counts(ts.id, counts.ctypes.data)
-    F, P = int(counts[2]), int(counts[1])
-    Q = int(sum(max(int(p) - 1, 0) for p in deg))
     dev = torch.device("cuda:0")
-    xd = torch.from_numpy(x).to(dev)
-    fac = 2.0 / N
+    c = prepare(D, lib, torch, wl, dev)
 
     if args.workload == "ngd2_k4":
-        return bench_ngd(args, D, torch, wl, net, xd, res, N, dev)
+        print(json.dumps(bench_ngd(args, D, torch, c, dev)), flush=True)
+        return 0
     if wl.kf is not None:
-        return bench_kf(args, D, torch, wl, net, w, x, res, terms, dev)
+        return bench_kf(args, D, torch, wl, c.net, c.w, c.x, c.res, c.terms, dev)
 
-    def run_mode(prec, name, wbytes, mma_peak, budget=0):
-        bw = D.BatchWork(net, N, prec, budget=budget)
-        info = bw.info()
-        kinfo = np.zeros(6, np.int32)
-        lib().dnnf90_net_kernel_info(net.id, prec, kinfo.ctypes.data)
-        keep_s = int(kinfo[0]) == 2
-        g = D.Grad(net)
-        s = torch.cuda.Stream(device=dev)
-        bw.set_stream(s.cuda_stream)
-        with torch.cuda.stream(s):
-            for _ in range(args.warmup):
-                bw.residual_step(xd, res, fac, g, want_rsq=False)
-            torch.cuda.synchronize()
-            # inside the timed region the roofline's kernel (the Faa di Bruno
-            # adjoint) is bracketed by HIP events on the work space's stream
-            # in the last step (the markers add launch gaps; one step's worth
-            # spread over K steps)
-            bw.kernel_ms()
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            t0 = time.perf_counter()
-            e0.record(s)
-            for i in range(args.steps):
-                if i == args.steps - 1:
-                    bw.profile(True, only="bell_bwd")
-                bw.residual_step(xd, res, fac, g, want_rsq=False)
-            e1.record(s)
-            torch.cuda.synchronize()
-            wall = time.perf_counter() - t0
-            ms = e0.elapsed_time(e1) / args.steps
-            bw.profile(False)
-            kms_timed = bw.kernel_ms()
-            # the per-kernel breakdown: the same steps again, untimed, with
-            # every kernel category bracketed (the markers add launch gaps,
-            # so the timed steps carry only the roofline kernel's)
-            bw.profile(True)
-            for _ in range(args.steps):
-                bw.residual_step(xd, res, fac, g, want_rsq=False)
-            torch.cuda.synchronize()
-            bw.profile(False)
-            kms = bw.kernel_ms()
-            kms["bell_bwd"] = tuple(v * args.steps for v in kms_timed["bell_bwd"])   # one step -> K
-            tout = bw.forward(xd)
-            torch.cuda.synchronize()
-        g.zero()
-        rsq = bw.residual_step(xd, res, fac, g)
-        grad = g.get()
-        t_host = tout.cpu().numpy()
-        bw.set_stream(None)
-        bw.free()
-        g.free()
-        work = algorithmic(wl, na, F, Q, P, N, wbytes, keep_s)
-        kern = {}
-        for c, (tot, cnt) in kms.items():
-            ent = {"ms_per_step": tot / args.steps, "launches_per_step": cnt / args.steps}
-            if c in work and tot > 0:
-                # judged against the roof that binds: MFMA for the linear
-                # maps unless their bytes take longer at 8 TB/s than their
-                # flops at the mode's MFMA peak; HBM for the Faa di Bruno kernels
-                flops, byts = work[c]
-                sec = tot / args.steps * 1e-3
-                mfma_bound = c in ("linear", "linear_t", "grad") and flops / (mma_peak * 1e12) > byts / (PEAK["hbm"] * 1e9)
-                if mfma_bound:
-                    ent.update(bound="mfma", achieved=flops / sec / 1e12, unit="TFLOP/s", peak=mma_peak)
-                else:
-                    ent.update(bound="hbm", achieved=byts / sec / 1e9, unit="GB/s", peak=PEAK["hbm"])
-                ent["frac"] = ent["achieved"] / ent["peak"]
-                ent["algorithmic_flops_per_step"] = flops
-                ent["algorithmic_bytes_per_step"] = byts
-            kern[c] = ent
-        info["keep_s"] = keep_s
-        return {"name": name, "ms_per_step": ms, "wall_ms_per_step": wall * 1e3 / args.steps, "info": info,
-                "points_per_s": N / (ms * 1e-3), "kernels": kern, "T": t_host, "grad": grad, "rsq": rsq}
-
-    fp64 = run_mode(D.PREC_FP64, "fp64", 8, PEAK["fp64"])
+    print(f"bench: {wl.name} N={c.N}", file=sys.stderr, flush=True)
+    fp64 = run_mode(args, D, torch, lib, c, dev, D.PREC_FP64, "fp64", 8, PEAK["fp64"])
     others = {}
     modes = (("fp32", D.PREC_FP32, PEAK["fp32"]), ("bf16x3", D.PREC_BF16X3, PEAK["bf16"] / 3.0),
              ("bf16x6", D.PREC_BF16X6, PEAK["bf16"] / 6.0))
     for name, prec, peak in modes:
-        r = run_mode(prec, name, 4, peak)
-        r["err_T"] = rel_by_order(r["T"], fp64["T"], deg)
-        r["err_g"] = float(np.abs(r["grad"] - fp64["grad"]).max() / np.abs(fp64["grad"]).max())
+        r = run_mode(args, D, torch, lib, c, dev, prec, name, 4, peak)
+        mixed_vs_fp64(r, fp64, c.deg)
         others[name] = r
     mixed = others[args.mixed]
-    err_T, err_g = mixed["err_T"], mixed["err_g"]
-    # the same mixed step with per-layer recompute of h in the reverse pass
+    # the same mixed step with per-layer recompute of the kept plane in the reverse pass
     mprec = {m[0]: m[1] for m in modes}[args.mixed]
     mpeak = {m[0]: m[2] for m in modes}[args.mixed]
-    rc_run = run_mode(mprec, args.mixed + "+recompute", 4, mpeak, budget=1)
-    recompute = {"mode": args.mixed, "ms_per_step": rc_run["ms_per_step"],
-                 "points_per_s": rc_run["points_per_s"],
+    rc_run = run_mode(args, D, torch, lib, c, dev, mprec, args.mixed + "+recompute", 4, mpeak, budget=1,
+                      breakdown=False)
+    recompute = {"mode": args.mixed, "ms_per_step": rc_run["ms_per_step"], "points_per_s": rc_run["points_per_s"],
                  "identical_grad": bool(np.array_equal(rc_run["grad"], mixed["grad"])),
                  "device_bytes": rc_run["info"]["bytes"], "device_bytes_all_kept": rc_run["info"]["bytes_stored"]}
 
-    # FP64 GPU against the oracle on a sample of points
-    oracle_check = None
-    cpu = None
+    oracle_check, cpu, cpu_port_run = None, None, None
     try:
-        import oracle
-        if not os.path.exists(oracle.LIB):
-            oracle.build(ref=False)
-        o = oracle.Oracle(wl.d0, wl.k, wl.seeds, wl.act)
-        ns = min(N, 256)
-        to = o.eval(wl.dims, w, x[:ns])
-        oracle_check = {"points": ns, "forward_max_rel": max(v["max"] for v in rel_by_order(fp64["T"][:ns], to, deg).values())}
+        o, oracle_check = oracle_sample(c, fp64["T"], 256)
         if not args.no_cpu:
-            ncores = os.cpu_count() or 1
-            n0 = 64
-            t0 = time.perf_counter()
-            o.pinn_step(wl.dims, w, x[:n0], terms, fac)
-            dt = time.perf_counter() - t0
-            ns = int(min(N, max(n0, n0 * args.cpu_seconds / max(dt, 1e-6))))
-            t0 = time.perf_counter()
-            o.pinn_step(wl.dims, w, x[:ns], terms, fac)
-            dt = time.perf_counter() - t0
-            cpu = {"value": ns / dt, "unit": "points/s", "cores": ncores, "kind": "port",
-                   "sample": f"first {ns} of the {N} points, oracle FP64 fwd+seed+adjoint+accumulate, "
-                             f"OpenMP over points, {ncores} threads, {dt:.1f} s"}
-    except Exception as exc:  # the CPU leg is a reported baseline, never the product
+            print("bench: CPU baselines", file=sys.stderr, flush=True)
+            cpu_port_run = cpu_port(c, o, args.cpu_seconds)
+            cpu = cpu_reference(c, args.cpu_seconds) or cpu_port_run
+    except Exception as exc:  # the CPU legs are reported baselines, never the product
         cpu = {"error": repr(exc)}
-
-    # the roofline line names one kernel: the dominant category whose
-    # launches in a step are all the same kernel (the generated Faa di Bruno
-    # kernels; the linear-map and gradient categories mix instantiations for
-    # the input, hidden and output layers -- every category is in "kernels")
-    kk = mixed["kernels"]
-    dom = max((c for c in ("bell_fwd", "bell_bwd") if "frac" in kk.get(c, {})), key=lambda c: kk[c]["ms_per_step"])
-    dk = kk[dom]
-    roof = {"bound": dk["bound"], "achieved": dk["achieved"], "peak": dk["peak"], "unit": dk["unit"],
-            "frac": dk["frac"], "traffic": traffic_of(wl.name, args.mixed, dom), "kernel": dom,
-            "launch_ms": dk["ms_per_step"] / max(dk["launches_per_step"], 1e-9)}
 
     line = {
         "metric": METRIC, "value": mixed["points_per_s"], "unit": "points/s", "n_gpus": 1,
@@ -492,25 +640,41 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": DTYPE[args.mixed], "data": "synthetic",
         "config": {"workload": f"{wl.name}: MLP {wl.dims} {'tanh' if wl.act == 0 else 'erf-GELU'}, "
-                               f"D0={wl.d0} K={wl.k} |A|={na}, N={N}, loss mean square of {wl.note}",
+                               f"D0={wl.d0} K={wl.k} |A|={c.na}, N={c.N}, loss mean square of {wl.note}",
                    "precision": args.mixed},
-        "roofline": roof,
+        "roofline": roofline_of(mixed["kernels"], wl.name, args.mixed),
         "cpu_baseline": cpu,
-        "fp64": {"value": fp64["points_per_s"], "ms_per_step": fp64["ms_per_step"], "kernels": fp64["kernels"],
-                 "vs_oracle": oracle_check},
+        "cpu_baseline_port": cpu_port_run if (cpu is not cpu_port_run) else None,
+        "timing": "median of per-step HIP-event times on the work space's stream",
+        "step_ms": mixed["step_ms"],
+        "fp64": {"value": fp64["points_per_s"], "ms_per_step": fp64["ms_per_step"], "step_ms": fp64["step_ms"],
+                 "kernels": fp64["kernels"], "vs_oracle": oracle_check},
         "speedup_vs_fp64": mixed["points_per_s"] / fp64["points_per_s"],
-        "err_vs_fp64": {"forward_by_order": err_T, "grad_max_rel": err_g},
+        "err_vs_fp64": {"forward_by_order": mixed["err_T"], "grad_max_rel": mixed["err_g"]},
         "kernels": mixed["kernels"],
         "kept_plane": {"fp64": "S" if fp64["info"]["keep_s"] else "h",
                        args.mixed: "S" if mixed["info"]["keep_s"] else "h"},
         "recompute": recompute,
         "modes": {k: {"points_per_s": r["points_per_s"], "ms_per_step": r["ms_per_step"],
                       "speedup_vs_fp64": r["points_per_s"] / fp64["points_per_s"],
-                      "max_err_by_order": {o: v["max"] for o, v in r["err_T"].items()},
+                      "max_err_by_order": {o_: v["max"] for o_, v in r["err_T"].items()},
                       "grad_max_rel": r["err_g"],
-                      "kernel_ms": {c: v["ms_per_step"] for c, v in r["kernels"].items()}}
+                      "kernel_ms": {k2: v["ms_per_step"] for k2, v in r["kernels"].items()}}
                   for k, r in others.items()},
     }
+    release(c)
+    del fp64, others, mixed, rc_run
+    if not args.no_configs and args.workload == HEADLINE:
+        cfg = {}
+        for name in CONFIGS:
+            cfg[name] = config_entry(args, D, torch, lib, WORKLOADS[name], dev)
+        print("bench: config ngd2_k4", file=sys.stderr, flush=True)
+        cn = prepare(D, lib, torch, WORKLOADS["ngd2_k4"], dev)
+        ngd = bench_ngd(args, D, torch, cn, dev)
+        release(cn)
+        cfg["ngd2_k4"] = {k: ngd[k] for k in ("value", "unit", "ms_per_step", "config", "breakdown_ms",
+                                              "dual_route", "metric_solve_flops")}
+        line["configs"] = cfg
     print(json.dumps(line), flush=True)
     return 0
 

@@ -440,6 +440,28 @@ class RefNet:
         assert rc == 0
         return m, sd
 
+    def pinn_step(self, x, terms, fac):
+        """refx_pinn_step: the collocation step on the reference's modules,
+        point by point.  terms: (coeff, slot 0-based, nonlin).  Returns
+        (grad, sum R^2, seconds of the point loop)."""
+        L = self.L
+        if not hasattr(L, "_refx_pinn_ready"):
+            L.refx_pinn_step.argtypes = [_i, _i, _i, _pi, _i, _i, _pi, _pd, _i, _pd, _i, _pd, _pi, _pi, _d,
+                                         _pd, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+            L.refx_pinn_step.restype = _i
+            L._refx_pinn_ready = True
+        x = _da(x)
+        co = _da([t[0] for t in terms])
+        sl = _ia([t[1] + 1 for t in terms])
+        nl = _ia([1 if t[2] else 0 for t in terms])
+        g = np.zeros(Oracle.nweights(self.dims))
+        rsq, secs = C.c_double(0.0), C.c_double(0.0)
+        rc = L.refx_pinn_step(self.d0, self.k, self.nseed, _ip(self.seeds), self.act, len(self.dims),
+                              _ip(self.dims), _dp(self.w), x.shape[0], _dp(x), len(terms), _dp(co), _ip(sl),
+                              _ip(nl), float(fac), _dp(g), C.byref(rsq), C.byref(secs))
+        assert rc == 0
+        return g, rsq.value, secs.value
+
     def grad(self, x, seed):
         x, seed = _da(x), _da(seed)
         g = np.zeros(Oracle.nweights(self.dims))

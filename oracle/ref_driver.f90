@@ -258,4 +258,70 @@ CONTAINS
     refx_comm_eval = 0
   END FUNCTION refx_comm_eval
 
+  !> The collocation step on output 1, one point at a time: forward, the
+  !! residual R = sum_k coeff(k) [T_1 *] T_slot(k) (slot 1-based; nonlin(k)
+  !! multiplies by the value), the seed fac R dR/dT, the adjoint into one
+  !! accumulator (app/pinn_module.f90:32-65, app/sgd_batch_module.f90:733-737
+  !! with the trainer's lnet_* calls replaced by the module routines).
+  !! grad(nw) += the batch gradient; rsq = sum_n R_n^2; secs = the seconds of
+  !! the point loop alone (set-up excluded).
+  INTEGER(c_int) FUNCTION refx_pinn_step( d0, k, nseed, seeds, iact, nlayer, dims, w, &
+                                          npt, x, nterm, coeff, slot, nonlin, fac, &
+                                          grad, rsq, secs ) bind(C)
+    integer(c_int),value :: d0, k, nseed, iact, nlayer, npt, nterm
+    integer(c_int),intent(IN) :: seeds(*), dims(*), slot(*), nonlin(*)
+    real(c_double),value :: fac
+    real(c_double),intent(IN) :: w(*), x(*), coeff(*)
+    real(c_double),intent(INOUT) :: grad(*)
+    real(c_double),intent(OUT) :: rsq, secs
+    type(tabset_t) :: ts
+    type(net_t) :: nt
+    type(twork_t) :: tw
+    type(grad_t) :: g
+    real(8),allocatable :: tm(:,:), sm(:,:)
+    real(8) :: R
+    integer :: n, nout, na, p, l, j, it
+    integer(8) :: c0, c1, crate
+    call build_net( d0, k, nseed, seeds, iact, nlayer, dims, w, ts, nt )
+    na = nt%tab%na;  nout = dims(nlayer)
+    call twork_init( tw, nt )
+    call grad_init( g, nt )
+    allocate( tm(nout,na), sm(nout,na) )
+    rsq = 0.d0
+    call system_clock( c0, crate )
+    do n = 1, npt
+       call net_forward_point_multi( nt, tw, x((n-1)*d0+1:n*d0), tm )
+       R = 0.d0
+       do it = 1, nterm
+          if ( nonlin(it) /= 0 ) then
+             R = R + coeff(it)*tm(1,1)*tm(1,slot(it))
+          else
+             R = R + coeff(it)*tm(1,slot(it))
+          end if
+       end do
+       rsq = rsq + R*R
+       sm = 0.d0
+       do it = 1, nterm
+          if ( nonlin(it) /= 0 ) then
+             sm(1,slot(it)) = sm(1,slot(it)) + fac*R*coeff(it)*tm(1,1)
+             sm(1,1) = sm(1,1) + fac*R*coeff(it)*tm(1,slot(it))
+          else
+             sm(1,slot(it)) = sm(1,slot(it)) + fac*R*coeff(it)
+          end if
+       end do
+       call net_backward_point_multi( nt, tw, sm, g )
+    end do
+    call system_clock( c1 )
+    secs = dble(c1 - c0) / dble(crate)
+    p = 0
+    do l = 2, nlayer
+       do j = 1, dims(l)
+          grad(p+1:p+dims(l-1)+1) = grad(p+1:p+dims(l-1)+1) + g%nabla(l,j,0:dims(l-1))
+          p = p + dims(l-1) + 1
+       end do
+    end do
+    call grad_free( g );  call twork_free( tw );  call net_free( nt );  call tabset_free( ts )
+    refx_pinn_step = 0
+  END FUNCTION refx_pinn_step
+
 END MODULE ref_driver

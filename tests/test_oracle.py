@@ -330,3 +330,21 @@ def test_grad_rows_sum_and_single_points(oracle_mod):
     assert rel(rows.sum(axis=0), o.grad(dims, w, x, seed)) <= 1e-14
     for n in range(x.shape[0]):
         assert np.array_equal(rows[n], o.grad(dims, w, x[n:n + 1], seed[n:n + 1], 1)), n
+
+
+@pytest.mark.parametrize("name", ["kdv", "closure4_k7"])
+def test_pinn_step_vs_compiled_reference(oracle_mod, name):
+    """The collocation step (the benchmark's unit of work) on the reference's
+    own modules, point by point, against the oracle on one thread: the same
+    bits for the gradient and sum R^2."""
+    if not oracle_mod.RefLib.available():
+        pytest.skip("oracle/_ref not built (needs the reference tree)")
+    from dnnf90_amd.workloads import WORKLOADS
+    wl = WORKLOADS[name]
+    w, x = wl.inputs(40)
+    o = oracle_mod.Oracle(wl.d0, wl.k, wl.seeds, wl.act)
+    terms = wl.terms(o.alpha())
+    g, rsq, _ = o.pinn_step(wl.dims, w, x, terms, 2.0 / 40, nthreads=1)
+    r = oracle_mod.RefNet(wl.d0, wl.k, wl.seeds, wl.act, wl.dims, w)
+    gr, rsqr, secs = r.pinn_step(x, terms, 2.0 / 40)
+    assert np.array_equal(g, gr) and rsq == rsqr and secs >= 0.0
