@@ -90,6 +90,7 @@ SIGNATURES = {
     "dnnf90_residual_step_batch": (_i, [_i, _i, _i, _vp, _i, _vp, _vp, _vp, _d, _i, _vp, _i]),
     "dnnf90_residual_rows_batch": (_i, [_i, _i, _i, _vp, _i, _vp, _vp, _vp, _vp, _l, _i]),
     "dnnf90_ngd_solve": (_i, [_i, _i, _i, _vp, _l, _vp, _i, _d, _i, _vp, _i]),
+    "dnnf90_grad_point_noalloc": (_i, [_i, _vp, _vp, _vp, _vp]),
 }
 
 REFERENCE_ABI = [n for n in SIGNATURES if not n.startswith("dnnf90_last_error")][:24]

@@ -1278,6 +1278,44 @@ static int kf_begin(Net &nt, BatchWork &b, KFilter &kf, int nobs, const double *
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// one-point gradient without allocation (dnnf90_grad_point_noalloc): the
+// planes of the generated one-point kernels, packed with slot stride 1
+// (every hidden layer's T and S, the input and output T, the seed plane),
+// live in a fixed device array.  Each GPU holds its own copy until the
+// process ends; its address is taken once per device with
+// hipGetSymbolAddress and used for every access from the host.  Every call
+// on one GPU shares it, so calls that overlap on one GPU race.
+static constexpr long NOALLOC_DOUBLES = 49152;   // 384 KiB per device
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+static constexpr int NOALLOC_MAXDEV = 64;
+static double *g_scratch_ptr[NOALLOC_MAXDEV] = {};
+static std::mutex g_scratch_mu;
+
+// the current device's copy of scratch (its address taken on the first call)
+static int scratch_of_device(double **out)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAXDEV) return fail(-1, "device index above the no-allocation table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+// seed plane [nout][na]: output 1 takes the seed, the others zero (as
+// dnnf90_grad_point seeds output 1 only)
+__global__ void k_noalloc_seed(const double *__restrict__ seed, int nout, int na, double *__restrict__ sb)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < nout * na) sb[u] = u < na ? seed[u] : 0.0;
+}
+
 }  // namespace dnnf90
 
 // ===========================================================================
@@ -1533,6 +1571,60 @@ int dnnf90_grad_point(int nid, int tid, const double *x, const double *seed, int
     rc = run_backward_seed<double>(*nt, b, sm.data(), *g_grad.obj[gid], false);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(b.stream));
+    return 0;
+}
+
+int dnnf90_grad_point_noalloc(int nid, const double *x, const double *seed, double *grad, void *stream)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt || !x || !seed || !grad) return fail(-1, "bad handle or argument");
+    {
+        std::lock_guard<std::mutex> lk(nt->mu);
+        if (!nt->dt.ready || !nt->gen_tried[1] || !nt->d_w64)
+            return fail(-1, "network not prepared on the device: create a work space of it first "
+                            "(dnnf90_twork_create builds its tables, kernels and weights there)");
+    }
+    const GenKernels &g = nt->gen[1];
+    const int L = nt->nlayer, na = nt->tab.na, nout = nt->dims[L - 1];
+    bool fits = codegen_on() && g.ok && g.pt_fwd && g.pt_bwd && g.pt_scratch == 0 && L <= PT_MAXL;
+    for (int l = 0; l < L && fits; ++l) fits = nt->dims[l] <= PT_MAXW && (long)nt->dims[l] * na <= PT_MAXWA;
+    if (!fits)
+        return fail(-1, "the network is outside the one-point kernels (at most 16 layers, widths <= 256, "
+                        "width x slots <= 1536, a family of at most 48 slots and 96 operand entries)");
+    long need = (long)nout * na;                                   // seed plane
+    for (int l = 0; l < L; ++l) need += (long)nt->dims[l] * na;    // T of every layer
+    for (int l = 1; l < L - 1; ++l) need += (long)nt->dims[l] * na;  // S of the hidden layers
+    if (need > NOALLOC_DOUBLES)
+        return fail(-1, "the one-point planes need " + std::to_string(need) + " doubles of scratch; the maximum is " +
+                            std::to_string(NOALLOC_DOUBLES));
+    double *sp = nullptr;
+    int rc = scratch_of_device(&sp);
+    if (rc) return rc;
+    if ((rc = nt->ensure_device(false))) return rc;   // prepared: at most a weight copy, no allocation
+    hipStream_t s = (hipStream_t)stream;
+    PtNet p{};
+    p.L = L;
+    p.Np = 1;
+    p.ldr = na;
+    double *q = sp;
+    double *sb = q;
+    q += (long)nout * na;
+    for (int l = 0; l < L; ++l) {
+        p.dims[l] = nt->dims[l];
+        p.woff[l] = nt->woff[l];
+        p.T[l] = q;
+        q += (long)nt->dims[l] * na;
+    }
+    for (int l = 1; l < L - 1; ++l) {
+        p.S[l] = q;
+        q += (long)nt->dims[l] * na;
+    }
+    hipLaunchKernelGGL(k_noalloc_seed, dim3((nout * na + 255) / 256), dim3(256), 0, s, seed, nout, na, sb);
+    hipError_t e = gen_launch_pt_fwd(g, nt->d_w64, x, nt->dt.ind_e1, p, nt->dt.act(), s);
+    if (e == hipSuccess) e = gen_launch_pt_bwd(g, nt->d_w64, grad, sb, p, nt->dt.act(), s);
+    nt->used_on(s);
+    if (e != hipSuccess) return hip_fail(e, "one-point launch");
+    HIPCHK(hipGetLastError());
     return 0;
 }
 

@@ -255,6 +255,23 @@ int dnnf90_kf_get(int kid, double *P, double *info);
 int dnnf90_comm_eval_batch(int nmem, const int *nids, int bid, int npt, const double *x,
                            double *tmean, double *tstd, int flags);
 
+/* dnnf90_grad_point without allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving or per-observation loops, harnesses that forbid
+ * hipMalloc).  One point through the generated one-point kernels: x [d0],
+ * seed [nderiv] (output 1; the others are seeded with zero) and grad [nw]
+ * are DEVICE pointers, the gradient is ADDED to grad, everything is
+ * enqueued on stream (a hipStream_t; NULL = the null stream) and nothing
+ * waits.  The planes live in a fixed device array of 49152 doubles
+ * (384 KiB) per GPU: networks whose planes need more, or that fall outside
+ * the one-point kernels (more than 16 layers, a width above 256, width x
+ * slots above 1536, a family above 48 slots or 96 operand entries), are
+ * refused with -1.  The network must have been prepared by a work space
+ * (dnnf90_twork_create or any other work-space creation); a weight change
+ * since then is copied, nothing is allocated.  All calls on one GPU share
+ * the array: calls that overlap on one GPU race. */
+int dnnf90_grad_point_noalloc(int nid, const double *x, const double *seed, double *grad, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
