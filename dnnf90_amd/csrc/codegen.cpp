@@ -291,6 +291,71 @@ std::string gen_bell_source(const TableSet &t, const FwdSchedule &s, bool fp64)
     for (int a = 0; a < na; ++a) o << "  const real t" << a << " = tb[" << a << "L * Np];\n";
     emit_pair_sums(o, t, fp64);
     o << "}\n";
+
+    // ---- the layers next to the input and the output.  The first hidden
+    // layer's S needs no plane: T0 holds x at slot 0 and 1 at slot e_i, so
+    // S_0 = b + sum_i w_i x_i (summed in FP64, then rounded), S at e_i = w_i
+    // and every other slot is zero (the zero operands fold away in the
+    // recurrence).  The last hidden layer's Tbar needs no plane either:
+    // Tbar_a = sum_k W_L(k, j) Sbar_L(k, a) over the nout output rows.
+    auto load_s_in = [&]() {
+        const int d0 = t.d0;
+        o << "  const real *wj = W1 + (long)j * " << d0 + 1 << ";\n"
+          << "  double s0 = (double)wj[0];\n";
+        for (int i = 0; i < d0; ++i) o << "  s0 += (double)wj[" << i + 1 << "] * (double)T0[" << i << "L * ldr + n];\n";
+        o << "  const real e0 = (real)s0;\n";
+        for (int a = 1; a < na; ++a) {
+            int src = -1;
+            for (int i = 0; i < d0; ++i)
+                if (t.ind_e1[i] == a) src = i;
+            if (src >= 0) o << "  const real e" << a << " = wj[" << src + 1 << "];\n";
+            else o << "  const real e" << a << " = (real)0;\n";
+        }
+        o << "  double d[" << kq << "];\n"
+          << "  act_derivs<" << kq << ", " << (bes ? "true" : "false") << ">(ap, (double)e0, d);\n";
+        for (int q = 0; q < kq; ++q) o << "  const real d" << q << " = (real)d[" << q << "];\n";
+    };
+    auto load_tb_out = [&]() {
+        for (int a = 0; a < na; ++a) o << "  real t" << a << " = (real)0;\n";
+        o << "  for (int k = 0; k < nout; ++k) {\n"
+          << "    const real wk = WL[(long)k * (nrow + 1) + 1 + j];\n"
+          << "    const real *sl = SbL + (long)k * ldr + n;\n";
+        for (int a = 0; a < na; ++a) o << "    t" << a << " += wk * sl[" << a << "L * Np];\n";
+        o << "  }\n";
+    };
+    head("dnnf90_bell_fwdt_in", "const real *__restrict__ T0, const real *__restrict__ W1, real *__restrict__ Tout, "
+                                "int nrow, long Np, ActParams ap");
+    o << "  real *Tj = Tout + (long)j * ldr + n;\n";
+    load_s_in();
+    o << "  Tj[0] = d0;\n";
+    emit_recurrence(o, t, s, fp64, REC_T);
+    o << "}\n";
+    const char *bargs = "const real *__restrict__ Tb, const real *__restrict__ S, real *__restrict__ Sb, int nrow, "
+                        "long Np, ActParams ap, const real *__restrict__ T0, const real *__restrict__ W1, "
+                        "const real *__restrict__ SbL, const real *__restrict__ WL, int nout";
+    for (int v = 1; v < 4; ++v) {
+        const bool in_s = v & 1, out_t = v & 2;
+        const std::string name = std::string("dnnf90_bell_bwds_") + (in_s ? "i" : "") + (out_t ? "o" : "");
+        head(name.c_str(), bargs);
+        o << "  real *sb = Sb + (long)j * ldr + n;\n";
+        if (in_s) {
+            load_s_in();
+        } else {
+            o << "  const real *Sj = S + (long)j * ldr + n;\n";
+            load_s();
+        }
+        o << "  const real h0 = d1;\n";
+        for (int a = 1; a < na; ++a) o << "  real h" << a << ";\n";
+        emit_recurrence(o, t, s, fp64, REC_HREG);
+        if (out_t) {
+            load_tb_out();
+        } else {
+            o << "  const real *tb = Tb + (long)j * ldr + n;\n";
+            for (int a = 0; a < na; ++a) o << "  const real t" << a << " = tb[" << a << "L * Np];\n";
+        }
+        emit_pair_sums(o, t, fp64);
+        o << "}\n";
+    }
     if (fp64 && gen_point_family(t, s)) emit_point_kernels(o, t, s, load_s);
     return o.str();
 }
@@ -352,6 +417,18 @@ int gen_bell_load(const std::vector<char> &code, bool fp64, GenKernels *g)
     (void)hipFuncGetAttribute(&g->bwds_scratch, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, g->bwds);
     (void)hipFuncGetAttribute(&g->fwdt_regs, HIP_FUNC_ATTRIBUTE_NUM_REGS, g->fwdt);
     (void)hipFuncGetAttribute(&g->bwds_regs, HIP_FUNC_ATTRIBUTE_NUM_REGS, g->bwds);
+    // the edge variants: used where they compile without scratch
+    {
+        hipFunction_t *f[4] = {&g->fwdt_in, &g->bwds_i, &g->bwds_o, &g->bwds_io};
+        const char *nm[4] = {"dnnf90_bell_fwdt_in", "dnnf90_bell_bwds_i", "dnnf90_bell_bwds_o", "dnnf90_bell_bwds_io"};
+        for (int i = 0; i < 4; ++i) {
+            int sc = 0;
+            if (!get(f[i], nm[i]) ||
+                hipFuncGetAttribute(&sc, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, *f[i]) != hipSuccess || sc != 0)
+                *f[i] = nullptr;
+        }
+        (void)hipGetLastError();
+    }
     // the one-point kernels are in the FP64 modules of small families only
     if (fp64 && hipModuleGetFunction(&g->pt_fwd, g->mod, "dnnf90_pt_fwd_f64") == hipSuccess &&
         hipModuleGetFunction(&g->pt_bwd, g->mod, "dnnf90_pt_bwd_f64") == hipSuccess) {
@@ -372,6 +449,7 @@ void GenKernels::release()
     if (mod) (void)hipModuleUnload(mod);
     mod = nullptr;
     fwd = bwd = fwdt = bwds = pt_fwd = pt_bwd = nullptr;
+    fwdt_in = bwds_i = bwds_o = bwds_io = nullptr;
     ok = false;
 }
 
@@ -407,6 +485,28 @@ hipError_t gen_launch_bwds(const GenKernels &g, const void *Tb, const void *S, v
     ActParams a = ap;
     void *params[] = {&Tb, &S, &Sb, &nrow, &Np, &a};
     return hipModuleLaunchKernel(g.bwds, (unsigned)(Np / 64), (unsigned)((nrow + 3) / 4), 1, 256, 1, 1, 0, s, params,
+                                 nullptr);
+}
+
+hipError_t gen_launch_fwdt_in(const GenKernels &g, const void *T0, const void *W1, void *Tout, int nrow, long Np,
+                              const ActParams &ap, hipStream_t s)
+{
+    ActParams a = ap;
+    void *params[] = {&T0, &W1, &Tout, &nrow, &Np, &a};
+    return hipModuleLaunchKernel(g.fwdt_in, (unsigned)(Np / 64), (unsigned)((nrow + 3) / 4), 1, 256, 1, 1, 0, s,
+                                 params, nullptr);
+}
+
+hipError_t gen_launch_bwds_edge(const GenKernels &g, bool in_s, bool out_t, const void *Tb, const void *S, void *Sb,
+                                int nrow, long Np, const ActParams &ap, const void *T0, const void *W1,
+                                const void *SbL, const void *WL, int nout, hipStream_t s)
+{
+    hipFunction_t f = in_s ? (out_t ? g.bwds_io : g.bwds_i) : (out_t ? g.bwds_o : g.bwds);
+    if (!f) return hipErrorInvalidDeviceFunction;
+    ActParams a = ap;
+    void *params[] = {&Tb, &S, &Sb, &nrow, &Np, &a, &T0, &W1, &SbL, &WL, &nout};
+    if (!in_s && !out_t) return gen_launch_bwds(g, Tb, S, Sb, nrow, Np, ap, s);
+    return hipModuleLaunchKernel(f, (unsigned)(Np / 64), (unsigned)((nrow + 3) / 4), 1, 256, 1, 1, 0, s, params,
                                  nullptr);
 }
 

@@ -34,6 +34,12 @@ struct GenKernels {
     hipModule_t mod = nullptr;
     hipFunction_t fwd = nullptr, bwd = nullptr, fwdt = nullptr, bwds = nullptr;
     hipFunction_t pt_fwd = nullptr, pt_bwd = nullptr;   // FP64 small families only
+    // edge variants: fwdt_in -- the first hidden layer, S formed from the
+    // input plane T0 and the layer's weights; bwds_i likewise in the adjoint;
+    // bwds_o -- the last hidden layer, Tbar formed from the output layer's
+    // Sbar and weights; bwds_io both (one hidden layer).  Null where they
+    // would need scratch.
+    hipFunction_t fwdt_in = nullptr, bwds_i = nullptr, bwds_o = nullptr, bwds_io = nullptr;
     std::vector<char> code;
     int fwd_regs = 0, bwd_regs = 0, fwd_scratch = 0, bwd_scratch = 0;
     int fwdt_regs = 0, bwds_regs = 0, fwdt_scratch = 0, bwds_scratch = 0;
@@ -63,6 +69,13 @@ hipError_t gen_launch_fwdt(const GenKernels &g, const void *S, void *Tout, int n
                            hipStream_t s);
 hipError_t gen_launch_bwds(const GenKernels &g, const void *Tb, const void *S, void *Sb, int nrow, long Np,
                            const ActParams &ap, hipStream_t s);
+hipError_t gen_launch_fwdt_in(const GenKernels &g, const void *T0, const void *W1, void *Tout, int nrow, long Np,
+                              const ActParams &ap, hipStream_t s);
+// the adjoint from Tb (or, out_t, from the output layer's SbL and WL over
+// nout rows) and S (or, in_s, S formed from T0 and W1), into Sb
+hipError_t gen_launch_bwds_edge(const GenKernels &g, bool in_s, bool out_t, const void *Tb, const void *S, void *Sb,
+                                int nrow, long Np, const ActParams &ap, const void *T0, const void *W1,
+                                const void *SbL, const void *WL, int nout, hipStream_t s);
 hipError_t gen_launch_pt_fwd(const GenKernels &g, const double *W, const double *x, const int *ind_e1, const PtNet &p,
                              const ActParams &ap, hipStream_t s);
 hipError_t gen_launch_pt_bwd(const GenKernels &g, const double *W, double *grad, const double *Sb, const PtNet &p,

@@ -52,6 +52,11 @@ static int g_store_s = 1;
 // one-point kernels); on unless switched off by dnnf90_set_point_kernels
 static int g_point = 1;
 
+// edge variants of the generated kernels (the first hidden layer's S formed
+// from the input plane, the last hidden layer's Tbar from the output
+// layer's Sbar): on unless switched off by dnnf90_set_edge_kernels
+static int g_edge = 1;
+
 static thread_local std::string g_err;
 
 static int fail(int rc, const std::string &m)
@@ -362,6 +367,7 @@ struct BatchWork {
     // Faa di Bruno forward when it holds h) into the plane hs
     bool recompute = false, s_kept = false;
     bool pt_used = false;   // the last forward went through the one-point kernel
+    bool edge_in = false;   // the last forward formed the first hidden layer's S from T0 (no S plane kept)
     bool rows_next = false; // the adjoint of this forward writes per-point rows (batched kernels)
     void *hs = nullptr;
     double bytes = 0.0, bytes_stored = 0.0;   // device bytes held / needed with every h kept
@@ -634,6 +640,7 @@ static hipError_t forward_t(const Net &nt, BatchWork &b, const double *d_x, int 
     hipStream_t s = b.stream;
     if (const GenKernels *pk = point_kernels<T>(nt, b)) {
         b.pt_used = true;
+        b.edge_in = false;
         hipError_t e;
         {
             PScope p(b, CAT_BELL_FWD);
@@ -649,7 +656,16 @@ static hipError_t forward_t(const Net &nt, BatchWork &b, const double *d_x, int 
         launch_input<T>(d_x, npt, nt.dims[0], na, nt.dt.ind_e1, (T *)b.t0, Np, s);
     }
     hipError_t e = hipSuccess;
-    for (int l = 1; l < L && e == hipSuccess; ++l) e = layer_forward<T>(nt, b, l, ncol, Np);
+    const GenKernels &gg = nt.gen[sizeof(T) == 8 ? 1 : 0];
+    b.edge_in = b.s_kept && g_edge && L >= 3 && gg.fwdt_in && gg.bwds_i;
+    int l0 = 1;
+    if (b.edge_in) {
+        // the first hidden layer straight from T0 (no K1, no S plane)
+        PScope p(b, CAT_BELL_FWD);
+        e = gen_launch_fwdt_in(gg, b.t0, weights<T>(nt) + nt.woff[1], b.tpl[1], nt.dims[1], Np, nt.dt.act(), s);
+        l0 = 2;
+    }
+    for (int l = l0; l < L && e == hipSuccess; ++l) e = layer_forward<T>(nt, b, l, ncol, Np);
     nt.used_on(s);
     return e;
 }
@@ -676,13 +692,22 @@ static hipError_t backward_t(const Net &nt, BatchWork &b, double *grad, float *r
         return e;
     }
     hipError_t e = hipSuccess;
+    // sbp holds Sbar of the layer being reversed, tbp Tbar of the layer
+    // below; they trade places after the last hidden layer when its Tbar is
+    // formed from the output layer's Sbar (no K6 of the output layer)
+    void *sbp = b.p0, *tbp = b.p1;
+    const GenKernels &gg = nt.gen[sizeof(T) == 8 ? 1 : 0];
+    const int nout = nt.dims[L - 1];
+    const bool out_fused = b.s_kept && g_edge && L >= 3 && ((L == 3 && b.edge_in) ? gg.bwds_io : gg.bwds_o);
     for (int l = L - 1; l >= 1 && e == hipSuccess; --l) {
         const int nd = nt.dims[l], ndm = nt.dims[l - 1];
         if (l < L - 1) {
+            const bool in_s = l == 1 && b.edge_in;
             // p0 is free here (its S-bar was consumed by layer l+1); the
             // recomputed T of layer l equals the stored one bit for bit.
-            // A layer that keeps S needs K1 alone.
-            if (b.recompute) {
+            // A layer that keeps S needs K1 alone; the first hidden layer
+            // of an edge forward keeps none.
+            if (b.recompute && !in_s) {
                 if (b.s_kept)
                     layer_linear<T>(nt, b, l, ncol, Np);
                 else if ((e = layer_forward<T>(nt, b, l, ncol, Np)) != hipSuccess)
@@ -690,37 +715,46 @@ static hipError_t backward_t(const Net &nt, BatchWork &b, double *grad, float *r
             }
             PScope p(b, CAT_BELL_BWD);
             const GenKernels *gk = gen_for<T>(nt);
-            if (b.s_kept)
-                e = gen_launch_bwds(nt.gen[sizeof(T) == 8 ? 1 : 0], b.p1, b.hplane(l), b.p0, nd, Np, nt.dt.act(), s);
-            else if (gk)
-                e = gen_launch_bwd(*gk, b.p1, b.hplane(l), b.p0, nd, Np, s);
-            else
-                launch_bell_bwd<T>((const T *)b.p1, (const T *)b.hplane(l), (T *)b.p0, nd, Np, pr, s);
+            if (b.s_kept) {
+                const T *W1 = W + nt.woff[1];
+                if (l == L - 2 && out_fused) {
+                    e = gen_launch_bwds_edge(gg, in_s, true, nullptr, b.hplane(l), tbp, nd, Np, nt.dt.act(), b.t0,
+                                             W1, sbp, W + nt.woff[L - 1], nout, s);
+                    std::swap(sbp, tbp);
+                } else {
+                    e = gen_launch_bwds_edge(gg, in_s, false, tbp, b.hplane(l), sbp, nd, Np, nt.dt.act(), b.t0, W1,
+                                             nullptr, nullptr, 0, s);
+                }
+            } else if (gk) {
+                e = gen_launch_bwd(*gk, tbp, b.hplane(l), sbp, nd, Np, s);
+            } else {
+                launch_bell_bwd<T>((const T *)tbp, (const T *)b.hplane(l), (T *)sbp, nd, Np, pr, s);
+            }
             if (e != hipSuccess) break;
         }
         const T *X = (const T *)(l == 1 ? b.t0 : b.tpl[l - 1]);
         if (rows_t) {
             PScope p(b, CAT_GRAD);
-            launch_point_rows<T>((const T *)b.p0, X, nd, ndm + 1, na, Np, rows_t + nt.woff[l] * ldr, ldr, s);
+            launch_point_rows<T>((const T *)sbp, X, nd, ndm + 1, na, Np, rows_t + nt.woff[l] * ldr, ldr, s);
         } else if (b.npt == 1) {
             // one point (the per-point ABI, the Kalman filter): one thread a
             // weight instead of the batched chunked product and its reduction
             PScope p(b, CAT_GRAD);
-            launch_grad_point1<T>((const T *)b.p0, X, nd, ndm + 1, na, Np, grad + nt.woff[l], s);
+            launch_grad_point1<T>((const T *)sbp, X, nd, ndm + 1, na, Np, grad + nt.woff[l], s);
         } else {
             int nc;
             long ch;
             grad_split(ncol, nd, ndm + 1, (int)b.esz, b.prec, &nc, &ch);
             {
                 PScope p(b, CAT_GRAD);
-                launch_gemm_grad<T>(b.prec, (const T *)b.p0, X, nd, ndm + 1, ncol, Np, b.partial, nc, ch, s);
+                launch_gemm_grad<T>(b.prec, (const T *)sbp, X, nd, ndm + 1, ncol, Np, b.partial, nc, ch, s);
             }
             PScope p(b, CAT_GRAD_REDUCE);
             launch_grad_reduce(b.partial, nc, (long)nd * (ndm + 1), grad + nt.woff[l], s);
         }
-        if (l > 1) {
+        if (l > 1 && !(l == L - 1 && out_fused)) {
             PScope p(b, CAT_LINEAR_T);
-            launch_gemm_wx<T>(b.prec, true, W + nt.woff[l], ndm + 1, (const T *)b.p0, (T *)b.p1, ndm, nd,
+            launch_gemm_wx<T>(b.prec, true, W + nt.woff[l], ndm + 1, (const T *)sbp, (T *)tbp, ndm, nd,
                               ncol, Np, s);
         }
     }
@@ -1956,6 +1990,13 @@ int dnnf90_set_point_kernels(int on)
 }
 
 int dnnf90_set_split_kernels(int on) { return set_lin_x6c(on ? 1 : 0); }
+
+int dnnf90_set_edge_kernels(int on)
+{
+    const int prev = g_edge;
+    g_edge = on ? 1 : 0;
+    return prev;
+}
 
 int dnnf90_set_store_s(int on)
 {

@@ -85,6 +85,14 @@ int dnnf90_set_point_kernels(int on);
  * every map (results identical bit for bit).  Returns the previous setting.
  * No counterpart in the reference. */
 int dnnf90_set_split_kernels(int on);
+/* with the generated kernels and S kept: the first hidden layer forms its
+ * S from the input plane and its weights in the Faa di Bruno kernels (no
+ * linear map, no S plane), and the last hidden layer forms its Tbar from
+ * the output layer's Sbar (no transposed map of the output layer);
+ * dnnf90_set_edge_kernels(0) runs those maps as separate kernels (results
+ * agree to rounding).  Returns the previous setting.  No counterpart in the
+ * reference. */
+int dnnf90_set_edge_kernels(int on);
 int dnnf90_net_kernel_info(int nid, int prec, int *info);
 
 int dnnf90_net_nweights(int nid);

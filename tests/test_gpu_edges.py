@@ -189,6 +189,17 @@ def test_profile_one_category(D):
     bw.forward(x, want=False)
     bw.profile(False)
     k = bw.kernel_ms()
+    # the first hidden layer forms its S inside the Faa di Bruno kernel (edge
+    # kernels): linear maps of the second hidden and the output layer only
+    assert k["linear"][1] == 2 and k["bell_fwd"][1] == 2
+    prev = lib().dnnf90_set_edge_kernels(0)
+    try:
+        bw.profile(True)
+        bw.forward(x, want=False)
+        bw.profile(False)
+        k = bw.kernel_ms()
+    finally:
+        lib().dnnf90_set_edge_kernels(prev)
     assert k["linear"][1] == 3 and k["bell_fwd"][1] == 2
     n = len(D.BatchWork.KERNEL_CATEGORIES)
     assert lib().dnnf90_bwork_profile(bw.id, 2 + n) == -1
@@ -261,3 +272,68 @@ def test_stale_dependents_refused(D):
                  (L.dnnf90_grad_free, gid), (L.dnnf90_bwork_free, bid)):
         assert f(h) == 0
     assert L.dnnf90_net_free(nid2) == 0
+
+
+EDGE_CASES = [("closure4", 4, 7, [[0, 0, 0, 1], [1, 0, 0, 0], [7, 0, 0, 0], [3, 2, 2, 0]], [4, 24, 24, 24, 1]),
+              ("one_hidden", 2, 4, SEEDS, [2, 40, 1]),
+              ("two_out", 3, 3, None, [3, 16, 16, 2])]
+
+
+@pytest.mark.parametrize("budget", [0, 1], ids=["keep", "recompute"])
+@pytest.mark.parametrize("prec", [0, 3], ids=["fp64", "bf16x6"])
+@pytest.mark.parametrize("case", EDGE_CASES, ids=[c[0] for c in EDGE_CASES])
+def test_edge_kernels(D, oracle_mod, case, prec, budget):
+    """dnnf90_set_edge_kernels: the first hidden layer forms S from the input
+    plane (no linear map, no S plane) and the last hidden layer forms Tbar
+    from the output layer's Sbar (no transposed map); one hidden layer takes
+    both at once, two outputs sum Tbar over both rows.  Forward and gradient
+    with the edge kernels on and off agree to rounding (1e-13 FP64, 1e-6
+    BF16x6) and the FP64 run matches the oracle to 1e-12; so do the
+    Gauss-Newton rows (the per-point adjoint takes the same edge kernels)."""
+    from dnnf90_amd._lib import lib
+    name, d0, k, seeds, dims = case
+    ts, net, w = _net(D, d0, k, seeds, 0, dims, seed=11)
+    rng = np.random.default_rng(12)
+    npt = 200
+    x = rng.uniform(-1, 1, (npt, d0))
+    seed = rng.standard_normal((npt, dims[-1], ts.na))
+    print(f"edge kernels: {name} dims={dims} prec={prec} budget={budget}", flush=True)
+    out = []
+    for on in (1, 0):
+        prev = lib().dnnf90_set_edge_kernels(on)
+        try:
+            bw = D.BatchWork(net, npt, prec, budget=budget)
+            g = D.Grad(net)
+            t = np.array(bw.forward(x), copy=True)
+            bw.backward(seed, g)
+            out.append((t, np.array(g.get(), copy=True)))
+            bw.free(); g.free()
+        finally:
+            lib().dnnf90_set_edge_kernels(prev)
+    (t1, g1), (t0, g0) = out
+    tol = 1e-13 if prec == 0 else 1e-6
+    assert rel(t1, t0) <= tol and rel(g1, g0) <= tol
+    if prec == 0:
+        o = oracle_mod.Oracle(d0, k, seeds)
+        assert rel(t1, o.eval(dims, w, x)) <= TOL64
+        assert rel(g1, o.grad(dims, w, x, seed)) <= TOL64
+    net.free()
+
+
+def test_edge_kernels_rows(D, oracle_mod):
+    """Gauss-Newton rows through the edge kernels against the oracle."""
+    import torch
+    ts, net, w = _net(D, 2, 4, SEEDS, 0, [2, 20, 20, 1], seed=13)
+    rng = np.random.default_rng(14)
+    npt = 100
+    x = rng.uniform(-1, 1, (npt, 2))
+    o = oracle_mod.Oracle(2, 4, SEEDS)
+    al = [tuple(a) for a in o.alpha()]
+    terms = [(1.0, al.index((4, 0)), False), (0.5, al.index((0, 2)), False), (2.0, al.index((1, 0)), True)]
+    bw = D.BatchWork(net, npt, D.PREC_FP64)
+    rows_t = torch.zeros((net.nw, 128), dtype=torch.float32, device="cuda")
+    bw.residual_rows(x, D.Residual(terms), rows_t, 128)
+    torch.cuda.synchronize()
+    rows = rows_t[:, :npt].double().cpu().numpy().T
+    assert rel(rows, o.pinn_rows([2, 20, 20, 1], w, x, terms)) <= 1e-6
+    bw.free(); net.free()

@@ -129,7 +129,9 @@ def test_generated_kernels_active(D, name):
 @pytest.mark.parametrize("name,npt", [("dense4_k3", 197), ("closure4_k7", 130), ("kdv", 100)])
 def test_store_s_identical(D, oracle_mod, name, npt, prec):
     """Keeping S and recomputing h in the adjoint gives the same forward
-    planes and gradient, bit for bit, as keeping h."""
+    planes and gradient, bit for bit, as keeping h (edge kernels off: they
+    need S kept, and their maps round differently; tests/test_gpu_edges.py
+    compares them)."""
     from dnnf90_amd._lib import lib
     from dnnf90_amd.workloads import WORKLOADS
     wl = WORKLOADS[name]
@@ -137,6 +139,7 @@ def test_store_s_identical(D, oracle_mod, name, npt, prec):
     seed = np.random.default_rng(7).standard_normal((npt, wl.dims[-1], o.na))
     out = []
     prev = lib().dnnf90_set_store_s(1)
+    prev_edge = lib().dnnf90_set_edge_kernels(0)
     try:
         for store_s in (1, 0):
             lib().dnnf90_set_store_s(store_s)
@@ -151,6 +154,7 @@ def test_store_s_identical(D, oracle_mod, name, npt, prec):
             g.free()
     finally:
         lib().dnnf90_set_store_s(prev)
+        lib().dnnf90_set_edge_kernels(prev_edge)
     (k1, t1, g1), (k0, t0, g0) = out
     print(name, "prec", prec, "kernels", k1, k0, "max |dT|", np.abs(t1 - t0).max(), "max |dg|", np.abs(g1 - g0).max())
     assert (k1, k0) == (2, 1)
