@@ -356,6 +356,61 @@ std::string gen_bell_source(const TableSet &t, const FwdSchedule &s, bool fp64)
         emit_pair_sums(o, t, fp64);
         o << "}\n";
     }
+    // ---- the first hidden layer's adjoint with its weight gradient.  T0 is
+    // x at slot 0 and 1 at slot e_i, so the layer's gradient needs Sbar at
+    // slot 0 and at the slots e_i alone: g(j, bias) = sum_n Sbar_0 and
+    // g(j, i) = sum_n (Sbar_0 x_i + Sbar_(e_i)).  The lanes of a workgroup
+    // (64 points) are summed in FP64 in point order; part[blockIdx.x][j][c]
+    // takes the sums and a fixed-order reduction adds the point blocks.  No
+    // Sbar plane is written.
+    for (int v = 1; v < 4; v += 2) {
+        const bool out_t = v & 2;
+        const int d0 = t.d0;
+        const std::string name = std::string("dnnf90_bell_bwds_i") + (out_t ? "o" : "") + "g";
+        o << "extern \"C\" __global__ __launch_bounds__(256) void " << name << sfx << "(" << bargs
+          << ", double *__restrict__ part)\n{\n"
+          << "  __shared__ double gred[256 * " << d0 + 1 << "];\n"
+          << "  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;\n"
+          << "  const int j0 = blockIdx.y * (blockDim.x >> 6) + wave;\n"
+          << "  const bool jok = j0 < nrow;\n"
+          << "  const int j = jok ? j0 : nrow - 1;   // every wave reaches the barrier\n"
+          << "  const long n = (long)blockIdx.x * 64 + lane;\n"
+          << "  const long ldr = " << na << "L * Np;\n";
+        load_s_in();
+        o << "  const real h0 = d1;\n";
+        for (int a = 1; a < na; ++a) o << "  real h" << a << ";\n";
+        emit_recurrence(o, t, s, fp64, REC_HREG);
+        if (out_t) {
+            load_tb_out();
+        } else {
+            o << "  const real *tb = Tb + (long)j * ldr + n;\n";
+            for (int a = 0; a < na; ++a) o << "  const real t" << a << " = tb[" << a << "L * Np];\n";
+        }
+        // Sbar at slot 0 and at the slots e_i, as emit_pair_sums forms them
+        o << "  real sbar0;\n  {\n    real v = t0 * h0;\n";
+        for (int a = 1; a < na; ++a) o << "    v += t" << a << " * h" << a << ";\n";
+        o << "    sbar0 = v;\n  }\n";
+        o << "  double *gr = gred + threadIdx.x * " << d0 + 1 << ";\n"
+          << "  gr[0] = jok ? (double)sbar0 : 0.0;\n";
+        for (int i = 0; i < d0; ++i) {
+            const int b = t.ind_e1[i];
+            o << "  {\n    double g = (double)sbar0 * (double)T0[" << i << "L * ldr + n];\n";
+            if (b >= 0) {
+                o << "    real v = (real)0;\n";
+                for (int ip = t.pair_start[b]; ip < t.pair_start[b] + t.pair_num[b]; ++ip)
+                    o << "    v += " << lit(t.pair_binom[ip], fp64) << " * t" << t.pair_ia[ip] << " * h"
+                      << t.pair_id[ip] << ";\n";
+                o << "    g += (double)v;\n";
+            }
+            o << "    gr[" << i + 1 << "] = jok ? g : 0.0;\n  }\n";
+        }
+        o << "  __syncthreads();\n"
+          << "  if (jok && lane < " << d0 + 1 << ") {\n"
+          << "    double acc = 0.0;\n"
+          << "    for (int p = 0; p < 64; ++p) acc += gred[(wave * 64 + p) * " << d0 + 1 << " + lane];\n"
+          << "    part[((long)blockIdx.x * nrow + j) * " << d0 + 1 << " + lane] = acc;\n"
+          << "  }\n}\n";
+    }
     if (fp64 && gen_point_family(t, s)) emit_point_kernels(o, t, s, load_s);
     return o.str();
 }
@@ -419,9 +474,10 @@ int gen_bell_load(const std::vector<char> &code, bool fp64, GenKernels *g)
     (void)hipFuncGetAttribute(&g->bwds_regs, HIP_FUNC_ATTRIBUTE_NUM_REGS, g->bwds);
     // the edge variants: used where they compile without scratch
     {
-        hipFunction_t *f[4] = {&g->fwdt_in, &g->bwds_i, &g->bwds_o, &g->bwds_io};
-        const char *nm[4] = {"dnnf90_bell_fwdt_in", "dnnf90_bell_bwds_i", "dnnf90_bell_bwds_o", "dnnf90_bell_bwds_io"};
-        for (int i = 0; i < 4; ++i) {
+        hipFunction_t *f[6] = {&g->fwdt_in, &g->bwds_i, &g->bwds_o, &g->bwds_io, &g->bwds_ig, &g->bwds_iog};
+        const char *nm[6] = {"dnnf90_bell_fwdt_in", "dnnf90_bell_bwds_i", "dnnf90_bell_bwds_o", "dnnf90_bell_bwds_io",
+                             "dnnf90_bell_bwds_ig", "dnnf90_bell_bwds_iog"};
+        for (int i = 0; i < 6; ++i) {
             int sc = 0;
             if (!get(f[i], nm[i]) ||
                 hipFuncGetAttribute(&sc, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, *f[i]) != hipSuccess || sc != 0)
@@ -449,7 +505,7 @@ void GenKernels::release()
     if (mod) (void)hipModuleUnload(mod);
     mod = nullptr;
     fwd = bwd = fwdt = bwds = pt_fwd = pt_bwd = nullptr;
-    fwdt_in = bwds_i = bwds_o = bwds_io = nullptr;
+    fwdt_in = bwds_i = bwds_o = bwds_io = bwds_ig = bwds_iog = nullptr;
     ok = false;
 }
 
@@ -506,6 +562,20 @@ hipError_t gen_launch_bwds_edge(const GenKernels &g, bool in_s, bool out_t, cons
     ActParams a = ap;
     void *params[] = {&Tb, &S, &Sb, &nrow, &Np, &a, &T0, &W1, &SbL, &WL, &nout};
     if (!in_s && !out_t) return gen_launch_bwds(g, Tb, S, Sb, nrow, Np, ap, s);
+    return hipModuleLaunchKernel(f, (unsigned)(Np / 64), (unsigned)((nrow + 3) / 4), 1, 256, 1, 1, 0, s, params,
+                                 nullptr);
+}
+
+hipError_t gen_launch_bwds_grad1(const GenKernels &g, bool out_t, const void *Tb, int nrow, long Np,
+                                 const ActParams &ap, const void *T0, const void *W1, const void *SbL, const void *WL,
+                                 int nout, double *part, hipStream_t s)
+{
+    hipFunction_t f = out_t ? g.bwds_iog : g.bwds_ig;
+    if (!f) return hipErrorInvalidDeviceFunction;
+    ActParams a = ap;
+    const void *S = nullptr;
+    void *Sb = nullptr;
+    void *params[] = {&Tb, &S, &Sb, &nrow, &Np, &a, &T0, &W1, &SbL, &WL, &nout, &part};
     return hipModuleLaunchKernel(f, (unsigned)(Np / 64), (unsigned)((nrow + 3) / 4), 1, 256, 1, 1, 0, s, params,
                                  nullptr);
 }

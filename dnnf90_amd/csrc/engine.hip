@@ -508,6 +508,12 @@ static int bwork_init(BatchWork &b, const Net &nt, int maxpt, int prec, double b
         grad_split((long)col, nt.dims[l], nt.dims[l - 1] + 1, (int)b.esz, prec, &nc, &ch, true);
         pc = std::max(pc, (long)(nc + grad_reduce_splits(nc)) * nt.dims[l] * (nt.dims[l - 1] + 1));
     }
+    if (L >= 3) {
+        // the first hidden layer's gradient from its adjoint kernel: one
+        // partial a 64-point block
+        const int nc = (int)(b.np_cap / 64);
+        pc = std::max(pc, (long)(nc + grad_reduce_splits(nc)) * nt.dims[1] * (nt.dims[0] + 1));
+    }
     const int nfwd = bell_fwd_threads(nt.sched.nentry, (int)b.esz);
     const double fixed = plane(nt.dims[0]) + plane(nt.dims[L - 1]) + 2 * plane(nmax) + 8.0 * pc +
                          (nfwd == 0 ? (double)nt.sched.nentry * nmax * b.np_cap * b.esz : 0.0) +
@@ -713,11 +719,27 @@ static hipError_t backward_t(const Net &nt, BatchWork &b, double *grad, float *r
                 else if ((e = layer_forward<T>(nt, b, l, ncol, Np)) != hipSuccess)
                     break;
             }
-            PScope p(b, CAT_BELL_BWD);
             const GenKernels *gk = gen_for<T>(nt);
+            const bool out_t = l == L - 2 && out_fused;
+            if (in_s && !rows_t && b.npt > 1 && (out_t ? gg.bwds_iog : gg.bwds_ig)) {
+                // the first hidden layer's adjoint forms the layer's weight
+                // gradient itself: no Sbar plane, no K5 of this layer
+                const int nc = (int)(Np / 64);
+                {
+                    PScope p(b, CAT_BELL_BWD);
+                    e = gen_launch_bwds_grad1(gg, out_t, tbp, nd, Np, nt.dt.act(), b.t0, W + nt.woff[1], sbp,
+                                              W + nt.woff[L - 1], nout, b.partial, s);
+                }
+                if (e == hipSuccess) {
+                    PScope p(b, CAT_GRAD_REDUCE);
+                    launch_grad_reduce(b.partial, nc, (long)nd * (ndm + 1), grad + nt.woff[l], s);
+                }
+                break;   // the first hidden layer is the last one reversed
+            }
+            PScope p(b, CAT_BELL_BWD);
             if (b.s_kept) {
                 const T *W1 = W + nt.woff[1];
-                if (l == L - 2 && out_fused) {
+                if (out_t) {
                     e = gen_launch_bwds_edge(gg, in_s, true, nullptr, b.hplane(l), tbp, nd, Np, nt.dt.act(), b.t0,
                                              W1, sbp, W + nt.woff[L - 1], nout, s);
                     std::swap(sbp, tbp);
