@@ -47,11 +47,13 @@ static int kq_of(int k)
 // Bell sums v of each q (terms in the reference's order, stored entries kept
 // as registers), T_a = d1 S_a + sum_q d_q v and h_a = d2 S_a + sum_q d_(q+1) v.
 // REC_T stores T_a, REC_H stores h_a, REC_HREG assigns h_a to register h<a>.
-enum { REC_T = 1, REC_H = 2, REC_HREG = 4 };
+// REC_TDOT adds T_a * Sbar_L(0, a) (the single output row, through slz)
+// into gdot.
+enum { REC_T = 1, REC_H = 2, REC_HREG = 4, REC_TDOT = 8 };
 
 static void emit_recurrence(std::ostringstream &o, const TableSet &t, const FwdSchedule &s, bool fp64, int m)
 {
-    const bool wt = (m & REC_T) != 0, wh = (m & (REC_H | REC_HREG)) != 0;
+    const bool wt = (m & (REC_T | REC_TDOT)) != 0, wh = (m & (REC_H | REC_HREG)) != 0;
     for (int e = t.na; e < s.nentry; ++e) o << "  real e" << e << ";\n";
     for (int a = 1; a < t.na; ++a) {
         const int g0 = s.slot_group[a], g1 = s.slot_group[a + 1];
@@ -72,6 +74,7 @@ static void emit_recurrence(std::ostringstream &o, const TableSet &t, const FwdS
         if (m & REC_T) o << "    Tj[" << a << "L * Np] = t;\n";
         if (m & REC_H) o << "    Hj[" << a << "L * Np] = h;\n";
         if (m & REC_HREG) o << "    h" << a << " = h;\n";
+        if (m & REC_TDOT) o << "    gdot += t * slz[" << a << "L * Np];\n";
         o << "  }\n";
     }
 }
@@ -411,6 +414,49 @@ std::string gen_bell_source(const TableSet &t, const FwdSchedule &s, bool fp64)
           << "    part[((long)blockIdx.x * nrow + j) * " << d0 + 1 << " + lane] = acc;\n"
           << "  }\n}\n";
     }
+    // ---- the last hidden layer's adjoint (Tbar from the single output
+    // row) with the output layer's weight gradient: T of the layer is
+    // recomputed beside h from the same sums, g(0, 1 + j) = sum_n sum_a
+    // Sbar_L(0, a) T_a(j) and the bias sum_n Sbar_L(0, 0), per 64-point block
+    // in part[blockIdx.x][c] (c = 0 bias, 1 + j weights), summed in FP64 in
+    // point order; the Sbar plane of the layer is written as bwds_o does
+    {
+        o << "extern \"C\" __global__ __launch_bounds__(256) void dnnf90_bell_bwds_og" << sfx << "(" << bargs
+          << ", double *__restrict__ part)\n{\n"
+          << "  __shared__ double gred[256 * 2];\n"
+          << "  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;\n"
+          << "  const int j0 = blockIdx.y * (blockDim.x >> 6) + wave;\n"
+          << "  const bool jok = j0 < nrow;\n"
+          << "  const int j = jok ? j0 : nrow - 1;   // every wave reaches the barrier\n"
+          << "  const long n = (long)blockIdx.x * 64 + lane;\n"
+          << "  const long ldr = " << na << "L * Np;\n"
+          << "  real *sb = Sb + (long)j * ldr + n;\n"
+          << "  const real *slz = SbL + n;   // the output row (nout is 1)\n"
+          << "  const real *Sj = S + (long)j * ldr + n;\n";
+        load_s();
+        o << "  const real sl00 = slz[0];\n"
+          << "  real gdot = d0 * sl00;\n"
+          << "  const real h0 = d1;\n";
+        for (int a = 1; a < na; ++a) o << "  real h" << a << ";\n";
+        emit_recurrence(o, t, s, fp64, REC_HREG | REC_TDOT);
+        load_tb_out();
+        o << "  if (jok) {\n";
+        emit_pair_sums(o, t, fp64);
+        o << "  }\n"
+          << "  gred[threadIdx.x * 2] = jok ? (double)gdot : 0.0;\n"
+          << "  gred[threadIdx.x * 2 + 1] = (double)sl00;\n"
+          << "  __syncthreads();\n"
+          << "  if (lane == 0 && jok) {\n"
+          << "    double acc = 0.0;\n"
+          << "    for (int p = 0; p < 64; ++p) acc += gred[(wave * 64 + p) * 2];\n"
+          << "    part[(long)blockIdx.x * (nrow + 1) + 1 + j] = acc;\n"
+          << "  }\n"
+          << "  if (lane == 1 && wave == 0 && blockIdx.y == 0) {\n"
+          << "    double acc = 0.0;\n"
+          << "    for (int p = 0; p < 64; ++p) acc += gred[p * 2 + 1];\n"
+          << "    part[(long)blockIdx.x * (nrow + 1)] = acc;\n"
+          << "  }\n}\n";
+    }
     if (fp64 && gen_point_family(t, s)) emit_point_kernels(o, t, s, load_s);
     return o.str();
 }
@@ -474,15 +520,24 @@ int gen_bell_load(const std::vector<char> &code, bool fp64, GenKernels *g)
     (void)hipFuncGetAttribute(&g->bwds_regs, HIP_FUNC_ATTRIBUTE_NUM_REGS, g->bwds);
     // the edge variants: used where they compile without scratch
     {
-        hipFunction_t *f[6] = {&g->fwdt_in, &g->bwds_i, &g->bwds_o, &g->bwds_io, &g->bwds_ig, &g->bwds_iog};
-        const char *nm[6] = {"dnnf90_bell_fwdt_in", "dnnf90_bell_bwds_i", "dnnf90_bell_bwds_o", "dnnf90_bell_bwds_io",
-                             "dnnf90_bell_bwds_ig", "dnnf90_bell_bwds_iog"};
-        for (int i = 0; i < 6; ++i) {
+        hipFunction_t *f[7] = {&g->fwdt_in, &g->bwds_i,  &g->bwds_o, &g->bwds_io,
+                               &g->bwds_ig,  &g->bwds_iog, &g->bwds_og};
+        const char *nm[7] = {"dnnf90_bell_fwdt_in", "dnnf90_bell_bwds_i",   "dnnf90_bell_bwds_o", "dnnf90_bell_bwds_io",
+                             "dnnf90_bell_bwds_ig", "dnnf90_bell_bwds_iog", "dnnf90_bell_bwds_og"};
+        for (int i = 0; i < 7; ++i) {
             int sc = 0;
             if (!get(f[i], nm[i]) ||
                 hipFuncGetAttribute(&sc, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, *f[i]) != hipSuccess || sc != 0)
                 *f[i] = nullptr;
         }
+        // the output-gradient variant holds T beside h: where that leaves one
+        // wave a SIMD (the registers of large families spill into the
+        // accumulation registers) it streams slower than the separate
+        // gradient kernel, so it is kept only at two resident workgroups a CU
+        int occ = 0;
+        if (g->bwds_og && (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ, g->bwds_og, 256, 0) != hipSuccess ||
+                           occ < 2))
+            g->bwds_og = nullptr;
         (void)hipGetLastError();
     }
     // the one-point kernels are in the FP64 modules of small families only
@@ -505,7 +560,7 @@ void GenKernels::release()
     if (mod) (void)hipModuleUnload(mod);
     mod = nullptr;
     fwd = bwd = fwdt = bwds = pt_fwd = pt_bwd = nullptr;
-    fwdt_in = bwds_i = bwds_o = bwds_io = bwds_ig = bwds_iog = nullptr;
+    fwdt_in = bwds_i = bwds_o = bwds_io = bwds_ig = bwds_iog = bwds_og = nullptr;
     ok = false;
 }
 
@@ -577,6 +632,18 @@ hipError_t gen_launch_bwds_grad1(const GenKernels &g, bool out_t, const void *Tb
     void *Sb = nullptr;
     void *params[] = {&Tb, &S, &Sb, &nrow, &Np, &a, &T0, &W1, &SbL, &WL, &nout, &part};
     return hipModuleLaunchKernel(f, (unsigned)(Np / 64), (unsigned)((nrow + 3) / 4), 1, 256, 1, 1, 0, s, params,
+                                 nullptr);
+}
+
+hipError_t gen_launch_bwds_gradout(const GenKernels &g, const void *S, void *Sb, int nrow, long Np, const ActParams &ap,
+                                   const void *SbL, const void *WL, double *part, hipStream_t s)
+{
+    if (!g.bwds_og) return hipErrorInvalidDeviceFunction;
+    ActParams a = ap;
+    const void *Tb = nullptr, *T0 = nullptr, *W1 = nullptr;
+    int nout = 1;
+    void *params[] = {&Tb, &S, &Sb, &nrow, &Np, &a, &T0, &W1, &SbL, &WL, &nout, &part};
+    return hipModuleLaunchKernel(g.bwds_og, (unsigned)(Np / 64), (unsigned)((nrow + 3) / 4), 1, 256, 1, 1, 0, s, params,
                                  nullptr);
 }
 

@@ -43,6 +43,9 @@ struct GenKernels {
     // the first hidden layer's adjoint forming that layer's weight gradient
     // (per point block) instead of its Sbar plane; iog with bwds_o's Tbar
     hipFunction_t bwds_ig = nullptr, bwds_iog = nullptr;
+    // the last hidden layer's adjoint (one output row) with the output
+    // layer's weight gradient
+    hipFunction_t bwds_og = nullptr;
     std::vector<char> code;
     int fwd_regs = 0, bwd_regs = 0, fwd_scratch = 0, bwd_scratch = 0;
     int fwdt_regs = 0, bwds_regs = 0, fwdt_scratch = 0, bwds_scratch = 0;
@@ -84,6 +87,11 @@ hipError_t gen_launch_bwds_edge(const GenKernels &g, bool in_s, bool out_t, cons
 hipError_t gen_launch_bwds_grad1(const GenKernels &g, bool out_t, const void *Tb, int nrow, long Np,
                                  const ActParams &ap, const void *T0, const void *W1, const void *SbL, const void *WL,
                                  int nout, double *part, hipStream_t s);
+// the last hidden layer's adjoint from the single output row SbL (weights
+// WL), writing Sb, with the output layer's gradient: part gets
+// [Np/64][nrow + 1] point-block sums (bias first)
+hipError_t gen_launch_bwds_gradout(const GenKernels &g, const void *S, void *Sb, int nrow, long Np, const ActParams &ap,
+                                   const void *SbL, const void *WL, double *part, hipStream_t s);
 hipError_t gen_launch_pt_fwd(const GenKernels &g, const double *W, const double *x, const int *ind_e1, const PtNet &p,
                              const ActParams &ap, hipStream_t s);
 hipError_t gen_launch_pt_bwd(const GenKernels &g, const double *W, double *grad, const double *Sb, const PtNet &p,

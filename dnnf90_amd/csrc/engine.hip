@@ -705,6 +705,9 @@ static hipError_t backward_t(const Net &nt, BatchWork &b, double *grad, float *r
     const GenKernels &gg = nt.gen[sizeof(T) == 8 ? 1 : 0];
     const int nout = nt.dims[L - 1];
     const bool out_fused = b.s_kept && g_edge && L >= 3 && ((L == 3 && b.edge_in) ? gg.bwds_io : gg.bwds_o);
+    // one output row: the last hidden layer's adjoint also forms the output
+    // layer's weight gradient (T of the layer recomputed beside h)
+    const bool out_grad = out_fused && nout == 1 && gg.bwds_og && !rows_t && b.npt > 1 && !(L == 3 && b.edge_in);
     for (int l = L - 1; l >= 1 && e == hipSuccess; --l) {
         const int nd = nt.dims[l], ndm = nt.dims[l - 1];
         if (l < L - 1) {
@@ -736,26 +739,42 @@ static hipError_t backward_t(const Net &nt, BatchWork &b, double *grad, float *r
                 }
                 break;   // the first hidden layer is the last one reversed
             }
-            PScope p(b, CAT_BELL_BWD);
-            if (b.s_kept) {
-                const T *W1 = W + nt.woff[1];
-                if (out_t) {
-                    e = gen_launch_bwds_edge(gg, in_s, true, nullptr, b.hplane(l), tbp, nd, Np, nt.dt.act(), b.t0,
-                                             W1, sbp, W + nt.woff[L - 1], nout, s);
-                    std::swap(sbp, tbp);
-                } else {
-                    e = gen_launch_bwds_edge(gg, in_s, false, tbp, b.hplane(l), sbp, nd, Np, nt.dt.act(), b.t0, W1,
-                                             nullptr, nullptr, 0, s);
+            if (out_t && out_grad) {
+                {
+                    PScope p(b, CAT_BELL_BWD);
+                    e = gen_launch_bwds_gradout(gg, b.hplane(l), tbp, nd, Np, nt.dt.act(), sbp, W + nt.woff[L - 1],
+                                                b.partial, s);
                 }
-            } else if (gk) {
-                e = gen_launch_bwd(*gk, tbp, b.hplane(l), sbp, nd, Np, s);
+                if (e != hipSuccess) break;
+                {
+                    PScope p(b, CAT_GRAD_REDUCE);
+                    launch_grad_reduce(b.partial, (int)(Np / 64), (long)nd + 1, grad + nt.woff[L - 1], s);
+                }
+                std::swap(sbp, tbp);
             } else {
-                launch_bell_bwd<T>((const T *)tbp, (const T *)b.hplane(l), (T *)sbp, nd, Np, pr, s);
+                PScope p(b, CAT_BELL_BWD);
+                if (b.s_kept) {
+                    const T *W1 = W + nt.woff[1];
+                    if (out_t) {
+                        e = gen_launch_bwds_edge(gg, in_s, true, nullptr, b.hplane(l), tbp, nd, Np, nt.dt.act(), b.t0,
+                                                 W1, sbp, W + nt.woff[L - 1], nout, s);
+                        std::swap(sbp, tbp);
+                    } else {
+                        e = gen_launch_bwds_edge(gg, in_s, false, tbp, b.hplane(l), sbp, nd, Np, nt.dt.act(), b.t0,
+                                                 W1, nullptr, nullptr, 0, s);
+                    }
+                } else if (gk) {
+                    e = gen_launch_bwd(*gk, tbp, b.hplane(l), sbp, nd, Np, s);
+                } else {
+                    launch_bell_bwd<T>((const T *)tbp, (const T *)b.hplane(l), (T *)sbp, nd, Np, pr, s);
+                }
             }
             if (e != hipSuccess) break;
         }
         const T *X = (const T *)(l == 1 ? b.t0 : b.tpl[l - 1]);
-        if (rows_t) {
+        if (l == L - 1 && out_grad) {
+            // the output layer's gradient comes with the next layer's adjoint
+        } else if (rows_t) {
             PScope p(b, CAT_GRAD);
             launch_point_rows<T>((const T *)sbp, X, nd, ndm + 1, na, Np, rows_t + nt.woff[l] * ldr, ldr, s);
         } else if (b.npt == 1) {
