@@ -68,29 +68,62 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
-FORTRAN_EXAMPLE = os.path.join(HERE, "fortran", "example_fd")
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+# the Fortran modules with the reference's names, in dependency order
+FORTRAN_MODULES = ["dnnf90_capi.f90", "multi_index_bell_module.f90", "net_module.f90", "train_module.f90"]
+FORTRAN_LIB = os.path.join(HERE, "_build", "libdnnf90_fortran.a")
+# reference host programs compiled unchanged against those modules (from
+# /root/reference/tools, where that tree is present); the binaries go to
+# fortran/_ref/ and are never committed
+REF_TOOLS = ["dbg_backward_fd", "example_multiout", "example_multiout_adj", "example_batch_act",
+             "example_act_roundtrip"]
+REF_TOOL_DIR = os.path.join(FORTRAN_DIR, "_ref")
+REFERENCE = os.environ.get("DNNF90_REFERENCE", "/root/reference")
 
 
 def build_fortran(verbose: bool = False) -> str | None:
-    """The Fortran binding (fortran/dnnf90_gpu.f90) and its example program,
-    compiled with amdflang and linked against the library."""
+    """The Fortran modules multi_index_bell_module, net_module and
+    train_module (over the C ABI, fortran/*.f90), compiled with amdflang into
+    _build/ (objects, .mod files, libdnnf90_fortran.a); then, where the
+    reference tree exists, its tools/*.f90 host programs listed in REF_TOOLS,
+    compiled as they are against those modules and linked with the library."""
     fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
     if not os.path.exists(fc):
         return None
     bdir = os.path.join(HERE, "_build")
     os.makedirs(bdir, exist_ok=True)
-    fdir = os.path.join(HERE, "fortran")
-    cmds = [[fc, "-O2", "-module-dir", bdir, "-c", os.path.join(fdir, "dnnf90_gpu.f90"),
-             "-o", os.path.join(bdir, "dnnf90_gpu.o")],
-            [fc, "-O2", "-I", bdir, os.path.join(fdir, "example_fd.f90"), os.path.join(bdir, "dnnf90_gpu.o"),
-             "-L", HERE, "-ldnnf90_amd", "-Wl,-rpath,$ORIGIN/..", "-o", FORTRAN_EXAMPLE]]
-    for cmd in cmds:
+    objs = []
+    for src in FORTRAN_MODULES:
+        obj = os.path.join(bdir, os.path.splitext(src)[0] + ".o")
+        cmd = [fc, "-O2", "-fPIC", "-module-dir", bdir, "-I", bdir, "-c", os.path.join(FORTRAN_DIR, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
-            raise RuntimeError("amdflang failed:\n" + r.stdout + r.stderr)
-    return FORTRAN_EXAMPLE
+            raise RuntimeError(f"amdflang failed on {src}:\n" + r.stdout + r.stderr)
+        objs.append(obj)
+    if os.path.exists(FORTRAN_LIB):
+        os.remove(FORTRAN_LIB)
+    r = subprocess.run(["ar", "rcs", FORTRAN_LIB] + objs, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("ar failed:\n" + r.stdout + r.stderr)
+    tools = os.path.join(REFERENCE, "tools")
+    if not os.path.isdir(tools):
+        return FORTRAN_LIB
+    os.makedirs(REF_TOOL_DIR, exist_ok=True)
+    for name in REF_TOOLS:
+        src = os.path.join(tools, name + ".f90")
+        if not os.path.exists(src):
+            continue
+        out = os.path.join(REF_TOOL_DIR, name)
+        cmd = [fc, "-O2", "-I", bdir, src, FORTRAN_LIB, "-L", HERE, "-ldnnf90_amd",
+               "-Wl,-rpath,$ORIGIN/../..", "-o", out]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True, cwd=bdir)
+        if r.returncode != 0:
+            raise RuntimeError(f"amdflang failed on the reference's {name}.f90:\n" + r.stdout + r.stderr)
+    return FORTRAN_LIB
 
 
 if __name__ == "__main__":

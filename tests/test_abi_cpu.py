@@ -216,3 +216,22 @@ def test_switches_return_previous_setting(L):
     ms = np.zeros(16)
     cnt = np.zeros(16, np.int64)
     assert L.dnnf90_bwork_kernel_ms(63, ms.ctypes.data, cnt.ctypes.data) == -1
+
+
+def test_fortran_modules_built():
+    """The reference-named Fortran modules compile (amdflang) into the
+    archive hosts link, with their .mod files; where the reference tree is
+    present its host programs link against them unchanged."""
+    import shutil
+    from dnnf90_amd import build as b
+    if not (shutil.which("amdflang") or os.path.exists("/opt/rocm/bin/amdflang")):
+        pytest.skip("amdflang not installed")
+    if not os.path.exists(b.FORTRAN_LIB):
+        b.build_fortran()
+    assert os.path.exists(b.FORTRAN_LIB)
+    bdir = os.path.dirname(b.FORTRAN_LIB)
+    for m in ("multi_index_bell_module", "net_module", "train_module", "dnnf90_capi"):
+        assert os.path.exists(os.path.join(bdir, m + ".mod")), m
+    if os.path.isdir(os.path.join(b.REFERENCE, "tools")):
+        for t in b.REF_TOOLS:
+            assert os.path.exists(os.path.join(b.REF_TOOL_DIR, t)), t
