@@ -77,6 +77,8 @@ FORTRAN_LIB = os.path.join(HERE, "_build", "libdnnf90_fortran.a")
 # fortran/_ref/ and are never committed
 REF_TOOLS = ["dbg_backward_fd", "example_multiout", "example_multiout_adj", "example_batch_act",
              "example_act_roundtrip"]
+# ...and the stock trainer (app/main.f90) with fortran/trainer/lib_net_module.f90
+# as its network bridge: REF_TOOL_DIR/serial_gpu
 REF_TOOL_DIR = os.path.join(FORTRAN_DIR, "_ref")
 REFERENCE = os.environ.get("DNNF90_REFERENCE", "/root/reference")
 
@@ -123,6 +125,12 @@ def build_fortran(verbose: bool = False) -> str | None:
         r = subprocess.run(cmd, capture_output=True, text=True, cwd=bdir)
         if r.returncode != 0:
             raise RuntimeError(f"amdflang failed on the reference's {name}.f90:\n" + r.stdout + r.stderr)
+    # the stock trainer with the GPU bridge (fortran/trainer/Makefile)
+    if os.path.isdir(os.path.join(REFERENCE, "app")):
+        r = subprocess.run(["make", "-s", "-B", "-C", os.path.join(FORTRAN_DIR, "trainer"), "gpu",
+                            f"REF={REFERENCE}"], capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("trainer bridge build failed:\n" + r.stdout[-4000:] + r.stderr[-4000:])
     return FORTRAN_LIB
 
 

@@ -185,6 +185,47 @@ MODULE dnnf90_capi
        real(c_double), intent(in) :: x(*), y(*)
        real(c_double), value :: coef
      end function
+     integer(c_int) function dnnf90_kf_create(nid, p0, lam_ini, lam0, decoupled) bind(C)
+       import :: c_int, c_double
+       integer(c_int), value :: nid, decoupled
+       real(c_double), value :: p0, lam_ini, lam0
+     end function
+     integer(c_int) function dnnf90_kf_set(kid, gate, q) bind(C)
+       import :: c_int, c_double
+       integer(c_int), value :: kid
+       real(c_double), value :: gate, q
+     end function
+     integer(c_int) function dnnf90_kf_free(kid) bind(C)
+       import :: c_int
+       integer(c_int), value :: kid
+     end function
+     integer(c_int) function dnnf90_kf_update_obs(nid, kid, bid, nobs, x, seed, target, pred, flags) bind(C)
+       import :: c_int, c_double, c_ptr
+       integer(c_int), value :: nid, kid, bid, nobs, flags
+       real(c_double), intent(in) :: x(*), seed(*), target(*)
+       type(c_ptr), value :: pred
+     end function
+     integer(c_int) function dnnf90_kf_update_resid(nid, kid, bid, nobs, x, seed, resid, rnoise, flags) bind(C)
+       import :: c_int, c_double
+       integer(c_int), value :: nid, kid, bid, nobs, flags
+       real(c_double), intent(in) :: x(*), seed(*), resid(*), rnoise(*)
+     end function
+     integer(c_int) function dnnf90_kf_iekf_begin(nid, kid) bind(C)
+       import :: c_int
+       integer(c_int), value :: nid, kid
+     end function
+     integer(c_int) function dnnf90_kf_iekf_iter(nid, kid, bid, x, seed, resid, rnoise, final, flags) bind(C)
+       import :: c_int, c_double
+       integer(c_int), value :: nid, kid, bid, final, flags
+       real(c_double), intent(in) :: x(*), seed(*)
+       real(c_double), value :: resid, rnoise
+     end function
+     integer(c_int) function dnnf90_kf_get(kid, P, info) bind(C)
+       import :: c_int, c_ptr, c_double
+       integer(c_int), value :: kid
+       type(c_ptr), value :: P
+       real(c_double), intent(out) :: info(*)
+     end function
      type(c_ptr) function dnnf90_last_error() bind(C)
        import :: c_ptr
      end function

@@ -41,6 +41,9 @@ def build(ref: bool | None = None) -> None:
     if ref:
         subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True,
                        stdout=subprocess.DEVNULL)
+        if os.path.isdir("/root/reference/app"):
+            subprocess.run(["make", "-s", "-C", HERE, "trainer"], check=True,
+                           stdout=subprocess.DEVNULL)
 
 
 def _ip(a):
