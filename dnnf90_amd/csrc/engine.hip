@@ -1721,8 +1721,20 @@ int dnnf90_adam_step(int nid, int gid, double eta, double beta1, double beta2, d
         launch_fill(g.d_m, g.nw, 0.0, s);
         launch_fill(g.d_v, g.nw, 0.0, s);
     }
+    // beta**istep as Fortran evaluates a real to an integer power: binary
+    // exponentiation, squaring the base and multiplying in the set bits
+    auto powi = [](double a, int b) {
+        double r = 1.0;
+        while (true) {
+            if (b & 1) r *= a;
+            b /= 2;
+            if (b == 0) break;
+            a *= a;
+        }
+        return r;
+    };
     const int is = std::max(istep, 1);
-    const double bc1 = 1.0 - std::pow(beta1, is), bc2 = 1.0 - std::pow(beta2, is);
+    const double bc1 = 1.0 - powi(beta1, is), bc2 = 1.0 - powi(beta2, is);
     launch_adam(nt->d_w64, g.d_m, g.d_v, g.d_g, g.nw, eta, beta1, beta2, eps, (double)nbatch, bc1, bc2, s);
     if ((rc = g.after(s))) return rc;
     std::lock_guard<std::mutex> lk(nt->mu);

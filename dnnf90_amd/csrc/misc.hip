@@ -192,6 +192,9 @@ __global__ void k_adam(double *__restrict__ w, double *__restrict__ m, double *_
                        const double *__restrict__ g, long n, double eta, double b1, double b2,
                        double eps, double nbatch, double bc1, double bc2)
 {
+    // the reference's arithmetic, operation for operation, unfused (its
+    // build multiplies and adds separately): the same bits
+#pragma clang fp contract(off)
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double gv = g[i] / nbatch;

@@ -365,6 +365,20 @@ class RefLib:
         return cls._lib
 
 
+def ref_adam(dims, w, g, eta, beta1, beta2, eps, nbatch, istep0, nsteps):
+    """nsteps of the reference's opt_adam_step with the fixed gradient g,
+    through ref_driver.f90's refx_adam; returns the weights."""
+    L = RefLib.lib()
+    if not hasattr(L, "_refx_adam_ready"):
+        L.refx_adam.argtypes = [_i, _pi, _pd, _pd, _d, _d, _d, _d, _i, _i, _i]
+        L.refx_adam.restype = _i
+        L._refx_adam_ready = True
+    dims, w, g = _ia(dims), _da(np.array(w, copy=True)), _da(g)
+    rc = L.refx_adam(len(dims), _ip(dims), _dp(w), _dp(g), eta, beta1, beta2, eps, nbatch, istep0, nsteps)
+    assert rc == 0
+    return w
+
+
 class RefNet:
     """One network on the compiled reference, driven through ref_driver.f90."""
 

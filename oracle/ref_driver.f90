@@ -14,7 +14,7 @@ MODULE ref_driver
   use multi_index_bell_module, only: tabset_t, tabset_init, tabset_free
   use net_module, only: net_t, net_init, net_free
   use train_module, only: twork_t, grad_t, twork_init, twork_free, grad_init, &
-       grad_free, net_forward_point_multi, net_backward_point_multi
+       grad_free, net_forward_point_multi, net_backward_point_multi, opt_adam_step
   use kalman_module, only: kalman_t, kf_init, kf_free, kf_update_obs, kf_update_resid, &
        kf_update_resid_multi, kf_iekf_begin, kf_iekf_iter
   use committee_module, only: committee_t, comm_init, comm_free, comm_eval
@@ -323,5 +323,45 @@ CONTAINS
     call grad_free( g );  call twork_free( tw );  call net_free( nt );  call tabset_free( ts )
     refx_pinn_step = 0
   END FUNCTION refx_pinn_step
+
+  !> nsteps Adam steps of the reference's opt_adam_step
+  !! (train_module.f90:656-683) with the fixed gradient g(nw), steps istep0,
+  !! istep0+1, ...; w(nw) in and out (flat, layer blocks, bias first).
+  INTEGER(c_int) FUNCTION refx_adam( nlayer, dims, w, g, eta, beta1, beta2, eps, &
+                                     nbatch, istep0, nsteps ) bind(C)
+    integer(c_int),value :: nlayer, nbatch, istep0, nsteps
+    integer(c_int),intent(IN) :: dims(*)
+    real(c_double),intent(INOUT) :: w(*)
+    real(c_double),intent(IN) :: g(*)
+    real(c_double),value :: eta, beta1, beta2, eps
+    type(tabset_t) :: ts
+    type(net_t) :: nt
+    type(grad_t) :: gr
+    integer :: s, l, j, p
+    integer(c_int) :: dummy(1)
+    dummy = 0
+    call build_net( dims(1), 1_c_int, 0_c_int, dummy, 0_c_int, nlayer, dims, w, ts, nt )
+    call grad_init( gr, nt )
+    gr%nabla = 0.d0
+    p = 0
+    do l = 2, nlayer
+       do j = 1, dims(l)
+          gr%nabla(l,j,0:dims(l-1)) = g(p+1:p+dims(l-1)+1)
+          p = p + dims(l-1) + 1
+       end do
+    end do
+    do s = 0, nsteps - 1
+       call opt_adam_step( nt, gr, eta, beta1, beta2, eps, int(nbatch), int(istep0 + s) )
+    end do
+    p = 0
+    do l = 2, nlayer
+       do j = 1, dims(l)
+          w(p+1:p+dims(l-1)+1) = nt%w(l,j,0:dims(l-1))
+          p = p + dims(l-1) + 1
+       end do
+    end do
+    call grad_free( gr );  call net_free( nt );  call tabset_free( ts )
+    refx_adam = 0
+  END FUNCTION refx_adam
 
 END MODULE ref_driver

@@ -233,6 +233,34 @@ def test_per_point_abi_and_adam(D, oracle_mod):
     assert np.abs(net.weights - ww).max() <= 1e-15
 
 
+def test_adam_vs_compiled_reference(D, oracle_mod):
+    """dnnf90_adam_step on the GPU against the reference's own opt_adam_step
+    (lib/train_module.f90:656-683, compiled from its sources into
+    oracle/_ref and driven by ref_driver.f90's refx_adam) on the same
+    weights and the same fixed gradient, over seven steps from istep 1 and
+    from istep 40: the same bits."""
+    if not oracle_mod.RefLib.available():
+        pytest.skip("oracle/_ref not built (needs the reference tree)")
+    dims = [3, 20, 12, 2]
+    ts = D.TableSet(3, 2)
+    net = D.Net(ts, dims)
+    rng = np.random.default_rng(21)
+    w = rng.normal(0, 0.4, net.nw)
+    gv = rng.normal(0, 2.0, net.nw)
+    for istep0 in (1, 40):
+        net.weights = w
+        g = D.Grad(net)
+        g.set(gv)
+        for it in range(istep0, istep0 + 7):
+            D.adam_step(net, g, 2e-3, 0.9, 0.999, 1e-8, 13, it)
+        wg = net.weights
+        wr = oracle_mod.ref_adam(dims, w, gv, 2e-3, 0.9, 0.999, 1e-8, 13, istep0, 7)
+        print("Adam from istep", istep0, "max |gpu - reference|", np.abs(wg - wr).max())
+        assert np.array_equal(wg, wr)
+        g.free()
+    net.free()
+
+
 @pytest.mark.parametrize("prec,tol", [(1, 1.5e-6), (2, 2e-4), (3, 1.5e-6)])
 @pytest.mark.parametrize("name,npt", [("dense4_k3", 197), ("closure4_k7", 130), ("gelu3_k4", 70)])
 def test_mixed_precision(D, oracle_mod, codegen, name, npt, prec, tol):
