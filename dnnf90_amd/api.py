@@ -341,7 +341,8 @@ class BatchWork:
         return rsq.value if want_rsq else None
 
     def residual_rows(self, x, res: Residual, rows_t, ld):
-        """Gauss-Newton rows dR/dw into the device FP32 buffer rows_t[nw][ld]."""
+        """Gauss-Newton rows dR/dw into the device FP32 buffer rows_t[nw][ld];
+        ld >= npt rounded up to 64 (padding rows are written as zeros)."""
         xk, px, dev = _ptr(x, np.float64)
         rk, pr, rdev = _ptr(rows_t, np.float32)
         if not rdev:

@@ -2448,10 +2448,10 @@ int dnnf90_comm_eval_batch(int nmem, const int *nids, int bid, int npt, const do
     launch_fill(m2, len, 0.0, s);
     for (int k = 0; k < nmem; ++k) {
         if (b.prec == PREC_FP64) {
-            forward_t<double>(*mem[k], b, dx, npt);
+            if (hipError_t e = forward_t<double>(*mem[k], b, dx, npt)) return hip_fail(e, "committee forward");
             launch_output<double>((const double *)b.tl, nout, na, npt, b.np, b.d_tout, s);
         } else {
-            forward_t<float>(*mem[k], b, dx, npt);
+            if (hipError_t e = forward_t<float>(*mem[k], b, dx, npt)) return hip_fail(e, "committee forward");
             launch_output<float>((const float *)b.tl, nout, na, npt, b.np, b.d_tout, s);
         }
         launch_comm_welford(b.d_tout, mean, m2, len, k + 1, s);

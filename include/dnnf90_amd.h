@@ -172,8 +172,11 @@ int dnnf90_residual_step_batch(int nid, int bid, int npt, const double *x,
                                const int *slot, const int *nonlin,
                                double fac, int gid, double *rsq, int flags);
 /* Gauss-Newton rows dR_n/dw of the same residual (seed dR/dT), written
- * transposed as FP32 rows_t[w][n], w < nweights, row stride ld >= npt
- * (device pointer always) */
+ * transposed as FP32 rows_t[w][n], w < nweights (device pointer always).
+ * The kernels write whole 64-point blocks, the rows of the padding points
+ * as zeros, so the row stride must be at least npt rounded up to a
+ * multiple of 64 (-1 otherwise); dnnf90_ngd_solve further needs ld to be a
+ * multiple of 16. */
 int dnnf90_residual_rows_batch(int nid, int bid, int npt, const double *x,
                                int nterm, const double *coeff,
                                const int *slot, const int *nonlin,

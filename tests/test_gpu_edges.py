@@ -337,3 +337,21 @@ def test_edge_kernels_rows(D, oracle_mod):
     rows = rows_t[:, :npt].double().cpu().numpy().T
     assert rel(rows, o.pinn_rows([2, 20, 20, 1], w, x, terms)) <= 1e-6
     bw.free(); net.free()
+
+
+def test_rows_stride_refused(D):
+    """dnnf90_residual_rows_batch writes whole 64-point blocks: a row stride
+    below npt rounded up to 64 is refused (-1), the rounded one accepted."""
+    import torch
+    ts, net, w = _net(D, 2, 4, SEEDS, 0, [2, 8, 1])
+    o_terms = [(1.0, 1, False)]
+    bw = D.BatchWork(net, 100, D.PREC_FP64)
+    x = np.random.default_rng(3).uniform(-1, 1, (100, 2))
+    small = torch.zeros((net.nw, 100), dtype=torch.float32, device="cuda")
+    with pytest.raises(D.DNNF90Error):
+        bw.residual_rows(x, D.Residual(o_terms), small, 100)
+    ok = torch.zeros((net.nw, 128), dtype=torch.float32, device="cuda")
+    bw.residual_rows(x, D.Residual(o_terms), ok, 128)
+    torch.cuda.synchronize()
+    assert float(ok[:, 100:].abs().max()) == 0.0
+    bw.free(); net.free()
