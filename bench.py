@@ -56,35 +56,72 @@ def traffic_of(workload, mode, category):
         return None
 
 
-def algorithmic(wl, na, F, Q, P, N, wbytes, keep_s):
-    """Per-step algorithmic work of each kernel category: (flops, bytes).
+def algorithmic(wl, na, F, Q, P, N, wbytes, keep_s, edge=(0, 0, 0, 0)):
+    """Per-step algorithmic work of each kernel category: (flops, bytes,
+    launches), for the kernels the step launches.
 
-    K1 and K5 are 2 n_l (n_{l-1}+1) |A| flops per point and layer, K6
-    2 n_l n_{l-1} |A| for l > 2; K1 reads T of the layer below and writes S,
-    K5 reads S-bar and T of the layer below, K6 reads S-bar and writes T-bar
-    (|A| words per row and point).  Keeping h, the Faa di Bruno forward is
-    n (3F + 4Q + 3(|A|-1)) flops and reads S, writes T and h; the adjoint is
-    n (3P + 2|A|) flops and reads T-bar and h, writes S-bar.  Keeping S
-    (keep_s), the forward is n (3F + 2Q + (|A|-1)) flops and reads S, writes
-    T; the adjoint recomputes h (n (3F + 2Q + (|A|-1)) flops more) and reads
-    T-bar and S, writes S-bar."""
+    A plane row is |A| words a point (w bytes a row over the N points).  K1
+    and K5 are 2 n_l (n_{l-1}+1) |A| flops per point and layer, K6
+    2 n_l n_{l-1} |A|; K1 reads T of the layer below and writes S, K5 reads
+    S-bar and T of the layer below, K6 reads S-bar and writes T-bar.
+    Keeping h, the Faa di Bruno forward is n (3F + 4Q + 3(|A|-1)) flops and
+    reads S, writes T and h; the adjoint is n (3P + 2|A|) flops and reads
+    T-bar and h, writes S-bar.  Keeping S (keep_s), the forward is
+    n (3F + 2Q + (|A|-1)) flops and reads S, writes T; the adjoint
+    recomputes h (the same flops again) and reads T-bar and S, writes S-bar.
+
+    edge = the edge kernels taken (dnnf90_net_edge_info): (in) the first
+    hidden layer's forward and adjoint read the input values (one word a
+    point and input; the input plane's other slots are constants) and the
+    layer's weights instead of its S plane, and its K1 is not launched;
+    (in_grad) that adjoint writes its weight gradient as per-64-point
+    partials (8-byte words) instead of S-bar, and its K5 is not launched;
+    (out) the last hidden layer's adjoint reads the output layer's S-bar
+    and weights instead of its T-bar, and the output layer's K6 is not
+    launched; (out_grad) that adjoint also writes the output layer's
+    gradient partials, and the output layer's K5 is not launched."""
+    e_in, e_in_grad, e_out, e_out_grad = (bool(v) for v in edge)
     d = wl.dims
     L = len(d)
     w = float(wbytes) * na * N
-    lin = sum(2.0 * d[l] * (d[l - 1] + 1) * na for l in range(1, L)) * N
-    lin_b = sum(d[l - 1] + d[l] for l in range(1, L)) * w
-    lin_t = sum(2.0 * d[l] * d[l - 1] * na for l in range(2, L)) * N
-    lin_t_b = sum(d[l] + d[l - 1] for l in range(2, L)) * w
-    hid = sum(d[l] for l in range(1, L - 1))
-    bb_flops = hid * (3.0 * P + 2.0 * na) * N
-    if keep_s:
-        h_flops = hid * (3.0 * F + 2.0 * Q + (na - 1)) * N
-        bell_fwd, bell_bwd = (h_flops, 2.0 * hid * w), (bb_flops + h_flops, 3.0 * hid * w)
-    else:
-        bell_fwd = (hid * (3.0 * F + 4.0 * Q + 3.0 * (na - 1)) * N, 3.0 * hid * w)
-        bell_bwd = (bb_flops, 3.0 * hid * w)
-    return {"linear": (lin, lin_b), "linear_t": (lin_t, lin_t_b), "grad": (lin, lin_b),
-            "bell_fwd": bell_fwd, "bell_bwd": bell_bwd}
+    xin = float(wbytes) * N
+    nblk = N / 64.0
+    cat = {k: [0.0, 0.0, 0] for k in ("linear", "linear_t", "grad", "bell_fwd", "bell_bwd")}
+
+    def add(k, flops, byts):
+        cat[k][0] += flops
+        cat[k][1] += byts
+        cat[k][2] += 1
+
+    h_fl = lambda l: d[l] * (3.0 * F + 2.0 * Q + (na - 1)) * N
+    for l in range(1, L):
+        if not (l == 1 and e_in):
+            add("linear", 2.0 * d[l] * (d[l - 1] + 1) * na * N, (d[l - 1] + d[l]) * w)
+        if l < L - 1:
+            if l == 1 and e_in:
+                add("bell_fwd", h_fl(l) + 2.0 * d[1] * (d[0] + 1) * N, d[0] * xin + d[1] * w)
+            elif keep_s:
+                add("bell_fwd", h_fl(l), 2.0 * d[l] * w)
+            else:
+                add("bell_fwd", d[l] * (3.0 * F + 4.0 * Q + 3.0 * (na - 1)) * N, 3.0 * d[l] * w)
+    for l in range(L - 1, 0, -1):
+        if l < L - 1:
+            in_s, out_t = l == 1 and e_in, l == L - 2 and e_out
+            fl = d[l] * (3.0 * P + 2.0 * na) * N + (h_fl(l) if keep_s else 0.0)
+            if not keep_s:
+                add("bell_bwd", fl, 3.0 * d[l] * w)
+            else:
+                rd = (d[0] * xin if in_s else d[l] * w) + (d[L - 1] * w if out_t else d[l] * w)
+                wr = nblk * d[1] * (d[0] + 1) * 8.0 if (in_s and e_in_grad) else d[l] * w
+                if out_t and e_out_grad:
+                    wr += nblk * (d[l] + 1) * 8.0
+                add("bell_bwd", fl, rd + wr)
+        skip_grad = (l == L - 1 and e_out_grad) or (l == 1 and e_in and e_in_grad)
+        if not skip_grad:
+            add("grad", 2.0 * d[l] * (d[l - 1] + 1) * na * N, (d[l] + d[l - 1]) * w)
+        if l > 1 and not (l == L - 1 and e_out):
+            add("linear_t", 2.0 * d[l] * d[l - 1] * na * N, (d[l] + d[l - 1]) * w)
+    return {k: tuple(v) for k, v in cat.items()}
 
 
 def rel_by_order(x, ref, deg):
@@ -197,6 +234,8 @@ def run_mode(args, D, torch, lib, c, dev, prec, name, wbytes, mma_peak, budget=0
     kinfo = np.zeros(6, np.int32)
     lib().dnnf90_net_kernel_info(net.id, prec, kinfo.ctypes.data)
     keep_s = int(kinfo[0]) == 2
+    edge = np.zeros(4, np.int32)
+    lib().dnnf90_net_edge_info(net.id, prec, edge.ctypes.data)
     g = D.Grad(net)
     s = torch.cuda.Stream(device=dev)
     bw.set_stream(s.cuda_stream)
@@ -239,15 +278,18 @@ def run_mode(args, D, torch, lib, c, dev, prec, name, wbytes, mma_peak, budget=0
     bw.set_stream(None)
     bw.free()
     g.free()
-    work = algorithmic(c.wl, c.na, c.F, c.Q, c.P, N, wbytes, keep_s)
+    work = algorithmic(c.wl, c.na, c.F, c.Q, c.P, N, wbytes, keep_s, edge)
     kern = {}
     for cat, (tot, cnt) in kms.items():
         ent = {"ms_per_step": tot / args.steps, "launches_per_step": cnt / args.steps}
-        if cat in work and tot > 0:
+        if cat in work and tot > 0 and work[cat][2] != cnt / args.steps:
+            # the byte model describes other launches than the step made
+            ent["model_launches_per_step"] = work[cat][2]
+        elif cat in work and tot > 0:
             # judged against the roof that binds: MFMA for the linear maps
             # unless their bytes take longer at 8 TB/s than their flops at the
             # mode's MFMA peak; HBM for the Faa di Bruno kernels
-            flops, byts = work[cat]
+            flops, byts, _ = work[cat]
             sec = tot / args.steps * 1e-3
             mfma_bound = cat in ("linear", "linear_t", "grad") and \
                 flops / (mma_peak * 1e12) > byts / (PEAK["hbm"] * 1e9)
@@ -260,6 +302,7 @@ def run_mode(args, D, torch, lib, c, dev, prec, name, wbytes, mma_peak, budget=0
             ent["algorithmic_bytes_per_step"] = byts
         kern[cat] = ent
     info["keep_s"] = keep_s
+    info["edge"] = [int(v) for v in edge]
     ms = float(np.median(step_ms))
     return {"name": name, "ms_per_step": ms, "step_ms": {"median": ms, "min": float(min(step_ms)),
                                                           "max": float(max(step_ms)),

@@ -233,14 +233,31 @@ std::string gen_bell_source(const TableSet &t, const FwdSchedule &s, bool fp64)
     o << "struct ActParams { int iact, nq; const double *tpc; int ldt; };\n"
       << kActSrc << "\n"
       << "typedef " << (fp64 ? "double" : "float") << " real;\n";
-    auto head = [&](const char *name, const char *args) {
+    // the edge kernels' row blocks of one point block share operands (the
+    // input values, the output layer's Sbar): the block index is remapped so
+    // that each XCD (workgroups go to the 8 XCDs in turn) runs a contiguous
+    // run of point blocks, all row blocks of each together, and the shared
+    // operands stay in that XCD's L2.  pb / rb: point and row block.
+    auto block_ids = [&](bool xcd) {
+        if (!xcd) {
+            o << "  const unsigned pb = blockIdx.x, rb = blockIdx.y;\n";
+            return;
+        }
+        o << "  unsigned lin = blockIdx.x + gridDim.x * blockIdx.y;\n"
+          << "  {\n    const unsigned G = gridDim.x * gridDim.y;\n"
+          << "    if ((G & 7u) == 0u) lin = (lin & 7u) * (G >> 3) + (lin >> 3);\n  }\n"
+          << "  const unsigned pb = lin / gridDim.y, rb = lin % gridDim.y;\n";
+    };
+    auto head_x = [&](const char *name, const char *args, bool xcd) {
         o << "extern \"C\" __global__ __launch_bounds__(256) void " << name << sfx << "(" << args << ")\n{\n"
-          << "  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;\n"
-          << "  const int j = blockIdx.y * (blockDim.x >> 6) + wave;\n"
+          << "  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;\n";
+        block_ids(xcd);
+        o << "  const int j = rb * (blockDim.x >> 6) + wave;\n"
           << "  if (j >= nrow) return;\n"
-          << "  const long n = (long)blockIdx.x * 64 + lane;\n"
+          << "  const long n = (long)pb * 64 + lane;\n"
           << "  const long ldr = " << na << "L * Np;\n";
     };
+    auto head = [&](const char *name, const char *args) { head_x(name, args, false); };
     // S of one lane and the activation derivatives sigma^(q)(S_0), q < kq, in FP64
     auto load_s = [&]() {
         for (int a = 0; a < na; ++a) o << "  const real e" << a << " = Sj[" << a << "L * Np];\n";
@@ -326,8 +343,8 @@ std::string gen_bell_source(const TableSet &t, const FwdSchedule &s, bool fp64)
         for (int a = 0; a < na; ++a) o << "    t" << a << " += wk * sl[" << a << "L * Np];\n";
         o << "  }\n";
     };
-    head("dnnf90_bell_fwdt_in", "const real *__restrict__ T0, const real *__restrict__ W1, real *__restrict__ Tout, "
-                                "int nrow, long Np, ActParams ap");
+    head_x("dnnf90_bell_fwdt_in", "const real *__restrict__ T0, const real *__restrict__ W1, real *__restrict__ Tout, "
+                                  "int nrow, long Np, ActParams ap", true);
     o << "  real *Tj = Tout + (long)j * ldr + n;\n";
     load_s_in();
     o << "  Tj[0] = d0;\n";
@@ -339,7 +356,7 @@ std::string gen_bell_source(const TableSet &t, const FwdSchedule &s, bool fp64)
     for (int v = 1; v < 4; ++v) {
         const bool in_s = v & 1, out_t = v & 2;
         const std::string name = std::string("dnnf90_bell_bwds_") + (in_s ? "i" : "") + (out_t ? "o" : "");
-        head(name.c_str(), bargs);
+        head_x(name.c_str(), bargs, true);
         o << "  real *sb = Sb + (long)j * ldr + n;\n";
         if (in_s) {
             load_s_in();
@@ -363,7 +380,7 @@ std::string gen_bell_source(const TableSet &t, const FwdSchedule &s, bool fp64)
     // x at slot 0 and 1 at slot e_i, so the layer's gradient needs Sbar at
     // slot 0 and at the slots e_i alone: g(j, bias) = sum_n Sbar_0 and
     // g(j, i) = sum_n (Sbar_0 x_i + Sbar_(e_i)).  The lanes of a workgroup
-    // (64 points) are summed in FP64 in point order; part[blockIdx.x][j][c]
+    // (64 points) are summed in FP64 in point order; part[pb][j][c]
     // takes the sums and a fixed-order reduction adds the point blocks.  No
     // Sbar plane is written.
     for (int v = 1; v < 4; v += 2) {
@@ -373,11 +390,12 @@ std::string gen_bell_source(const TableSet &t, const FwdSchedule &s, bool fp64)
         o << "extern \"C\" __global__ __launch_bounds__(256) void " << name << sfx << "(" << bargs
           << ", double *__restrict__ part)\n{\n"
           << "  __shared__ double gred[256 * " << d0 + 1 << "];\n"
-          << "  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;\n"
-          << "  const int j0 = blockIdx.y * (blockDim.x >> 6) + wave;\n"
+          << "  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;\n";
+        block_ids(true);
+        o << "  const int j0 = rb * (blockDim.x >> 6) + wave;\n"
           << "  const bool jok = j0 < nrow;\n"
           << "  const int j = jok ? j0 : nrow - 1;   // every wave reaches the barrier\n"
-          << "  const long n = (long)blockIdx.x * 64 + lane;\n"
+          << "  const long n = (long)pb * 64 + lane;\n"
           << "  const long ldr = " << na << "L * Np;\n";
         load_s_in();
         o << "  const real h0 = d1;\n";
@@ -411,24 +429,25 @@ std::string gen_bell_source(const TableSet &t, const FwdSchedule &s, bool fp64)
           << "  if (jok && lane < " << d0 + 1 << ") {\n"
           << "    double acc = 0.0;\n"
           << "    for (int p = 0; p < 64; ++p) acc += gred[(wave * 64 + p) * " << d0 + 1 << " + lane];\n"
-          << "    part[((long)blockIdx.x * nrow + j) * " << d0 + 1 << " + lane] = acc;\n"
+          << "    part[((long)pb * nrow + j) * " << d0 + 1 << " + lane] = acc;\n"
           << "  }\n}\n";
     }
     // ---- the last hidden layer's adjoint (Tbar from the single output
     // row) with the output layer's weight gradient: T of the layer is
     // recomputed beside h from the same sums, g(0, 1 + j) = sum_n sum_a
     // Sbar_L(0, a) T_a(j) and the bias sum_n Sbar_L(0, 0), per 64-point block
-    // in part[blockIdx.x][c] (c = 0 bias, 1 + j weights), summed in FP64 in
+    // in part[pb][c] (c = 0 bias, 1 + j weights), summed in FP64 in
     // point order; the Sbar plane of the layer is written as bwds_o does
     {
         o << "extern \"C\" __global__ __launch_bounds__(256) void dnnf90_bell_bwds_og" << sfx << "(" << bargs
           << ", double *__restrict__ part)\n{\n"
           << "  __shared__ double gred[256 * 2];\n"
-          << "  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;\n"
-          << "  const int j0 = blockIdx.y * (blockDim.x >> 6) + wave;\n"
+          << "  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;\n";
+        block_ids(true);
+        o << "  const int j0 = rb * (blockDim.x >> 6) + wave;\n"
           << "  const bool jok = j0 < nrow;\n"
           << "  const int j = jok ? j0 : nrow - 1;   // every wave reaches the barrier\n"
-          << "  const long n = (long)blockIdx.x * 64 + lane;\n"
+          << "  const long n = (long)pb * 64 + lane;\n"
           << "  const long ldr = " << na << "L * Np;\n"
           << "  real *sb = Sb + (long)j * ldr + n;\n"
           << "  const real *slz = SbL + n;   // the output row (nout is 1)\n"
@@ -449,12 +468,12 @@ std::string gen_bell_source(const TableSet &t, const FwdSchedule &s, bool fp64)
           << "  if (lane == 0 && jok) {\n"
           << "    double acc = 0.0;\n"
           << "    for (int p = 0; p < 64; ++p) acc += gred[(wave * 64 + p) * 2];\n"
-          << "    part[(long)blockIdx.x * (nrow + 1) + 1 + j] = acc;\n"
+          << "    part[(long)pb * (nrow + 1) + 1 + j] = acc;\n"
           << "  }\n"
-          << "  if (lane == 1 && wave == 0 && blockIdx.y == 0) {\n"
+          << "  if (lane == 1 && wave == 0 && rb == 0) {\n"
           << "    double acc = 0.0;\n"
           << "    for (int p = 0; p < 64; ++p) acc += gred[p * 2 + 1];\n"
-          << "    part[(long)blockIdx.x * (nrow + 1)] = acc;\n"
+          << "    part[(long)pb * (nrow + 1)] = acc;\n"
           << "  }\n}\n";
     }
     if (fp64 && gen_point_family(t, s)) emit_point_kernels(o, t, s, load_s);

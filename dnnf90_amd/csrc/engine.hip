@@ -558,6 +558,30 @@ static const GenKernels *gen_for(const Net &nt)
     return (codegen_on() && g.ok) ? &g : nullptr;
 }
 
+// the edge kernels a batched step of many points takes (a step of one point,
+// or one writing per-point rows, drops the two gradient-forming ones):
+// in -- the first hidden layer forms S from the input plane (forward and
+// adjoint); in_grad -- its adjoint also forms that layer's weight gradient;
+// out -- the last hidden layer forms Tbar from the output layer's Sbar;
+// out_grad -- that adjoint also forms the output layer's weight gradient
+struct EdgePlan {
+    bool in = false, in_grad = false, out = false, out_grad = false;
+};
+
+static EdgePlan edge_plan(const Net &nt, bool fp64, bool s_kept)
+{
+    EdgePlan p;
+    const GenKernels &g = nt.gen[fp64 ? 1 : 0];
+    const int L = nt.nlayer;
+    if (!s_kept || !g_edge || L < 3) return p;
+    p.in = g.fwdt_in && g.bwds_i;
+    const bool one_hidden = L == 3 && p.in;
+    p.out = one_hidden ? g.bwds_io : g.bwds_o;
+    p.out_grad = p.out && nt.dims[L - 1] == 1 && g.bwds_og && !one_hidden;
+    p.in_grad = p.in && ((one_hidden && p.out) ? g.bwds_iog : g.bwds_ig);
+    return p;
+}
+
 template <typename T> static const T *weights(const Net &nt);
 template <> const double *weights<double>(const Net &nt) { return nt.d_w64; }
 template <> const float *weights<float>(const Net &nt) { return nt.d_w32; }
@@ -663,7 +687,7 @@ static hipError_t forward_t(const Net &nt, BatchWork &b, const double *d_x, int 
     }
     hipError_t e = hipSuccess;
     const GenKernels &gg = nt.gen[sizeof(T) == 8 ? 1 : 0];
-    b.edge_in = b.s_kept && g_edge && L >= 3 && gg.fwdt_in && gg.bwds_i;
+    b.edge_in = edge_plan(nt, sizeof(T) == 8, b.s_kept).in;
     int l0 = 1;
     if (b.edge_in) {
         // the first hidden layer straight from T0 (no K1, no S plane)
@@ -704,10 +728,11 @@ static hipError_t backward_t(const Net &nt, BatchWork &b, double *grad, float *r
     void *sbp = b.p0, *tbp = b.p1;
     const GenKernels &gg = nt.gen[sizeof(T) == 8 ? 1 : 0];
     const int nout = nt.dims[L - 1];
-    const bool out_fused = b.s_kept && g_edge && L >= 3 && ((L == 3 && b.edge_in) ? gg.bwds_io : gg.bwds_o);
+    const EdgePlan plan = edge_plan(nt, sizeof(T) == 8, b.s_kept);
+    const bool out_fused = plan.out;
     // one output row: the last hidden layer's adjoint also forms the output
     // layer's weight gradient (T of the layer recomputed beside h)
-    const bool out_grad = out_fused && nout == 1 && gg.bwds_og && !rows_t && b.npt > 1 && !(L == 3 && b.edge_in);
+    const bool out_grad = plan.out_grad && !rows_t && b.npt > 1;
     for (int l = L - 1; l >= 1 && e == hipSuccess; --l) {
         const int nd = nt.dims[l], ndm = nt.dims[l - 1];
         if (l < L - 1) {
@@ -724,7 +749,7 @@ static hipError_t backward_t(const Net &nt, BatchWork &b, double *grad, float *r
             }
             const GenKernels *gk = gen_for<T>(nt);
             const bool out_t = l == L - 2 && out_fused;
-            if (in_s && !rows_t && b.npt > 1 && (out_t ? gg.bwds_iog : gg.bwds_ig)) {
+            if (in_s && !rows_t && b.npt > 1 && plan.in_grad) {
                 // the first hidden layer's adjoint forms the layer's weight
                 // gradient itself: no Sbar plane, no K5 of this layer
                 const int nc = (int)(Np / 64);
@@ -2072,6 +2097,22 @@ int dnnf90_net_kernel_info(int nid, int prec, int *info)
     info[3] = on ? (ks ? g.fwdt_scratch : g.fwd_scratch) : 0;
     info[4] = on ? (ks ? g.bwds_scratch : g.bwd_scratch) : 0;
     info[5] = nt->sched.nentry;
+    return 0;
+}
+
+int dnnf90_net_edge_info(int nid, int prec, int *info)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt || !info || prec < 0 || prec > 3) return fail(-1, "bad network handle or argument");
+    int rc = nt->ensure_device(prec != PREC_FP64);
+    if (rc) return rc;
+    const GenKernels &g = nt->gen[prec == PREC_FP64 ? 1 : 0];
+    const bool s_kept = codegen_on() && g.ok && g_store_s && g.s_ok();
+    const EdgePlan p = edge_plan(*nt, prec == PREC_FP64, s_kept);
+    info[0] = p.in;
+    info[1] = p.in_grad;
+    info[2] = p.out;
+    info[3] = p.out_grad;
     return 0;
 }
 

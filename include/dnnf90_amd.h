@@ -94,6 +94,15 @@ int dnnf90_set_split_kernels(int on);
  * reference. */
 int dnnf90_set_edge_kernels(int on);
 int dnnf90_net_kernel_info(int nid, int prec, int *info);
+/* the edge kernels a batched step of many points takes in precision prec,
+ * info[4] (1 taken, 0 not): [0] the first hidden layer forms S from the
+ * input plane, forward and adjoint; [1] its adjoint also forms that layer's
+ * weight gradient; [2] the last hidden layer forms Tbar from the output
+ * layer's Sbar; [3] that adjoint also forms the output layer's weight
+ * gradient (one output row).  A step of one point, or one writing per-point
+ * rows, takes neither gradient-forming kernel.  No counterpart in the
+ * reference. */
+int dnnf90_net_edge_info(int nid, int prec, int *info);
 
 int dnnf90_net_nweights(int nid);
 int dnnf90_net_nlayer(int nid);
