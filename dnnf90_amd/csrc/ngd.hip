@@ -80,19 +80,21 @@ __device__ __forceinline__ double ldot(const double *__restrict__ x, const doubl
     return (s0 + s1) + (s2 + s3);
 }
 
-// Cholesky of the diagonal block at (k, k) (nb <= PB rows), in place in G
-// (lower), then its inverse and the inverse's transpose, written to
-// Linv[PB][PB] and LinvT[PB][PB] (zero off the triangle; rows past nb hold
-// the identity).  Right-looking over 16-column steps: wave 0 factors the
-// 16 x 16 diagonal block in registers (one row a lane, columns exchanged by
-// lane shuffles), one thread a row solves the panel below against it, and
-// the trailing lower triangle takes the rank-16 update in 4 x 4 register
-// tiles.  The inverse X = L^-1 is blocked the same way: the eight 16 x 16
-// diagonal inverses by forward substitution (one column a thread), then
-// block row by block row X_ij = -X_ii (sum_{j <= m < i} L_im X_mj) on FP64
-// MFMA, a wave a block.  X (below
-// the diagonal) is kept transposed in the free upper triangle of S, its
-// diagonal in dg.  info > 0 marks the first non-positive pivot (index + 1).
+// Cholesky of the diagonal block at (k, k) (nb <= PB rows) in LDS, and
+// its inverse X = L^-1 and the inverse's transpose, written to Linv[PB][PB]
+// and LinvT[PB][PB] (zero off the triangle; rows past nb hold the
+// identity).  Right-looking over 16-column steps: wave 0 factors the
+// 16 x 16 diagonal block in registers (one row a lane, the pivot column
+// broadcast by readlane) and forms that block's inverse X_kk by forward
+// substitution (one column a lane); the rows below take L21 = A21 X_kk^T
+// and the trailing lower blocks the rank-16 update, both on
+// v_mfma_f64_16x16x4f64 (16 x 16 blocks, a wave a block).  The off-diagonal
+// blocks of X follow block row by block row, X_ij = -X_ii (sum_{j <= m < i}
+// L_im X_mj), on FP64 MFMA, a wave a block.  X (below the diagonal) is kept
+// transposed in the free upper triangle of S, its diagonal in dg.  The
+// block of G itself is left as the trailing updates made it (the solves
+// and the panel read Linv, LinvT and the rows below the block).  info > 0
+// marks the first non-positive pivot (index + 1).
 static constexpr int DB = 16;         // step width inside the diagonal block
 static constexpr int ND = PB / DB;    // steps
 
@@ -102,7 +104,16 @@ __device__ __forceinline__ double xinv(const double (*S)[PBL], const double *dg,
     return R > C ? S[C][R] : (R == C ? dg[R] : 0.0);
 }
 
-__global__ __launch_bounds__(256) void k_chol_diag(double *__restrict__ A, long lda, int k, int nb,
+// the value of lane src (a constant after unrolling) in every lane
+__device__ __forceinline__ double lane_bcast(double v, int src)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned)b, src);
+    const int hi = __builtin_amdgcn_readlane((int)(unsigned)(b >> 32), src);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+__global__ __launch_bounds__(256) void k_chol_diag(const double *__restrict__ A, long lda, int k, int nb,
                                                    double *__restrict__ Linv, double *__restrict__ LinvT,
                                                    int *__restrict__ info)
 {
@@ -124,19 +135,22 @@ __global__ __launch_bounds__(256) void k_chol_diag(double *__restrict__ A, long 
         }
     }
     __syncthreads();
+    const int lane = tid & 63, li = lane & 15, lk = lane >> 4;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     for (int kb = 0; kb < ND; ++kb) {
         const int o = kb * DB;
-        // (a) the diagonal 16 x 16 block; lane r holds row o + r (lanes 16..63
-        // repeat rows 0..15 so that every shuffle source is live)
-        if (tid < 64) {
-            const int r = tid & 15;
-            double d[DB];
+        // (a) wave 0: the 16 x 16 diagonal block, lane r holding row o + r
+        // (lanes 16..63 repeat rows 0..15), then its inverse, lane b forming
+        // column b of X_kk
+        if (wave == 0) {
+            const int r = lane & 15;
+            double d[DB], il_[DB];
 #pragma unroll
             for (int c = 0; c < DB; ++c) d[c] = S[o + r][o + c];
 #pragma unroll
             for (int j = 0; j < DB; ++j) {
-                const double djj = __shfl(d[j], j);
-                if (tid == j && !(djj > 0.0) && *info == 0) *info = k + o + j + 1;
+                const double djj = lane_bcast(d[j], j);
+                if (lane == 0 && !(djj > 0.0) && *info == 0) *info = k + o + j + 1;
                 // 1/sqrt from v_rsq_f64 and two Newton steps (each doubles
                 // the correct bits), l = d * (1/sqrt d): no IEEE sqrt and
                 // division sequences on the column's dependent path
@@ -144,95 +158,84 @@ __global__ __launch_bounds__(256) void k_chol_diag(double *__restrict__ A, long 
                 double il = __builtin_amdgcn_rsq(dd);
                 il = il * (1.5 - 0.5 * dd * il * il);
                 il = il * (1.5 - 0.5 * dd * il * il);
+                il_[j] = il;
+                if (lane == 0) dg[o + j] = il;
                 const double l = dd * il;
                 if (r == j) d[j] = l;
                 else if (r > j) d[j] = d[j] * il;
 #pragma unroll
                 for (int c = j + 1; c < DB; ++c) {
-                    const double lcj = __shfl(d[j], c);   // L[o + c][o + j]
+                    const double lcj = lane_bcast(d[j], c);   // L[o + c][o + j]
                     if (c <= r) d[c] -= d[j] * lcj;
                 }
-                if (tid == 0) dg[o + j] = il;
             }
-            if (tid < DB) {
+            if (lane < DB) {
 #pragma unroll
                 for (int c = 0; c < DB; ++c)
                     if (c <= r) S[o + r][o + c] = d[c];
             }
-        }
-        __syncthreads();
-        const int t0 = o + DB, T = PB - t0;
-        if (T == 0) break;
-        // (b) the panel below: row t0 + t solves x L11^T = a
-        if (tid < T) {
-            const int row = t0 + tid;
+            // X[a][b] = -X[a][a] sum_{b <= m < a} L[a][m] X[m][b]
+            const int b = r;
             double x[DB];
 #pragma unroll
-            for (int c = 0; c < DB; ++c) x[c] = S[row][o + c];
+            for (int a = 0; a < DB; ++a) {
+                double s = 0.0;
 #pragma unroll
-            for (int c = 0; c < DB; ++c) {
-                double v = x[c];
-#pragma unroll
-                for (int m = 0; m < c; ++m) v -= x[m] * S[o + c][o + m];
-                x[c] = v * dg[o + c];
+                for (int m = 0; m < a; ++m) s += lane_bcast(d[m], a) * x[m];   // L[o + a][o + m]
+                x[a] = a < b ? 0.0 : (a == b ? il_[a] : -il_[a] * s);
             }
+            if (lane < DB) {
 #pragma unroll
-            for (int c = 0; c < DB; ++c) S[row][o + c] = x[c];
+                for (int a = 0; a < DB; ++a)
+                    if (a > b) S[o + b][o + a] = x[a];   // X[a][b] at S[C][R]
+            }
         }
         __syncthreads();
-        // (c) trailing lower triangle -= L21 L21^T in 4 x 4 tiles
-        const int tq = T / 4, ntile = tq * (tq + 1) / 2;
-        for (int u = tid; u < ntile; u += 256) {
-            int ti = (int)((sqrt(8.0 * (double)u + 1.0) - 1.0) * 0.5);
-            while ((ti + 1) * (ti + 2) / 2 <= u) ++ti;
-            while (ti * (ti + 1) / 2 > u) --ti;
-            const int tj = u - ti * (ti + 1) / 2;
-            const int r0 = t0 + 4 * ti, c0 = t0 + 4 * tj;
-            double acc[4][4];
+        const int t0 = o + DB, nrb = ND - 1 - kb;
+        if (nrb == 0) break;
+        // (b) the rows below: L21 = A21 X_kk^T, a 16-row block a wave turn
+        // (A operand: lane (i = li, k = lk); B: (k = lk, j = li); result
+        // element r at row lk + 4 r, column li)
+        {
+            double bx[4];
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
+            for (int c = 0; c < 4; ++c) bx[c] = xinv(S, dg, o + li, o + 4 * c + lk);   // X_kk[li][4 c + lk]
+            for (int rb = wave; rb < nrb; rb += 4) {
+                const int r0 = t0 + DB * rb;
+                double av[4];
 #pragma unroll
-                for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-#pragma unroll 4
-            for (int m = 0; m < DB; ++m) {
+                for (int c = 0; c < 4; ++c) av[c] = S[r0 + li][o + 4 * c + lk];
+                dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int c = 0; c < 4; ++c) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[c], bx[c], acc, 0, 0, 0);
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) S[r0 + lk + 4 * rr][o + li] = acc[rr];
+            }
+        }
+        __syncthreads();
+        // (c) trailing lower blocks (ib >= jb > kb) -= L(ib, kb) L(jb, kb)^T
+        {
+            const int nblk = nrb * (nrb + 1) / 2;
+            for (int u = wave; u < nblk; u += 4) {
+                int ti = 0;
+                while ((ti + 1) * (ti + 2) / 2 <= u) ++ti;
+                const int tj = u - ti * (ti + 1) / 2;
+                const int ra = t0 + DB * ti, rc = t0 + DB * tj;
                 double av[4], bv[4];
 #pragma unroll
-                for (int a = 0; a < 4; ++a) {
-                    av[a] = S[r0 + a][o + m];
-                    bv[a] = S[c0 + a][o + m];
+                for (int c = 0; c < 4; ++c) {
+                    av[c] = S[ra + li][o + 4 * c + lk];
+                    bv[c] = S[rc + li][o + 4 * c + lk];
                 }
+                dbl4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                for (int a = 0; a < 4; ++a)
+                for (int c = 0; c < 4; ++c) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[c], bv[c], acc, 0, 0, 0);
 #pragma unroll
-                    for (int b = 0; b < 4; ++b) acc[a][b] += av[a] * bv[b];
+                for (int rr = 0; rr < 4; ++rr) S[ra + lk + 4 * rr][rc + li] -= acc[rr];
             }
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int b = 0; b < 4; ++b)
-                    if (c0 + b <= r0 + a) S[r0 + a][c0 + b] -= acc[a][b];
         }
         __syncthreads();
     }
-    for (int u = tid; u < nb * nb; u += 256) {
-        const int r = u / nb, c = u % nb;
-        if (c <= r) A[(long)(k + r) * lda + k + c] = S[r][c];
-    }
-    // (d) the diagonal blocks of X: thread (jb, b) solves column b of block jb
-    // (the sums over m unrolled, so their LDS loads are in flight together)
-    if (tid < PB) {
-        const int base = (tid / DB) * DB, b = tid % DB;
-        for (int a = b + 1; a < DB; ++a) {
-            double s0 = S[base + a][base + b] * dg[base + b], s1 = 0.0;
-#pragma unroll
-            for (int m = 1; m < DB; m += 2) {
-                if (m > b && m < a) s0 += S[base + a][base + m] * S[base + b][base + m];
-                if (m + 1 > b && m + 1 < a && m + 1 < DB) s1 += S[base + a][base + m + 1] * S[base + b][base + m + 1];
-            }
-            S[base + b][base + a] = -(s0 + s1) * dg[base + a];
-        }
-    }
-    __syncthreads();
     // (e) block row ib = 1 .. 7: each wave takes blocks jb < ib in turn and
     // forms Y = sum_{jb <= mb < ib} L_(ib,mb) X_(mb,jb) on
     // v_mfma_f64_16x16x4f64 (A operand: lane (i = lane % 16, k = lane / 16);
@@ -243,8 +246,6 @@ __global__ __launch_bounds__(256) void k_chol_diag(double *__restrict__ A, long 
     // in block ib) are read by nothing in the same block row; the blocks of
     // row ib are complete at the barrier that ends it.
     {
-        const int li = tid & 15, lk = (tid & 63) >> 4;
-        const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
         for (int ib = 1; ib < ND; ++ib) {
             for (int jb = wave; jb < ib; jb += 4) {
                 dbl4 y = {0.0, 0.0, 0.0, 0.0};
@@ -270,14 +271,22 @@ __global__ __launch_bounds__(256) void k_chol_diag(double *__restrict__ A, long 
             __syncthreads();
         }
     }
-    // X and its transpose, each written in its own row order (coalesced)
-    for (int u = tid; u < PB * PB; u += 256) {
+    // X and its transpose, each in its own row order, two adjacent elements
+    // a thread at a time
+    typedef double dbl2 __attribute__((ext_vector_type(2)));
+    for (int u = 2 * tid; u < PB * PB; u += 512) {
         const int r = u / PB, c = u % PB;
-        Linv[u] = r == c ? dg[r] : (r > c ? S[c][r] : 0.0);    // X[r][c]
+        dbl2 v;
+        v[0] = r == c ? dg[r] : (r > c ? S[c][r] : 0.0);               // X[r][c]
+        v[1] = r == c + 1 ? dg[r] : (r > c + 1 ? S[c + 1][r] : 0.0);   // X[r][c + 1]
+        *reinterpret_cast<dbl2 *>(Linv + u) = v;
     }
-    for (int u = tid; u < PB * PB; u += 256) {
+    for (int u = 2 * tid; u < PB * PB; u += 512) {
         const int c = u / PB, r = u % PB;
-        LinvT[u] = r == c ? dg[r] : (r > c ? S[c][r] : 0.0);   // X[r][c] at LinvT[c][r]
+        dbl2 v;
+        v[0] = r == c ? dg[r] : (r > c ? S[c][r] : 0.0);                   // X[r][c] at LinvT[c][r]
+        v[1] = r + 1 == c ? dg[r + 1] : (r + 1 > c ? S[c][r + 1] : 0.0);   // X[r + 1][c]
+        *reinterpret_cast<dbl2 *>(LinvT + u) = v;
     }
 }
 
