@@ -56,7 +56,7 @@ def traffic_of(workload, mode, category):
         return None
 
 
-def algorithmic(wl, na, F, Q, P, N, wbytes, keep_s, edge=(0, 0, 0, 0)):
+def algorithmic(wl, na, F, Q, P, N, wbytes, keep_s, edge=(0, 0, 0, 0, 0)):
     """Per-step algorithmic work of each kernel category: (flops, bytes,
     launches), for the kernels the step launches.
 
@@ -70,7 +70,7 @@ def algorithmic(wl, na, F, Q, P, N, wbytes, keep_s, edge=(0, 0, 0, 0)):
     n (3F + 2Q + (|A|-1)) flops and reads S, writes T; the adjoint
     recomputes h (the same flops again) and reads T-bar and S, writes S-bar.
 
-    edge = the edge kernels taken (dnnf90_net_edge_info): (in) the first
+    edge = the fused kernels taken (dnnf90_net_step_info): (in) the first
     hidden layer's forward and adjoint read the input values (one word a
     point and input; the input plane's other slots are constants) and the
     layer's weights instead of its S plane, and its K1 is not launched;
@@ -79,8 +79,11 @@ def algorithmic(wl, na, F, Q, P, N, wbytes, keep_s, edge=(0, 0, 0, 0)):
     (out) the last hidden layer's adjoint reads the output layer's S-bar
     and weights instead of its T-bar, and the output layer's K6 is not
     launched; (out_grad) that adjoint also writes the output layer's
-    gradient partials, and the output layer's K5 is not launched."""
-    e_in, e_in_grad, e_out, e_out_grad = (bool(v) for v in edge)
+    gradient partials, and the output layer's K5 is not launched; (bit l of
+    the fifth entry) layer l's K5 and K6 are one kernel, counted under K5:
+    it reads S-bar once, T of the layer below, and writes T-bar."""
+    e_in, e_in_grad, e_out, e_out_grad = (bool(v) for v in edge[:4])
+    fused = int(edge[4]) if len(edge) > 4 else 0
     d = wl.dims
     L = len(d)
     w = float(wbytes) * na * N
@@ -117,9 +120,11 @@ def algorithmic(wl, na, F, Q, P, N, wbytes, keep_s, edge=(0, 0, 0, 0)):
                     wr += nblk * (d[l] + 1) * 8.0
                 add("bell_bwd", fl, rd + wr)
         skip_grad = (l == L - 1 and e_out_grad) or (l == 1 and e_in and e_in_grad)
+        both = (fused >> l) & 1 and not skip_grad
         if not skip_grad:
-            add("grad", 2.0 * d[l] * (d[l - 1] + 1) * na * N, (d[l] + d[l - 1]) * w)
-        if l > 1 and not (l == L - 1 and e_out):
+            fl = 2.0 * d[l] * (d[l - 1] + 1) * na * N + (2.0 * d[l] * d[l - 1] * na * N if both else 0.0)
+            add("grad", fl, (d[l] + d[l - 1] + (d[l - 1] if both else 0)) * w)
+        if l > 1 and not (l == L - 1 and e_out) and not both:
             add("linear_t", 2.0 * d[l] * d[l - 1] * na * N, (d[l] + d[l - 1]) * w)
     return {k: tuple(v) for k, v in cat.items()}
 
@@ -234,8 +239,8 @@ def run_mode(args, D, torch, lib, c, dev, prec, name, wbytes, mma_peak, budget=0
     kinfo = np.zeros(6, np.int32)
     lib().dnnf90_net_kernel_info(net.id, prec, kinfo.ctypes.data)
     keep_s = int(kinfo[0]) == 2
-    edge = np.zeros(4, np.int32)
-    lib().dnnf90_net_edge_info(net.id, prec, edge.ctypes.data)
+    edge = np.zeros(5, np.int32)
+    lib().dnnf90_net_step_info(net.id, prec, edge.ctypes.data)
     g = D.Grad(net)
     s = torch.cuda.Stream(device=dev)
     bw.set_stream(s.cuda_stream)
@@ -601,7 +606,7 @@ def kf_ref_timing(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default=HEADLINE)
     ap.add_argument("--mixed", default="bf16x6", choices=["bf16x6", "fp32", "bf16x3"])

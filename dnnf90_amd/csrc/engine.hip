@@ -700,6 +700,13 @@ static hipError_t forward_t(const Net &nt, BatchWork &b, const double *d_x, int 
     return e;
 }
 
+// whether layer l's K5 and K6 run as one kernel in a batched BF16X6 step
+// (layers 2 .. L-1 whose K6 is not inside the last hidden layer's adjoint)
+static bool lint_fused(const BatchWork &b, int l, int L, bool out_fused, int nd, int ndm)
+{
+    return b.prec == PREC_BF16X6 && l > 1 && !(l == L - 1 && out_fused) && grad_lint_x6_ok(nd, ndm + 1);
+}
+
 // adjoint from the output seed already in p0 (rows < nout).  rows_t == null:
 // gradient added to grad; else the per-point rows are written to rows_t.
 template <typename T>
@@ -797,6 +804,7 @@ static hipError_t backward_t(const Net &nt, BatchWork &b, double *grad, float *r
             if (e != hipSuccess) break;
         }
         const T *X = (const T *)(l == 1 ? b.t0 : b.tpl[l - 1]);
+        bool k6_done = false;   // this layer's K6 ran inside its K5
         if (l == L - 1 && out_grad) {
             // the output layer's gradient comes with the next layer's adjoint
         } else if (rows_t) {
@@ -813,12 +821,19 @@ static hipError_t backward_t(const Net &nt, BatchWork &b, double *grad, float *r
             grad_split(ncol, nd, ndm + 1, (int)b.esz, b.prec, &nc, &ch);
             {
                 PScope p(b, CAT_GRAD);
-                launch_gemm_grad<T>(b.prec, (const T *)sbp, X, nd, ndm + 1, ncol, Np, b.partial, nc, ch, s);
+                // K5 and this layer's K6 in one pass over S-bar where that kernel applies
+                if constexpr (sizeof(T) == 4)
+                    if (lint_fused(b, l, L, out_fused, nd, ndm))
+                        k6_done = launch_grad_lint_x6((const float *)sbp, (const float *)X,
+                                                      (const float *)(W + nt.woff[l]), (float *)tbp, nd, ndm + 1,
+                                                      ncol, Np, b.partial, nc, ch, s);
+                if (!k6_done)
+                    launch_gemm_grad<T>(b.prec, (const T *)sbp, X, nd, ndm + 1, ncol, Np, b.partial, nc, ch, s);
             }
             PScope p(b, CAT_GRAD_REDUCE);
             launch_grad_reduce(b.partial, nc, (long)nd * (ndm + 1), grad + nt.woff[l], s);
         }
-        if (l > 1 && !(l == L - 1 && out_fused)) {
+        if (l > 1 && !(l == L - 1 && out_fused) && !k6_done) {
             PScope p(b, CAT_LINEAR_T);
             launch_gemm_wx<T>(b.prec, true, W + nt.woff[l], ndm + 1, (const T *)sbp, (T *)tbp, ndm, nd,
                               ncol, Np, s);
@@ -2076,6 +2091,8 @@ int dnnf90_set_edge_kernels(int on)
     return prev;
 }
 
+int dnnf90_set_fused_grad(int on) { return set_grad_lint_x6(on ? 1 : 0); }
+
 int dnnf90_set_store_s(int on)
 {
     const int prev = g_store_s;
@@ -2100,7 +2117,7 @@ int dnnf90_net_kernel_info(int nid, int prec, int *info)
     return 0;
 }
 
-int dnnf90_net_edge_info(int nid, int prec, int *info)
+int dnnf90_net_step_info(int nid, int prec, int *info)
 {
     Net *nt = g_net.get(nid);
     if (!nt || !info || prec < 0 || prec > 3) return fail(-1, "bad network handle or argument");
@@ -2113,6 +2130,12 @@ int dnnf90_net_edge_info(int nid, int prec, int *info)
     info[1] = p.in_grad;
     info[2] = p.out;
     info[3] = p.out_grad;
+    // layers whose K5 and K6 are one kernel (bit l)
+    info[4] = 0;
+    const int L = nt->nlayer;
+    for (int l = 2; l < L; ++l)
+        if (prec == PREC_BF16X6 && !(l == L - 1 && p.out) && grad_lint_x6_ok(nt->dims[l], nt->dims[l - 1] + 1))
+            info[4] |= 1 << l;
     return 0;
 }
 

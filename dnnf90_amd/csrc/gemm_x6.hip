@@ -1038,6 +1038,246 @@ static bool launch_grad_x6s(const float *A, const float *B, int M, int Kd, long 
     return false;
 }
 
+// K5 and K6 of one layer in one pass over S-bar (BF16x6): the one-block K5
+// above (the whole M x Kd weight block in one workgroup, one column chunk
+// a workgroup), and for the same 32-column steps T-bar = W' S-bar, so that
+// S-bar is read from HBM once for both.  K6 needs S-bar as its B operand,
+// eight consecutive rows of one column a lane, where K5 holds eight
+// consecutive columns of one row: the threads that load S-bar's fragments
+// also write their raw values to LDS transposed (St[column][row]), and K6
+// reads its fragments from there (two 16-byte reads a lane) and splits them.
+// W' is split once into LDS in operand order as k_lin_x6c holds it (Wt[rb]
+// [kc][piece][lane]), and each 16 x 16 block of T-bar takes the six products
+// of every 32-row chunk of S-bar in chunk order with an FP32 accumulator, as
+// k_lin_x6c does: the same products in the same order, so T-bar has the same
+// bits as the separate K6 (and the K5 partials those of k_grad_x6s).  For
+// layers of 32 or 64 rows over at most 64 inputs (MB in {2, 4}, KB <= 4).
+template <int MB, int KB, int WR>
+__global__ __launch_bounds__(256, 2) void k_grad_lint_x6(const float *__restrict__ A, const float *__restrict__ B,
+                                                         const float *__restrict__ W, float *__restrict__ Tb, int M,
+                                                         int Kd, long ncol, long Np, long chunk, int nchunk,
+                                                         double *__restrict__ partial)
+{
+    constexpr int WK = 4 / WR, RW = MB / WR, CW = KB / WK;
+    static_assert(RW * WR == MB && CW * WK == KB, "wave grid divides the block grid");
+    constexpr int NF = (MB + KB) * 64;   // fragments a step
+    constexpr int NJ = (NF + 255) / 256; // fragments a thread
+    constexpr int NKC = MB / 2;          // 32-row chunks of S-bar (K6's reduction)
+    constexpr int STL = 16 * MB + 4;     // St row length (floats)
+    static_assert(MB == 2 || MB == 4, "K6 reduction of whole 32-row chunks");
+    __shared__ bf16x8 pc[(MB + KB) * 3 * 64];
+    __shared__ bf16x8 Wt[KB * NKC * 3 * 64];
+    __shared__ __attribute__((aligned(16))) float St[64 * STL];
+    const int G = gridDim.x, bid = blockIdx.x;
+    const int chk = (bid & 7) * (G >> 3) + (bid >> 3);
+    if (chk >= nchunk) return;
+    const int Kg = Kd - 1;
+    const long cb = (long)chk * chunk, ce = min(ncol, cb + chunk);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wr = wave % WR, wk = wave / WR;
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    // W' pieces: Wt rows are the Kg outputs of K6, its reduction the M rows
+    for (int u = tid; u < KB * NKC * 64; u += 256) {
+        const int rb = u / (NKC * 64), rem = u - rb * NKC * 64, kc = rem >> 6, ln = rem & 63;
+        const int m = 16 * rb + (ln & 15), k = 32 * kc + 8 * (ln >> 4);
+        float v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = (m < Kg && k + e < M) ? W[(long)(k + e) * Kd + 1 + m] : 0.f;
+        bf16x8 h, mm, l;
+        split3(v, h, mm, l);
+        bf16x8 *dst = Wt + (rb * NKC + kc) * 3 * 64 + ln;
+        dst[0] = h;
+        dst[64] = mm;
+        dst[128] = l;
+    }
+    const float *src[NJ];
+    bool live[NJ], isA[NJ];
+    double bsum[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+        const int f = tid + 256 * j, blk = f >> 6, ln = f & 63, r16 = ln & 15, qq = ln >> 4;
+        bsum[j] = 0.0;
+        isA[j] = f < NF && blk < MB;
+        if (f >= NF) {
+            live[j] = false;
+            src[j] = A;
+        } else if (blk < MB) {
+            const int m = 16 * blk + r16;
+            live[j] = m < M;
+            src[j] = A + (long)min(m, M - 1) * ncol + cb + 8 * qq;
+        } else {
+            const int k = 16 * (blk - MB) + r16;
+            live[j] = k < Kg;
+            src[j] = B + (long)min(k, Kg - 1) * ncol + cb + 8 * qq;
+        }
+    }
+    float raw[NJ][8];
+    auto load = [&](long off) {
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            if (!live[j]) continue;
+            const f4 x = *reinterpret_cast<const f4 *>(src[j] + off), y = *reinterpret_cast<const f4 *>(src[j] + off + 4);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                raw[j][e] = x[e];
+                raw[j][4 + e] = y[e];
+            }
+        }
+    };
+    flt4 acc[RW][CW];
+#pragma unroll
+    for (int i = 0; i < RW; ++i)
+#pragma unroll
+        for (int j = 0; j < CW; ++j) acc[i][j] = flt4{0, 0, 0, 0};
+    if (cb < ce) load(0);
+    const int q = lane >> 4, n16 = lane & 15;
+    for (long c = cb; c < ce; c += 32) {
+        const bool bias = c < Np;   // a 32-column step never straddles Np
+        // steps pair up: St holds the S-bar columns of two steps, and K6 runs
+        // on the second over all 64 (a chunk is a whole number of 64-column pairs)
+        const int half = (int)((c - cb) >> 5) & 1;
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const int f = tid + 256 * j;
+            if (f >= NF) continue;
+            bf16x8 h, m, l;
+            if (live[j]) {
+                if (isA[j] && bias) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) t += (double)raw[j][e];
+                    bsum[j] += t;
+                }
+                split3(raw[j], h, m, l);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) h[e] = m[e] = l[e] = (__bf16)0.f;
+            }
+            bf16x8 *d = pc + (f >> 6) * 3 * 64 + (f & 63);
+            d[0] = h;
+            d[64] = m;
+            d[128] = l;
+            if (isA[j]) {
+                // row 16 blk + r16 of S-bar, columns 8 qq .. 8 qq + 7, transposed
+                const int row = 16 * (f >> 6) + (f & 15), c8 = 32 * half + 8 * ((f & 63) >> 4);
+#pragma unroll
+                for (int e = 0; e < 8; ++e) St[(c8 + e) * STL + row] = live[j] ? raw[j][e] : 0.f;
+            }
+        }
+        if (c + 32 < ce) load(c + 32 - cb);
+        __syncthreads();
+        bf16x8 bh[CW], bm[CW], bl[CW];
+#pragma unroll
+        for (int j = 0; j < CW; ++j) {
+            const bf16x8 *s = pc + (MB + wk * CW + j) * 3 * 64 + lane;
+            bh[j] = s[0];
+            bm[j] = s[64];
+            bl[j] = s[128];
+        }
+#pragma unroll
+        for (int i = 0; i < RW; ++i) {
+            const bf16x8 *s = pc + (wr * RW + i) * 3 * 64 + lane;
+            const bf16x8 ah = s[0], am = s[64], al = s[128];
+#pragma unroll
+            for (int j = 0; j < CW; ++j) acc[i][j] = mma_x6(ah, am, al, bh[j], bm[j], bl[j], acc[i][j]);
+        }
+        // K6 on the second step of a pair: row block rb = wave, wave + 4, ...
+        // of T-bar over the pair's 64 columns as four MFMA column blocks,
+        // block j taking column 4 n + j (n = lane % 16), so a lane stores four
+        // adjacent columns of a row (sixteen lanes: 256 bytes)
+        if (half) {
+            for (int rb = wave; rb < KB; rb += 4) {
+                flt4 t[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    t[j] = flt4{0, 0, 0, 0};
+#pragma unroll
+                    for (int kc = 0; kc < NKC; ++kc) {
+                        const float *sp = St + (4 * n16 + j) * STL + 32 * kc + 8 * q;
+                        const f4 x = *reinterpret_cast<const f4 *>(sp), y = *reinterpret_cast<const f4 *>(sp + 4);
+                        float v[8];
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            v[e] = x[e];
+                            v[4 + e] = y[e];
+                        }
+                        bf16x8 sh, sm, sl;
+                        split3(v, sh, sm, sl);
+                        const bf16x8 *a = Wt + (rb * NKC + kc) * 3 * 64 + lane;
+                        t[j] = mma_x6(a[0], a[64], a[128], sh, sm, sl, kc == 0 ? flt4{0, 0, 0, 0} : t[j]);
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * rb + 4 * q + r;
+                    if (row < Kg)
+                        *reinterpret_cast<f4 *>(Tb + (long)row * ncol + (c - 32) + 4 * n16) =
+                            f4{t[0][r], t[1][r], t[2][r], t[3][r]};
+                }
+            }
+        }
+        __syncthreads();
+    }
+    double *P = partial + (long)chk * M * Kd;
+#pragma unroll
+    for (int i = 0; i < RW; ++i)
+#pragma unroll
+        for (int j = 0; j < CW; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = 16 * (wr * RW + i) + 4 * q + r, k = 16 * (wk * CW + j) + n16;
+                if (m < M && k < Kg) P[(long)m * Kd + 1 + k] = (double)acc[i][j][r];
+            }
+    double *red = reinterpret_cast<double *>(pc);   // 16 MB x 4 doubles, free after the last barrier
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+        const int f = tid + 256 * j;
+        if (isA[j]) red[((f >> 6) * 16 + (f & 15)) * 4 + ((f & 63) >> 4)] = bsum[j];
+    }
+    __syncthreads();
+    for (int m = tid; m < 16 * MB && m < M; m += 256) {
+        const double *rr = red + m * 4;
+        P[(long)m * Kd] = ((rr[0] + rr[1]) + rr[2]) + rr[3];
+    }
+}
+
+// 0: K5 and K6 of a layer as separate kernels even where the fused one applies
+static int g_grad_lint = 1;
+int set_grad_lint_x6(int on)
+{
+    const int prev = g_grad_lint;
+    if (on >= 0) g_grad_lint = on ? 1 : 0;
+    return prev;
+}
+
+// whether the fused K5 + K6 takes an M x Kd weight block (M rows: the
+// layer; Kd - 1: the layer below); chunks as grad_x6s_wgs gives them
+bool grad_lint_x6_ok(int M, int Kd)
+{
+    const int kb = pow2_blocks(Kd - 1);
+    return g_grad_lint && set_lin_x6c(-1) && (M == 32 || M == 64) && Kd - 1 >= 1 && Kd - 1 <= 64 &&
+           !(M == 32 && kb == 1);
+}
+
+bool launch_grad_lint_x6(const float *A, const float *B, const float *W, float *Tb, int M, int Kd, long ncol, long Np,
+                         double *partial, int nchunk, long chunk, hipStream_t s)
+{
+    if (!grad_lint_x6_ok(M, Kd) || chunk % 64 != 0 || ncol % 64 != 0) return false;
+    const int MB = M / 16, KB = pow2_blocks(Kd - 1);
+    long g = ((long)nchunk + 7) / 8 * 8;
+#define GL(a, b)                                                                                                     \
+    if (MB == a && KB == b) {                                                                                        \
+        constexpr int WR = (a >= 2 && b >= 2) ? 2 : (a >= 4 ? 4 : 1);                                               \
+        hipLaunchKernelGGL((k_grad_lint_x6<a, b, WR>), dim3((unsigned)g), dim3(256), 0, s, A, B, W, Tb, M, Kd, ncol,  \
+                           Np, chunk, nchunk, partial);                                                              \
+        return true;                                                                                                 \
+    }
+    GL(4, 4) GL(4, 2) GL(4, 1) GL(2, 4) GL(2, 2)
+#undef GL
+    return false;
+}
+
 // register stages of raw rows of the split K5 at 128 and at 64 rows (two
 // stages spill at 128 rows, and gain nothing measurable at 64)
 #ifndef GRAD_X6_ST8

@@ -94,15 +94,22 @@ int dnnf90_set_split_kernels(int on);
  * reference. */
 int dnnf90_set_edge_kernels(int on);
 int dnnf90_net_kernel_info(int nid, int prec, int *info);
-/* the edge kernels a batched step of many points takes in precision prec,
- * info[4] (1 taken, 0 not): [0] the first hidden layer forms S from the
- * input plane, forward and adjoint; [1] its adjoint also forms that layer's
- * weight gradient; [2] the last hidden layer forms Tbar from the output
- * layer's Sbar; [3] that adjoint also forms the output layer's weight
- * gradient (one output row).  A step of one point, or one writing per-point
- * rows, takes neither gradient-forming kernel.  No counterpart in the
- * reference. */
-int dnnf90_net_edge_info(int nid, int prec, int *info);
+/* the fused kernels a batched step of many points takes in precision
+ * prec, info[5]: [0] the first hidden layer forms S from the input plane,
+ * forward and adjoint; [1] its adjoint also forms that layer's weight
+ * gradient; [2] the last hidden layer forms Tbar from the output layer's
+ * Sbar; [3] that adjoint also forms the output layer's weight gradient (one
+ * output row) -- 1 taken, 0 not; [4] the layers l (bit l, 1-based as the
+ * ABI counts them) whose weight gradient and transposed map are one
+ * kernel.  A step of one point, or one writing per-point rows, takes none
+ * of the gradient-forming kernels.  No counterpart in the reference. */
+int dnnf90_net_step_info(int nid, int prec, int *info);
+/* BF16X6 work spaces: a hidden layer's weight gradient and transposed map
+ * (Tbar of the layer below) in one kernel that reads Sbar once, for layers
+ * of 32 or 64 neurons over at most 64 inputs; results identical bit for
+ * bit to the separate kernels.  On by default; returns the previous
+ * setting.  No counterpart in the reference. */
+int dnnf90_set_fused_grad(int on);
 
 int dnnf90_net_nweights(int nid);
 int dnnf90_net_nlayer(int nid);

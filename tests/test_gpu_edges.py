@@ -241,6 +241,53 @@ def test_split_kernels_vs_general(D, dims):
     net.free()
 
 
+@pytest.mark.parametrize("budget", [0, 1], ids=["keep", "recompute"])
+@pytest.mark.parametrize("dims", [[2, 64, 64, 64, 1], [3, 32, 32, 32, 2], [2, 64, 40, 64, 64, 1], [4, 24, 64, 1]],
+                         ids=["w64", "w32", "w40", "w24"])
+def test_fused_grad_identical(D, dims, budget):
+    """dnnf90_set_fused_grad: a BF16x6 hidden layer of 32 or 64 neurons over
+    at most 64 inputs forms its weight gradient and the layer below's Tbar
+    in one kernel that reads Sbar once.  Same products in the same order as
+    the separate kernels: the gradient must be identical bit for bit, and
+    the step must launch fewer transposed maps (dnnf90_net_step_info says
+    which layers)."""
+    from dnnf90_amd._lib import lib
+    ts, net, w = _net(D, dims[0], 4, None, 0, dims, seed=17)
+    rng = np.random.default_rng(18)
+    npt = 300
+    x = rng.uniform(-1, 1, (npt, dims[0]))
+    seed = rng.standard_normal((npt, dims[-1], ts.na))
+    print(f"fused grad: dims={dims} budget={budget}", flush=True)
+    info = np.zeros(5, np.int32)
+    assert lib().dnnf90_net_step_info(net.id, D.PREC_BF16X6, info.ctypes.data) == 0
+    out, nlin_t = [], []
+    for on in (1, 0):
+        prev = lib().dnnf90_set_fused_grad(on)
+        try:
+            bw = D.BatchWork(net, npt, D.PREC_BF16X6, budget=budget)
+            g = D.Grad(net)
+            t = np.array(bw.forward(x), copy=True)
+            bw.kernel_ms()
+            bw.profile(True)
+            bw.backward(seed, g)
+            bw.profile(False)
+            nlin_t.append(bw.kernel_ms()["linear_t"][1])
+            out.append((t, np.array(g.get(), copy=True)))
+            bw.free(); g.free()
+        finally:
+            lib().dnnf90_set_fused_grad(prev)
+    (t1, g1), (t0, g0) = out
+    assert np.all(np.isfinite(g1))
+    assert np.array_equal(t1, t0)
+    assert np.array_equal(g1, g0)
+    nf = bin(int(info[4])).count("1")
+    print("fused layers mask", int(info[4]), "linear_t launches", nlin_t)
+    assert nlin_t[0] == nlin_t[1] - nf
+    if dims[1:-1] == [64, 64, 64] or dims[1:-1] == [32, 32, 32]:
+        assert nf == 2   # layers 2 and 3; the last hidden layer's Tbar comes from the output layer's Sbar
+    net.free()
+
+
 def test_stale_dependents_refused(D):
     """Handles of work spaces and accumulators built on a network that was
     freed are refused (-1) after the network's slot is reused: the
