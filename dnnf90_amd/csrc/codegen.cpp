@@ -232,7 +232,17 @@ std::string gen_bell_source(const TableSet &t, const FwdSchedule &s, bool fp64)
     // hipRTC supplies the HIP device environment itself (no runtime header)
     o << "struct ActParams { int iact, nq; const double *tpc; int ldt; };\n"
       << kActSrc << "\n"
-      << "typedef " << (fp64 ? "double" : "float") << " real;\n";
+      << "typedef " << (fp64 ? "double" : "float") << " real;\n"
+      // the sum of v over the wave's 64 lanes by a fixed butterfly (lane 0's
+      // result is used: one fixed order)
+      << "__device__ __forceinline__ double wsum64(double v)\n{\n"
+      << "  for (int o = 32; o > 0; o >>= 1) {\n"
+      << "    const int src = (int)(((threadIdx.x & 63) ^ o) << 2);\n"
+      << "    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);\n"
+      << "    const int lo = __builtin_amdgcn_ds_bpermute(src, (int)(unsigned)b);\n"
+      << "    const int hi = __builtin_amdgcn_ds_bpermute(src, (int)(unsigned)(b >> 32));\n"
+      << "    v += __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);\n"
+      << "  }\n  return v;\n}\n";
     // the edge kernels' row blocks of one point block share operands (the
     // input values, the output layer's Sbar): the block index is remapped so
     // that each XCD (workgroups go to the 8 XCDs in turn) runs a contiguous
@@ -380,21 +390,20 @@ std::string gen_bell_source(const TableSet &t, const FwdSchedule &s, bool fp64)
     // x at slot 0 and 1 at slot e_i, so the layer's gradient needs Sbar at
     // slot 0 and at the slots e_i alone: g(j, bias) = sum_n Sbar_0 and
     // g(j, i) = sum_n (Sbar_0 x_i + Sbar_(e_i)).  The lanes of a workgroup
-    // (64 points) are summed in FP64 in point order; part[pb][j][c]
-    // takes the sums and a fixed-order reduction adds the point blocks.  No
-    // Sbar plane is written.
+    // (64 points) are summed in FP64 by a fixed butterfly over the wave
+    // (wsum64); part[pb][j][c] takes the sums and a fixed-order reduction
+    // adds the point blocks.  No Sbar plane is written.
     for (int v = 1; v < 4; v += 2) {
         const bool out_t = v & 2;
         const int d0 = t.d0;
         const std::string name = std::string("dnnf90_bell_bwds_i") + (out_t ? "o" : "") + "g";
         o << "extern \"C\" __global__ __launch_bounds__(256) void " << name << sfx << "(" << bargs
           << ", double *__restrict__ part)\n{\n"
-          << "  __shared__ double gred[256 * " << d0 + 1 << "];\n"
           << "  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;\n";
         block_ids(true);
         o << "  const int j0 = rb * (blockDim.x >> 6) + wave;\n"
           << "  const bool jok = j0 < nrow;\n"
-          << "  const int j = jok ? j0 : nrow - 1;   // every wave reaches the barrier\n"
+          << "  const int j = jok ? j0 : nrow - 1;   // every lane takes part in the butterflies\n"
           << "  const long n = (long)pb * 64 + lane;\n"
           << "  const long ldr = " << na << "L * Np;\n";
         load_s_in();
@@ -411,8 +420,9 @@ std::string gen_bell_source(const TableSet &t, const FwdSchedule &s, bool fp64)
         o << "  real sbar0;\n  {\n    real v = t0 * h0;\n";
         for (int a = 1; a < na; ++a) o << "    v += t" << a << " * h" << a << ";\n";
         o << "    sbar0 = v;\n  }\n";
-        o << "  double *gr = gred + threadIdx.x * " << d0 + 1 << ";\n"
-          << "  gr[0] = jok ? (double)sbar0 : 0.0;\n";
+        o << "  double *pj = part + ((long)pb * nrow + j) * " << d0 + 1 << ";\n"
+          << "  {\n    const double g = wsum64(jok ? (double)sbar0 : 0.0);\n"
+          << "    if (jok && lane == 0) pj[0] = g;\n  }\n";
         for (int i = 0; i < d0; ++i) {
             const int b = t.ind_e1[i];
             o << "  {\n    double g = (double)sbar0 * (double)T0[" << i << "L * ldr + n];\n";
@@ -423,30 +433,25 @@ std::string gen_bell_source(const TableSet &t, const FwdSchedule &s, bool fp64)
                       << t.pair_id[ip] << ";\n";
                 o << "    g += (double)v;\n";
             }
-            o << "    gr[" << i + 1 << "] = jok ? g : 0.0;\n  }\n";
+            o << "    g = wsum64(jok ? g : 0.0);\n"
+              << "    if (jok && lane == 0) pj[" << i + 1 << "] = g;\n  }\n";
         }
-        o << "  __syncthreads();\n"
-          << "  if (jok && lane < " << d0 + 1 << ") {\n"
-          << "    double acc = 0.0;\n"
-          << "    for (int p = 0; p < 64; ++p) acc += gred[(wave * 64 + p) * " << d0 + 1 << " + lane];\n"
-          << "    part[((long)pb * nrow + j) * " << d0 + 1 << " + lane] = acc;\n"
-          << "  }\n}\n";
+        o << "}\n";
     }
     // ---- the last hidden layer's adjoint (Tbar from the single output
     // row) with the output layer's weight gradient: T of the layer is
     // recomputed beside h from the same sums, g(0, 1 + j) = sum_n sum_a
     // Sbar_L(0, a) T_a(j) and the bias sum_n Sbar_L(0, 0), per 64-point block
-    // in part[pb][c] (c = 0 bias, 1 + j weights), summed in FP64 in
-    // point order; the Sbar plane of the layer is written as bwds_o does
+    // in part[pb][c] (c = 0 bias, 1 + j weights), summed in FP64 by the
+    // wave butterfly; the Sbar plane of the layer is written as bwds_o does
     {
         o << "extern \"C\" __global__ __launch_bounds__(256) void dnnf90_bell_bwds_og" << sfx << "(" << bargs
           << ", double *__restrict__ part)\n{\n"
-          << "  __shared__ double gred[256 * 2];\n"
           << "  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;\n";
         block_ids(true);
         o << "  const int j0 = rb * (blockDim.x >> 6) + wave;\n"
           << "  const bool jok = j0 < nrow;\n"
-          << "  const int j = jok ? j0 : nrow - 1;   // every wave reaches the barrier\n"
+          << "  const int j = jok ? j0 : nrow - 1;   // every lane takes part in the butterflies\n"
           << "  const long n = (long)pb * 64 + lane;\n"
           << "  const long ldr = " << na << "L * Np;\n"
           << "  real *sb = Sb + (long)j * ldr + n;\n"
@@ -462,18 +467,11 @@ std::string gen_bell_source(const TableSet &t, const FwdSchedule &s, bool fp64)
         o << "  if (jok) {\n";
         emit_pair_sums(o, t, fp64);
         o << "  }\n"
-          << "  gred[threadIdx.x * 2] = jok ? (double)gdot : 0.0;\n"
-          << "  gred[threadIdx.x * 2 + 1] = (double)sl00;\n"
-          << "  __syncthreads();\n"
-          << "  if (lane == 0 && jok) {\n"
-          << "    double acc = 0.0;\n"
-          << "    for (int p = 0; p < 64; ++p) acc += gred[(wave * 64 + p) * 2];\n"
-          << "    part[(long)pb * (nrow + 1) + 1 + j] = acc;\n"
-          << "  }\n"
-          << "  if (lane == 1 && wave == 0 && rb == 0) {\n"
-          << "    double acc = 0.0;\n"
-          << "    for (int p = 0; p < 64; ++p) acc += gred[p * 2 + 1];\n"
-          << "    part[(long)pb * (nrow + 1)] = acc;\n"
+          << "  {\n    const double g = wsum64(jok ? (double)gdot : 0.0);\n"
+          << "    if (lane == 0 && jok) part[(long)pb * (nrow + 1) + 1 + j] = g;\n  }\n"
+          << "  if (wave == 0 && rb == 0) {\n"
+          << "    const double g = wsum64((double)sl00);\n"
+          << "    if (lane == 0) part[(long)pb * (nrow + 1)] = g;\n"
           << "  }\n}\n";
     }
     if (fp64 && gen_point_family(t, s)) emit_point_kernels(o, t, s, load_s);
