@@ -662,7 +662,11 @@ static void lin_x6_pick(const float *W, int ldw, const float *B, float *C, int M
     while (rbt > 1 && 16 * (rbt / 2) >= M) rbt /= 2;
     while (rbt > 1 && rbt * per_rb > (size_t)144 * 1024) rbt /= 2;
     switch (rbt) {
-    case 8: lin_x6_go<kTrans, kAcc64, 8>(W, ldw, B, C, M, Kd, ncol, Np, s); break;
+    case 8:
+        // only K6 reaches 128-row tiles (K1's cap is LIN_X6_RBT_MAX); K1 at
+        // 128 rows would spill its FP64 accumulators, so it is not built
+        if constexpr (kTrans || LIN_X6_RBT_MAX >= 8) lin_x6_go<kTrans, kAcc64, 8>(W, ldw, B, C, M, Kd, ncol, Np, s);
+        break;
     case 4: lin_x6_go<kTrans, kAcc64, 4>(W, ldw, B, C, M, Kd, ncol, Np, s); break;
     case 2: lin_x6_go<kTrans, kAcc64, 2>(W, ldw, B, C, M, Kd, ncol, Np, s); break;
     default: lin_x6_go<kTrans, kAcc64, 1>(W, ldw, B, C, M, Kd, ncol, Np, s); break;
