@@ -368,6 +368,9 @@ struct BatchWork {
     bool recompute = false, s_kept = false;
     bool pt_used = false;   // the last forward went through the one-point kernel
     bool edge_in = false;   // the last forward formed the first hidden layer's S from T0 (no S plane kept)
+    // the caller drains the stream before returning (the filter's
+    // observation loop), so the forwards and adjoints record no reader event
+    bool defer_use = false;
     bool rows_next = false; // the adjoint of this forward writes per-point rows (batched kernels)
     void *hs = nullptr;
     double bytes = 0.0, bytes_stored = 0.0;   // device bytes held / needed with every h kept
@@ -677,7 +680,7 @@ static hipError_t forward_t(const Net &nt, BatchWork &b, const double *d_x, int 
             e = gen_launch_pt_fwd(*pk, (const double *)weights<T>(nt), d_x, nt.dt.ind_e1, point_net(nt, b),
                                   nt.dt.act(), s);
         }
-        nt.used_on(s);
+        if (!b.defer_use) nt.used_on(s);
         return e;
     }
     b.pt_used = false;
@@ -696,7 +699,7 @@ static hipError_t forward_t(const Net &nt, BatchWork &b, const double *d_x, int 
         l0 = 2;
     }
     for (int l = l0; l < L && e == hipSuccess; ++l) e = layer_forward<T>(nt, b, l, ncol, Np);
-    nt.used_on(s);
+    if (!b.defer_use) nt.used_on(s);
     return e;
 }
 
@@ -725,7 +728,7 @@ static hipError_t backward_t(const Net &nt, BatchWork &b, double *grad, float *r
             e = gen_launch_pt_bwd(nt.gen[1], (const double *)W, grad, (const double *)b.p0, point_net(nt, b),
                                   nt.dt.act(), s);
         }
-        nt.used_on(s);
+        if (!b.defer_use) nt.used_on(s);
         return e;
     }
     hipError_t e = hipSuccess;
@@ -839,7 +842,7 @@ static hipError_t backward_t(const Net &nt, BatchWork &b, double *grad, float *r
                               ncol, Np, s);
         }
     }
-    nt.used_on(s);
+    if (!b.defer_use) nt.used_on(s);
     return e;
 }
 
@@ -1325,6 +1328,8 @@ static void kf_observe(Net &nt, BatchWork &b, KFilter &kf, const double *dx, con
         PScope p(b, CAT_OTHER);
         launch_fill(kf.g, kf.nw, 0.0, s);
     }
+    // kf_finish drains the stream after the observations: no reader events
+    b.defer_use = true;
     hipError_t e = forward_t<double>(nt, b, dx, 1);
     if (e && !kf.lerr) kf.lerr = e;
     {
@@ -1337,6 +1342,7 @@ static void kf_observe(Net &nt, BatchWork &b, KFilter &kf, const double *dx, con
             launch_seed_user<double>(dseed, 1, nout, na, (double *)b.p0, b.np, s);
     }
     e = backward_t<double>(nt, b, kf.g, nullptr, 0);
+    b.defer_use = false;
     if (e && !kf.lerr) kf.lerr = e;
     kf_step(nt, kf, kf.g, scale, xmode, v, dseed, b.d_tout, nout * na, iekf, final, s, &b);
 }
