@@ -1627,8 +1627,13 @@ int dnnf90_eval(int nid, int wid, const double *x, double *t)
     BatchWork &b = *g_work.obj[wid];
     const int na = nt->tab.na, nout = nt->dims[nt->nlayer - 1];
     std::vector<double> tout((size_t)nout * na);
+    b.defer_use = true;   // the stream is drained below: no reader event
     int rc = run_forward<double>(*nt, b, 1, x, tout.data(), false);
-    if (rc) return rc;
+    b.defer_use = false;
+    if (rc) {
+        nt->used_on(b.stream);   // not drained on this path
+        return rc;
+    }
     HIPCHK(hipStreamSynchronize(b.stream));
     std::memcpy(t, tout.data(), sizeof(double) * na);  // output 1
     return 0;
@@ -1684,13 +1689,22 @@ int dnnf90_grad_point(int nid, int tid, const double *x, const double *seed, int
     if (!nt || !dependent_ok(g_twork, tid, nid) || !grad_ok(gid, nid) || !x || !seed)
         return fail(-1, "bad handle or argument");
     BatchWork &b = *g_twork.obj[tid];
+    b.defer_use = true;   // the stream is drained below: no reader events
     int rc = run_forward<double>(*nt, b, 1, x, nullptr, false);
-    if (rc) return rc;
+    if (rc) {
+        b.defer_use = false;
+        nt->used_on(b.stream);   // not drained on this path
+        return rc;
+    }
     const int na = nt->tab.na, nout = nt->dims[nt->nlayer - 1];
     std::vector<double> sm((size_t)nout * na, 0.0);
     std::memcpy(sm.data(), seed, sizeof(double) * na);  // output 1 only
     rc = run_backward_seed<double>(*nt, b, sm.data(), *g_grad.obj[gid], false);
-    if (rc) return rc;
+    b.defer_use = false;
+    if (rc) {
+        nt->used_on(b.stream);   // not drained on this path
+        return rc;
+    }
     HIPCHK(hipStreamSynchronize(b.stream));
     return 0;
 }
