@@ -475,11 +475,15 @@ static int valloc(void **p, size_t bytes)
 // one-block split K5 takes one chunk per workgroup it keeps resident, the
 // tiled kernels theirs (grad_chunks).  any: the larger of the two, whatever
 // the split-kernel switch (a work space's partials are sized once)
-static void grad_split(long ncol, int M, int Kd, int esz, int prec, int *nchunk, long *chunk, bool any = false)
+static void grad_split(long ncol, int M, int Kd, int esz, int prec, int *nchunk, long *chunk, bool any = false,
+                       bool fused = false)
 {
     grad_chunks(ncol, M, Kd, esz, nchunk, chunk);
     if (prec != PREC_BF16X6) return;
-    const int wgs = grad_x6s_wgs(M, Kd, any);
+    // fused: the layer's K5 runs with its K6 (k_grad_lint_x6), which keeps
+    // three workgroups a CU; any: room for whichever runs
+    int wgs = fused ? grad_lint_x6_wgs(M, Kd) : grad_x6s_wgs(M, Kd, any);
+    if (any) wgs = std::max(wgs, grad_lint_x6_wgs(M, Kd, true));
     if (wgs <= 0) return;
     const long nc = std::min((long)wgs, std::max(1L, ncol / 128));
     const long ch = (((ncol + nc - 1) / nc) + 127) / 128 * 128;
@@ -821,12 +825,13 @@ static hipError_t backward_t(const Net &nt, BatchWork &b, double *grad, float *r
         } else {
             int nc;
             long ch;
-            grad_split(ncol, nd, ndm + 1, (int)b.esz, b.prec, &nc, &ch);
+            const bool fuse = lint_fused(b, l, L, out_fused, nd, ndm);
+            grad_split(ncol, nd, ndm + 1, (int)b.esz, b.prec, &nc, &ch, false, fuse);
             {
                 PScope p(b, CAT_GRAD);
                 // K5 and this layer's K6 in one pass over S-bar where that kernel applies
                 if constexpr (sizeof(T) == 4)
-                    if (lint_fused(b, l, L, out_fused, nd, ndm))
+                    if (fuse)
                         k6_done = launch_grad_lint_x6((const float *)sbp, (const float *)X,
                                                       (const float *)(W + nt.woff[l]), (float *)tbp, nd, ndm + 1,
                                                       ncol, Np, b.partial, nc, ch, s);

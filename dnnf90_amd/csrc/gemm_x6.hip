@@ -1050,14 +1050,16 @@ static bool launch_grad_x6s(const float *A, const float *B, int M, int Kd, long 
 // consecutive columns of one row: the threads that load S-bar's fragments
 // also write their raw values to LDS transposed (St[column][row]), and K6
 // reads its fragments from there (two 16-byte reads a lane) and splits them.
-// W' is split once into LDS in operand order as k_lin_x6c holds it (Wt[rb]
-// [kc][piece][lane]), and each 16 x 16 block of T-bar takes the six products
+// Each wave holds its row block of W' split in registers, in the operand
+// order of k_lin_x6c's LDS, and each 16 x 16 block of T-bar takes the six products
 // of every 32-row chunk of S-bar in chunk order with an FP32 accumulator, as
 // k_lin_x6c does: the same products in the same order, so T-bar has the same
-// bits as the separate K6 (and the K5 partials those of k_grad_x6s).  For
-// layers of 32 or 64 rows over at most 64 inputs (MB in {2, 4}, KB <= 4).
+// bits as the separate K6.  The K5 partials are k_grad_x6s's over this
+// kernel's column chunks (three workgroups a CU: W' in registers, 42 KB of
+// LDS).  For layers of 32 or 64 rows over at most 64 inputs (MB in {2, 4},
+// KB <= 4).
 template <int MB, int KB, int WR>
-__global__ __launch_bounds__(256, 2) void k_grad_lint_x6(const float *__restrict__ A, const float *__restrict__ B,
+__global__ __launch_bounds__(256, 3) void k_grad_lint_x6(const float *__restrict__ A, const float *__restrict__ B,
                                                          const float *__restrict__ W, float *__restrict__ Tb, int M,
                                                          int Kd, long ncol, long Np, long chunk, int nchunk,
                                                          double *__restrict__ partial)
@@ -1069,8 +1071,8 @@ __global__ __launch_bounds__(256, 2) void k_grad_lint_x6(const float *__restrict
     constexpr int NKC = MB / 2;          // 32-row chunks of S-bar (K6's reduction)
     constexpr int STL = 16 * MB + 4;     // St row length (floats)
     static_assert(MB == 2 || MB == 4, "K6 reduction of whole 32-row chunks");
+    static_assert(KB <= 4, "one K6 row block a wave");
     __shared__ bf16x8 pc[(MB + KB) * 3 * 64];
-    __shared__ bf16x8 Wt[KB * NKC * 3 * 64];
     __shared__ __attribute__((aligned(16))) float St[64 * STL];
     const int G = gridDim.x, bid = blockIdx.x;
     const int chk = (bid & 7) * (G >> 3) + (bid >> 3);
@@ -1081,19 +1083,21 @@ __global__ __launch_bounds__(256, 2) void k_grad_lint_x6(const float *__restrict
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wr = wave % WR, wk = wave / WR;
     typedef float f4 __attribute__((ext_vector_type(4)));
-    // W' pieces: Wt rows are the Kg outputs of K6, its reduction the M rows
-    for (int u = tid; u < KB * NKC * 64; u += 256) {
-        const int rb = u / (NKC * 64), rem = u - rb * NKC * 64, kc = rem >> 6, ln = rem & 63;
-        const int m = 16 * rb + (ln & 15), k = 32 * kc + 8 * (ln >> 4);
-        float v[8];
+    // W' pieces of the wave's K6 row block rb = wave (rows of T-bar: the Kg
+    // outputs; reduction: the M rows of S-bar), in registers, as k_lin_x6c's
+    // LDS holds them: lane (m = lane % 16, q = lane / 16) has rows
+    // 32 kc + 8 q .. +7 of output row 16 rb + m
+    bf16x8 wh[NKC], wm[NKC], wl[NKC];
+    {
+        const int m = 16 * wave + (lane & 15);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) v[e] = (m < Kg && k + e < M) ? W[(long)(k + e) * Kd + 1 + m] : 0.f;
-        bf16x8 h, mm, l;
-        split3(v, h, mm, l);
-        bf16x8 *dst = Wt + (rb * NKC + kc) * 3 * 64 + ln;
-        dst[0] = h;
-        dst[64] = mm;
-        dst[128] = l;
+        for (int kc = 0; kc < NKC; ++kc) {
+            const int k = 32 * kc + 8 * (lane >> 4);
+            float v[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] = (wave < KB && m < Kg && k + e < M) ? W[(long)(k + e) * Kd + 1 + m] : 0.f;
+            split3(v, wh[kc], wm[kc], wl[kc]);
+        }
     }
     const float *src[NJ];
     bool live[NJ], isA[NJ];
@@ -1186,12 +1190,13 @@ __global__ __launch_bounds__(256, 2) void k_grad_lint_x6(const float *__restrict
 #pragma unroll
             for (int j = 0; j < CW; ++j) acc[i][j] = mma_x6(ah, am, al, bh[j], bm[j], bl[j], acc[i][j]);
         }
-        // K6 on the second step of a pair: row block rb = wave, wave + 4, ...
-        // of T-bar over the pair's 64 columns as four MFMA column blocks,
-        // block j taking column 4 n + j (n = lane % 16), so a lane stores four
-        // adjacent columns of a row (sixteen lanes: 256 bytes)
-        if (half) {
-            for (int rb = wave; rb < KB; rb += 4) {
+        // K6 on the second step of a pair: row block rb = wave of T-bar over
+        // the pair's 64 columns as four MFMA column blocks, block j taking
+        // column 4 n + j (n = lane % 16), so a lane stores four adjacent
+        // columns of a row (sixteen lanes: 256 bytes)
+        if (half && wave < KB) {
+            {
+                const int rb = wave;
                 flt4 t[4];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
@@ -1208,8 +1213,7 @@ __global__ __launch_bounds__(256, 2) void k_grad_lint_x6(const float *__restrict
                         }
                         bf16x8 sh, sm, sl;
                         split3(v, sh, sm, sl);
-                        const bf16x8 *a = Wt + (rb * NKC + kc) * 3 * 64 + lane;
-                        t[j] = mma_x6(a[0], a[64], a[128], sh, sm, sl, kc == 0 ? flt4{0, 0, 0, 0} : t[j]);
+                        t[j] = mma_x6(wh[kc], wm[kc], wl[kc], sh, sm, sl, kc == 0 ? flt4{0, 0, 0, 0} : t[j]);
                     }
                 }
 #pragma unroll
@@ -1257,11 +1261,19 @@ int set_grad_lint_x6(int on)
 
 // whether the fused K5 + K6 takes an M x Kd weight block (M rows: the
 // layer; Kd - 1: the layer below); chunks as grad_x6s_wgs gives them
-bool grad_lint_x6_ok(int M, int Kd)
+static bool grad_lint_x6_fits(int M, int Kd)
 {
     const int kb = pow2_blocks(Kd - 1);
-    return g_grad_lint && set_lin_x6c(-1) && (M == 32 || M == 64) && Kd - 1 >= 1 && Kd - 1 <= 64 &&
-           !(M == 32 && kb == 1);
+    return (M == 32 || M == 64) && Kd - 1 >= 1 && Kd - 1 <= 64 && !(M == 32 && kb == 1);
+}
+
+bool grad_lint_x6_ok(int M, int Kd) { return g_grad_lint && set_lin_x6c(-1) && grad_lint_x6_fits(M, Kd); }
+
+// column chunks the fused kernel takes: one resident round at three
+// workgroups a CU (its LDS allows three), 0 where it does not apply
+int grad_lint_x6_wgs(int M, int Kd, bool any)
+{
+    return (any ? grad_lint_x6_fits(M, Kd) : grad_lint_x6_ok(M, Kd)) ? 256 * 3 : 0;
 }
 
 bool launch_grad_lint_x6(const float *A, const float *B, const float *W, float *Tb, int M, int Kd, long ncol, long Np,

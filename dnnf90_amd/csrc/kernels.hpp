@@ -106,8 +106,9 @@ void launch_grad_x6(const float *A, const float *B, int M, int Kd, long ncol, lo
                     int nchunk, long chunk, hipStream_t s);
 
 // K5 and K6 of one layer in one kernel on BF16 three-way split products:
-// the K5 chunk partials of launch_grad_x6 and Tb[Kd-1][ncol] = W' A as
-// launch_lin_x6(true, ...) gives them (the same bits), S-bar read once.
+// K5 chunk partials as launch_grad_x6 forms them (over its own chunks) and
+// Tb[Kd-1][ncol] = W' A as launch_lin_x6(true, ...) gives it (the same
+// bits), S-bar read once.
 // grad_lint_x6_ok: whether it takes an M x Kd weight block (the switch on,
 // M 32 or 64, Kd - 1 <= 64); chunks as grad_x6s_wgs gives them.
 // set_grad_lint_x6: on (1) / off (0) / query (-1), returns the previous
@@ -115,6 +116,9 @@ bool grad_lint_x6_ok(int M, int Kd);
 bool launch_grad_lint_x6(const float *A, const float *B, const float *W, float *Tb, int M, int Kd, long ncol, long Np,
                          double *partial, int nchunk, long chunk, hipStream_t s);
 int set_grad_lint_x6(int on);
+// column chunks the fused kernel takes for an M x Kd block (0: not
+// applicable); any: whatever the switches say (work-space sizing)
+int grad_lint_x6_wgs(int M, int Kd, bool any = false);
 
 // column chunks of launch_gemm_grad for an M x Kd weight block over ncol
 // columns of esz-byte elements (chunk: a multiple of 128 columns)

@@ -106,9 +106,11 @@ int dnnf90_net_kernel_info(int nid, int prec, int *info);
 int dnnf90_net_step_info(int nid, int prec, int *info);
 /* BF16X6 work spaces: a hidden layer's weight gradient and transposed map
  * (Tbar of the layer below) in one kernel that reads Sbar once, for layers
- * of 32 or 64 neurons over at most 64 inputs; results identical bit for
- * bit to the separate kernels.  On by default; returns the previous
- * setting.  No counterpart in the reference. */
+ * of 32 or 64 neurons over at most 64 inputs.  Tbar is identical bit for
+ * bit to the separate map's; the layer's gradient sums the same products
+ * over other column chunks (agreement to FP32 rounding, about 1e-7
+ * relative).  On by default; returns the previous setting.  No counterpart
+ * in the reference. */
 int dnnf90_set_fused_grad(int on);
 
 int dnnf90_net_nweights(int nid);

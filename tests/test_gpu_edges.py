@@ -247,9 +247,12 @@ def test_split_kernels_vs_general(D, dims):
 def test_fused_grad_identical(D, dims, budget):
     """dnnf90_set_fused_grad: a BF16x6 hidden layer of 32 or 64 neurons over
     at most 64 inputs forms its weight gradient and the layer below's Tbar
-    in one kernel that reads Sbar once.  Same products in the same order as
-    the separate kernels: the gradient must be identical bit for bit, and
-    the step must launch fewer transposed maps (dnnf90_net_step_info says
+    in one kernel that reads Sbar once.  Tbar takes the same products in the
+    same order as the separate map, so everything computed from it -- the
+    first hidden layer's gradient here -- is identical bit for bit; the fused
+    layers' own gradients sum the same products over more column chunks
+    (three workgroups a CU instead of two), within 1e-6 of max |gradient|.
+    The step must launch fewer transposed maps (dnnf90_net_step_info says
     which layers)."""
     from dnnf90_amd._lib import lib
     ts, net, w = _net(D, dims[0], 4, None, 0, dims, seed=17)
@@ -279,7 +282,9 @@ def test_fused_grad_identical(D, dims, budget):
     (t1, g1), (t0, g0) = out
     assert np.all(np.isfinite(g1))
     assert np.array_equal(t1, t0)
-    assert np.array_equal(g1, g0)
+    n1 = dims[1] * (dims[0] + 1)   # the first hidden layer's weights lead the vector
+    assert np.array_equal(g1[:n1], g0[:n1])
+    assert rel(g1, g0) <= 1e-6
     nf = bin(int(info[4])).count("1")
     print("fused layers mask", int(info[4]), "linear_t launches", nlin_t)
     assert nlin_t[0] == nlin_t[1] - nf
