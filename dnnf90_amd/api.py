@@ -287,7 +287,10 @@ class BatchWork:
         return {c: (float(ms[i]), int(cnt[i])) for i, c in enumerate(self.KERNEL_CATEGORIES)}
 
     def forward(self, x, tout=None, want=True):
-        """x [npt][d0]; returns T [npt][nout][na] (numpy, or the given tout)."""
+        """x [npt][d0]; returns T [npt][nout][na] (numpy, or the given tout).
+        With device tensors the kernels run on the work space's stream: the
+        result is complete after sync(), or in stream order on the stream
+        given to set_stream()."""
         xk, px, dev = _ptr(x, np.float64)
         npt = int(x.shape[0])
         if want and tout is None:

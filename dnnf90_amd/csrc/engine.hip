@@ -372,6 +372,10 @@ struct BatchWork {
     // observation loop), so the forwards and adjoints record no reader event
     bool defer_use = false;
     bool rows_next = false; // the adjoint of this forward writes per-point rows (batched kernels)
+    // debug mode (dnnf90_set_poison): the planes, scratch and partials below
+    // are filled with NaN at creation and before every batched forward
+    bool poison = false;
+    std::vector<std::pair<void *, size_t>> poisoned;
     void *hs = nullptr;
     double bytes = 0.0, bytes_stored = 0.0;   // device bytes held / needed with every h kept
     void *hplane(int l) const { return recompute ? hs : hpl[l]; }
@@ -457,6 +461,17 @@ struct PScope {
 };
 
 static long round_up(long v, long m) { return (v + m - 1) / m * m; }
+
+static int g_poison = 0;
+
+// every byte 0xFF: a NaN in FP32 and in FP64
+static hipError_t poison_planes(BatchWork &b)
+{
+    for (auto &pb : b.poisoned)
+        if (pb.first && pb.second)
+            if (hipError_t e = hipMemsetAsync(pb.first, 0xFF, pb.second, b.stream)) return e;
+    return hipSuccess;
+}
 
 static int valloc(void **p, size_t bytes)
 {
@@ -552,6 +567,20 @@ static int bwork_init(BatchWork &b, const Net &nt, int maxpt, int prec, double b
     if ((rc = dalloc(&b.d_tout, (size_t)b.np_cap * nt.dims[L - 1] * na))) return rc;
     if ((rc = dalloc(&b.d_rsq_part, (size_t)(b.np_cap / 256 + 1)))) return rc;
     if ((rc = dalloc(&b.d_rsq, 1))) return rc;
+    b.poison = g_poison != 0;
+    if (b.poison) {
+        const size_t pl = col * b.esz;
+        b.poisoned = {{b.t0, nt.dims[0] * pl}, {b.tl, nt.dims[L - 1] * pl}, {b.p0, nmax * pl}, {b.p1, nmax * pl},
+                      {b.partial, sizeof(double) * (size_t)pc}};
+        for (int l = 1; l < L - 1; ++l) {
+            b.poisoned.push_back({b.tpl[l], nt.dims[l] * pl});
+            if (b.hpl[l]) b.poisoned.push_back({b.hpl[l], nt.dims[l] * pl});
+        }
+        if (b.hs) b.poisoned.push_back({b.hs, nmax * pl});
+        if (b.scratch) b.poisoned.push_back({b.scratch, (size_t)nt.sched.nentry * nmax * b.np_cap * b.esz});
+        HIPCHK(poison_planes(b));
+        HIPCHK(hipStreamSynchronize(b.stream));
+    }
     return 0;
 }
 
@@ -688,6 +717,8 @@ static hipError_t forward_t(const Net &nt, BatchWork &b, const double *d_x, int 
         return e;
     }
     b.pt_used = false;
+    if (b.poison)
+        if (hipError_t e = poison_planes(b)) return e;
     {
         PScope p(b, CAT_OTHER);
         launch_input<T>(d_x, npt, nt.dims[0], na, nt.dt.ind_e1, (T *)b.t0, Np, s);
@@ -2117,6 +2148,13 @@ int dnnf90_set_edge_kernels(int on)
 }
 
 int dnnf90_set_fused_grad(int on) { return set_grad_lint_x6(on ? 1 : 0); }
+
+int dnnf90_set_poison(int on)
+{
+    const int prev = g_poison;
+    g_poison = on ? 1 : 0;
+    return prev;
+}
 
 int dnnf90_set_store_s(int on)
 {

@@ -112,6 +112,14 @@ int dnnf90_net_step_info(int nid, int prec, int *info);
  * relative).  On by default; returns the previous setting.  No counterpart
  * in the reference. */
 int dnnf90_set_fused_grad(int on);
+/* Debug mode for reads of unwritten memory: batched work spaces created
+ * while it is on fill their planes, operand scratch and gradient partials
+ * with NaN (every byte 0xFF) at creation and again before every batched
+ * forward, so a value that a step reads without having written it
+ * propagates as NaN.  Results must not change (tested).  Off by default;
+ * returns the previous setting.  No counterpart in the reference (whose
+ * hardened build poisons its reals with signalling NaN, Makefile:254-258). */
+int dnnf90_set_poison(int on);
 
 int dnnf90_net_nweights(int nid);
 int dnnf90_net_nlayer(int nid);
