@@ -1,5 +1,6 @@
 """Parity at the benchmark sizes (dense4_k3: N = 65 536, |A| = 35, width 128;
-closure4_k7: N = 16 384, |A| = 41, width 64), through properties that hold
+closure4_k7: N = 16 384, |A| = 41, width 64; gelu3_k4: N = 131 072, |A| = 35,
+width 256, erf-GELU; kdv: N = 1 024, |A| = 5), through properties that hold
 at any size, so that the oracle (the C restatement of
 lib/train_module.f90:241-450) only runs on samples:
 
@@ -33,7 +34,7 @@ def D():
     return dnnf90_amd
 
 
-@pytest.mark.parametrize("name", ["dense4_k3", "closure4_k7"])
+@pytest.mark.parametrize("name", ["dense4_k3", "closure4_k7", "gelu3_k4", "kdv"])
 def test_full_size(D, oracle_mod, name):
     from dnnf90_amd.workloads import WORKLOADS
     wl = WORKLOADS[name]
