@@ -328,7 +328,11 @@ def test_ngd_solve_vs_oracle(D, oracle_mod, mode, mu, dual):
     """Gram metric (FP32 MFMA) + FP64 Cholesky, primal (nw x nw) or dual
     (nrow x nrow, app/optimizer_module.f90:408-446), against the oracle's
     primal Gaussian elimination (app/optimizer_module.f90:221-522) on the
-    same FP32 rows; both routes give the same step to rounding."""
+    same FP32 rows.  The dual route is FP64 throughout (its Gram matrix of
+    the FP32 rows is exact to FP64 rounding), so it must agree to 1e-14
+    times the condition number; the primal route's metric comes from FP32
+    MFMA, which bounds it at 1e-7 times the condition number.  Both use the
+    same blocked FP64 Cholesky kernels."""
     dims = [2, 16, 16, 1]
     npt = 200
     net, bw, rows_t, ld, b, o, w, x, terms = _ngd_inputs(D, oracle_mod, dims, npt)
@@ -340,7 +344,7 @@ def test_ngd_solve_vs_oracle(D, oracle_mod, mode, mu, dual):
     cond = np.linalg.cond(G + c * np.eye(len(b)))
     e = rel(u, ur)
     print("ngd mode", mode, "rel err vs oracle", e, "cond", cond)
-    assert e <= 1e-7 * cond
+    assert e <= (1e-14 if dual else 1e-7) * cond
     # the oracle's rows (FP64) agree with the GPU's FP32 rows to FP32 rounding
     assert rel(J, o.pinn_rows(dims, w, x, terms)) <= 1e-6
 
@@ -360,14 +364,14 @@ def test_ngd_full_size_backward_error(D, oracle_mod):
     gnorm = (Jt * Jt).sum().item() / npt + 1e-3   # Frobenius-type bound of the damped metric
     be = r.norm().item() / (gnorm * ut.norm().item() + np.linalg.norm(b) / npt)
     print("ngd full-size backward error", be)
-    assert be <= 1e-6
+    assert be <= 1e-7   # the FP32-MFMA metric's rounding (2.7e-8 measured)
     # the dual (Gram, 4096 x 4096) route solves the same system
     ud = bw.ngd_solve(rows_t, ld, npt, b, npt, 1e-3, 1, dual=True)
     udt = torch.from_numpy(ud).cuda()
     rd = (Jt @ (Jt.T @ udt)) / npt + 1e-3 * udt - torch.from_numpy(b).cuda() / npt
     bed = rd.norm().item() / (gnorm * udt.norm().item() + np.linalg.norm(b) / npt)
     print("ngd dual backward error", bed, "rel diff to primal", rel(ud, u))
-    assert bed <= 1e-6
+    assert bed <= 1e-11   # FP64 throughout (2.9e-13 measured)
 
 
 @pytest.mark.parametrize("prec,tol", [(0, TOL64), (3, 2e-6)], ids=["fp64", "bf16x6"])
