@@ -69,6 +69,8 @@ SIGNATURES = {
     "dnnf90_net_kernel_info": (_i, [_i, _i, _vp]),
     "dnnf90_net_step_info": (_i, [_i, _i, _vp]),
     "dnnf90_set_fused_grad": (_i, [_i]),
+    "dnnf90_set_fused_layers": (_i, [_i]),
+    "dnnf90_net_fused_info": (_i, [_i, _vp]),
     "dnnf90_set_poison": (_i, [_i]),
     "dnnf90_net_nweights": (_i, [_i]),
     "dnnf90_net_nlayer": (_i, [_i]),

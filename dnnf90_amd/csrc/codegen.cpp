@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <functional>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -48,15 +49,19 @@ static int kq_of(int k)
 // as registers), T_a = d1 S_a + sum_q d_q v and h_a = d2 S_a + sum_q d_(q+1) v.
 // REC_T stores T_a, REC_H stores h_a, REC_HREG assigns h_a to register h<a>.
 // REC_TDOT adds T_a * Sbar_L(0, a) (the single output row, through slz)
-// into gdot.
-enum { REC_T = 1, REC_H = 2, REC_HREG = 4, REC_TDOT = 8 };
+// into gdot.  REC_TOUT hands T_a to the kernel's FZ_OUT(a, t) (the fused
+// last hidden layer's output row).  st: the slot stride of the T store and
+// of slz.
+enum { REC_T = 1, REC_H = 2, REC_HREG = 4, REC_TDOT = 8, REC_TOUT = 16 };
 
-static void emit_recurrence(std::ostringstream &o, const TableSet &t, const FwdSchedule &s, bool fp64, int m)
+static void emit_recurrence(std::ostringstream &o, const TableSet &t, const FwdSchedule &s, bool fp64, int m,
+                            const char *st = "Np", const std::function<void(int)> &pre_slot = nullptr)
 {
-    const bool wt = (m & (REC_T | REC_TDOT)) != 0, wh = (m & (REC_H | REC_HREG)) != 0;
+    const bool wt = (m & (REC_T | REC_TDOT | REC_TOUT)) != 0, wh = (m & (REC_H | REC_HREG)) != 0;
     for (int e = t.na; e < s.nentry; ++e) o << "  real e" << e << ";\n";
     for (int a = 1; a < t.na; ++a) {
         const int g0 = s.slot_group[a], g1 = s.slot_group[a + 1];
+        if (pre_slot) pre_slot(a);
         o << "  {\n";
         if (wt) o << "    real t = d1 * e" << a << ";\n";
         if (wh) o << "    real h = d2 * e" << a << ";\n";
@@ -71,27 +76,38 @@ static void emit_recurrence(std::ostringstream &o, const TableSet &t, const FwdS
             if (wh) o << "      h += d" << q + 1 << " * v;\n";
             o << "    }\n";
         }
-        if (m & REC_T) o << "    Tj[" << a << "L * Np] = t;\n";
+        if (m & REC_T) o << "    Tj[" << a << "L * " << st << "] = t;\n";
         if (m & REC_H) o << "    Hj[" << a << "L * Np] = h;\n";
         if (m & REC_HREG) o << "    h" << a << " = h;\n";
-        if (m & REC_TDOT) o << "    gdot += t * slz[" << a << "L * Np];\n";
+        if (m & REC_TDOT) o << "    gdot += t * slz[" << a << "L * " << st << "];\n";
+        if (m & REC_TOUT) o << "    FZ_OUT(" << a << ", t);\n";
         o << "  }\n";
     }
 }
 
 // Sbar_0 = sum_a Tbar_a h_a and the pair sums Sbar_b = sum binom Tbar_ia h_id
-// (lib/multi_index_bell_module.f90:267-297) from registers t<a>, h<a>
-static void emit_pair_sums(std::ostringstream &o, const TableSet &t, bool fp64)
+// (lib/multi_index_bell_module.f90:267-297) from registers t<a>, h<a>; into
+// sb[b * Np], or (reg) into registers sb<b> the caller declares
+static void emit_pair_sums(std::ostringstream &o, const TableSet &t, bool fp64, bool reg = false,
+                           const std::function<void(int)> &post_slot = nullptr)
 {
+    auto put = [&](int b) {
+        if (reg) o << "    sb" << b << " = v;\n";
+        else o << "    sb[" << b << "L * Np] = v;\n";
+    };
     o << "  {\n    real v = t0 * h0;\n";
     for (int a = 1; a < t.na; ++a) o << "    v += t" << a << " * h" << a << ";\n";
-    o << "    sb[0] = v;\n  }\n";
+    put(0);
+    o << "  }\n";
+    if (post_slot) post_slot(0);
     for (int b = 1; b < t.na; ++b) {
         o << "  {\n    real v = (real)0;\n";
         for (int ip = t.pair_start[b]; ip < t.pair_start[b] + t.pair_num[b]; ++ip)
             o << "    v += " << lit(t.pair_binom[ip], fp64) << " * t" << t.pair_ia[ip] << " * h" << t.pair_id[ip]
               << ";\n";
-        o << "    sb[" << b << "L * Np] = v;\n  }\n";
+        put(b);
+        o << "  }\n";
+        if (post_slot) post_slot(b);
     }
 }
 
@@ -484,13 +500,12 @@ bool gen_bell_usable(const TableSet &t, const FwdSchedule &s)
     return t.nfq() <= 4096 && t.npair() <= 4096 && s.nentry <= 320 && t.na <= 160;
 }
 
-int gen_bell_compile(const TableSet &t, const FwdSchedule &s, bool fp64, GenKernels *g, std::string *log)
+// hipRTC for gfx950 (packed FP32 instructions off, as for every kernel of
+// the library); 0 and the code object, or -3 with the log
+static int rtc_compile(const std::string &src, const char *name, std::vector<char> *code, std::string *log)
 {
-    *g = GenKernels();
-    const std::string src = gen_bell_source(t, s, fp64);
     hiprtcProgram prog;
-    if (hiprtcCreateProgram(&prog, src.c_str(), "dnnf90_bell_gen.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
-        return -3;
+    if (hiprtcCreateProgram(&prog, src.c_str(), name, 0, nullptr, nullptr) != HIPRTC_SUCCESS) return -3;
     const char *opts[] = {"--gpu-architecture=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature",
                           "-Xclang", "-packed-fp32-ops"};
     const hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
@@ -505,9 +520,17 @@ int gen_bell_compile(const TableSet &t, const FwdSchedule &s, bool fp64, GenKern
     }
     size_t sz = 0;
     hiprtcGetCodeSize(prog, &sz);
-    std::vector<char> code(sz);
-    hiprtcGetCode(prog, code.data());
+    code->assign(sz, 0);
+    hiprtcGetCode(prog, code->data());
     hiprtcDestroyProgram(&prog);
+    return 0;
+}
+
+int gen_bell_compile(const TableSet &t, const FwdSchedule &s, bool fp64, GenKernels *g, std::string *log)
+{
+    *g = GenKernels();
+    std::vector<char> code;
+    if (rtc_compile(gen_bell_source(t, s, fp64), "dnnf90_bell_gen.hip", &code, log)) return -3;
     return gen_bell_load(code, fp64, g);
 }
 
@@ -680,6 +703,536 @@ hipError_t gen_launch_pt_bwd(const GenKernels &g, const double *W, double *grad,
     PtNet q = p;
     void *params[] = {&W, &grad, &Sb, &q, &a};
     return hipModuleLaunchKernel(g.pt_bwd, 1, 1, 1, 256, 1, 1, 0, s, params, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// Fused layer kernels (FP32 storage) for networks whose hidden layers are all
+// FZ_W = 64 wide, with at least two hidden layers and one output row.
+//
+// Their planes are point-blocked: a block of FZ_P = 4 points holds, slot by
+// slot and point by point, the 64 neurons' values ([block][slot][point]
+// [neuron], na KB a block, contiguous; a wave's 64 lanes then touch four
+// 64-byte runs in lane order, which streams at 6.7 TB/s where the neuron-
+// outer order took 4.3), and one workgroup of four waves covers one
+// block, every neuron and every slot.  Wave w, lane l owns neuron
+// j = 16 w + (l mod 16) at point p = l / 16.  A block the linear maps read is
+// staged in LDS as X[slot * FZ_XS + point * FZ_PJ + neuron] (its own order,
+// rows padded to FZ_PJ and slots to FZ_XS floats); the maps run on
+// v_mfma_f32_16x16x4f32:
+//   K1 (lib/train_module.f90:262-278) and K6 (:430-444): rows (point, slot),
+//   columns the 16 neurons of the wave, k the 64 neurons of the block; the
+//   result leaves lane l holding slots 4t .. 4t+3 of its (neuron, point) --
+//   the recurrence's operands, in registers, so S and T-bar never go to
+//   memory.  K1 moves every FZ_FL k-steps (4 FZ_FL terms) into FP64.
+//   K5 (:408-422): rows the wave's neurons, columns 16 inputs, k the four
+//   points of the block, slot by slot: the lane's own S-bar is the A operand;
+//   FP32 over a block, FP64 across blocks, per-workgroup FP64 partials
+//   summed in a fixed order by the gradient reduction.
+// The maps are emitted interleaved with the per-slot arithmetic: slot a of
+// the recurrence needs S of its sub-indices only (earlier slots in the graded
+// order), so the MFMAs of tile q + 1 go out before the recurrence of tile q's
+// slots, and each slot's K5 right after its pair sum -- the matrix pipe runs
+// while the wave issues the vector work.
+// Forward: the first hidden layer forms S from the inputs (fwd_in); every
+// other hidden layer reads the block of T below, maps it and writes its own
+// T (fwd); the last also forms the output row, summed over the neurons in
+// FP64 (fwd_last).  Adjoint: the last hidden layer takes T-bar from the
+// output row's S-bar and recomputes T beside h for the output layer's
+// gradient (bwd_last); the middle layers form T-bar from the S-bar block of
+// the layer above (bwd); each recomputes S from the block of T below, h in
+// registers, the pair sums, the layer's K5 and writes its S-bar block; the
+// first hidden layer (bwd_in) forms S from the inputs and its gradient from
+// S-bar at slot 0 and the slots e_i, writing no plane.
+static const char *kFusedHelpers = R"FZ(
+// lane offsets pass through an empty asm at the top of every block: nothing
+// derived from them is hoisted out of the block loop into live registers
+#define FZ_OPAQUE(x) asm volatile("" : "+v"(x))
+// one block [slot][point][neuron] (global) into X[slot * XS + point * PJ + neuron]
+__device__ __forceinline__ void fz_stage(const float *__restrict__ src, float *__restrict__ X)
+{
+  constexpr int NU = (NA * 64 + 255) / 256;
+  const int tid = threadIdx.x;
+  int so = tid, xo = (tid >> 6) * XS + ((tid >> 4) & 3) * PJ + (tid & 15) * 4;
+  FZ_OPAQUE(so);
+  FZ_OPAQUE(xo);
+  flt4 v[NU];
+#pragma unroll
+  for (int k = 0; k < NU; ++k)
+    if (tid + 256 * k < NA * 64) v[k] = reinterpret_cast<const flt4 *>(src)[so + 256 * k];
+#pragma unroll
+  for (int k = 0; k < NU; ++k)
+    if (tid + 256 * k < NA * 64) *reinterpret_cast<flt4 *>(X + xo + k * 4 * XS) = v[k];
+}
+__device__ __forceinline__ void fz_zero_pad(float *__restrict__ X)
+{
+  for (int u = threadIdx.x; u < (4 * NT - NA) * XS; u += 256) X[NA * XS + u] = 0.f;
+}
+// K1/K6 tile t: rows (point (lane mod 16) / 4, slot 4 t + lane mod 4), k the
+// input lane / 16 + 4 s at step s; xa = the lane's offset for t = 0.  Lane l
+// gets slots 4 t + r of its (neuron, point l / 16).
+__device__ __forceinline__ flt4 fz_tile(const float *__restrict__ X, int xa, int t, const float (&b)[16])
+{
+  flt4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int s = 0; s < 16; ++s) c = __builtin_amdgcn_mfma_f32_16x16x4f32(X[xa + t * 4 * XS + 4 * s], b[s], c, 0, 0, 0);
+  return c;
+}
+// the forward map: every FL k-steps the partial sum moves into FP64; slot 0 takes the bias
+__device__ __forceinline__ flt4 fz_tile64(const float *__restrict__ X, int xa, int t, const float (&b)[16], float bias)
+{
+  double d[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int c0 = 0; c0 < 16; c0 += FL) {
+    flt4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = c0; s < c0 + FL; ++s)
+      c = __builtin_amdgcn_mfma_f32_16x16x4f32(X[xa + t * 4 * XS + 4 * s], b[s], c, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) d[r] += (double)c[r];
+  }
+  flt4 out;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) out[r] = (float)(d[r] + ((t == 0 && r == 0) ? (double)bias : 0.0));
+  return out;
+}
+// sum over the lanes l, l ^ 16, l ^ 32, l ^ 48 (the four points of a neuron)
+__device__ __forceinline__ double fz_sum4(double v)
+{
+  for (int o = 16; o <= 32; o <<= 1) {
+    const int src = (int)(((threadIdx.x & 63) ^ o) << 2);
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_ds_bpermute(src, (int)(unsigned)b);
+    const int hi = __builtin_amdgcn_ds_bpermute(src, (int)(unsigned)(b >> 32));
+    v += __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+  }
+  return v;
+}
+// K5 of one slot: g<ib> += Sbar_a (row: the lane's neuron, k: its point) x
+// T_a of the block below (k: point, column: input 16 ib + lane mod 16);
+// xg = (lane / 16) PJ + lane mod 16
+#define FZ_G(a, sbv)                                                                               \
+  {                                                                                                \
+    g0 = __builtin_amdgcn_mfma_f32_16x16x4f32((sbv), X[xg + (a) * XS], g0, 0, 0, 0);               \
+    g1 = __builtin_amdgcn_mfma_f32_16x16x4f32((sbv), X[xg + (a) * XS + 16], g1, 0, 0, 0);          \
+    g2 = __builtin_amdgcn_mfma_f32_16x16x4f32((sbv), X[xg + (a) * XS + 32], g2, 0, 0, 0);          \
+    g3 = __builtin_amdgcn_mfma_f32_16x16x4f32((sbv), X[xg + (a) * XS + 48], g3, 0, 0, 0);          \
+  }
+)FZ";
+
+bool gen_fused_usable(const TableSet &t, const FwdSchedule &s)
+{
+    return gen_bell_usable(t, s) && t.na <= FZ_MAXA && t.d0 <= FZ_MAXD0;
+}
+
+std::string gen_fused_source(const TableSet &t, const FwdSchedule &s)
+{
+    std::ostringstream o;
+    const int na = t.na, kq = kq_of(t.k), d0 = t.d0, nt = (na + 3) / 4;
+    const bool bes = t.iact == ACT_BESSEL_J0 || t.iact == ACT_BESSEL_J1;
+    // the activation, its derivative count and the polynomial row length
+    // are the family's: compile-time here (the argument layout is the
+    // host's ActParams), so one activation's code is compiled and nothing of
+    // it is hoisted out of the block loops
+    o << "struct ActParams { int iact_rt, nq_rt; const double *tpc; int ldt_rt; static constexpr int iact = "
+      << t.iact << ", nq = " << t.k + 1 << ", ldt = " << t.k + 3 << "; };\n"
+      << kActSrc << "\n"
+      << "typedef float real;\n"
+      << "typedef float flt4 __attribute__((ext_vector_type(4)));\n"
+      << "#define NA " << na << "\n#define NT " << nt << "\n#define D0 " << d0 << "\n#define XS " << FZ_XS << "\n#define PJ " << FZ_PJ
+      << "\n#define FL " << FZ_FL << "\n#define BLK (NA * 256)\n"
+      << kFusedHelpers;
+    auto derivs = [&]() {
+        // the table pointer passes through an empty asm in every block: its
+        // loads are not hoisted out of the loop into live registers
+        o << "  double d[" << kq << "];\n"
+          << "  ActParams apl = ap;\n  asm volatile(\"\" : \"+s\"(apl.tpc));\n"
+          << "  apl.tpc = (const double *)(((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)((unsigned long long)apl.tpc >> 32)) << 32) | (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(unsigned long long)apl.tpc));\n"
+          << "  act_derivs<" << kq << ", " << (bes ? "true" : "false") << ">(apl, (double)e0, d);\n";
+        for (int q = 0; q < kq; ++q) o << "  const real d" << q << " = (real)d[" << q << "];\n";
+    };
+    // S of the lane from the block of T below (K1, FP64-flushed, bias at slot
+    // 0), tile by tile: tiles 0 and 1 and the activation derivatives here,
+    // tile q + 1 before the recurrence reaches tile q's slots (tile_hook)
+    auto tile = [&](int q) {
+        o << "  {\n    const flt4 sv = fz_tile64(X, xa, " << q << ", b, bias);\n";
+        for (int r = 0; r < 4 && 4 * q + r < na; ++r) o << "    e" << 4 * q + r << " = sv[" << r << "];\n";
+        o << "  }\n";
+    };
+    auto s_from_x = [&](const char *bsrc) {
+        o << "  float b[16];\n#pragma unroll\n  for (int s = 0; s < 16; ++s) b[s] = " << bsrc << ";\n";
+        for (int a = 0; a < na; ++a) o << "  real e" << a << ";\n";
+        tile(0);
+        if (nt > 1) tile(1);
+        derivs();
+    };
+    const std::function<void(int)> tile_hook = [&](int a) {
+        if (a % 4 == 0 && a / 4 + 1 < nt) tile(a / 4 + 1);
+    };
+    // S of the lane from the inputs (the FP32 input values, S_0 summed in FP64)
+    auto s_from_in = [&]() {
+        o << "  const long n = (long)pb * 4 + p;\n"
+          << "  const float *wj = W1 + j * " << d0 + 1 << ";\n"
+          << "  double s0 = (double)wj[0];\n";
+        for (int i = 0; i < d0; ++i)
+            o << "  const float xv" << i << " = n < npt ? (float)xin[n * " << d0 << " + " << i << "] : 0.f;\n"
+              << "  s0 += (double)wj[" << i + 1 << "] * (double)xv" << i << ";\n";
+        o << "  const real e0 = (real)s0;\n";
+        for (int a = 1; a < na; ++a) {
+            int src = -1;
+            for (int i = 0; i < d0; ++i)
+                if (t.ind_e1[i] == a) src = i;
+            if (src >= 0) o << "  const real e" << a << " = wj[" << src + 1 << "];\n";
+            else o << "  const real e" << a << " = (real)0;\n";
+        }
+        derivs();
+    };
+    auto head = [&](const char *name, const char *args, int waves_per_eu) {
+        o << "extern \"C\" __global__ __launch_bounds__(256, " << waves_per_eu << ") void " << name << "(" << args
+          << ")\n{\n"
+          << "  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;\n"
+          << "  const int j = 16 * w + (lane & 15), p = lane >> 4;\n"
+          << "  const long Np = 256;   // slot stride inside a block\n"
+          // lane offsets: K1/K6 A-operand row (xa), K5 B operand (xg), the
+          // transposed S-bar row (xs), the (neuron, point) in a block (lo),
+          // the weight rows of the B operands (wb: K1, wv: K6)
+          << "  int xa = (lane & 3) * XS + ((lane & 15) >> 2) * PJ + (lane >> 4);\n"
+          << "  int xg = (lane >> 4) * PJ + (lane & 15), xs = p * PJ + j, lo = p * 64 + j;\n"
+          << "  int wb = j * 65 + 1 + (lane >> 4), wv = (lane >> 4) * 65 + 1 + j;\n";
+    };
+    auto opaque = [&]() {
+        o << "  FZ_OPAQUE(xa);\n  FZ_OPAQUE(xg);\n  FZ_OPAQUE(xs);\n  FZ_OPAQUE(lo);\n  FZ_OPAQUE(wb);\n  FZ_OPAQUE(wv);\n";
+    };
+    auto xbuf = [&]() {
+        o << "  __shared__ __attribute__((aligned(16))) float X[4 * NT * XS];\n"
+          << "  fz_zero_pad(X);\n";
+    };
+    // K5 of the block into the workgroup's FP64 accumulators in LDS (each
+    // lane owns 16 of them: rows 16 w + 4 (lane / 16) + r, columns
+    // 16 ib + lane mod 16), FP32 within the block; its MFMAs go out slot by
+    // slot right after each pair sum (k5_hook)
+    auto k5_begin = [&]() { o << "  flt4 g0 = {0.f, 0.f, 0.f, 0.f}, g1 = g0, g2 = g0, g3 = g0;\n"; };
+    const std::function<void(int)> k5_hook = [&](int b) { o << "  FZ_G(" << b << ", sb" << b << ")\n"; };
+    auto k5_end = [&]() {
+        o << "  {\n    double *gl = Gs + (16 * w + 4 * (lane >> 4)) * 64 + (lane & 15);\n"
+          << "#pragma unroll\n    for (int r = 0; r < 4; ++r) {\n"
+          << "      gl[r * 64] += (double)g0[r];\n      gl[r * 64 + 16] += (double)g1[r];\n"
+          << "      gl[r * 64 + 32] += (double)g2[r];\n      gl[r * 64 + 48] += (double)g3[r];\n    }\n"
+          << "    gb += (double)sb0;\n  }\n";
+    };
+    auto k5_out = [&]() {
+        o << "  gb = fz_sum4(gb);\n"
+          << "  double *pw = part + (long)blockIdx.x * (64 * 65);\n"
+          << "  if (lane < 16) pw[j * 65] = gb;\n"
+          << "  {\n    const double *gl = Gs + (16 * w + 4 * (lane >> 4)) * 64 + (lane & 15);\n"
+          << "#pragma unroll\n    for (int ib = 0; ib < 4; ++ib)\n#pragma unroll\n      for (int r = 0; r < 4; ++r)\n"
+          << "        pw[(16 * w + 4 * (lane >> 4) + r) * 65 + 1 + 16 * ib + (lane & 15)] = gl[r * 64 + 16 * ib];\n  }\n";
+    };
+    auto decl_h = [&]() {
+        o << "  const real h0 = d1;\n";
+        for (int a = 1; a < na; ++a) o << "  real h" << a << ";\n";
+    };
+    auto decl_sb = [&]() {
+        for (int a = 0; a < na; ++a) o << "  real sb" << a << ";\n";
+    };
+
+    // ---- forward, first hidden layer
+    head("dnnf90_fz_fwd_in",
+         "const double *__restrict__ xin, int npt, const float *__restrict__ W1, float *__restrict__ Tout, int nblk, "
+         "ActParams ap",
+         1);
+    o << "  const int pb = blockIdx.x;\n  if (pb >= nblk) return;\n";
+    s_from_in();
+    o << "  real *Tj = Tout + (long)pb * BLK + lo;\n  Tj[0] = d0;\n";
+    emit_recurrence(o, t, s, false, REC_T);
+    o << "}\n";
+
+    // ---- forward, other hidden layers; one block a workgroup.  The last
+    // hidden layer also forms the output row (summed over the neurons in
+    // FP64 from T held in the staged block's place in LDS), the residual of
+    // each point and the seed S-bar of the output row (k_seed_residual's
+    // arithmetic, misc.hip; app/pinn_module.f90:32-65), the output layer's
+    // weight-gradient partials of the block (bias first) and its sum of R^2:
+    // no output plane, no seed kernel, no T plane of that layer.
+    o << "struct DevResidual { int nterm; const double *coeff; const int *slot; const int *nonlin; int distinct; };\n";
+    for (int last = 0; last < 2; ++last) {
+        if (last)
+            head("dnnf90_fz_fwd_last",
+                 "const float *__restrict__ Tin, const float *__restrict__ W, int nblk, ActParams ap, "
+                 "const float *__restrict__ WL, int npt, DevResidual rs, double fac, float *__restrict__ Sbo, "
+                 "double *__restrict__ gpart, double *__restrict__ rpart",
+                 2);
+        else
+            head("dnnf90_fz_fwd",
+                 "const float *__restrict__ Tin, const float *__restrict__ W, float *__restrict__ Tout, int nblk, "
+                 "ActParams ap",
+                 2);
+        xbuf();
+        o << "  const int pb = blockIdx.x;\n  if (pb >= nblk) return;\n";
+        if (last) o << "#define FZ_OUT(a, tv) X[xs + (a) * XS] = (tv);\n";
+        o << "  const float bias = W[j * 65];\n"
+          << "  fz_stage(Tin + (long)pb * BLK, X);\n  __syncthreads();\n";
+        s_from_x("W[wb + 4 * s]");
+        // the last hidden layer's T goes to the output row only, through the
+        // staged block's place in LDS once every wave has read it (all tiles)
+        if (last) {
+            for (int q = 2; q < nt; ++q) tile(q);
+            o << "  __syncthreads();\n  FZ_OUT(0, d0);\n";
+        } else {
+            o << "  real *Tj = Tout + (long)pb * BLK + lo;\n  Tj[0] = d0;\n";
+        }
+        emit_recurrence(o, t, s, false, last ? REC_TOUT : REC_T, "Np", last ? nullptr : tile_hook);
+        if (last)
+            o << "#undef FZ_OUT\n"
+              << "  __shared__ double Ot[NA * 4];   // the output row, FP32-rounded, (slot, point)\n"
+              << "  __shared__ double Rq[4];        // R of the block's points\n"
+              << "  __shared__ float Sz[NA * 4];    // the seed\n"
+              << "  __syncthreads();\n"
+              << "  if (tid < NA * 4) {\n"
+              << "    double v = 0.0;\n"
+              << "    for (int m = 0; m < 64; ++m) v += (double)WL[1 + m] * (double)X[(tid >> 2) * XS + (tid & 3) * PJ + m];\n"
+              << "    Ot[tid] = (double)(float)(v + (tid < 4 ? (double)WL[0] : 0.0));\n"
+              << "  }\n"
+              << "  __syncthreads();\n"
+              << "  if (tid < 4) {\n"
+              << "    double R = 0.0;\n"
+              << "    if ((long)pb * 4 + tid < npt) {\n"
+              << "      const double t0 = Ot[tid];\n"
+              << "      for (int k = 0; k < rs.nterm; ++k) {\n"
+              << "        const double tv = Ot[rs.slot[k] * 4 + tid];\n"
+              << "        R += rs.nonlin[k] ? rs.coeff[k] * t0 * tv : rs.coeff[k] * tv;\n"
+              << "      }\n"
+              << "    }\n"
+              << "    Rq[tid] = R;\n"
+              << "  }\n"
+              << "  __syncthreads();\n"
+              << "  if (tid < NA * 4) {\n"
+              << "    const int a = tid >> 2, q = tid & 3;\n"
+              << "    float sv = 0.f;\n"
+              << "    if ((long)pb * 4 + q < npt) {\n"
+              << "      const double R = Rq[q], t0 = Ot[q];\n"
+              << "      double s0 = 0.0;\n"
+              << "      for (int k = 0; k < rs.nterm; ++k) {\n"
+              << "        const double add = rs.nonlin[k] ? fac * R * rs.coeff[k] * t0 : fac * R * rs.coeff[k];\n"
+              << "        if (rs.nonlin[k]) s0 += fac * R * rs.coeff[k] * Ot[rs.slot[k] * 4 + q];\n"
+              << "        if (rs.slot[k] == a) sv = rs.distinct ? (float)(0.0 + add) : (float)((double)sv + add);\n"
+              << "      }\n"
+              << "      if (a == 0) sv = rs.distinct ? (float)(0.0 + s0) : (float)((double)sv + s0);\n"
+              << "    }\n"
+              << "    Sz[tid] = sv;\n"
+              << "    Sbo[(long)pb * (NA * 4) + tid] = sv;\n"
+              << "  }\n"
+              << "  __syncthreads();\n"
+              << "  if (tid < 64) {\n"
+              << "    double g = 0.0;\n"
+              << "    for (int u = 0; u < NA * 4; ++u) g += (double)Sz[u] * (double)X[(u >> 2) * XS + (u & 3) * PJ + tid];\n"
+              << "    gpart[(long)pb * 65 + 1 + tid] = g;\n"
+              << "  } else if (tid == 64) {\n"
+              << "    gpart[(long)pb * 65] = (double)Sz[0] + (double)Sz[1] + (double)Sz[2] + (double)Sz[3];\n"
+              << "  } else if (tid == 65) {\n"
+              << "    rpart[pb] = Rq[0] * Rq[0] + Rq[1] * Rq[1] + Rq[2] * Rq[2] + Rq[3] * Rq[3];\n"
+              << "  }\n";
+        o << "}\n";
+    }
+
+    // ---- adjoint of a hidden layer l >= 2 (last: l = L-1, T-bar = the
+    // output weight times the output row's seed; else T-bar of l read from
+    // the plane the layer above wrote).  Per block: S from the staged T of
+    // l-1 (K1), h, the pair sums into S-bar (registers), the layer's K5
+    // against the staged block, then S-bar transposed into LDS and T-bar of
+    // l-1 = W' S-bar (K6) written out for the layer below.
+    for (int last = 0; last < 2; ++last) {
+        head(last ? "dnnf90_fz_bwd_last" : "dnnf90_fz_bwd",
+             "const float *__restrict__ Tin, const float *__restrict__ Tb, const float *__restrict__ W, "
+             "const float *__restrict__ WL, float *__restrict__ Tbout, double *__restrict__ part, int nblk, "
+             "ActParams ap",
+             2);
+        xbuf();
+        o << "  __shared__ double Gs[64 * 64];\n"
+          << "  for (int u = tid; u < 64 * 64; u += 256) Gs[u] = 0.0;\n"
+          << "  const float bias = W[j * 65];\n"
+          << "  double gb = 0.0;\n";
+        if (last) o << "  const real wl = WL[1 + j];\n  __shared__ float Sz[NA * 4];   // the block's output-row seed\n";
+        o << "  for (int pb = blockIdx.x; pb < nblk; pb += gridDim.x) {\n";
+        opaque();
+        o << "  __syncthreads();\n  fz_stage(Tin + (long)pb * BLK, X);\n";
+        if (last) o << "  if (tid < NA * 4) Sz[tid] = Tb[(long)pb * (NA * 4) + tid];\n";
+        o << "  __syncthreads();\n";
+        s_from_x("W[wb + 4 * s]");
+        decl_h();
+        emit_recurrence(o, t, s, false, REC_HREG, "Np", tile_hook);
+        // T-bar loads after the recurrence (hoisted, they would hold 41
+        // registers through it)
+        o << "  __builtin_amdgcn_sched_barrier(0);\n";
+        if (last) {
+            for (int a = 0; a < na; ++a) o << "  const real t" << a << " = wl * Sz[" << a * 4 << " + p];\n";
+        } else {
+            o << "  const real *tb = Tb + (long)pb * BLK + lo;\n";
+            for (int a = 0; a < na; ++a) o << "  const real t" << a << " = tb[" << a * 256 << "];\n";
+        }
+        decl_sb();
+        k5_begin();
+        emit_pair_sums(o, t, false, true, k5_hook);
+        k5_end();
+        // xa once more through the asm: K6 then forms its own LDS addresses
+        // instead of keeping K1's alive across the block
+        o << "  __syncthreads();\n  {\n";
+        for (int a = 0; a < na; ++a) o << "    X[xs + " << a << " * XS] = sb" << a << ";\n";
+        o << "  }\n  __syncthreads();\n  FZ_OPAQUE(xa);\n  {\n    float v[16];\n"
+          << "#pragma unroll\n    for (int s = 0; s < 16; ++s) v[s] = W[wv + 4 * s * 65];\n"
+          << "    real *tbo = Tbout + (long)pb * BLK + lo;\n";
+        for (int q = 0; q < nt; ++q) {
+            o << "    {\n      const flt4 c = fz_tile(X, xa, " << q << ", v);\n";
+            for (int r = 0; r < 4 && 4 * q + r < na; ++r) o << "      tbo[" << (4 * q + r) * 256 << "] = c[" << r << "];\n";
+            o << "    }\n    __builtin_amdgcn_sched_barrier(0);\n";
+        }
+        o << "  }\n  }\n";
+        k5_out();
+        o << "}\n";
+    }
+
+    // ---- adjoint, first hidden layer: S from the inputs, T-bar from the
+    // plane layer 2 wrote, the layer's gradient from S-bar at slot 0 and the
+    // slots e_i (no plane written)
+    head("dnnf90_fz_bwd_in",
+         "const double *__restrict__ xin, int npt, const float *__restrict__ Tb, const float *__restrict__ W1, "
+         "double *__restrict__ part, int nblk, ActParams ap",
+         2);
+    o << "  double g[" << d0 + 1 << "];\n";
+    for (int c = 0; c <= d0; ++c) o << "  g[" << c << "] = 0.0;\n";
+    o << "  for (int pb = blockIdx.x; pb < nblk; pb += gridDim.x) {\n  FZ_OPAQUE(lo);\n";
+    s_from_in();
+    decl_h();
+    emit_recurrence(o, t, s, false, REC_HREG);
+    o << "  const real *tb = Tb + (long)pb * BLK + lo;\n";
+    for (int a = 0; a < na; ++a) o << "  const real t" << a << " = tb[" << a * 256 << "];\n";
+    o << "  real sbar0;\n  {\n    real v = t0 * h0;\n";
+    for (int a = 1; a < na; ++a) o << "    v += t" << a << " * h" << a << ";\n";
+    o << "    sbar0 = v;\n  }\n"
+      << "  g[0] += (double)sbar0;\n";
+    for (int i = 0; i < d0; ++i) {
+        const int bb = t.ind_e1[i];
+        o << "  {\n    double gi = (double)sbar0 * (double)xv" << i << ";\n";
+        if (bb >= 0) {
+            o << "    real v = (real)0;\n";
+            for (int ip = t.pair_start[bb]; ip < t.pair_start[bb] + t.pair_num[bb]; ++ip)
+                o << "    v += " << lit(t.pair_binom[ip], false) << " * t" << t.pair_ia[ip] << " * h"
+                  << t.pair_id[ip] << ";\n";
+            o << "    gi += (double)v;\n";
+        }
+        o << "    g[" << i + 1 << "] += gi;\n  }\n";
+    }
+    o << "  }\n";
+    for (int c = 0; c <= d0; ++c) o << "  g[" << c << "] = fz_sum4(g[" << c << "]);\n";
+    o << "  if (lane < 16) {\n    double *pj = part + ((long)blockIdx.x * 64 + j) * " << d0 + 1 << ";\n";
+    for (int c = 0; c <= d0; ++c) o << "    pj[" << c << "] = g[" << c << "];\n";
+    o << "  }\n}\n";
+    return o.str();
+}
+
+int gen_fused_compile(const TableSet &t, const FwdSchedule &s, FusedKernels *f, std::string *log)
+{
+    *f = FusedKernels();
+    std::vector<char> code;
+    if (rtc_compile(gen_fused_source(t, s), "dnnf90_fz_gen.hip", &code, log)) {
+        f->status = -2;
+        return -3;
+    }
+    return gen_fused_load(code, f);
+}
+
+int gen_fused_load(const std::vector<char> &code, FusedKernels *f)
+{
+    *f = FusedKernels();
+    f->code = code;
+    hipFunction_t *fn[FusedKernels::NK] = {&f->fwd_in, &f->fwd, &f->fwd_last, &f->bwd_last, &f->bwd, &f->bwd_in};
+    const char *nm[FusedKernels::NK] = {"dnnf90_fz_fwd_in", "dnnf90_fz_fwd",  "dnnf90_fz_fwd_last",
+                                        "dnnf90_fz_bwd_last", "dnnf90_fz_bwd", "dnnf90_fz_bwd_in"};
+    int st = hipModuleLoadData(&f->mod, f->code.data()) == hipSuccess ? 1 : -3;
+    int regs[FusedKernels::NK] = {}, scratch[FusedKernels::NK] = {};
+    for (int i = 0; st == 1 && i < FusedKernels::NK; ++i) {
+        if (hipModuleGetFunction(fn[i], f->mod, nm[i]) != hipSuccess ||
+            hipFuncGetAttribute(&regs[i], HIP_FUNC_ATTRIBUTE_NUM_REGS, *fn[i]) != hipSuccess ||
+            hipFuncGetAttribute(&scratch[i], HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, *fn[i]) != hipSuccess)
+            st = -3;
+        // a few loop-invariant values may live in scratch (reloaded a few
+        // times a block); a kernel spilling more is not used (the unfused
+        // kernels take the step)
+        else if (scratch[i] > FZ_MAX_SCRATCH)
+            st = -4;
+    }
+    // the adjoint kernels stream their blocks in one resident round
+    int occ = 0, dev = 0, ncu = 0;
+    if (st == 1 && !(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ, f->bwd, 256, 0) == hipSuccess &&
+                     hipGetDevice(&dev) == hipSuccess &&
+                     hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && occ > 0))
+        st = -5;
+    if (st != 1) {
+        (void)hipGetLastError();
+        f->release();
+        std::copy(regs, regs + FusedKernels::NK, f->regs);
+        std::copy(scratch, scratch + FusedKernels::NK, f->scratch);
+        f->status = st;
+        return -3;
+    }
+    std::copy(regs, regs + FusedKernels::NK, f->regs);
+    std::copy(scratch, scratch + FusedKernels::NK, f->scratch);
+    f->grid = occ * ncu;
+    f->status = 1;
+    f->ok = true;
+    return 0;
+}
+
+void FusedKernels::release()
+{
+    if (mod) (void)hipModuleUnload(mod);
+    mod = nullptr;
+    fwd_in = fwd = fwd_last = bwd_last = bwd = bwd_in = nullptr;
+    ok = false;
+}
+
+hipError_t fz_launch_fwd_in(const FusedKernels &f, const double *x, int npt, const float *W1, float *Tout, int nblk,
+                            const ActParams &ap, hipStream_t s)
+{
+    ActParams a = ap;
+    void *params[] = {&x, &npt, &W1, &Tout, &nblk, &a};
+    return hipModuleLaunchKernel(f.fwd_in, (unsigned)nblk, 1, 1, 256, 1, 1, 0, s, params, nullptr);
+}
+
+hipError_t fz_launch_fwd(const FusedKernels &f, const float *Tin, const float *W, float *Tout, int nblk,
+                         const ActParams &ap, hipStream_t s)
+{
+    ActParams a = ap;
+    void *params[] = {&Tin, &W, &Tout, &nblk, &a};
+    return hipModuleLaunchKernel(f.fwd, (unsigned)nblk, 1, 1, 256, 1, 1, 0, s, params, nullptr);
+}
+
+hipError_t fz_launch_fwd_last(const FusedKernels &f, const float *Tin, const float *W, int nblk, const ActParams &ap,
+                              const float *WL, int npt, const DevResidual &r, double fac, float *Sbo, double *gpart,
+                              double *rpart, hipStream_t s)
+{
+    ActParams a = ap;
+    DevResidual rr = r;
+    void *params[] = {&Tin, &W, &nblk, &a, &WL, &npt, &rr, &fac, &Sbo, &gpart, &rpart};
+    return hipModuleLaunchKernel(f.fwd_last, (unsigned)nblk, 1, 1, 256, 1, 1, 0, s, params, nullptr);
+}
+
+// the adjoint of a hidden layer l >= 2 (last: T-bar from the output row's
+// seed in Tb)
+hipError_t fz_launch_bwd(const FusedKernels &f, bool last, const float *Tin, const float *Tb, const float *W,
+                         const float *WL, float *Tbout, double *part, int nblk, const ActParams &ap, int grid,
+                         hipStream_t s)
+{
+    ActParams a = ap;
+    void *params[] = {&Tin, &Tb, &W, &WL, &Tbout, &part, &nblk, &a};
+    return hipModuleLaunchKernel(last ? f.bwd_last : f.bwd, (unsigned)grid, 1, 1, 256, 1, 1, 0, s, params, nullptr);
+}
+
+hipError_t fz_launch_bwd_in(const FusedKernels &f, const double *x, int npt, const float *Tb, const float *W1,
+                            double *part, int nblk, const ActParams &ap, int grid, hipStream_t s)
+{
+    ActParams a = ap;
+    void *params[] = {&x, &npt, &Tb, &W1, &part, &nblk, &a};
+    return hipModuleLaunchKernel(f.bwd_in, (unsigned)grid, 1, 1, 256, 1, 1, 0, s, params, nullptr);
 }
 
 }  // namespace dnnf90

@@ -97,4 +97,51 @@ hipError_t gen_launch_pt_fwd(const GenKernels &g, const double *W, const double 
 hipError_t gen_launch_pt_bwd(const GenKernels &g, const double *W, double *grad, const double *Sb, const PtNet &p,
                              const ActParams &ap, hipStream_t s);
 
+// Fused layer kernels (FP32 storage; codegen.cpp): hidden layers FZ_W wide,
+// point blocks of FZ_P points, slots at most FZ_MAXA, inputs at most
+// FZ_MAXD0.  A staged block in LDS: rows of FZ_PJ floats (one point's 64
+// neurons and 4 of padding, 16-byte aligned for the DMA), slots of FZ_XS
+// (four rows), which keeps the linear maps' operand reads at most 2-way
+// bank-conflicted; FZ_FL: k-steps of the
+// forward map between FP64 flushes (four terms a step).
+constexpr int FZ_W = 64, FZ_P = 4, FZ_MAXA = 48, FZ_MAXD0 = 8, FZ_PJ = 68, FZ_XS = 272, FZ_FL = 4;
+// scratch (bytes a lane) a fused kernel may hold: loop-invariant values the
+// register allocator parks there and reloads a few times a block
+constexpr int FZ_MAX_SCRATCH = 256;
+struct FusedKernels {
+    static constexpr int NK = 6;
+    int grid = 0;   // workgroups of the adjoint kernels: one resident round
+    // 1 loaded; else why not: -2 hipRTC failed, -3 module or kernel missing,
+    // -4 a kernel spills past FZ_MAX_SCRATCH, -5 no occupancy reported
+    int status = 0;
+    hipModule_t mod = nullptr;
+    hipFunction_t fwd_in = nullptr, fwd = nullptr, fwd_last = nullptr;
+    hipFunction_t bwd_last = nullptr, bwd = nullptr, bwd_in = nullptr;
+    int regs[NK] = {}, scratch[NK] = {};
+    std::vector<char> code;
+    bool ok = false;
+    void release();
+};
+bool gen_fused_usable(const TableSet &t, const FwdSchedule &s);
+std::string gen_fused_source(const TableSet &t, const FwdSchedule &s);
+int gen_fused_compile(const TableSet &t, const FwdSchedule &s, FusedKernels *f, std::string *log);
+// 0, or -3 (a kernel missing or spilling) with f released
+int gen_fused_load(const std::vector<char> &code, FusedKernels *f);
+hipError_t fz_launch_fwd_in(const FusedKernels &f, const double *x, int npt, const float *W1, float *Tout, int nblk,
+                            const ActParams &ap, hipStream_t s);
+// forward of a hidden layer l >= 2 but the last, one block a workgroup
+hipError_t fz_launch_fwd(const FusedKernels &f, const float *Tin, const float *W, float *Tout, int nblk,
+                         const ActParams &ap, hipStream_t s);
+// forward of the last hidden layer with the output row, the collocation
+// seed into Sbo ([block][slot][4 points]), the output layer's gradient
+// partials gpart[block][65] (bias first) and rpart[block] = sum of R^2
+hipError_t fz_launch_fwd_last(const FusedKernels &f, const float *Tin, const float *W, int nblk, const ActParams &ap,
+                              const float *WL, int npt, const DevResidual &r, double fac, float *Sbo, double *gpart,
+                              double *rpart, hipStream_t s);
+hipError_t fz_launch_bwd(const FusedKernels &f, bool last, const float *Tin, const float *Tb, const float *W,
+                         const float *WL, float *Tbout, double *part, int nblk, const ActParams &ap, int grid,
+                         hipStream_t s);
+hipError_t fz_launch_bwd_in(const FusedKernels &f, const double *x, int npt, const float *Tb, const float *W1,
+                            double *part, int nblk, const ActParams &ap, int grid, hipStream_t s);
+
 }  // namespace dnnf90

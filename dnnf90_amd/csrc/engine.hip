@@ -57,6 +57,11 @@ static int g_point = 1;
 // layer's Sbar): on unless switched off by dnnf90_set_edge_kernels
 static int g_edge = 1;
 
+// the fused layer kernels for the collocation step of FP32 work spaces
+// (codegen.cpp, fused section): on unless switched off by
+// dnnf90_set_fused_layers
+static int g_fused_layers = 1;
+
 static thread_local std::string g_err;
 
 static int fail(int rc, const std::string &m)
@@ -187,6 +192,8 @@ struct Net {
     float *d_w32 = nullptr;
     GenKernels gen[2];       // [0] FP32, [1] FP64
     bool gen_tried[2] = {false, false};
+    FusedKernels fz;         // the fused layer kernels (FP32), compiled on first use
+    bool fz_tried = false;
     std::mutex mu;
     // weight uploads and device-side weight steps run on the network's own
     // stream after the last launch of every stream that read the weights:
@@ -295,6 +302,30 @@ struct Net {
         dfree(d_w32);
         gen[0].release();
         gen[1].release();
+        fz.release();
+    }
+
+    // the fused layer kernels of the family (compiled on first use; code
+    // objects cached by source text)
+    const FusedKernels *fused()
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!codegen_on()) return nullptr;
+        if (!fz_tried) {
+            fz_tried = true;
+            if (gen_fused_usable(tab, sched)) {
+                const std::string src = gen_fused_source(tab, sched);
+                std::lock_guard<std::mutex> lg(g_gen_mu);
+                auto it = g_gen_cache.find(src);
+                if (it != g_gen_cache.end()) {
+                    (void)gen_fused_load(it->second, &fz);
+                } else {
+                    std::string log;
+                    if (gen_fused_compile(tab, sched, &fz, &log) == 0) g_gen_cache[src] = fz.code;
+                }
+            }
+        }
+        return fz.ok ? &fz : nullptr;
     }
 };
 
@@ -1171,6 +1202,134 @@ static int set_residual(BatchWork &b, int nterm, const double *coeff, const int 
     return 0;
 }
 
+// the fused layer kernels take a collocation step (not per-point rows) of an
+// FP32 work space when the network has at least two hidden layers, every
+// one FZ_W wide, one output row, at most FZ_MAXD0 inputs, and the family's
+// fused module compiled (codegen.cpp)
+static const FusedKernels *fused_plan(Net &nt, const BatchWork &b)
+{
+    const int L = nt.nlayer;
+    if (!g_fused_layers || !codegen_on() || b.prec != PREC_FP32 || L < 4) return nullptr;
+    if (nt.dims[L - 1] != 1 || nt.dims[0] > FZ_MAXD0) return nullptr;
+    for (int l = 1; l < L - 1; ++l)
+        if (nt.dims[l] != FZ_W) return nullptr;
+    return nt.fused();
+}
+
+// grow the gradient partials to n doubles (the work space's stream drained first)
+static int grow_partial(BatchWork &b, long n)
+{
+    if (n <= b.partial_cap) return 0;
+    HIPCHK(hipStreamSynchronize(b.stream));
+    for (auto &pb : b.poisoned)
+        if (pb.first == b.partial) pb = {nullptr, 0};
+    dfree(b.partial);
+    b.partial_cap = 0;
+    int rc = dalloc(&b.partial, (size_t)n);
+    if (rc) return rc;
+    b.partial_cap = n;
+    if (b.poison) {
+        b.poisoned.push_back({b.partial, sizeof(double) * (size_t)n});
+        HIPCHK(hipMemsetAsync(b.partial, 0xFF, sizeof(double) * (size_t)n, b.stream));
+    }
+    return 0;
+}
+
+// The collocation step on the fused layer kernels.  The work space's planes
+// hold the point-blocked layout ([block][slot][4 points][neuron]): T of the
+// hidden layers but the last (tpl), the seed and the T-bar planes the
+// adjoint passes down (p0, p1).  Forward: the first hidden layer from the
+// inputs, then one kernel a layer (K1 + Faa di Bruno); the last one also
+// forms the output row, the residual, the seed, the output layer's gradient
+// partials and the sums of R^2.  Then per hidden layer from the last: one
+// kernel (K1 recomputed, h, pair sums, K5, T-bar of the layer below), the
+// first hidden layer's forming its gradient from the inputs.  The planes are
+// left in that layout: no backward_batch may follow.
+static int run_residual_fused(Net &nt, BatchWork &b, const FusedKernels &f, int npt, const double *x,
+                              const DevResidual &r, double fac, Grad &g, double *rsq, bool dev)
+{
+    if (npt < 0 || npt > b.maxpt) return fail(-1, "npt outside [0, maxpt]");
+    int rc = nt.ensure_device(true);
+    if (rc) return rc;
+    const int L = nt.nlayer, na = nt.tab.na, d0 = nt.dims[0];
+    hipStream_t s = b.stream;
+    const double *dx = x;
+    if (!dev) {
+        if (npt > 0) HIPCHK(hipMemcpyAsync(b.d_x, x, sizeof(double) * npt * d0, hipMemcpyHostToDevice, s));
+        dx = b.d_x;
+    }
+    const long Np = round_up(std::max(npt, 1), 64);
+    const int nblk = (int)(Np / FZ_P);
+    const int grid = std::min(nblk, f.grid);
+    b.np = Np;
+    b.npt = npt;
+    b.pt_used = false;
+    b.fwd_valid = false;
+    // partials: the adjoint kernels' (grid x 64 x 65), before them the
+    // output layer's per block (nblk x 65) and the blocks' sums of R^2
+    if ((rc = grow_partial(b, std::max((long)grid * FZ_W * (FZ_W + 1), (long)nblk * (FZ_W + 2))))) return rc;
+    if (b.poison)
+        if (hipError_t e = poison_planes(b)) return hip_fail(e, "poison");
+    const float *W = nt.d_w32;
+    const ActParams act = nt.dt.act();
+    if ((rc = g.before(s))) return rc;
+    hipError_t e;
+    {
+        PScope p(b, CAT_BELL_FWD);
+        e = fz_launch_fwd_in(f, dx, npt, W + nt.woff[1], (float *)b.tpl[1], nblk, act, s);
+    }
+    for (int l = 2; l < L - 2 && e == hipSuccess; ++l) {
+        PScope p(b, CAT_BELL_FWD);
+        e = fz_launch_fwd(f, (const float *)b.tpl[l - 1], W + nt.woff[l], (float *)b.tpl[l], nblk, act, s);
+    }
+    double *gpart = b.partial, *rpart = b.partial + (long)nblk * (FZ_W + 1);
+    if (e == hipSuccess) {
+        PScope p(b, CAT_BELL_FWD);
+        e = fz_launch_fwd_last(f, (const float *)b.tpl[L - 3], W + nt.woff[L - 2], nblk, act, W + nt.woff[L - 1], npt,
+                               r, fac, (float *)b.p0, gpart, rpart, s);
+    }
+    if (e == hipSuccess) {
+        PScope p(b, CAT_GRAD_REDUCE);
+        launch_grad_reduce(gpart, nblk, FZ_W + 1, g.d_g + nt.woff[L - 1], s);
+        if (rsq) launch_reduce_sum(rpart, nblk, b.d_rsq, s);
+    }
+    double *part = b.partial;
+    void *tb = b.p0, *tbo = b.p1;
+    for (int l = L - 2; l >= 2 && e == hipSuccess; --l) {
+        const bool last = l == L - 2;
+        {
+            PScope p(b, CAT_BELL_BWD);
+            e = fz_launch_bwd(f, last, (const float *)b.tpl[l - 1], (const float *)tb, W + nt.woff[l],
+                              W + nt.woff[L - 1], (float *)tbo, part, nblk, act, grid, s);
+        }
+        if (e) break;
+        {
+            PScope p(b, CAT_GRAD_REDUCE);
+            launch_grad_reduce(part, grid, (long)FZ_W * (FZ_W + 1), g.d_g + nt.woff[l], s);
+        }
+        std::swap(tb, tbo);
+    }
+    if (e == hipSuccess) {
+        {
+            PScope p(b, CAT_BELL_BWD);
+            e = fz_launch_bwd_in(f, dx, npt, (const float *)tb, W + nt.woff[1], part, nblk, act, grid, s);
+        }
+        if (e == hipSuccess) {
+            PScope p(b, CAT_GRAD_REDUCE);
+            launch_grad_reduce(part, grid, (long)FZ_W * (d0 + 1), g.d_g + nt.woff[1], s);
+        }
+    }
+    if ((rc = g.after(s))) return rc;
+    if (e) return hip_fail(e, "fused step launch");
+    if (!b.defer_use) nt.used_on(s);
+    if (rsq) {
+        HIPCHK(hipMemcpyAsync(rsq, b.d_rsq, sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 template <typename T>
 static int run_residual(Net &nt, BatchWork &b, int npt, const double *x, int nterm,
                         const double *coeff, const int *slot, const int *nonlin, double fac,
@@ -1180,6 +1339,9 @@ static int run_residual(Net &nt, BatchWork &b, int npt, const double *x, int nte
     DevResidual r;
     int rc = set_residual(b, nterm, coeff, slot, nonlin, na, &r);
     if (rc) return rc;
+    if constexpr (sizeof(T) == 4)
+        if (!rows_t && g)
+            if (const FusedKernels *f = fused_plan(nt, b)) return run_residual_fused(nt, b, *f, npt, x, r, fac, *g, rsq, dev);
     b.rows_next = rows_t != nullptr;
     rc = run_forward<T>(nt, b, npt, x, nullptr, dev);
     b.rows_next = false;
@@ -2149,6 +2311,13 @@ int dnnf90_set_edge_kernels(int on)
 
 int dnnf90_set_fused_grad(int on) { return set_grad_lint_x6(on ? 1 : 0); }
 
+int dnnf90_set_fused_layers(int on)
+{
+    const int prev = g_fused_layers;
+    g_fused_layers = on ? 1 : 0;
+    return prev;
+}
+
 int dnnf90_set_poison(int on)
 {
     const int prev = g_poison;
@@ -2177,6 +2346,36 @@ int dnnf90_net_kernel_info(int nid, int prec, int *info)
     info[3] = on ? (ks ? g.fwdt_scratch : g.fwd_scratch) : 0;
     info[4] = on ? (ks ? g.bwds_scratch : g.bwd_scratch) : 0;
     info[5] = nt->sched.nentry;
+    return 0;
+}
+
+int dnnf90_net_fused_info(int nid, int *info)
+{
+    Net *nt = g_net.get(nid);
+    if (!nt || !info) return fail(-1, "bad network handle or argument");
+    int rc = nt->ensure_device(true);
+    if (rc) return rc;
+    BatchWork probe;
+    probe.prec = PREC_FP32;
+    const FusedKernels *f = fused_plan(*nt, probe);
+    for (int i = 0; i < 14; ++i) info[i] = 0;
+    if (!f) {
+        // the network's shape or the switch rules them out (0), or the
+        // module did not come up (FusedKernels::status)
+        if (nt->fz_tried) info[0] = nt->fz.status == 0 ? -1 : nt->fz.status;
+        if (nt->fz_tried)
+            for (int i = 0; i < FusedKernels::NK; ++i) {
+                info[1 + i] = nt->fz.regs[i];
+                info[7 + i] = nt->fz.scratch[i];
+            }
+        return 0;
+    }
+    info[0] = 1;
+    for (int i = 0; i < FusedKernels::NK; ++i) {
+        info[1 + i] = f->regs[i];
+        info[7 + i] = f->scratch[i];
+    }
+    info[13] = f->grid;
     return 0;
 }
 

@@ -38,6 +38,13 @@ static void source(int d0, int k, int nseed, const int *seeds)
         CHECK(src.find("dnnf90_bell_fwd") != std::string::npos);
         CHECK(src.find("dnnf90_bell_bwds") != std::string::npos);
     }
+    // the fused layer kernels of the family (FP32 collocation step)
+    if (dnnf90::gen_fused_usable(t, s)) {
+        const std::string src = dnnf90::gen_fused_source(t, s);
+        for (const char *k : {"dnnf90_fz_fwd_in", "dnnf90_fz_fwd", "dnnf90_fz_fwd_last", "dnnf90_fz_bwd_last",
+                              "dnnf90_fz_bwd", "dnnf90_fz_bwd_in"})
+            CHECK(src.find(k) != std::string::npos);
+    }
 }
 
 static void lists(int tid)
