@@ -129,6 +129,62 @@ def algorithmic(wl, na, F, Q, P, N, wbytes, keep_s, edge=(0, 0, 0, 0, 0)):
     return {k: tuple(v) for k, v in cat.items()}
 
 
+def algorithmic_fused(wl, na, F, Q, P, N, grid, kept_h=True):
+    """Per-step algorithmic work of the fused FP32 layer kernels (4-byte
+    planes, point blocks of 4): (flops, bytes, launches) under bell_fwd (the
+    forward kernels, each hidden layer's K1 inside) and bell_bwd (one
+    adjoint kernel a hidden layer: S recomputed from T of the layer below,
+    h, the pair sums, K5 and, above the first hidden layer, K6).
+
+    Forward: the first hidden layer reads the inputs (8-byte words) and
+    writes T; the others read T of the layer below and write T, but the last,
+    which forms the output row and the collocation seed (one plane row) and
+    writes the output layer's gradient partials (one 65-word row of 8 bytes
+    per point block) and the sums of R^2.  Adjoint: the last hidden layer
+    reads T of the layer below and the seed, the others T of the layer below
+    and their T-bar; all but the first write T-bar of the layer below; each
+    writes grid x n_l x (n_{l-1} + 1) gradient partials (8-byte words).
+    kept_h: each forward also writes h of its layer (one plane) and the
+    adjoint reads it instead of recomputing S (K1) and h from T of the layer
+    below.  The fourth entry is the linear maps' share of the flops (FP32
+    MFMA)."""
+    d = wl.dims
+    L = len(d)
+    w = 4.0 * na * N
+    xin = 8.0 * N
+    nblk = N / 4.0
+    cat = {k: [0.0, 0.0, 0, 0.0] for k in ("bell_fwd", "bell_bwd")}
+
+    def add(k, flops, byts, mfma):
+        cat[k][0] += flops
+        cat[k][1] += byts
+        cat[k][2] += 1
+        cat[k][3] += mfma
+
+    h_fl = lambda l: d[l] * (3.0 * F + 2.0 * Q + (na - 1)) * N
+    k1 = lambda l: 2.0 * d[l] * (d[l - 1] + 1) * (N if l == 1 else na * N)
+    last = L - 2
+    for l in range(1, L - 1):
+        rd = d[0] * xin if l == 1 else d[l - 1] * w
+        hw = d[l] * w if kept_h else 0.0
+        hf = d[l] * 2.0 * Q * N if kept_h else 0.0
+        if l < last:
+            add("bell_fwd", k1(l) + h_fl(l) + hf, rd + d[l] * w + hw, k1(l))
+        else:
+            fl = k1(l) + h_fl(l) + hf + 2.0 * (d[l] + 1) * na * N + 2.0 * (d[l] + 1) * N
+            add("bell_fwd", fl, rd + w + hw + nblk * (d[l] + 1) * 8.0 + nblk * 8.0, k1(l))
+    for l in range(L - 2, 0, -1):
+        rd = (d[0] * xin if l == 1 else d[l - 1] * w) + (w if l == last else d[l] * w)
+        wr = (0.0 if l == 1 else d[l - 1] * w) + grid * d[l] * (d[l - 1] + 1) * 8.0
+        mm = 2.0 * d[l] * (d[l - 1] + 1) * na * N + (2.0 * d[l] * d[l - 1] * na * N if l > 1 else 0.0)
+        if kept_h:
+            add("bell_bwd", mm + d[l] * (3.0 * P + 2.0 * na) * N, rd + d[l] * w + wr, mm)
+        else:
+            mm += k1(l)
+            add("bell_bwd", mm + 2.0 * h_fl(l) + d[l] * (3.0 * P + 2.0 * na) * N, rd + wr, mm)
+    return {k: tuple(v) for k, v in cat.items()}
+
+
 def rel_by_order(x, ref, deg):
     out = {}
     for p in sorted(set(int(v) for v in deg)):
@@ -241,6 +297,9 @@ def run_mode(args, D, torch, lib, c, dev, prec, name, wbytes, mma_peak, budget=0
     keep_s = int(kinfo[0]) == 2
     edge = np.zeros(5, np.int32)
     lib().dnnf90_net_step_info(net.id, prec, edge.ctypes.data)
+    finfo = np.zeros(27, np.int32)
+    lib().dnnf90_net_fused_info(net.id, finfo.ctypes.data)
+    fused_path = prec == D.PREC_FP32 and int(finfo[0]) == 1
     g = D.Grad(net)
     s = torch.cuda.Stream(device=dev)
     bw.set_stream(s.cuda_stream)
@@ -283,7 +342,11 @@ def run_mode(args, D, torch, lib, c, dev, prec, name, wbytes, mma_peak, budget=0
     bw.set_stream(None)
     bw.free()
     g.free()
-    work = algorithmic(c.wl, c.na, c.F, c.Q, c.P, N, wbytes, keep_s, edge)
+    if fused_path:
+        kept_h = not info.get("recompute", False)
+        work = algorithmic_fused(c.wl, c.na, c.F, c.Q, c.P, N, int(finfo[26 if kept_h else 25]), kept_h)
+    else:
+        work = algorithmic(c.wl, c.na, c.F, c.Q, c.P, N, wbytes, keep_s, edge)
     kern = {}
     for cat, (tot, cnt) in kms.items():
         ent = {"ms_per_step": tot / args.steps, "launches_per_step": cnt / args.steps}
@@ -294,10 +357,15 @@ def run_mode(args, D, torch, lib, c, dev, prec, name, wbytes, mma_peak, budget=0
             # judged against the roof that binds: MFMA for the linear maps
             # unless their bytes take longer at 8 TB/s than their flops at the
             # mode's MFMA peak; HBM for the Faa di Bruno kernels
-            flops, byts, _ = work[cat]
+            flops, byts = work[cat][:2]
             sec = tot / args.steps * 1e-3
             mfma_bound = cat in ("linear", "linear_t", "grad") and \
                 flops / (mma_peak * 1e12) > byts / (PEAK["hbm"] * 1e9)
+            if fused_path:
+                # the fused kernels' linear maps on FP32 MFMA against their bytes
+                mfma_bound = work[cat][3] / (mma_peak * 1e12) > byts / (PEAK["hbm"] * 1e9)
+                if mfma_bound:
+                    flops = work[cat][3]
             if mfma_bound:
                 ent.update(bound="mfma", achieved=flops / sec / 1e12, unit="TFLOP/s", peak=mma_peak)
             else:
@@ -308,6 +376,10 @@ def run_mode(args, D, torch, lib, c, dev, prec, name, wbytes, mma_peak, budget=0
         kern[cat] = ent
     info["keep_s"] = keep_s
     info["edge"] = [int(v) for v in edge]
+    if fused_path:
+        info["fused_layers"] = {"kept_h": kept_h, "regs": [int(v) for v in finfo[1:13]],
+                                "scratch": [int(v) for v in finfo[13:25]],
+                                "adjoint_grid": int(finfo[26 if kept_h else 25])}
     ms = float(np.median(step_ms))
     return {"name": name, "ms_per_step": ms, "step_ms": {"median": ms, "min": float(min(step_ms)),
                                                           "max": float(max(step_ms)),

@@ -833,7 +833,11 @@ std::string gen_fused_source(const TableSet &t, const FwdSchedule &s)
     // are the family's: compile-time here (the argument layout is the
     // host's ActParams), so one activation's code is compiled and nothing of
     // it is hoisted out of the block loops
-    o << "struct ActParams { int iact_rt, nq_rt; const double *tpc; int ldt_rt; static constexpr int iact = "
+    // multiply-adds fused inside an expression only: h comes out of the
+    // forward's recurrence (beside T) and of the adjoint's (alone) in the
+    // same bits
+    o << "#pragma clang fp contract(on)\n"
+      << "struct ActParams { int iact_rt, nq_rt; const double *tpc; int ldt_rt; static constexpr int iact = "
       << t.iact << ", nq = " << t.k + 1 << ", ldt = " << t.k + 3 << "; };\n"
       << kActSrc << "\n"
       << "typedef float real;\n"
@@ -935,6 +939,22 @@ std::string gen_fused_source(const TableSet &t, const FwdSchedule &s)
         for (int a = 0; a < na; ++a) o << "  real sb" << a << ";\n";
     };
 
+    // The kernels of the hidden layers l >= 2 in two plane policies: (kh =
+    // 0) the adjoint recomputes S and h of the layer from T of the layer
+    // below (nothing kept but T); (kh = 1, names ending _h) the forward also
+    // writes h of the layer into the plane Hout (point-blocked like T) and
+    // the adjoint reads it, with no linear map, activation derivatives or
+    // recurrence of its own.  Both compute h by the same expressions (the
+    // module is compiled with contraction inside expressions only, so each
+    // multiply-add is fused the same way wherever it is emitted): the same
+    // bits.  The first hidden layer's adjoint always recomputes (S from the
+    // inputs takes no linear map; writing and reading h cost more).
+    auto kname = [](const char *base, int kh) { return std::string(base) + (kh ? "_h" : ""); };
+    auto hout = [&](int kh) {
+        if (kh) o << "  real *Hj = Hout + (long)pb * BLK + lo;\n  Hj[0] = d1;\n";
+    };
+    const int rec_h = REC_H;
+
     // ---- forward, first hidden layer
     head("dnnf90_fz_fwd_in",
          "const double *__restrict__ xin, int npt, const float *__restrict__ W1, float *__restrict__ Tout, int nblk, "
@@ -954,17 +974,19 @@ std::string gen_fused_source(const TableSet &t, const FwdSchedule &s)
     // weight-gradient partials of the block (bias first) and its sum of R^2:
     // no output plane, no seed kernel, no T plane of that layer.
     o << "struct DevResidual { int nterm; const double *coeff; const int *slot; const int *nonlin; int distinct; };\n";
+    for (int kh = 0; kh < 2; ++kh)
     for (int last = 0; last < 2; ++last) {
+        const char *hx = kh ? ", float *__restrict__ Hout" : "";
         if (last)
-            head("dnnf90_fz_fwd_last",
-                 "const float *__restrict__ Tin, const float *__restrict__ W, int nblk, ActParams ap, "
-                 "const float *__restrict__ WL, int npt, DevResidual rs, double fac, float *__restrict__ Sbo, "
-                 "double *__restrict__ gpart, double *__restrict__ rpart",
+            head(kname("dnnf90_fz_fwd_last", kh).c_str(),
+                 (std::string("const float *__restrict__ Tin, const float *__restrict__ W, int nblk, ActParams ap, "
+                              "const float *__restrict__ WL, int npt, DevResidual rs, double fac, float *__restrict__ Sbo, "
+                              "double *__restrict__ gpart, double *__restrict__ rpart") + hx).c_str(),
                  2);
         else
-            head("dnnf90_fz_fwd",
-                 "const float *__restrict__ Tin, const float *__restrict__ W, float *__restrict__ Tout, int nblk, "
-                 "ActParams ap",
+            head(kname("dnnf90_fz_fwd", kh).c_str(),
+                 (std::string("const float *__restrict__ Tin, const float *__restrict__ W, float *__restrict__ Tout, "
+                              "int nblk, ActParams ap") + hx).c_str(),
                  2);
         xbuf();
         o << "  const int pb = blockIdx.x;\n  if (pb >= nblk) return;\n";
@@ -980,7 +1002,9 @@ std::string gen_fused_source(const TableSet &t, const FwdSchedule &s)
         } else {
             o << "  real *Tj = Tout + (long)pb * BLK + lo;\n  Tj[0] = d0;\n";
         }
-        emit_recurrence(o, t, s, false, last ? REC_TOUT : REC_T, "Np", last ? nullptr : tile_hook);
+        hout(kh);
+        emit_recurrence(o, t, s, false, (last ? REC_TOUT : REC_T) | (kh ? rec_h : 0), "Np",
+                        last ? nullptr : tile_hook);
         if (last)
             o << "#undef FZ_OUT\n"
               << "  __shared__ double Ot[NA * 4];   // the output row, FP32-rounded, (slot, point)\n"
@@ -1040,26 +1064,38 @@ std::string gen_fused_source(const TableSet &t, const FwdSchedule &s)
     // l-1 (K1), h, the pair sums into S-bar (registers), the layer's K5
     // against the staged block, then S-bar transposed into LDS and T-bar of
     // l-1 = W' S-bar (K6) written out for the layer below.
+    for (int kh = 0; kh < 2; ++kh)
     for (int last = 0; last < 2; ++last) {
-        head(last ? "dnnf90_fz_bwd_last" : "dnnf90_fz_bwd",
-             "const float *__restrict__ Tin, const float *__restrict__ Tb, const float *__restrict__ W, "
-             "const float *__restrict__ WL, float *__restrict__ Tbout, double *__restrict__ part, int nblk, "
-             "ActParams ap",
+        head(kname(last ? "dnnf90_fz_bwd_last" : "dnnf90_fz_bwd", kh).c_str(),
+             kh ? "const float *__restrict__ Tin, const float *__restrict__ Tb, const float *__restrict__ W, "
+                  "const float *__restrict__ WL, float *__restrict__ Tbout, double *__restrict__ part, int nblk, "
+                  "ActParams ap, const float *__restrict__ H"
+                : "const float *__restrict__ Tin, const float *__restrict__ Tb, const float *__restrict__ W, "
+                  "const float *__restrict__ WL, float *__restrict__ Tbout, double *__restrict__ part, int nblk, "
+                  "ActParams ap",
              2);
         xbuf();
         o << "  __shared__ double Gs[64 * 64];\n"
-          << "  for (int u = tid; u < 64 * 64; u += 256) Gs[u] = 0.0;\n"
-          << "  const float bias = W[j * 65];\n"
-          << "  double gb = 0.0;\n";
+          << "  for (int u = tid; u < 64 * 64; u += 256) Gs[u] = 0.0;\n";
+        if (!kh) o << "  const float bias = W[j * 65];\n";
+        o << "  double gb = 0.0;\n";
         if (last) o << "  const real wl = WL[1 + j];\n  __shared__ float Sz[NA * 4];   // the block's output-row seed\n";
         o << "  for (int pb = blockIdx.x; pb < nblk; pb += gridDim.x) {\n";
         opaque();
         o << "  __syncthreads();\n  fz_stage(Tin + (long)pb * BLK, X);\n";
         if (last) o << "  if (tid < NA * 4) Sz[tid] = Tb[(long)pb * (NA * 4) + tid];\n";
-        o << "  __syncthreads();\n";
-        s_from_x("W[wb + 4 * s]");
-        decl_h();
-        emit_recurrence(o, t, s, false, REC_HREG, "Np", tile_hook);
+        if (kh) {
+            // h of the lane's (neuron, point) from the forward's plane; the
+            // staged block below is only K5's operand
+            o << "  const real *hp = H + (long)pb * BLK + lo;\n";
+            for (int a = 0; a < na; ++a) o << "  const real h" << a << " = hp[" << a * 256 << "];\n";
+            o << "  __syncthreads();\n";
+        } else {
+            o << "  __syncthreads();\n";
+            s_from_x("W[wb + 4 * s]");
+            decl_h();
+            emit_recurrence(o, t, s, false, REC_HREG, "Np", tile_hook);
+        }
         // T-bar loads after the recurrence (hoisted, they would hold 41
         // registers through it)
         o << "  __builtin_amdgcn_sched_barrier(0);\n";
@@ -1144,15 +1180,19 @@ int gen_fused_load(const std::vector<char> &code, FusedKernels *f)
 {
     *f = FusedKernels();
     f->code = code;
-    hipFunction_t *fn[FusedKernels::NK] = {&f->fwd_in, &f->fwd, &f->fwd_last, &f->bwd_last, &f->bwd, &f->bwd_in};
-    const char *nm[FusedKernels::NK] = {"dnnf90_fz_fwd_in", "dnnf90_fz_fwd",  "dnnf90_fz_fwd_last",
-                                        "dnnf90_fz_bwd_last", "dnnf90_fz_bwd", "dnnf90_fz_bwd_in"};
+    static const char *const nm[FusedKernels::NR] = {"dnnf90_fz_fwd_in",   "dnnf90_fz_fwd", "dnnf90_fz_fwd_last",
+                                                      "dnnf90_fz_bwd_last", "dnnf90_fz_bwd", "dnnf90_fz_bwd_in"};
     int st = hipModuleLoadData(&f->mod, f->code.data()) == hipSuccess ? 1 : -3;
     int regs[FusedKernels::NK] = {}, scratch[FusedKernels::NK] = {};
     for (int i = 0; st == 1 && i < FusedKernels::NK; ++i) {
-        if (hipModuleGetFunction(fn[i], f->mod, nm[i]) != hipSuccess ||
-            hipFuncGetAttribute(&regs[i], HIP_FUNC_ATTRIBUTE_NUM_REGS, *fn[i]) != hipSuccess ||
-            hipFuncGetAttribute(&scratch[i], HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, *fn[i]) != hipSuccess)
+        const int kh = i / FusedKernels::NR, r = i % FusedKernels::NR;
+        hipFunction_t *fn = &f->fn[kh][r];
+        // the first hidden layer's kernels are one form in both policies
+        const bool one = r == FusedKernels::FWD_IN || r == FusedKernels::BWD_IN;
+        const std::string name = std::string(nm[r]) + (kh && !one ? "_h" : "");
+        if (hipModuleGetFunction(fn, f->mod, name.c_str()) != hipSuccess ||
+            hipFuncGetAttribute(&regs[i], HIP_FUNC_ATTRIBUTE_NUM_REGS, *fn) != hipSuccess ||
+            hipFuncGetAttribute(&scratch[i], HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, *fn) != hipSuccess)
             st = -3;
         // a few loop-invariant values may live in scratch (reloaded a few
         // times a block); a kernel spilling more is not used (the unfused
@@ -1161,22 +1201,26 @@ int gen_fused_load(const std::vector<char> &code, FusedKernels *f)
             st = -4;
     }
     // the adjoint kernels stream their blocks in one resident round
-    int occ = 0, dev = 0, ncu = 0;
-    if (st == 1 && !(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ, f->bwd, 256, 0) == hipSuccess &&
-                     hipGetDevice(&dev) == hipSuccess &&
-                     hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && occ > 0))
+    int occ[2] = {}, dev = 0, ncu = 0;
+    if (st == 1 &&
+        !(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ[0], f->fn[0][FusedKernels::BWD], 256, 0) ==
+              hipSuccess &&
+          hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ[1], f->fn[1][FusedKernels::BWD], 256, 0) ==
+              hipSuccess &&
+          hipGetDevice(&dev) == hipSuccess &&
+          hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && occ[0] > 0 &&
+          occ[1] > 0))
         st = -5;
+    std::copy(regs, regs + FusedKernels::NK, f->regs);
+    std::copy(scratch, scratch + FusedKernels::NK, f->scratch);
     if (st != 1) {
         (void)hipGetLastError();
         f->release();
-        std::copy(regs, regs + FusedKernels::NK, f->regs);
-        std::copy(scratch, scratch + FusedKernels::NK, f->scratch);
         f->status = st;
         return -3;
     }
-    std::copy(regs, regs + FusedKernels::NK, f->regs);
-    std::copy(scratch, scratch + FusedKernels::NK, f->scratch);
-    f->grid = occ * ncu;
+    f->grid[0] = occ[0] * ncu;
+    f->grid[1] = occ[1] * ncu;
     f->status = 1;
     f->ok = true;
     return 0;
@@ -1186,8 +1230,19 @@ void FusedKernels::release()
 {
     if (mod) (void)hipModuleUnload(mod);
     mod = nullptr;
-    fwd_in = fwd = fwd_last = bwd_last = bwd = bwd_in = nullptr;
+    for (auto &row : fn)
+        for (auto &k : row) k = nullptr;
     ok = false;
+}
+
+// the kernels taking h (H non-null) have it as their last argument
+static hipError_t fz_launch(const FusedKernels &f, int role, const float *H, unsigned grid, void **params, int np,
+                            hipStream_t s)
+{
+    void *pv[16];
+    std::copy(params, params + np, pv);
+    if (H) pv[np] = (void *)&H;
+    return hipModuleLaunchKernel(f.fn[H ? 1 : 0][role], grid, 1, 1, 256, 1, 1, 0, s, pv, nullptr);
 }
 
 hipError_t fz_launch_fwd_in(const FusedKernels &f, const double *x, int npt, const float *W1, float *Tout, int nblk,
@@ -1195,36 +1250,36 @@ hipError_t fz_launch_fwd_in(const FusedKernels &f, const double *x, int npt, con
 {
     ActParams a = ap;
     void *params[] = {&x, &npt, &W1, &Tout, &nblk, &a};
-    return hipModuleLaunchKernel(f.fwd_in, (unsigned)nblk, 1, 1, 256, 1, 1, 0, s, params, nullptr);
+    return fz_launch(f, FusedKernels::FWD_IN, nullptr, (unsigned)nblk, params, 6, s);
 }
 
-hipError_t fz_launch_fwd(const FusedKernels &f, const float *Tin, const float *W, float *Tout, int nblk,
+hipError_t fz_launch_fwd(const FusedKernels &f, const float *Tin, const float *W, float *Tout, float *H, int nblk,
                          const ActParams &ap, hipStream_t s)
 {
     ActParams a = ap;
     void *params[] = {&Tin, &W, &Tout, &nblk, &a};
-    return hipModuleLaunchKernel(f.fwd, (unsigned)nblk, 1, 1, 256, 1, 1, 0, s, params, nullptr);
+    return fz_launch(f, FusedKernels::FWD, H, (unsigned)nblk, params, 5, s);
 }
 
-hipError_t fz_launch_fwd_last(const FusedKernels &f, const float *Tin, const float *W, int nblk, const ActParams &ap,
-                              const float *WL, int npt, const DevResidual &r, double fac, float *Sbo, double *gpart,
-                              double *rpart, hipStream_t s)
+hipError_t fz_launch_fwd_last(const FusedKernels &f, const float *Tin, const float *W, float *H, int nblk,
+                              const ActParams &ap, const float *WL, int npt, const DevResidual &r, double fac,
+                              float *Sbo, double *gpart, double *rpart, hipStream_t s)
 {
     ActParams a = ap;
     DevResidual rr = r;
     void *params[] = {&Tin, &W, &nblk, &a, &WL, &npt, &rr, &fac, &Sbo, &gpart, &rpart};
-    return hipModuleLaunchKernel(f.fwd_last, (unsigned)nblk, 1, 1, 256, 1, 1, 0, s, params, nullptr);
+    return fz_launch(f, FusedKernels::FWD_LAST, H, (unsigned)nblk, params, 11, s);
 }
 
 // the adjoint of a hidden layer l >= 2 (last: T-bar from the output row's
 // seed in Tb)
-hipError_t fz_launch_bwd(const FusedKernels &f, bool last, const float *Tin, const float *Tb, const float *W,
-                         const float *WL, float *Tbout, double *part, int nblk, const ActParams &ap, int grid,
-                         hipStream_t s)
+hipError_t fz_launch_bwd(const FusedKernels &f, bool last, const float *Tin, const float *Tb, const float *H,
+                         const float *W, const float *WL, float *Tbout, double *part, int nblk, const ActParams &ap,
+                         int grid, hipStream_t s)
 {
     ActParams a = ap;
     void *params[] = {&Tin, &Tb, &W, &WL, &Tbout, &part, &nblk, &a};
-    return hipModuleLaunchKernel(last ? f.bwd_last : f.bwd, (unsigned)grid, 1, 1, 256, 1, 1, 0, s, params, nullptr);
+    return fz_launch(f, last ? FusedKernels::BWD_LAST : FusedKernels::BWD, H, (unsigned)grid, params, 8, s);
 }
 
 hipError_t fz_launch_bwd_in(const FusedKernels &f, const double *x, int npt, const float *Tb, const float *W1,
@@ -1232,7 +1287,7 @@ hipError_t fz_launch_bwd_in(const FusedKernels &f, const double *x, int npt, con
 {
     ActParams a = ap;
     void *params[] = {&x, &npt, &Tb, &W1, &part, &nblk, &a};
-    return hipModuleLaunchKernel(f.bwd_in, (unsigned)grid, 1, 1, 256, 1, 1, 0, s, params, nullptr);
+    return fz_launch(f, FusedKernels::BWD_IN, nullptr, (unsigned)grid, params, 7, s);
 }
 
 }  // namespace dnnf90

@@ -1237,14 +1237,16 @@ static int grow_partial(BatchWork &b, long n)
 
 // The collocation step on the fused layer kernels.  The work space's planes
 // hold the point-blocked layout ([block][slot][4 points][neuron]): T of the
-// hidden layers but the last (tpl), the seed and the T-bar planes the
-// adjoint passes down (p0, p1).  Forward: the first hidden layer from the
-// inputs, then one kernel a layer (K1 + Faa di Bruno); the last one also
-// forms the output row, the residual, the seed, the output layer's gradient
-// partials and the sums of R^2.  Then per hidden layer from the last: one
-// kernel (K1 recomputed, h, pair sums, K5, T-bar of the layer below), the
-// first hidden layer's forming its gradient from the inputs.  The planes are
-// left in that layout: no backward_batch may follow.
+// hidden layers but the last (tpl), h of the hidden layers but the first
+// (hpl; not with per-layer recompute), the seed and the T-bar planes the adjoint passes
+// down (p0, p1).  Forward: the first hidden layer from the inputs, then one
+// kernel a layer (K1 + Faa di Bruno, T and h); the last one also forms the
+// output row, the residual, the seed, the output layer's gradient partials
+// and the sums of R^2.  Then per hidden layer from the last: one kernel (h
+// read, or with per-layer recompute S and h recomputed from T below; pair
+// sums, K5, T-bar of the layer below), the first hidden layer's forming its
+// gradient from the inputs.  The planes are left in that layout: no
+// backward_batch may follow.
 static int run_residual_fused(Net &nt, BatchWork &b, const FusedKernels &f, int npt, const double *x,
                               const DevResidual &r, double fac, Grad &g, double *rsq, bool dev)
 {
@@ -1260,7 +1262,9 @@ static int run_residual_fused(Net &nt, BatchWork &b, const FusedKernels &f, int 
     }
     const long Np = round_up(std::max(npt, 1), 64);
     const int nblk = (int)(Np / FZ_P);
-    const int grid = std::min(nblk, f.grid);
+    const bool kh = !b.recompute;   // h planes kept (hpl)
+    const int grid = std::min(nblk, f.grid[kh ? 1 : 0]);
+    auto H = [&](int l) { return kh ? (float *)b.hpl[l] : nullptr; };
     b.np = Np;
     b.npt = npt;
     b.pt_used = false;
@@ -1280,13 +1284,13 @@ static int run_residual_fused(Net &nt, BatchWork &b, const FusedKernels &f, int 
     }
     for (int l = 2; l < L - 2 && e == hipSuccess; ++l) {
         PScope p(b, CAT_BELL_FWD);
-        e = fz_launch_fwd(f, (const float *)b.tpl[l - 1], W + nt.woff[l], (float *)b.tpl[l], nblk, act, s);
+        e = fz_launch_fwd(f, (const float *)b.tpl[l - 1], W + nt.woff[l], (float *)b.tpl[l], H(l), nblk, act, s);
     }
     double *gpart = b.partial, *rpart = b.partial + (long)nblk * (FZ_W + 1);
     if (e == hipSuccess) {
         PScope p(b, CAT_BELL_FWD);
-        e = fz_launch_fwd_last(f, (const float *)b.tpl[L - 3], W + nt.woff[L - 2], nblk, act, W + nt.woff[L - 1], npt,
-                               r, fac, (float *)b.p0, gpart, rpart, s);
+        e = fz_launch_fwd_last(f, (const float *)b.tpl[L - 3], W + nt.woff[L - 2], H(L - 2), nblk, act,
+                               W + nt.woff[L - 1], npt, r, fac, (float *)b.p0, gpart, rpart, s);
     }
     if (e == hipSuccess) {
         PScope p(b, CAT_GRAD_REDUCE);
@@ -1299,7 +1303,7 @@ static int run_residual_fused(Net &nt, BatchWork &b, const FusedKernels &f, int 
         const bool last = l == L - 2;
         {
             PScope p(b, CAT_BELL_BWD);
-            e = fz_launch_bwd(f, last, (const float *)b.tpl[l - 1], (const float *)tb, W + nt.woff[l],
+            e = fz_launch_bwd(f, last, (const float *)b.tpl[l - 1], (const float *)tb, H(l), W + nt.woff[l],
                               W + nt.woff[L - 1], (float *)tbo, part, nblk, act, grid, s);
         }
         if (e) break;
@@ -2358,24 +2362,26 @@ int dnnf90_net_fused_info(int nid, int *info)
     BatchWork probe;
     probe.prec = PREC_FP32;
     const FusedKernels *f = fused_plan(*nt, probe);
-    for (int i = 0; i < 14; ++i) info[i] = 0;
+    constexpr int NK = FusedKernels::NK;
+    for (int i = 0; i < 3 + 2 * NK; ++i) info[i] = 0;
     if (!f) {
         // the network's shape or the switch rules them out (0), or the
         // module did not come up (FusedKernels::status)
         if (nt->fz_tried) info[0] = nt->fz.status == 0 ? -1 : nt->fz.status;
         if (nt->fz_tried)
-            for (int i = 0; i < FusedKernels::NK; ++i) {
+            for (int i = 0; i < NK; ++i) {
                 info[1 + i] = nt->fz.regs[i];
-                info[7 + i] = nt->fz.scratch[i];
+                info[1 + NK + i] = nt->fz.scratch[i];
             }
         return 0;
     }
     info[0] = 1;
-    for (int i = 0; i < FusedKernels::NK; ++i) {
+    for (int i = 0; i < NK; ++i) {
         info[1 + i] = f->regs[i];
-        info[7 + i] = f->scratch[i];
+        info[1 + NK + i] = f->scratch[i];
     }
-    info[13] = f->grid;
+    info[1 + 2 * NK] = f->grid[0];
+    info[2 + 2 * NK] = f->grid[1];
     return 0;
 }
 

@@ -116,10 +116,12 @@ int dnnf90_set_fused_grad(int on);
  * layers, all 64 wide, one output row and at most 8 inputs take the
  * collocation step (dnnf90_residual_step_batch) on fused layer kernels:
  * point-blocked planes ([block][slot][4 points][neuron]), each hidden
- * layer's linear map inside its Faa di Bruno forward (S never stored), and
- * one adjoint kernel a layer that recomputes S, forms h, the pair sums, the
- * layer's weight gradient and Tbar of the layer below (Sbar never stored);
- * linear maps on FP32 MFMA, the forward map moved into FP64 every 16 terms.
+ * layer's linear map inside its Faa di Bruno forward (S never stored; T and
+ * h written), and one adjoint kernel a layer that reads h (with per-layer
+ * recompute, dnnf90_bwork_create_budget: recomputes S and h from T of the
+ * layer below), forms the pair sums, the layer's weight gradient and Tbar
+ * of the layer below (Sbar never stored); linear maps on FP32 MFMA, the
+ * forward map moved into FP64 every 16 terms.
  * The work space's planes are then in that layout, so a dnnf90_backward_batch
  * must follow a dnnf90_forward_batch, not the step (refused otherwise, as
  * always).  dnnf90_set_fused_layers(0) takes the layer-by-layer kernels
@@ -128,13 +130,15 @@ int dnnf90_set_fused_grad(int on);
  * point at a time). */
 int dnnf90_set_fused_layers(int on);
 /* whether the fused layer kernels take this network's FP32 collocation
- * step, info[14]: [0] 1 taken; 0 not (shape or switch); -1 the family is
+ * step, info[27]: [0] 1 taken; 0 not (shape or switch); -1 the family is
  * too large to unroll, -2 hipRTC failed, -3 module or kernel missing, -4 a
- * kernel spills past its allowance, -5 no occupancy; [1..6] registers and [7..12] scratch
- * bytes a lane of the six kernels (first hidden layer forward, forward,
- * last hidden layer forward with the output row, last hidden layer adjoint,
- * adjoint, first hidden layer adjoint); [13] the adjoint kernels' grid.
- * Compiles the module on first use.  No counterpart in the reference. */
+ * kernel spills past its allowance, -5 no occupancy; [1..12] registers and
+ * [13..24] scratch bytes a lane of the twelve kernels: first hidden layer
+ * forward, forward, last hidden layer forward with the output row, last
+ * hidden layer adjoint, adjoint, first hidden layer adjoint, first with S
+ * and h recomputed in the adjoint, then with h kept; [25], [26] the
+ * adjoint kernels' grid in the two.  Compiles the module on first use.  No
+ * counterpart in the reference. */
 int dnnf90_net_fused_info(int nid, int *info);
 /* Debug mode for reads of unwritten memory: batched work spaces created
  * while it is on fill their planes, operand scratch and gradient partials

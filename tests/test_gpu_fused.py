@@ -3,8 +3,10 @@
 An FP32 work space whose network has at least two hidden layers, all 64
 wide, and one output row takes dnnf90_residual_step_batch on the fused
 kernels (point-blocked planes; each hidden layer's linear map inside its
-Faa di Bruno forward, one adjoint kernel a layer forming S, h, the pair
-sums, the weight gradient and T-bar of the layer below).  The step's
+Faa di Bruno forward, which also writes h; one adjoint kernel a layer
+reading h -- or, with per-layer recompute, forming S and h again from T of
+the layer below -- and forming the pair sums, the weight gradient and T-bar
+of the layer below).  The two plane policies give the same bits.  The step's
 gradient and sum of squared residuals are checked against the oracle's FP64
 pinn_step (the restatement of app/pinn_module.f90:32-65 over
 lib/train_module.f90:241-450, pinned bitwise to the reference), at the FP32
@@ -45,12 +47,13 @@ def _cases():
     }
 
 
-def _step(D, net, x, terms, npt, fused, poison=False):
+def _step(D, net, x, terms, npt, fused, poison=False, recompute=False):
     from dnnf90_amd._lib import lib
     prev = lib().dnnf90_set_fused_layers(1 if fused else 0)
     prev_p = lib().dnnf90_set_poison(1 if poison else 0)
     try:
-        bw = D.BatchWork(net, npt, D.PREC_FP32)
+        bw = D.BatchWork(net, npt, D.PREC_FP32, budget=1 if recompute else 0)
+        assert bw.info()["recompute"] == recompute
         g = D.Grad(net)
         rsq = bw.residual_step(x, D.Residual(terms), 2.0 / npt, g)
         out = g.get().copy()
@@ -71,10 +74,11 @@ def test_fused_step_vs_oracle(D, oracle_mod, name):
     net = D.Net(ts, wl.dims)
     w, x = wl.inputs(npt)
     net.weights = w
-    info = np.zeros(14, np.int32)
+    info = np.zeros(27, np.int32)
     assert lib().dnnf90_net_fused_info(net.id, info.ctypes.data) == 0
     print("fused info", info.tolist(), flush=True)
     assert info[0] == 1, "the fused kernels must take this step"
+    assert info[25] > 0 and info[26] > 0 and max(info[13:25]) <= 256
     o = oracle_mod.Oracle(wl.d0, wl.k, wl.seeds, wl.act)
     terms = wl.terms(o.alpha())
     gr, rsq_r, _ = o.pinn_step(wl.dims, w, x, terms, 2.0 / npt)
@@ -90,6 +94,12 @@ def test_fused_step_vs_oracle(D, oracle_mod, name):
     assert np.array_equal(gf, gf2) and rsq_f == rsq_f2
     gp, rsq_p = _step(D, net, x, terms, npt, True, poison=True)
     assert np.array_equal(gf, gp) and rsq_f == rsq_p
+    # S and h recomputed in the adjoint (per-layer recompute): the same bits,
+    # also with the planes poisoned
+    gr2, rsq_r2 = _step(D, net, x, terms, npt, True, recompute=True)
+    assert np.array_equal(gf, gr2) and rsq_f == rsq_r2
+    gr3, rsq_r3 = _step(D, net, x, terms, npt, True, poison=True, recompute=True)
+    assert np.array_equal(gf, gr3) and rsq_f == rsq_r3
     net.free()
 
 
@@ -101,7 +111,7 @@ def test_fused_not_taken(D):
         wl = WORKLOADS[name]
         ts = D.TableSet(wl.d0, wl.k, wl.seeds, wl.act)
         net = D.Net(ts, wl.dims)
-        info = np.ones(14, np.int32)
+        info = np.ones(27, np.int32)
         assert lib().dnnf90_net_fused_info(net.id, info.ctypes.data) == 0
         assert info[0] == 0, name
         net.free()
@@ -110,7 +120,8 @@ def test_fused_not_taken(D):
 @pytest.mark.parametrize("name", ["closure4_k7", "ngd2_k4"])
 def test_fused_full_size(D, name):
     """The benchmark sizes (closure4_k7 N = 16 384: 4 096 point blocks, eight
-    a workgroup in the adjoint; ngd2_k4 N = 4 096): the fused FP32 step
+    a workgroup in the adjoint; ngd2_k4 N = 4 096): the fused FP32 step (h
+    kept and, bit for bit the same, recomputed)
     against the FP64 step (the FP64 path matches the oracle to 1e-12 at these
     sizes, tests/test_gpu_fullsize.py) at the FP32 modes' tolerance, against
     the layer-by-layer FP32 kernels, and additive over four quarter batches
@@ -134,6 +145,8 @@ def test_fused_full_size(D, name):
     g64.free()
     gf, rsq_f = _step(D, net, x, terms, N, True)
     gl, rsq_l = _step(D, net, x, terms, N, False)
+    gr2, rsq_r2 = _step(D, net, x, terms, N, True, recompute=True)
+    assert np.array_equal(gf, gr2) and rsq_f == rsq_r2
     quarters = np.zeros_like(gf)
     rq = 0.0
     for k in range(4):
