@@ -763,6 +763,27 @@ __device__ __forceinline__ void fz_stage(const float *__restrict__ src, float *_
   for (int k = 0; k < NU; ++k)
     if (tid + 256 * k < NA * 64) *reinterpret_cast<flt4 *>(X + xo + k * 4 * XS) = v[k];
 }
+// the same in two halves: the global loads into registers (issued a block
+// ahead) and the LDS writes
+#define FZ_NU ((NA * 64 + 255) / 256)
+__device__ __forceinline__ void fz_stage_load(const float *__restrict__ src, flt4 (&v)[FZ_NU])
+{
+  const int tid = threadIdx.x;
+  int so = tid;
+  FZ_OPAQUE(so);
+#pragma unroll
+  for (int k = 0; k < FZ_NU; ++k)
+    if (tid + 256 * k < NA * 64) v[k] = reinterpret_cast<const flt4 *>(src)[so + 256 * k];
+}
+__device__ __forceinline__ void fz_stage_store(const flt4 (&v)[FZ_NU], float *__restrict__ X)
+{
+  const int tid = threadIdx.x;
+  int xo = (tid >> 6) * XS + ((tid >> 4) & 3) * PJ + (tid & 15) * 4;
+  FZ_OPAQUE(xo);
+#pragma unroll
+  for (int k = 0; k < FZ_NU; ++k)
+    if (tid + 256 * k < NA * 64) *reinterpret_cast<flt4 *>(X + xo + k * 4 * XS) = v[k];
+}
 __device__ __forceinline__ void fz_zero_pad(float *__restrict__ X)
 {
   for (int u = threadIdx.x; u < (4 * NT - NA) * XS; u += 256) X[NA * XS + u] = 0.f;
@@ -776,6 +797,18 @@ __device__ __forceinline__ flt4 fz_tile(const float *__restrict__ X, int xa, int
 #pragma unroll
   for (int s = 0; s < 16; ++s) c = __builtin_amdgcn_mfma_f32_16x16x4f32(X[xa + t * 4 * XS + 4 * s], b[s], c, 0, 0, 0);
   return c;
+}
+// two tiles at once: two independent accumulator chains for the MFMA pipe
+__device__ __forceinline__ void fz_tile2(const float *__restrict__ X, int xa, int t0, const float (&b)[16], flt4 &c0,
+                                         flt4 &c1)
+{
+  c0 = flt4{0.f, 0.f, 0.f, 0.f};
+  c1 = c0;
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(X[xa + t0 * 4 * XS + 4 * s], b[s], c0, 0, 0, 0);
+    c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(X[xa + (t0 + 1) * 4 * XS + 4 * s], b[s], c1, 0, 0, 0);
+  }
 }
 // the forward map: every FL k-steps the partial sum moves into FP64; slot 0 takes the bias
 __device__ __forceinline__ flt4 fz_tile64(const float *__restrict__ X, int xa, int t, const float (&b)[16], float bias)
@@ -871,6 +904,16 @@ std::string gen_fused_source(const TableSet &t, const FwdSchedule &s)
     };
     const std::function<void(int)> tile_hook = [&](int a) {
         if (a % 4 == 0 && a / 4 + 1 < nt) tile(a / 4 + 1);
+    };
+    // the same with a barrier after each tile: tile q reads the staged
+    // slots 4q .. 4q+3 only, so once every wave has passed the barrier after
+    // tile q + 1 the slots up to 4q + 7 may take the layer's own T (the last
+    // hidden layer's output row reads it from there)
+    const std::function<void(int)> tile_hook_sync = [&](int a) {
+        if (a % 4 == 0 && a / 4 + 1 < nt) {
+            tile(a / 4 + 1);
+            o << "  __syncthreads();\n";
+        }
     };
     // S of the lane from the inputs (the FP32 input values, S_0 summed in FP64)
     auto s_from_in = [&]() {
@@ -990,66 +1033,94 @@ std::string gen_fused_source(const TableSet &t, const FwdSchedule &s)
                  2);
         xbuf();
         o << "  const int pb = blockIdx.x;\n  if (pb >= nblk) return;\n";
-        if (last) o << "#define FZ_OUT(a, tv) X[xs + (a) * XS] = (tv);\n";
+        if (last)
+            // the residual's terms in LDS, loaded beside the staged block
+            // (the output section reads them many times a block)
+            o << "#define FZ_OUT(a, tv) X[xs + (a) * XS] = (tv);\n"
+              << "  __shared__ double Rc[" << FZ_MAXT << "];\n  __shared__ int Rsl[" << FZ_MAXT << "], Rnl["
+              << FZ_MAXT << "];\n"
+              << "  if (tid < rs.nterm) {\n    Rc[tid] = rs.coeff[tid];\n    Rsl[tid] = rs.slot[tid];\n"
+              << "    Rnl[tid] = rs.nonlin[tid];\n  }\n"
+              << "  __shared__ float WLs[64];\n  if (tid < 64) WLs[tid] = WL[1 + tid];\n";
         o << "  const float bias = W[j * 65];\n"
           << "  fz_stage(Tin + (long)pb * BLK, X);\n  __syncthreads();\n";
         s_from_x("W[wb + 4 * s]");
         // the last hidden layer's T goes to the output row only, through the
-        // staged block's place in LDS once every wave has read it (all tiles)
+        // staged block's place in LDS, each slot once every wave has read it
+        // (the tiles keep their place in the recurrence, with a barrier each)
         if (last) {
-            for (int q = 2; q < nt; ++q) tile(q);
             o << "  __syncthreads();\n  FZ_OUT(0, d0);\n";
         } else {
             o << "  real *Tj = Tout + (long)pb * BLK + lo;\n  Tj[0] = d0;\n";
         }
         hout(kh);
         emit_recurrence(o, t, s, false, (last ? REC_TOUT : REC_T) | (kh ? rec_h : 0), "Np",
-                        last ? nullptr : tile_hook);
+                        last ? tile_hook_sync : tile_hook);
         if (last)
             o << "#undef FZ_OUT\n"
               << "  __shared__ double Ot[NA * 4];   // the output row, FP32-rounded, (slot, point)\n"
               << "  __shared__ double Rq[4];        // R of the block's points\n"
               << "  __shared__ float Sz[NA * 4];    // the seed\n"
+              << "  __shared__ double Op[4 * NA * 4 > 256 ? 4 * NA * 4 : 256];   // per-wave partial sums\n"
               << "  __syncthreads();\n"
-              << "  if (tid < NA * 4) {\n"
-              << "    double v = 0.0;\n"
-              << "    for (int m = 0; m < 64; ++m) v += (double)WL[1 + m] * (double)X[(tid >> 2) * XS + (tid & 3) * PJ + m];\n"
-              << "    Ot[tid] = (double)(float)(v + (tid < 4 ? (double)WL[0] : 0.0));\n"
+              // wave w sums its 16 neurons of each (slot, point), the four
+              // partials meet in wave order
+              << "#pragma unroll\n"
+              << "  for (int k = 0; k < 3; ++k) {\n"
+              << "    const int u = lane + 64 * k;\n"
+              << "    if (u < NA * 4) {\n"
+              << "      const float *xu = X + (u >> 2) * XS + (u & 3) * PJ + 16 * w;\n"
+              << "      double v = 0.0;\n"
+              << "#pragma unroll\n"
+              << "      for (int m = 0; m < 16; ++m) v += (double)WLs[16 * w + m] * (double)xu[m];\n"
+              << "      Op[w * (NA * 4) + u] = v;\n"
+              << "    }\n"
               << "  }\n"
               << "  __syncthreads();\n"
+              << "  if (tid < NA * 4) {\n"
+              << "    const double v = ((Op[tid] + Op[NA * 4 + tid]) + Op[2 * NA * 4 + tid]) + Op[3 * NA * 4 + tid];\n"
+              << "    Ot[tid] = (double)(float)(v + (tid < 4 ? (double)WL[0] : 0.0));\n"
+              << "    Sz[tid] = 0.f;\n"
+              << "  }\n"
+              << "  __syncthreads();\n"
+              // one lane a point: R, then the seed of each term's slot in
+              // the terms' order (k_seed_residual's arithmetic), slot 0's
+              // nonlinear part last
               << "  if (tid < 4) {\n"
               << "    double R = 0.0;\n"
               << "    if ((long)pb * 4 + tid < npt) {\n"
               << "      const double t0 = Ot[tid];\n"
               << "      for (int k = 0; k < rs.nterm; ++k) {\n"
-              << "        const double tv = Ot[rs.slot[k] * 4 + tid];\n"
-              << "        R += rs.nonlin[k] ? rs.coeff[k] * t0 * tv : rs.coeff[k] * tv;\n"
+              << "        const double tv = Ot[Rsl[k] * 4 + tid];\n"
+              << "        R += Rnl[k] ? Rc[k] * t0 * tv : Rc[k] * tv;\n"
               << "      }\n"
+              << "      double s0 = 0.0;\n"
+              << "      for (int k = 0; k < rs.nterm; ++k) {\n"
+              << "        const double add = Rnl[k] ? fac * R * Rc[k] * t0 : fac * R * Rc[k];\n"
+              << "        if (Rnl[k]) s0 += fac * R * Rc[k] * Ot[Rsl[k] * 4 + tid];\n"
+              << "        float &sv = Sz[Rsl[k] * 4 + tid];\n"
+              << "        sv = rs.distinct ? (float)(0.0 + add) : (float)((double)sv + add);\n"
+              << "      }\n"
+              << "      Sz[tid] = rs.distinct ? (float)(0.0 + s0) : (float)((double)Sz[tid] + s0);\n"
               << "    }\n"
               << "    Rq[tid] = R;\n"
               << "  }\n"
               << "  __syncthreads();\n"
-              << "  if (tid < NA * 4) {\n"
-              << "    const int a = tid >> 2, q = tid & 3;\n"
-              << "    float sv = 0.f;\n"
-              << "    if ((long)pb * 4 + q < npt) {\n"
-              << "      const double R = Rq[q], t0 = Ot[q];\n"
-              << "      double s0 = 0.0;\n"
-              << "      for (int k = 0; k < rs.nterm; ++k) {\n"
-              << "        const double add = rs.nonlin[k] ? fac * R * rs.coeff[k] * t0 : fac * R * rs.coeff[k];\n"
-              << "        if (rs.nonlin[k]) s0 += fac * R * rs.coeff[k] * Ot[rs.slot[k] * 4 + q];\n"
-              << "        if (rs.slot[k] == a) sv = rs.distinct ? (float)(0.0 + add) : (float)((double)sv + add);\n"
-              << "      }\n"
-              << "      if (a == 0) sv = rs.distinct ? (float)(0.0 + s0) : (float)((double)sv + s0);\n"
+              << "  if (tid < NA * 4) Sbo[(long)pb * (NA * 4) + tid] = Sz[tid];\n"
+              // wave w sums slots [w NA, (w + 1) NA) of the (slot, point)
+              // pairs for every neuron, the partials meet in wave order
+              << "  {\n"
+              << "    double g = 0.0;\n"
+              << "#pragma unroll\n"
+              << "    for (int i = 0; i < NA; ++i) {\n"
+              << "      const int u = w * NA + i;\n"
+              << "      g += (double)Sz[u] * (double)X[(u >> 2) * XS + (u & 3) * PJ + lane];\n"
               << "    }\n"
-              << "    Sz[tid] = sv;\n"
-              << "    Sbo[(long)pb * (NA * 4) + tid] = sv;\n"
+              << "    Op[w * 64 + lane] = g;\n"
               << "  }\n"
               << "  __syncthreads();\n"
               << "  if (tid < 64) {\n"
-              << "    double g = 0.0;\n"
-              << "    for (int u = 0; u < NA * 4; ++u) g += (double)Sz[u] * (double)X[(u >> 2) * XS + (u & 3) * PJ + tid];\n"
-              << "    gpart[(long)pb * 65 + 1 + tid] = g;\n"
+              << "    gpart[(long)pb * 65 + 1 + tid] = ((Op[tid] + Op[64 + tid]) + Op[128 + tid]) + Op[192 + tid];\n"
               << "  } else if (tid == 64) {\n"
               << "    gpart[(long)pb * 65] = (double)Sz[0] + (double)Sz[1] + (double)Sz[2] + (double)Sz[3];\n"
               << "  } else if (tid == 65) {\n"
@@ -1080,17 +1151,41 @@ std::string gen_fused_source(const TableSet &t, const FwdSchedule &s)
         if (!kh) o << "  const float bias = W[j * 65];\n";
         o << "  double gb = 0.0;\n";
         if (last) o << "  const real wl = WL[1 + j];\n  __shared__ float Sz[NA * 4];   // the block's output-row seed\n";
+        // kept h: each block's staged block below, h and the seed are
+        // loaded into registers during the block before it (h and the seed
+        // before its pair sums, the block below before its K6); T-bar of
+        // the layer above is loaded at the top of the block (both planes a
+        // block ahead would not fit the registers)
+        auto next_ops = [&](const char *pbv) {
+            o << "    const real *hp = H + (long)" << pbv << " * BLK + lo;\n";
+            for (int a = 0; a < na; ++a) o << "    hn" << a << " = hp[" << a * 256 << "];\n";
+            if (last) o << "    szn = tid < NA * 4 ? Tb[(long)" << pbv << " * (NA * 4) + tid] : 0.f;\n";
+        };
+        if (kh) {
+            o << "  flt4 xn[FZ_NU];\n";
+            for (int a = 0; a < na; ++a) o << "  real hn" << a << ";\n";
+            if (last) o << "  float szn;\n";
+            o << "  if ((int)blockIdx.x < nblk) {\n    fz_stage_load(Tin + (long)blockIdx.x * BLK, xn);\n";
+            next_ops("blockIdx.x");
+            o << "  }\n";
+        }
         o << "  for (int pb = blockIdx.x; pb < nblk; pb += gridDim.x) {\n";
         opaque();
-        o << "  __syncthreads();\n  fz_stage(Tin + (long)pb * BLK, X);\n";
-        if (last) o << "  if (tid < NA * 4) Sz[tid] = Tb[(long)pb * (NA * 4) + tid];\n";
         if (kh) {
-            // h of the lane's (neuron, point) from the forward's plane; the
-            // staged block below is only K5's operand
-            o << "  const real *hp = H + (long)pb * BLK + lo;\n";
-            for (int a = 0; a < na; ++a) o << "  const real h" << a << " = hp[" << a * 256 << "];\n";
-            o << "  __syncthreads();\n";
+            o << "  const int pn = pb + gridDim.x;\n"
+              << "  __syncthreads();\n  fz_stage_store(xn, X);\n";
+            if (last) o << "  if (tid < NA * 4) Sz[tid] = szn;\n";
+            if (!last) {
+                o << "  const real *tb = Tb + (long)pb * BLK + lo;\n";
+                for (int a = 0; a < na; ++a) o << "  const real t" << a << " = tb[" << a * 256 << "];\n";
+            }
+            for (int a = 0; a < na; ++a) o << "  const real h" << a << " = hn" << a << ";\n";
+            o << "  if (pn < nblk) {\n";
+            next_ops("pn");
+            o << "  }\n  __syncthreads();\n";
         } else {
+            o << "  __syncthreads();\n  fz_stage(Tin + (long)pb * BLK, X);\n";
+            if (last) o << "  if (tid < NA * 4) Sz[tid] = Tb[(long)pb * (NA * 4) + tid];\n";
             o << "  __syncthreads();\n";
             s_from_x("W[wb + 4 * s]");
             decl_h();
@@ -1101,7 +1196,7 @@ std::string gen_fused_source(const TableSet &t, const FwdSchedule &s)
         o << "  __builtin_amdgcn_sched_barrier(0);\n";
         if (last) {
             for (int a = 0; a < na; ++a) o << "  const real t" << a << " = wl * Sz[" << a * 4 << " + p];\n";
-        } else {
+        } else if (!kh) {
             o << "  const real *tb = Tb + (long)pb * BLK + lo;\n";
             for (int a = 0; a < na; ++a) o << "  const real t" << a << " = tb[" << a * 256 << "];\n";
         }
@@ -1109,6 +1204,7 @@ std::string gen_fused_source(const TableSet &t, const FwdSchedule &s)
         k5_begin();
         emit_pair_sums(o, t, false, true, k5_hook);
         k5_end();
+        if (kh) o << "  if (pn < nblk) fz_stage_load(Tin + (long)pn * BLK, xn);\n";
         // xa once more through the asm: K6 then forms its own LDS addresses
         // instead of keeping K1's alive across the block
         o << "  __syncthreads();\n  {\n";
@@ -1117,6 +1213,17 @@ std::string gen_fused_source(const TableSet &t, const FwdSchedule &s)
           << "#pragma unroll\n    for (int s = 0; s < 16; ++s) v[s] = W[wv + 4 * s * 65];\n"
           << "    real *tbo = Tbout + (long)pb * BLK + lo;\n";
         for (int q = 0; q < nt; ++q) {
+            if (kh && q + 1 < nt) {
+                // kept h leaves the registers for two tiles at a time
+                o << "    {\n      flt4 c, c1;\n      fz_tile2(X, xa, " << q << ", v, c, c1);\n";
+                for (int r = 0; r < 4 && 4 * q + r < na; ++r)
+                    o << "      tbo[" << (4 * q + r) * 256 << "] = c[" << r << "];\n";
+                for (int r = 0; r < 4 && 4 * q + 4 + r < na; ++r)
+                    o << "      tbo[" << (4 * q + 4 + r) * 256 << "] = c1[" << r << "];\n";
+                o << "    }\n    __builtin_amdgcn_sched_barrier(0);\n";
+                ++q;
+                continue;
+            }
             o << "    {\n      const flt4 c = fz_tile(X, xa, " << q << ", v);\n";
             for (int r = 0; r < 4 && 4 * q + r < na; ++r) o << "      tbo[" << (4 * q + r) * 256 << "] = c[" << r << "];\n";
             o << "    }\n    __builtin_amdgcn_sched_barrier(0);\n";

@@ -105,6 +105,8 @@ hipError_t gen_launch_pt_bwd(const GenKernels &g, const double *W, double *grad,
 // bank-conflicted; FZ_FL: k-steps of the
 // forward map between FP64 flushes (four terms a step).
 constexpr int FZ_W = 64, FZ_P = 4, FZ_MAXA = 48, FZ_MAXD0 = 8, FZ_PJ = 68, FZ_XS = 272, FZ_FL = 4;
+// terms of the collocation residual the fused last forward holds in LDS
+constexpr int FZ_MAXT = 64;
 // scratch (bytes a lane) a fused kernel may hold: loop-invariant values the
 // register allocator parks there and reloads a few times a block
 constexpr int FZ_MAX_SCRATCH = 256;

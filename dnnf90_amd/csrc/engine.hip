@@ -1203,13 +1203,14 @@ static int set_residual(BatchWork &b, int nterm, const double *coeff, const int 
 }
 
 // the fused layer kernels take a collocation step (not per-point rows) of an
-// FP32 work space when the network has at least two hidden layers, every
-// one FZ_W wide, one output row, at most FZ_MAXD0 inputs, and the family's
-// fused module compiled (codegen.cpp)
+// FP32 work space when the network has at least two hidden layers (at most
+// RS_MAX layers in all: one reduction segment a layer), every one FZ_W
+// wide, one output row, at most FZ_MAXD0 inputs, and the family's fused
+// module compiled (codegen.cpp)
 static const FusedKernels *fused_plan(Net &nt, const BatchWork &b)
 {
     const int L = nt.nlayer;
-    if (!g_fused_layers || !codegen_on() || b.prec != PREC_FP32 || L < 4) return nullptr;
+    if (!g_fused_layers || !codegen_on() || b.prec != PREC_FP32 || L < 4 || L > RS_MAX) return nullptr;
     if (nt.dims[L - 1] != 1 || nt.dims[0] > FZ_MAXD0) return nullptr;
     for (int l = 1; l < L - 1; ++l)
         if (nt.dims[l] != FZ_W) return nullptr;
@@ -1269,9 +1270,12 @@ static int run_residual_fused(Net &nt, BatchWork &b, const FusedKernels &f, int 
     b.npt = npt;
     b.pt_used = false;
     b.fwd_valid = false;
-    // partials: the adjoint kernels' (grid x 64 x 65), before them the
-    // output layer's per block (nblk x 65) and the blocks' sums of R^2
-    if ((rc = grow_partial(b, std::max((long)grid * FZ_W * (FZ_W + 1), (long)nblk * (FZ_W + 2))))) return rc;
+    // partials, each layer's in its own range, all reduced by one launch
+    // after the adjoint: the output layer's per block (nblk x 65), the
+    // blocks' sums of R^2 (nblk), then the adjoint kernels' of layers L-2 ..
+    // 2 (grid x 64 x 65 each) and of layer 1 (grid x 64 x (d0 + 1))
+    const long hid = (long)grid * FZ_W * (FZ_W + 1);
+    if ((rc = grow_partial(b, (long)nblk * (FZ_W + 2) + (L - 3) * hid + (long)grid * FZ_W * (d0 + 1)))) return rc;
     if (b.poison)
         if (hipError_t e = poison_planes(b)) return hip_fail(e, "poison");
     const float *W = nt.d_w32;
@@ -1292,25 +1296,17 @@ static int run_residual_fused(Net &nt, BatchWork &b, const FusedKernels &f, int 
         e = fz_launch_fwd_last(f, (const float *)b.tpl[L - 3], W + nt.woff[L - 2], H(L - 2), nblk, act,
                                W + nt.woff[L - 1], npt, r, fac, (float *)b.p0, gpart, rpart, s);
     }
-    if (e == hipSuccess) {
-        PScope p(b, CAT_GRAD_REDUCE);
-        launch_grad_reduce(gpart, nblk, FZ_W + 1, g.d_g + nt.woff[L - 1], s);
-        if (rsq) launch_reduce_sum(rpart, nblk, b.d_rsq, s);
-    }
-    double *part = b.partial;
+    ReduceSegs red;
+    red.add(gpart, nblk, FZ_W + 1, g.d_g + nt.woff[L - 1]);
+    if (rsq) red.add(rpart, nblk, 1, b.d_rsq, true);
+    double *part = rpart + nblk;
     void *tb = b.p0, *tbo = b.p1;
-    for (int l = L - 2; l >= 2 && e == hipSuccess; --l) {
+    for (int l = L - 2; l >= 2 && e == hipSuccess; --l, part += hid) {
         const bool last = l == L - 2;
-        {
-            PScope p(b, CAT_BELL_BWD);
-            e = fz_launch_bwd(f, last, (const float *)b.tpl[l - 1], (const float *)tb, H(l), W + nt.woff[l],
-                              W + nt.woff[L - 1], (float *)tbo, part, nblk, act, grid, s);
-        }
-        if (e) break;
-        {
-            PScope p(b, CAT_GRAD_REDUCE);
-            launch_grad_reduce(part, grid, (long)FZ_W * (FZ_W + 1), g.d_g + nt.woff[l], s);
-        }
+        PScope p(b, CAT_BELL_BWD);
+        e = fz_launch_bwd(f, last, (const float *)b.tpl[l - 1], (const float *)tb, H(l), W + nt.woff[l],
+                          W + nt.woff[L - 1], (float *)tbo, part, nblk, act, grid, s);
+        red.add(part, grid, (long)FZ_W * (FZ_W + 1), g.d_g + nt.woff[l]);
         std::swap(tb, tbo);
     }
     if (e == hipSuccess) {
@@ -1318,9 +1314,10 @@ static int run_residual_fused(Net &nt, BatchWork &b, const FusedKernels &f, int 
             PScope p(b, CAT_BELL_BWD);
             e = fz_launch_bwd_in(f, dx, npt, (const float *)tb, W + nt.woff[1], part, nblk, act, grid, s);
         }
+        red.add(part, grid, (long)FZ_W * (d0 + 1), g.d_g + nt.woff[1]);
         if (e == hipSuccess) {
             PScope p(b, CAT_GRAD_REDUCE);
-            launch_grad_reduce(part, grid, (long)FZ_W * (d0 + 1), g.d_g + nt.woff[1], s);
+            launch_reduce_segs(red, s);
         }
     }
     if ((rc = g.after(s))) return rc;
@@ -1344,7 +1341,7 @@ static int run_residual(Net &nt, BatchWork &b, int npt, const double *x, int nte
     int rc = set_residual(b, nterm, coeff, slot, nonlin, na, &r);
     if (rc) return rc;
     if constexpr (sizeof(T) == 4)
-        if (!rows_t && g)
+        if (!rows_t && g && r.nterm <= FZ_MAXT)
             if (const FusedKernels *f = fused_plan(nt, b)) return run_residual_fused(nt, b, *f, npt, x, r, fac, *g, rsq, dev);
     b.rows_next = rows_t != nullptr;
     rc = run_forward<T>(nt, b, npt, x, nullptr, dev);

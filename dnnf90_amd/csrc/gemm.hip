@@ -581,8 +581,6 @@ __global__ void k_reduce_pass2(const double *__restrict__ part2, int nsplit, lon
     grad[i] += s;
 }
 
-int grad_reduce_splits(int nchunk) { return nchunk >= 64 ? 16 : (nchunk >= 16 ? 4 : 1); }
-
 // the same two-level sum in one launch: thread (slice, weight) sums its
 // chunk range in order (k_reduce_pass1's loop), the slice sums meet in LDS
 // and are added in slice order (k_reduce_pass2's loop) -- the same
@@ -610,6 +608,46 @@ __global__ __launch_bounds__(256) void k_reduce_fused(const double *__restrict__
         for (int k = 0; k < nsplit; ++k) s += red[k * wpb + wi];
         grad[i] += s;
     }
+}
+
+// several of k_reduce_fused's reductions in one launch (blockIdx.y picks
+// the segment), each with its own split count: the same bits as one launch
+// a segment; set: the segment's output receives the sum instead of adding it
+__global__ __launch_bounds__(256) void k_reduce_segs(ReduceSegs r)
+{
+    __shared__ double red[256];
+    const int g = blockIdx.y;
+    const long len = r.len[g];
+    const int nchunk = r.nchunk[g], nsplit = grad_reduce_splits(nchunk);
+    const int wpb = 256 / nsplit, t = threadIdx.x;
+    if ((long)blockIdx.x * wpb >= len) return;   // the whole workgroup: this segment is shorter
+    const int wi = t % wpb, sidx = t / wpb;
+    const long i = (long)blockIdx.x * wpb + wi;
+    const double *partial = r.partial[g];
+    if (i < len) {
+        const int per = (nchunk + nsplit - 1) / nsplit;
+        const int c0 = sidx * per, c1 = min(nchunk, c0 + per);
+        double s = 0.0;
+#pragma unroll 8
+        for (int c = c0; c < c1; ++c) s += partial[(long)c * len + i];
+        red[t] = s;
+    }
+    __syncthreads();
+    if (sidx == 0 && i < len) {
+        double s = 0.0;
+        for (int k = 0; k < nsplit; ++k) s += red[k * wpb + wi];
+        double *out = r.out[g];
+        out[i] = r.set[g] ? s : out[i] + s;
+    }
+}
+
+void launch_reduce_segs(const ReduceSegs &r, hipStream_t s)
+{
+    long nb = 0;
+    for (int g = 0; g < r.n; ++g) nb = std::max(nb, (r.len[g] + 256 / grad_reduce_splits(r.nchunk[g]) - 1) /
+                                                        (256 / grad_reduce_splits(r.nchunk[g])));
+    if (r.n > 0 && nb > 0)
+        hipLaunchKernelGGL(k_reduce_segs, dim3((unsigned)nb, (unsigned)r.n), dim3(256), 0, s, r);
 }
 
 #ifndef GRAD_WGS

@@ -87,6 +87,27 @@ void launch_gemm_grad(int prec, const T *A, const T *B, int M, int Kd, long ncol
                       double *partial, int nchunk, long chunk_cols, hipStream_t s);
 // grad[len] += sum over chunks of partial[chunk][len], in a fixed order
 void launch_grad_reduce(double *partial, int nchunk, long len, double *grad, hipStream_t s);
+// up to RS_MAX such reductions in one launch (set: out = the sum instead of
+// out += it); each segment's bits are launch_grad_reduce's
+constexpr int RS_MAX = 16;
+struct ReduceSegs {
+    int n = 0;
+    const double *partial[RS_MAX];
+    double *out[RS_MAX];
+    long len[RS_MAX];
+    int nchunk[RS_MAX];
+    int set[RS_MAX];
+    void add(const double *p, int nc, long l, double *o, bool st = false)
+    {
+        partial[n] = p;
+        nchunk[n] = nc;
+        len[n] = l;
+        out[n] = o;
+        set[n] = st ? 1 : 0;
+        ++n;
+    }
+};
+void launch_reduce_segs(const ReduceSegs &r, hipStream_t s);
 
 // K1 (trans_a false) / K6 on BF16 three-way split products (gemm_x6.hip);
 // arguments as launch_gemm_wx
@@ -126,7 +147,7 @@ void grad_chunks(long ncol, int M, int Kd, int esz, int *nchunk, long *chunk);
 
 // partial buffer length needed by launch_gemm_grad / launch_gram:
 // (nchunk + grad_reduce_splits(nchunk)) * M * Kd doubles
-int grad_reduce_splits(int nchunk);
+__host__ __device__ inline int grad_reduce_splits(int nchunk) { return nchunk >= 64 ? 16 : (nchunk >= 16 ? 4 : 1); }
 
 // NGD: G <- G/nb + cI, b <- b/nb, then G = L L^T (FP64 blocked Cholesky, in
 // place, lower) and b <- (L L^T)^{-1} b; *info > 0 flags a non-positive
