@@ -38,7 +38,8 @@ CONFIGS = ("kdv", "dense4_k3", "gelu3_k4")
 
 METRIC = "fwd+adjoint points/s, mixed-prec vs FP64 on GPU, at stated err vs FP64 oracle"
 PEAK = {"hbm": 8000.0, "fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
-DTYPE = {"fp32": "f32 (FP64-accumulated forward map)",
+DTYPE = {"fp32": "f32 storage, FP32 MFMA linear maps (forward map flushed to FP64 every 16 terms), FP64 "
+                 "activation derivatives, residual and gradient sums",
          "bf16x3": "f32 storage, bf16 hi+lo split MFMA",
          "bf16x6": "f32 storage, bf16 three-way split MFMA (FP64-accumulated forward map)"}
 TRAFFIC = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic.json")
@@ -129,25 +130,23 @@ def algorithmic(wl, na, F, Q, P, N, wbytes, keep_s, edge=(0, 0, 0, 0, 0)):
     return {k: tuple(v) for k, v in cat.items()}
 
 
-def algorithmic_fused(wl, na, F, Q, P, N, grid, kept_h=True):
+def algorithmic_fused(wl, na, F, Q, P, N, grid):
     """Per-step algorithmic work of the fused FP32 layer kernels (4-byte
-    planes, point blocks of 4): (flops, bytes, launches) under bell_fwd (the
-    forward kernels, each hidden layer's K1 inside) and bell_bwd (one
-    adjoint kernel a hidden layer: S recomputed from T of the layer below,
-    h, the pair sums, K5 and, above the first hidden layer, K6).
+    planes, point blocks of 4): (flops, bytes, launches, MFMA seconds) under
+    bell_fwd (the forward kernels, each hidden layer's K1 inside) and
+    bell_bwd (one adjoint kernel a hidden layer).
 
     Forward: the first hidden layer reads the inputs (8-byte words) and
-    writes T; the others read T of the layer below and write T, but the last,
-    which forms the output row and the collocation seed (one plane row) and
-    writes the output layer's gradient partials (one 65-word row of 8 bytes
-    per point block) and the sums of R^2.  Adjoint: the last hidden layer
-    reads T of the layer below and the seed, the others T of the layer below
-    and their T-bar; all but the first write T-bar of the layer below; each
-    writes grid x n_l x (n_{l-1} + 1) gradient partials (8-byte words).
-    kept_h: each forward also writes h of its layer (one plane) and the
-    adjoint reads it instead of recomputing S (K1) and h from T of the layer
-    below.  The fourth entry is the linear maps' share of the flops (FP32
-    MFMA)."""
+    writes T; the others read T of the layer below and write T and h, but
+    the last, which writes h, the output row and the collocation seed (one
+    plane row each: the output row in 8-byte words), the output layer's
+    gradient partials (one 65-word row of 8 bytes per point block) and the
+    sums of R^2; their K1 runs on BF16 three-way split products (six
+    products at the BF16 peak).  Adjoint: the last hidden layer reads T of
+    the layer below, h and the seed, the others T below, h and T-bar; all but
+    the first write T-bar of the layer below (K5 and K6 on FP32 MFMA); the
+    first reads the inputs and T-bar and forms h again; each writes grid x
+    n_l x (n_{l-1} + 1) gradient partials (8-byte words)."""
     d = wl.dims
     L = len(d)
     w = 4.0 * na * N
@@ -155,33 +154,35 @@ def algorithmic_fused(wl, na, F, Q, P, N, grid, kept_h=True):
     nblk = N / 4.0
     cat = {k: [0.0, 0.0, 0, 0.0] for k in ("bell_fwd", "bell_bwd")}
 
-    def add(k, flops, byts, mfma):
+    def add(k, flops, byts, mfma_s):
         cat[k][0] += flops
         cat[k][1] += byts
         cat[k][2] += 1
-        cat[k][3] += mfma
+        cat[k][3] += mfma_s
 
     h_fl = lambda l: d[l] * (3.0 * F + 2.0 * Q + (na - 1)) * N
     k1 = lambda l: 2.0 * d[l] * (d[l - 1] + 1) * (N if l == 1 else na * N)
+    x6 = lambda fl: 6.0 * fl / (PEAK["bf16"] * 1e12)
+    f32 = lambda fl: fl / (PEAK["fp32"] * 1e12)
     last = L - 2
     for l in range(1, L - 1):
-        rd = d[0] * xin if l == 1 else d[l - 1] * w
-        hw = d[l] * w if kept_h else 0.0
-        hf = d[l] * 2.0 * Q * N if kept_h else 0.0
-        if l < last:
-            add("bell_fwd", k1(l) + h_fl(l) + hf, rd + d[l] * w + hw, k1(l))
+        if l == 1:
+            add("bell_fwd", k1(l) + h_fl(l), d[0] * xin + d[l] * w, 0.0)
+        elif l < last:
+            add("bell_fwd", k1(l) + h_fl(l) + d[l] * 2.0 * Q * N, 3.0 * d[l] * w, x6(k1(l)))
         else:
-            fl = k1(l) + h_fl(l) + hf + 2.0 * (d[l] + 1) * na * N + 2.0 * (d[l] + 1) * N
-            add("bell_fwd", fl, rd + w + hw + nblk * (d[l] + 1) * 8.0 + nblk * 8.0, k1(l))
+            fl = k1(l) + h_fl(l) + d[l] * 2.0 * Q * N + 2.0 * (d[l] + 1) * na * N + 2.0 * (d[l] + 1) * N
+            byts = d[l - 1] * w + d[l] * w + w + 2.0 * w + nblk * (d[l] + 1) * 8.0 + nblk * 8.0
+            add("bell_fwd", fl, byts, x6(k1(l)))
     for l in range(L - 2, 0, -1):
-        rd = (d[0] * xin if l == 1 else d[l - 1] * w) + (w if l == last else d[l] * w)
-        wr = (0.0 if l == 1 else d[l - 1] * w) + grid * d[l] * (d[l - 1] + 1) * 8.0
-        mm = 2.0 * d[l] * (d[l - 1] + 1) * na * N + (2.0 * d[l] * d[l - 1] * na * N if l > 1 else 0.0)
-        if kept_h:
-            add("bell_bwd", mm + d[l] * (3.0 * P + 2.0 * na) * N, rd + d[l] * w + wr, mm)
+        part = grid * d[l] * (d[l - 1] + 1) * 8.0
+        pairs = d[l] * (3.0 * P + 2.0 * na) * N
+        if l == 1:
+            add("bell_bwd", h_fl(l) + pairs + 2.0 * d[l] * (d[0] + 1) * N, d[0] * xin + d[l] * w + part, 0.0)
         else:
-            mm += k1(l)
-            add("bell_bwd", mm + 2.0 * h_fl(l) + d[l] * (3.0 * P + 2.0 * na) * N, rd + wr, mm)
+            mm = 2.0 * d[l] * (d[l - 1] + 1) * na * N + 2.0 * d[l] * d[l - 1] * na * N
+            rd = d[l - 1] * w + d[l] * w + (w if l == last else d[l] * w)
+            add("bell_bwd", mm + pairs, rd + d[l - 1] * w + part, f32(mm))
     return {k: tuple(v) for k, v in cat.items()}
 
 
@@ -297,9 +298,9 @@ def run_mode(args, D, torch, lib, c, dev, prec, name, wbytes, mma_peak, budget=0
     keep_s = int(kinfo[0]) == 2
     edge = np.zeros(5, np.int32)
     lib().dnnf90_net_step_info(net.id, prec, edge.ctypes.data)
-    finfo = np.zeros(27, np.int32)
+    finfo = np.zeros(14, np.int32)
     lib().dnnf90_net_fused_info(net.id, finfo.ctypes.data)
-    fused_path = prec == D.PREC_FP32 and int(finfo[0]) == 1
+    fused_path = prec == D.PREC_FP32 and int(finfo[0]) == 1 and not info["recompute"]
     g = D.Grad(net)
     s = torch.cuda.Stream(device=dev)
     bw.set_stream(s.cuda_stream)
@@ -333,18 +334,17 @@ def run_mode(args, D, torch, lib, c, dev, prec, name, wbytes, mma_peak, budget=0
             kms["bell_bwd"] = tuple(v * args.steps for v in kms_timed["bell_bwd"])   # one step -> K
         else:
             kms = {"bell_bwd": tuple(v * args.steps for v in kms_timed["bell_bwd"])}
-        tout = bw.forward(xd)
-        torch.cuda.synchronize()
     g.zero()
     rsq = bw.residual_step(xd, res, c.fac, g)
     grad = g.get()
-    t_host = tout.cpu().numpy()
+    # the outputs and derivatives the step itself formed (the fused kernels'
+    # output row where they take the step)
+    t_host = bw.step_output()
     bw.set_stream(None)
     bw.free()
     g.free()
     if fused_path:
-        kept_h = not info.get("recompute", False)
-        work = algorithmic_fused(c.wl, c.na, c.F, c.Q, c.P, N, int(finfo[26 if kept_h else 25]), kept_h)
+        work = algorithmic_fused(c.wl, c.na, c.F, c.Q, c.P, N, int(finfo[13]))
     else:
         work = algorithmic(c.wl, c.na, c.F, c.Q, c.P, N, wbytes, keep_s, edge)
     kern = {}
@@ -362,10 +362,10 @@ def run_mode(args, D, torch, lib, c, dev, prec, name, wbytes, mma_peak, budget=0
             mfma_bound = cat in ("linear", "linear_t", "grad") and \
                 flops / (mma_peak * 1e12) > byts / (PEAK["hbm"] * 1e9)
             if fused_path:
-                # the fused kernels' linear maps on FP32 MFMA against their bytes
-                mfma_bound = work[cat][3] / (mma_peak * 1e12) > byts / (PEAK["hbm"] * 1e9)
-                if mfma_bound:
-                    flops = work[cat][3]
+                # the fused kernels' MFMA time floor against their bytes'
+                mfma_bound = work[cat][3] > byts / (PEAK["hbm"] * 1e9)
+                if mfma_bound:   # as FP32-MFMA-equivalent flops
+                    flops = work[cat][3] * mma_peak * 1e12
             if mfma_bound:
                 ent.update(bound="mfma", achieved=flops / sec / 1e12, unit="TFLOP/s", peak=mma_peak)
             else:
@@ -377,9 +377,8 @@ def run_mode(args, D, torch, lib, c, dev, prec, name, wbytes, mma_peak, budget=0
     info["keep_s"] = keep_s
     info["edge"] = [int(v) for v in edge]
     if fused_path:
-        info["fused_layers"] = {"kept_h": kept_h, "regs": [int(v) for v in finfo[1:13]],
-                                "scratch": [int(v) for v in finfo[13:25]],
-                                "adjoint_grid": int(finfo[26 if kept_h else 25])}
+        info["fused_layers"] = {"regs": [int(v) for v in finfo[1:7]], "scratch": [int(v) for v in finfo[7:13]],
+                                "adjoint_grid": int(finfo[13])}
     ms = float(np.median(step_ms))
     return {"name": name, "ms_per_step": ms, "step_ms": {"median": ms, "min": float(min(step_ms)),
                                                           "max": float(max(step_ms)),
@@ -480,6 +479,13 @@ def ref_timing(args):
     with open(args.out, "w") as f:
         json.dump({"seconds": secs, "points": count}, f)
     return 0
+
+
+ERR_BAR = 1e-6   # relative error through derivative order 4 the mixed modes are held to
+
+
+def within_bar(r):
+    return max(v["max"] for o_, v in r["err_T"].items() if int(o_) <= 4) <= ERR_BAR and r["err_g"] <= ERR_BAR
 
 
 def mixed_vs_fp64(r, fp64, deg):
@@ -681,7 +687,8 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default=HEADLINE)
-    ap.add_argument("--mixed", default="bf16x6", choices=["bf16x6", "fp32", "bf16x3"])
+    ap.add_argument("--mixed", default="auto", choices=["auto", "bf16x6", "fp32", "bf16x3"],
+                    help="the headline mode; auto: the fastest mixed mode within the error bar")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only copy a buffer of this many MiB 5 times (HBM counter calibration)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -734,14 +741,18 @@ def main():
         r = run_mode(args, D, torch, lib, c, dev, prec, name, 4, peak)
         mixed_vs_fp64(r, fp64, c.deg)
         others[name] = r
+    if args.mixed == "auto":
+        # the fastest mode whose step output (every derivative through order
+        # 4) and gradient are within ERR_BAR of the FP64 step's
+        ok = [k for k, r in others.items() if within_bar(r)]
+        args.mixed = min(ok, key=lambda k: others[k]["ms_per_step"]) if ok else "bf16x6"
     mixed = others[args.mixed]
-    # the same mixed step with per-layer recompute of the kept plane in the reverse pass
-    mprec = {m[0]: m[1] for m in modes}[args.mixed]
-    mpeak = {m[0]: m[2] for m in modes}[args.mixed]
-    rc_run = run_mode(args, D, torch, lib, c, dev, mprec, args.mixed + "+recompute", 4, mpeak, budget=1,
-                      breakdown=False)
-    recompute = {"mode": args.mixed, "ms_per_step": rc_run["ms_per_step"], "points_per_s": rc_run["points_per_s"],
-                 "identical_grad": bool(np.array_equal(rc_run["grad"], mixed["grad"])),
+    # the BF16x6 step with per-layer recompute of the kept plane in the
+    # reverse pass (layer kernels: the fused FP32 kernels keep their planes)
+    rc_run = run_mode(args, D, torch, lib, c, dev, D.PREC_BF16X6, "bf16x6+recompute", 4, PEAK["bf16"] / 6.0,
+                      budget=1, breakdown=False)
+    recompute = {"mode": "bf16x6", "ms_per_step": rc_run["ms_per_step"], "points_per_s": rc_run["points_per_s"],
+                 "identical_grad": bool(np.array_equal(rc_run["grad"], others["bf16x6"]["grad"])),
                  "device_bytes": rc_run["info"]["bytes"], "device_bytes_all_kept": rc_run["info"]["bytes_stored"]}
 
     oracle_check, cpu, cpu_port_run = None, None, None
@@ -761,7 +772,9 @@ def main():
         "dtype": DTYPE[args.mixed], "data": "synthetic",
         "config": {"workload": f"{wl.name}: MLP {wl.dims} {'tanh' if wl.act == 0 else 'erf-GELU'}, "
                                f"D0={wl.d0} K={wl.k} |A|={c.na}, N={c.N}, loss mean square of {wl.note}",
-                   "precision": args.mixed},
+                   "precision": args.mixed,
+                   "precision_choice": "the fastest mixed mode whose step output through derivative order 4 "
+                                       "and gradient are within 1e-6 (relative to the maximum) of the FP64 step"},
         "roofline": roofline_of(mixed["kernels"], wl.name, args.mixed),
         "cpu_baseline": cpu,
         "cpu_baseline_port": cpu_port_run if (cpu is not cpu_port_run) else None,
@@ -773,7 +786,8 @@ def main():
         "err_vs_fp64": {"forward_by_order": mixed["err_T"], "grad_max_rel": mixed["err_g"]},
         "kernels": mixed["kernels"],
         "kept_plane": {"fp64": "S" if fp64["info"]["keep_s"] else "h",
-                       args.mixed: "S" if mixed["info"]["keep_s"] else "h"},
+                       args.mixed: "h (fused layer kernels)" if "fused_layers" in mixed["info"]
+                       else ("S" if mixed["info"]["keep_s"] else "h")},
         "recompute": recompute,
         "modes": {k: {"points_per_s": r["points_per_s"], "ms_per_step": r["ms_per_step"],
                       "speedup_vs_fp64": r["points_per_s"] / fp64["points_per_s"],

@@ -85,6 +85,7 @@ SIGNATURES = {
     "dnnf90_bwork_create": (_i, [_i, _i, _i]),
     "dnnf90_bwork_create_budget": (_i, [_i, _i, _i, _d]),
     "dnnf90_bwork_info": (_i, [_i, _vp]),
+    "dnnf90_bwork_step_output": (_i, [_i, _vp, _i]),
     "dnnf90_bwork_free": (_i, [_i]),
     "dnnf90_bwork_set_stream": (_i, [_i, _vp]),
     "dnnf90_bwork_sync": (_i, [_i]),

@@ -343,6 +343,13 @@ class BatchWork:
         self.npt = int(x.shape[0])
         return rsq.value if want_rsq else None
 
+    def step_output(self):
+        """The output row [npt][nout][na] the last residual_step formed
+        (dnnf90_bwork_step_output), as forward() returns it."""
+        out = np.zeros((self.npt, self.net.dims[-1], self.net.na))
+        _chk(lib().dnnf90_bwork_step_output(self.id, out.ctypes.data, 0), "dnnf90_bwork_step_output")
+        return out
+
     def residual_rows(self, x, res: Residual, rows_t, ld):
         """Gauss-Newton rows dR/dw into the device FP32 buffer rows_t[nw][ld];
         ld >= npt rounded up to 64 (padding rows are written as zeros)."""

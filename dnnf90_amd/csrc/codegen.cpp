@@ -715,57 +715,52 @@ hipError_t gen_launch_pt_bwd(const GenKernels &g, const double *W, double *grad,
 // 64-byte runs in lane order, which streams at 6.7 TB/s where the neuron-
 // outer order took 4.3), and one workgroup of four waves covers one
 // block, every neuron and every slot.  Wave w, lane l owns neuron
-// j = 16 w + (l mod 16) at point p = l / 16.  A block the linear maps read is
-// staged in LDS as X[slot * FZ_XS + point * FZ_PJ + neuron] (its own order,
-// rows padded to FZ_PJ and slots to FZ_XS floats); the maps run on
-// v_mfma_f32_16x16x4f32:
-//   K1 (lib/train_module.f90:262-278) and K6 (:430-444): rows (point, slot),
-//   columns the 16 neurons of the wave, k the 64 neurons of the block; the
-//   result leaves lane l holding slots 4t .. 4t+3 of its (neuron, point) --
-//   the recurrence's operands, in registers, so S and T-bar never go to
-//   memory.  K1 moves every FZ_FL k-steps (4 FZ_FL terms) into FP64.
-//   K5 (:408-422): rows the wave's neurons, columns 16 inputs, k the four
-//   points of the block, slot by slot: the lane's own S-bar is the A operand;
-//   FP32 over a block, FP64 across blocks, per-workgroup FP64 partials
-//   summed in a fixed order by the gradient reduction.
+// j = 16 w + (l mod 16) at point p = l / 16.
+//   K1 (lib/train_module.f90:262-278), in the forward: the block of T below
+//   is split on staging into three bf16 pieces (x = h + m + l, each the
+//   round-to-nearest bf16 of what the previous ones leave) held as
+//   X2[slot][piece][point][64 neurons], 128-byte rows whose 16-byte chunks
+//   sit swizzled by the row (chunk c at c ^ (row mod 8): the eight lanes
+//   of an A-operand read meet no bank twice); the weights are split the same
+//   way into registers; v_mfma_f32_16x16x32_bf16 forms the six significant
+//   products of a 32-term k-step, moved into FP64 after each k-step (bias at
+//   slot 0): rows (point, slot), columns the 16 neurons of the wave, k 32
+//   neurons of the block, leaving lane l with slots 4t .. 4t+3 of its
+//   (neuron, point l / 16) -- the recurrence's operands, in registers, so S
+//   never goes to memory.
+//   K5 (:408-422), in the adjoint: rows the wave's neurons, columns 16
+//   inputs, k the four points of the block, slot by slot on
+//   v_mfma_f32_16x16x4f32 against the FP32 block of T below staged as
+//   X[slot * FZ_XS + point * FZ_PJ + neuron]; the lane's own S-bar is the A
+//   operand; FP32 over a block, FP64 across blocks (LDS accumulators),
+//   per-workgroup FP64 partials summed in a fixed order by the gradient
+//   reduction.
+//   K6 (:430-444): S-bar transposed through X, rows (point, slot), columns
+//   the wave's 16 inputs, k the 64 neurons, FP32 MFMA, two tiles at a time.
 // The maps are emitted interleaved with the per-slot arithmetic: slot a of
 // the recurrence needs S of its sub-indices only (earlier slots in the graded
 // order), so the MFMAs of tile q + 1 go out before the recurrence of tile q's
 // slots, and each slot's K5 right after its pair sum -- the matrix pipe runs
 // while the wave issues the vector work.
-// Forward: the first hidden layer forms S from the inputs (fwd_in); every
-// other hidden layer reads the block of T below, maps it and writes its own
-// T (fwd); the last also forms the output row, summed over the neurons in
-// FP64 (fwd_last).  Adjoint: the last hidden layer takes T-bar from the
-// output row's S-bar and recomputes T beside h for the output layer's
-// gradient (bwd_last); the middle layers form T-bar from the S-bar block of
-// the layer above (bwd); each recomputes S from the block of T below, h in
-// registers, the pair sums, the layer's K5 and writes its S-bar block; the
-// first hidden layer (bwd_in) forms S from the inputs and its gradient from
-// S-bar at slot 0 and the slots e_i, writing no plane.
+// Forward: the first hidden layer forms S from the inputs (fwd_in, T
+// written); every other hidden layer reads the block of T below, maps it and
+// writes its own T and h (fwd); the last writes h only and forms the output
+// row from T held in LDS, summed over the neurons in FP64, with the
+// residual, the seed, the output layer's gradient partials and the sums of
+// R^2 (fwd_last).  Adjoint: the last hidden layer takes T-bar = the output
+// weight times the seed (bwd_last), the others read it (bwd); both read h,
+// form the pair sums, the layer's K5 and T-bar of the layer below (K6); the
+// first hidden layer (bwd_in) forms S and h again from the inputs (no linear
+// map: cheaper than a plane of h) and its gradient from S-bar at slot 0 and
+// the slots e_i, writing no plane.
 static const char *kFusedHelpers = R"FZ(
 // lane offsets pass through an empty asm at the top of every block: nothing
 // derived from them is hoisted out of the block loop into live registers
 #define FZ_OPAQUE(x) asm volatile("" : "+v"(x))
-// one block [slot][point][neuron] (global) into X[slot * XS + point * PJ + neuron]
-__device__ __forceinline__ void fz_stage(const float *__restrict__ src, float *__restrict__ X)
-{
-  constexpr int NU = (NA * 64 + 255) / 256;
-  const int tid = threadIdx.x;
-  int so = tid, xo = (tid >> 6) * XS + ((tid >> 4) & 3) * PJ + (tid & 15) * 4;
-  FZ_OPAQUE(so);
-  FZ_OPAQUE(xo);
-  flt4 v[NU];
-#pragma unroll
-  for (int k = 0; k < NU; ++k)
-    if (tid + 256 * k < NA * 64) v[k] = reinterpret_cast<const flt4 *>(src)[so + 256 * k];
-#pragma unroll
-  for (int k = 0; k < NU; ++k)
-    if (tid + 256 * k < NA * 64) *reinterpret_cast<flt4 *>(X + xo + k * 4 * XS) = v[k];
-}
-// the same in two halves: the global loads into registers (issued a block
-// ahead) and the LDS writes
 #define FZ_NU ((NA * 64 + 255) / 256)
+// one FP32 block [slot][point][neuron] (global) into X[slot * XS + point *
+// PJ + neuron], in two halves: the global loads into registers (issued a
+// block ahead) and the LDS writes
 __device__ __forceinline__ void fz_stage_load(const float *__restrict__ src, flt4 (&v)[FZ_NU])
 {
   const int tid = threadIdx.x;
@@ -788,9 +783,111 @@ __device__ __forceinline__ void fz_zero_pad(float *__restrict__ X)
 {
   for (int u = threadIdx.x; u < (4 * NT - NA) * XS; u += 256) X[NA * XS + u] = 0.f;
 }
-// K1/K6 tile t: rows (point (lane mod 16) / 4, slot 4 t + lane mod 4), k the
-// input lane / 16 + 4 s at step s; xa = the lane's offset for t = 0.  Lane l
-// gets slots 4 t + r of its (neuron, point l / 16).
+// bf16 pieces: a pair converts in one instruction and widens back by a
+// shift and a mask
+__device__ __forceinline__ flt2 fz_widen2(bf16x2 v)
+{
+  const unsigned b = __builtin_bit_cast(unsigned, v);
+  return flt2{__builtin_bit_cast(float, b << 16), __builtin_bit_cast(float, b & 0xffff0000u)};
+}
+__device__ __forceinline__ void fz_split2(flt2 x, bf16x2 &h, bf16x2 &m, bf16x2 &l)
+{
+  h = __builtin_convertvector(x, bf16x2);
+  const flt2 r = x - fz_widen2(h);
+  m = __builtin_convertvector(r, bf16x2);
+  l = __builtin_convertvector(r - fz_widen2(m), bf16x2);
+}
+// element (slot, piece, point, neuron col) of the split block: rows
+// (slot * 3 + piece) * 4 + point of 64 bf16, the 16-byte chunk col / 8 at
+// (col / 8) ^ (2 (slot mod 4) + point mod 2): the eight lanes of one
+// A-operand read (four slots, two points) meet eight different chunks
+__device__ __forceinline__ int fz_x2(int slot, int c, int pt, int col)
+{
+  return ((slot * 3 + c) * 4 + pt) * 64 + ((((col >> 3) ^ (((slot & 3) << 1) | (pt & 1))) << 3) | (col & 7));
+}
+// the FP32 block of T below into X2 as three bf16 pieces (row (slot * 3 +
+// piece) * 4 + point); each thread four neurons of a (slot, point) at a time
+__device__ __forceinline__ void fz_stage_x6(const float *__restrict__ src, __bf16 *__restrict__ X2)
+{
+  const int tid = threadIdx.x;
+  int so = tid;
+  FZ_OPAQUE(so);
+  flt4 v[FZ_NU];
+#pragma unroll
+  for (int k = 0; k < FZ_NU; ++k)
+    if (tid + 256 * k < NA * 64) v[k] = reinterpret_cast<const flt4 *>(src)[so + 256 * k];
+  const int pt = (tid >> 4) & 3, col = (tid & 15) * 4;
+#pragma unroll
+  for (int k = 0; k < FZ_NU; ++k)
+    if (tid + 256 * k < NA * 64) {
+      const int slot = (tid >> 6) + 4 * k;
+      bf16x2 h0, m0, l0, h1, m1, l1;
+      fz_split2(flt2{v[k][0], v[k][1]}, h0, m0, l0);
+      fz_split2(flt2{v[k][2], v[k][3]}, h1, m1, l1);
+      *reinterpret_cast<bf16x4 *>(X2 + fz_x2(slot, 0, pt, col)) = bf16x4{h0[0], h0[1], h1[0], h1[1]};
+      *reinterpret_cast<bf16x4 *>(X2 + fz_x2(slot, 1, pt, col)) = bf16x4{m0[0], m0[1], m1[0], m1[1]};
+      *reinterpret_cast<bf16x4 *>(X2 + fz_x2(slot, 2, pt, col)) = bf16x4{l0[0], l0[1], l1[0], l1[1]};
+    }
+}
+__device__ __forceinline__ void fz_zero_pad_x6(__bf16 *__restrict__ X2)
+{
+  for (int u = threadIdx.x; u < (4 * NT - NA) * XS2; u += 256) X2[NA * XS2 + u] = (__bf16)0.f;
+}
+// the lane's A-operand offset into X2 for tile 0, piece 0, k-step 0: row
+// (point, slot) lane mod 16, k = 8 (lane / 16) .. +8.  Tile t adds 4 t slots
+// (48 t rows; the same swizzle); the next k-step is chunk ^ 4 (offset ^ 32);
+// piece c is 4 c rows on (+256 c; the same swizzle).
+__device__ __forceinline__ int fz_x6_offset()
+{
+  const int lane = threadIdx.x & 63;
+  return fz_x2(lane & 3, 0, (lane & 15) >> 2, 8 * (lane >> 4));
+}
+// the weight pieces of the lane's column (neuron j), k as above
+__device__ __forceinline__ void fz_w_split(const float *__restrict__ wj, bf16x8 (&wh)[2], bf16x8 (&wm)[2],
+                                           bf16x8 (&wl)[2])
+{
+  const int kq = 8 * ((threadIdx.x & 63) >> 4);
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+    for (int e = 0; e < 8; e += 2) {
+      bf16x2 h, m, l;
+      fz_split2(flt2{wj[32 * ks + kq + e], wj[32 * ks + kq + e + 1]}, h, m, l);
+      wh[ks][e] = h[0]; wh[ks][e + 1] = h[1];
+      wm[ks][e] = m[0]; wm[ks][e + 1] = m[1];
+      wl[ks][e] = l[0]; wl[ks][e + 1] = l[1];
+    }
+}
+// K1 tile t: the six products of each 32-term k-step (small terms first)
+// accumulated in FP32, then moved into FP64; slot 0 takes the bias
+__device__ __forceinline__ flt4 fz_tile_x6(const __bf16 *__restrict__ X2, int xb0, int xb1, int t,
+                                           const bf16x8 (&wh)[2], const bf16x8 (&wm)[2], const bf16x8 (&wl)[2],
+                                           float bias)
+{
+  double d[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks) {
+    const int x0 = ks ? xb1 : xb0;
+    const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(X2 + x0 + t * 48 * 64);
+    const bf16x8 am = *reinterpret_cast<const bf16x8 *>(X2 + x0 + 256 + t * 48 * 64);
+    const bf16x8 al = *reinterpret_cast<const bf16x8 *>(X2 + x0 + 512 + t * 48 * 64);
+    flt4 c = {0.f, 0.f, 0.f, 0.f};
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, wh[ks], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, wl[ks], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, wm[ks], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, wh[ks], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, wm[ks], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, wh[ks], c, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) d[r] += (double)c[r];
+  }
+  flt4 out;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) out[r] = (float)(d[r] + ((t == 0 && r == 0) ? (double)bias : 0.0));
+  return out;
+}
+// K6 tile t: rows (point (lane mod 16) / 4, slot 4 t + lane mod 4), k the
+// input lane / 16 + 4 s at step s; xa = the lane's offset for t = 0
 __device__ __forceinline__ flt4 fz_tile(const float *__restrict__ X, int xa, int t, const float (&b)[16])
 {
   flt4 c = {0.f, 0.f, 0.f, 0.f};
@@ -809,24 +906,6 @@ __device__ __forceinline__ void fz_tile2(const float *__restrict__ X, int xa, in
     c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(X[xa + t0 * 4 * XS + 4 * s], b[s], c0, 0, 0, 0);
     c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(X[xa + (t0 + 1) * 4 * XS + 4 * s], b[s], c1, 0, 0, 0);
   }
-}
-// the forward map: every FL k-steps the partial sum moves into FP64; slot 0 takes the bias
-__device__ __forceinline__ flt4 fz_tile64(const float *__restrict__ X, int xa, int t, const float (&b)[16], float bias)
-{
-  double d[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-  for (int c0 = 0; c0 < 16; c0 += FL) {
-    flt4 c = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int s = c0; s < c0 + FL; ++s)
-      c = __builtin_amdgcn_mfma_f32_16x16x4f32(X[xa + t * 4 * XS + 4 * s], b[s], c, 0, 0, 0);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) d[r] += (double)c[r];
-  }
-  flt4 out;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) out[r] = (float)(d[r] + ((t == 0 && r == 0) ? (double)bias : 0.0));
-  return out;
 }
 // sum over the lanes l, l ^ 16, l ^ 32, l ^ 48 (the four points of a neuron)
 __device__ __forceinline__ double fz_sum4(double v)
@@ -865,18 +944,23 @@ std::string gen_fused_source(const TableSet &t, const FwdSchedule &s)
     // the activation, its derivative count and the polynomial row length
     // are the family's: compile-time here (the argument layout is the
     // host's ActParams), so one activation's code is compiled and nothing of
-    // it is hoisted out of the block loops
-    // multiply-adds fused inside an expression only: h comes out of the
-    // forward's recurrence (beside T) and of the adjoint's (alone) in the
-    // same bits
+    // it is hoisted out of the block loops.  Multiply-adds are fused inside
+    // an expression only (each the same way wherever it is emitted).
     o << "#pragma clang fp contract(on)\n"
       << "struct ActParams { int iact_rt, nq_rt; const double *tpc; int ldt_rt; static constexpr int iact = "
       << t.iact << ", nq = " << t.k + 1 << ", ldt = " << t.k + 3 << "; };\n"
       << kActSrc << "\n"
       << "typedef float real;\n"
       << "typedef float flt4 __attribute__((ext_vector_type(4)));\n"
-      << "#define NA " << na << "\n#define NT " << nt << "\n#define D0 " << d0 << "\n#define XS " << FZ_XS << "\n#define PJ " << FZ_PJ
-      << "\n#define FL " << FZ_FL << "\n#define BLK (NA * 256)\n"
+      << "typedef float flt2 __attribute__((ext_vector_type(2)));\n"
+      << "typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));\n"
+      << "typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));\n"
+      << "typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));\n"
+      << "#define NA " << na << "\n#define NT " << nt << "\n#define D0 " << d0 << "\n#define XS " << FZ_XS
+      << "\n#define PJ " << FZ_PJ << "\n#define BLK (NA * 256)\n"
+      // the split block: 3 pieces x 4 points x 64 bf16 a slot; the last
+      // hidden layer's T (FP32, rows of PJ) takes slot a's place once read
+      << "#define XS2 (3 * 4 * 64)\n#define XT (XS2 / 2)\n"
       << kFusedHelpers;
     auto derivs = [&]() {
         // the table pointer passes through an empty asm in every block: its
@@ -887,16 +971,17 @@ std::string gen_fused_source(const TableSet &t, const FwdSchedule &s)
           << "  act_derivs<" << kq << ", " << (bes ? "true" : "false") << ">(apl, (double)e0, d);\n";
         for (int q = 0; q < kq; ++q) o << "  const real d" << q << " = (real)d[" << q << "];\n";
     };
-    // S of the lane from the block of T below (K1, FP64-flushed, bias at slot
-    // 0), tile by tile: tiles 0 and 1 and the activation derivatives here,
-    // tile q + 1 before the recurrence reaches tile q's slots (tile_hook)
+    // S of the lane from the staged block of T below (K1): tiles 0 and 1 and
+    // the activation derivatives here, tile q + 1 before the recurrence
+    // reaches tile q's slots (tile_hook)
     auto tile = [&](int q) {
-        o << "  {\n    const flt4 sv = fz_tile64(X, xa, " << q << ", b, bias);\n";
+        o << "  {\n    const flt4 sv = fz_tile_x6(X2, xb0, xb1, " << q << ", wh, wm, wl, bias);\n";
         for (int r = 0; r < 4 && 4 * q + r < na; ++r) o << "    e" << 4 * q + r << " = sv[" << r << "];\n";
         o << "  }\n";
     };
-    auto s_from_x = [&](const char *bsrc) {
-        o << "  float b[16];\n#pragma unroll\n  for (int s = 0; s < 16; ++s) b[s] = " << bsrc << ";\n";
+    auto s_from_x = [&]() {
+        o << "  bf16x8 wh[2], wm[2], wl[2];\n  fz_w_split(W + j * 65 + 1, wh, wm, wl);\n"
+          << "  const int xb0 = fz_x6_offset(), xb1 = xb0 ^ 32;\n";
         for (int a = 0; a < na; ++a) o << "  real e" << a << ";\n";
         tile(0);
         if (nt > 1) tile(1);
@@ -939,15 +1024,16 @@ std::string gen_fused_source(const TableSet &t, const FwdSchedule &s)
           << "  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;\n"
           << "  const int j = 16 * w + (lane & 15), p = lane >> 4;\n"
           << "  const long Np = 256;   // slot stride inside a block\n"
-          // lane offsets: K1/K6 A-operand row (xa), K5 B operand (xg), the
-          // transposed S-bar row (xs), the (neuron, point) in a block (lo),
-          // the weight rows of the B operands (wb: K1, wv: K6)
+          // lane offsets: K6 A-operand row (xa), K5 B operand (xg), the
+          // transposed S-bar row and the last layer's T in LDS (xs), the
+          // (neuron, point) in a block (lo), the weight rows of K6's B
+          // operand (wv)
           << "  int xa = (lane & 3) * XS + ((lane & 15) >> 2) * PJ + (lane >> 4);\n"
           << "  int xg = (lane >> 4) * PJ + (lane & 15), xs = p * PJ + j, lo = p * 64 + j;\n"
-          << "  int wb = j * 65 + 1 + (lane >> 4), wv = (lane >> 4) * 65 + 1 + j;\n";
+          << "  int wv = (lane >> 4) * 65 + 1 + j;\n";
     };
     auto opaque = [&]() {
-        o << "  FZ_OPAQUE(xa);\n  FZ_OPAQUE(xg);\n  FZ_OPAQUE(xs);\n  FZ_OPAQUE(lo);\n  FZ_OPAQUE(wb);\n  FZ_OPAQUE(wv);\n";
+        o << "  FZ_OPAQUE(xa);\n  FZ_OPAQUE(xg);\n  FZ_OPAQUE(xs);\n  FZ_OPAQUE(lo);\n  FZ_OPAQUE(wv);\n";
     };
     auto xbuf = [&]() {
         o << "  __shared__ __attribute__((aligned(16))) float X[4 * NT * XS];\n"
@@ -982,22 +1068,6 @@ std::string gen_fused_source(const TableSet &t, const FwdSchedule &s)
         for (int a = 0; a < na; ++a) o << "  real sb" << a << ";\n";
     };
 
-    // The kernels of the hidden layers l >= 2 in two plane policies: (kh =
-    // 0) the adjoint recomputes S and h of the layer from T of the layer
-    // below (nothing kept but T); (kh = 1, names ending _h) the forward also
-    // writes h of the layer into the plane Hout (point-blocked like T) and
-    // the adjoint reads it, with no linear map, activation derivatives or
-    // recurrence of its own.  Both compute h by the same expressions (the
-    // module is compiled with contraction inside expressions only, so each
-    // multiply-add is fused the same way wherever it is emitted): the same
-    // bits.  The first hidden layer's adjoint always recomputes (S from the
-    // inputs takes no linear map; writing and reading h cost more).
-    auto kname = [](const char *base, int kh) { return std::string(base) + (kh ? "_h" : ""); };
-    auto hout = [&](int kh) {
-        if (kh) o << "  real *Hj = Hout + (long)pb * BLK + lo;\n  Hj[0] = d1;\n";
-    };
-    const int rec_h = REC_H;
-
     // ---- forward, first hidden layer
     head("dnnf90_fz_fwd_in",
          "const double *__restrict__ xin, int npt, const float *__restrict__ W1, float *__restrict__ Tout, int nblk, "
@@ -1017,34 +1087,36 @@ std::string gen_fused_source(const TableSet &t, const FwdSchedule &s)
     // weight-gradient partials of the block (bias first) and its sum of R^2:
     // no output plane, no seed kernel, no T plane of that layer.
     o << "struct DevResidual { int nterm; const double *coeff; const int *slot; const int *nonlin; int distinct; };\n";
-    for (int kh = 0; kh < 2; ++kh)
     for (int last = 0; last < 2; ++last) {
-        const char *hx = kh ? ", float *__restrict__ Hout" : "";
         if (last)
-            head(kname("dnnf90_fz_fwd_last", kh).c_str(),
-                 (std::string("const float *__restrict__ Tin, const float *__restrict__ W, int nblk, ActParams ap, "
-                              "const float *__restrict__ WL, int npt, DevResidual rs, double fac, float *__restrict__ Sbo, "
-                              "double *__restrict__ gpart, double *__restrict__ rpart") + hx).c_str(),
+            head("dnnf90_fz_fwd_last",
+                 "const float *__restrict__ Tin, const float *__restrict__ W, int nblk, ActParams ap, "
+                 "const float *__restrict__ WL, int npt, DevResidual rs, double fac, float *__restrict__ Sbo, "
+                 "double *__restrict__ gpart, double *__restrict__ rpart, double *__restrict__ tout, "
+                 "float *__restrict__ Hout",
                  2);
         else
-            head(kname("dnnf90_fz_fwd", kh).c_str(),
-                 (std::string("const float *__restrict__ Tin, const float *__restrict__ W, float *__restrict__ Tout, "
-                              "int nblk, ActParams ap") + hx).c_str(),
+            head("dnnf90_fz_fwd",
+                 "const float *__restrict__ Tin, const float *__restrict__ W, float *__restrict__ Tout, int nblk, "
+                 "ActParams ap, float *__restrict__ Hout",
                  2);
-        xbuf();
+        o << "  __shared__ __attribute__((aligned(16))) __bf16 X2[4 * NT * XS2];\n"
+          << "  fz_zero_pad_x6(X2);\n";
         o << "  const int pb = blockIdx.x;\n  if (pb >= nblk) return;\n";
         if (last)
-            // the residual's terms in LDS, loaded beside the staged block
-            // (the output section reads them many times a block)
-            o << "#define FZ_OUT(a, tv) X[xs + (a) * XS] = (tv);\n"
+            // the residual's terms and the output weights in LDS, loaded
+            // beside the staged block (the output section reads them many
+            // times a block)
+            o << "  float *Xf = reinterpret_cast<float *>(X2);\n"
+              << "#define FZ_OUT(a, tv) Xf[xs + (a) * XT] = (tv);\n"
               << "  __shared__ double Rc[" << FZ_MAXT << "];\n  __shared__ int Rsl[" << FZ_MAXT << "], Rnl["
               << FZ_MAXT << "];\n"
               << "  if (tid < rs.nterm) {\n    Rc[tid] = rs.coeff[tid];\n    Rsl[tid] = rs.slot[tid];\n"
               << "    Rnl[tid] = rs.nonlin[tid];\n  }\n"
               << "  __shared__ float WLs[64];\n  if (tid < 64) WLs[tid] = WL[1 + tid];\n";
         o << "  const float bias = W[j * 65];\n"
-          << "  fz_stage(Tin + (long)pb * BLK, X);\n  __syncthreads();\n";
-        s_from_x("W[wb + 4 * s]");
+          << "  fz_stage_x6(Tin + (long)pb * BLK, X2);\n  __syncthreads();\n";
+        s_from_x();
         // the last hidden layer's T goes to the output row only, through the
         // staged block's place in LDS, each slot once every wave has read it
         // (the tiles keep their place in the recurrence, with a barrier each)
@@ -1053,9 +1125,8 @@ std::string gen_fused_source(const TableSet &t, const FwdSchedule &s)
         } else {
             o << "  real *Tj = Tout + (long)pb * BLK + lo;\n  Tj[0] = d0;\n";
         }
-        hout(kh);
-        emit_recurrence(o, t, s, false, (last ? REC_TOUT : REC_T) | (kh ? rec_h : 0), "Np",
-                        last ? tile_hook_sync : tile_hook);
+        o << "  real *Hj = Hout + (long)pb * BLK + lo;\n  Hj[0] = d1;\n";
+        emit_recurrence(o, t, s, false, (last ? REC_TOUT : REC_T) | REC_H, "Np", last ? tile_hook_sync : tile_hook);
         if (last)
             o << "#undef FZ_OUT\n"
               << "  __shared__ double Ot[NA * 4];   // the output row, FP32-rounded, (slot, point)\n"
@@ -1069,7 +1140,7 @@ std::string gen_fused_source(const TableSet &t, const FwdSchedule &s)
               << "  for (int k = 0; k < 3; ++k) {\n"
               << "    const int u = lane + 64 * k;\n"
               << "    if (u < NA * 4) {\n"
-              << "      const float *xu = X + (u >> 2) * XS + (u & 3) * PJ + 16 * w;\n"
+              << "      const float *xu = Xf + (u >> 2) * XT + (u & 3) * PJ + 16 * w;\n"
               << "      double v = 0.0;\n"
               << "#pragma unroll\n"
               << "      for (int m = 0; m < 16; ++m) v += (double)WLs[16 * w + m] * (double)xu[m];\n"
@@ -1079,8 +1150,11 @@ std::string gen_fused_source(const TableSet &t, const FwdSchedule &s)
               << "  __syncthreads();\n"
               << "  if (tid < NA * 4) {\n"
               << "    const double v = ((Op[tid] + Op[NA * 4 + tid]) + Op[2 * NA * 4 + tid]) + Op[3 * NA * 4 + tid];\n"
-              << "    Ot[tid] = (double)(float)(v + (tid < 4 ? (double)WL[0] : 0.0));\n"
+              << "    const double ov = (double)(float)(v + (tid < 4 ? (double)WL[0] : 0.0));\n"
+              << "    Ot[tid] = ov;\n"
               << "    Sz[tid] = 0.f;\n"
+              << "    const long n = (long)pb * 4 + (tid & 3);\n"
+              << "    if (n < npt) tout[n * NA + (tid >> 2)] = ov;   // [point][slot], as the forward returns it\n"
               << "  }\n"
               << "  __syncthreads();\n"
               // one lane a point: R, then the seed of each term's slot in
@@ -1114,7 +1188,7 @@ std::string gen_fused_source(const TableSet &t, const FwdSchedule &s)
               << "#pragma unroll\n"
               << "    for (int i = 0; i < NA; ++i) {\n"
               << "      const int u = w * NA + i;\n"
-              << "      g += (double)Sz[u] * (double)X[(u >> 2) * XS + (u & 3) * PJ + lane];\n"
+              << "      g += (double)Sz[u] * (double)Xf[(u >> 2) * XT + (u & 3) * PJ + lane];\n"
               << "    }\n"
               << "    Op[w * 64 + lane] = g;\n"
               << "  }\n"
@@ -1131,90 +1205,65 @@ std::string gen_fused_source(const TableSet &t, const FwdSchedule &s)
 
     // ---- adjoint of a hidden layer l >= 2 (last: l = L-1, T-bar = the
     // output weight times the output row's seed; else T-bar of l read from
-    // the plane the layer above wrote).  Per block: S from the staged T of
-    // l-1 (K1), h, the pair sums into S-bar (registers), the layer's K5
-    // against the staged block, then S-bar transposed into LDS and T-bar of
-    // l-1 = W' S-bar (K6) written out for the layer below.
-    for (int kh = 0; kh < 2; ++kh)
+    // the plane the layer above wrote).  Per block: h read, the pair sums
+    // into S-bar (registers), the layer's K5 against the staged block of T
+    // below, then S-bar transposed into LDS and T-bar of l-1 = W' S-bar (K6)
+    // written out for the layer below.  Each block's staged block below, h
+    // and the seed are loaded into registers during the block before it (h
+    // and the seed before its pair sums, the block below before its K6);
+    // T-bar of the layer above is loaded at the top of the block (both
+    // planes a block ahead would not fit the registers).
     for (int last = 0; last < 2; ++last) {
-        head(kname(last ? "dnnf90_fz_bwd_last" : "dnnf90_fz_bwd", kh).c_str(),
-             kh ? "const float *__restrict__ Tin, const float *__restrict__ Tb, const float *__restrict__ W, "
-                  "const float *__restrict__ WL, float *__restrict__ Tbout, double *__restrict__ part, int nblk, "
-                  "ActParams ap, const float *__restrict__ H"
-                : "const float *__restrict__ Tin, const float *__restrict__ Tb, const float *__restrict__ W, "
-                  "const float *__restrict__ WL, float *__restrict__ Tbout, double *__restrict__ part, int nblk, "
-                  "ActParams ap",
+        head(last ? "dnnf90_fz_bwd_last" : "dnnf90_fz_bwd",
+             "const float *__restrict__ Tin, const float *__restrict__ Tb, const float *__restrict__ W, "
+             "const float *__restrict__ WL, float *__restrict__ Tbout, double *__restrict__ part, int nblk, "
+             "ActParams ap, const float *__restrict__ H",
              2);
         xbuf();
         o << "  __shared__ double Gs[64 * 64];\n"
-          << "  for (int u = tid; u < 64 * 64; u += 256) Gs[u] = 0.0;\n";
-        if (!kh) o << "  const float bias = W[j * 65];\n";
-        o << "  double gb = 0.0;\n";
+          << "  for (int u = tid; u < 64 * 64; u += 256) Gs[u] = 0.0;\n"
+          << "  double gb = 0.0;\n";
         if (last) o << "  const real wl = WL[1 + j];\n  __shared__ float Sz[NA * 4];   // the block's output-row seed\n";
-        // kept h: each block's staged block below, h and the seed are
-        // loaded into registers during the block before it (h and the seed
-        // before its pair sums, the block below before its K6); T-bar of
-        // the layer above is loaded at the top of the block (both planes a
-        // block ahead would not fit the registers)
         auto next_ops = [&](const char *pbv) {
             o << "    const real *hp = H + (long)" << pbv << " * BLK + lo;\n";
             for (int a = 0; a < na; ++a) o << "    hn" << a << " = hp[" << a * 256 << "];\n";
             if (last) o << "    szn = tid < NA * 4 ? Tb[(long)" << pbv << " * (NA * 4) + tid] : 0.f;\n";
         };
-        if (kh) {
-            o << "  flt4 xn[FZ_NU];\n";
-            for (int a = 0; a < na; ++a) o << "  real hn" << a << ";\n";
-            if (last) o << "  float szn;\n";
-            o << "  if ((int)blockIdx.x < nblk) {\n    fz_stage_load(Tin + (long)blockIdx.x * BLK, xn);\n";
-            next_ops("blockIdx.x");
-            o << "  }\n";
-        }
+        o << "  flt4 xn[FZ_NU];\n";
+        for (int a = 0; a < na; ++a) o << "  real hn" << a << ";\n";
+        if (last) o << "  float szn;\n";
+        o << "  if ((int)blockIdx.x < nblk) {\n    fz_stage_load(Tin + (long)blockIdx.x * BLK, xn);\n";
+        next_ops("blockIdx.x");
+        o << "  }\n";
         o << "  for (int pb = blockIdx.x; pb < nblk; pb += gridDim.x) {\n";
         opaque();
-        if (kh) {
-            o << "  const int pn = pb + gridDim.x;\n"
-              << "  __syncthreads();\n  fz_stage_store(xn, X);\n";
-            if (last) o << "  if (tid < NA * 4) Sz[tid] = szn;\n";
-            if (!last) {
-                o << "  const real *tb = Tb + (long)pb * BLK + lo;\n";
-                for (int a = 0; a < na; ++a) o << "  const real t" << a << " = tb[" << a * 256 << "];\n";
-            }
-            for (int a = 0; a < na; ++a) o << "  const real h" << a << " = hn" << a << ";\n";
-            o << "  if (pn < nblk) {\n";
-            next_ops("pn");
-            o << "  }\n  __syncthreads();\n";
-        } else {
-            o << "  __syncthreads();\n  fz_stage(Tin + (long)pb * BLK, X);\n";
-            if (last) o << "  if (tid < NA * 4) Sz[tid] = Tb[(long)pb * (NA * 4) + tid];\n";
-            o << "  __syncthreads();\n";
-            s_from_x("W[wb + 4 * s]");
-            decl_h();
-            emit_recurrence(o, t, s, false, REC_HREG, "Np", tile_hook);
-        }
-        // T-bar loads after the recurrence (hoisted, they would hold 41
-        // registers through it)
-        o << "  __builtin_amdgcn_sched_barrier(0);\n";
-        if (last) {
-            for (int a = 0; a < na; ++a) o << "  const real t" << a << " = wl * Sz[" << a * 4 << " + p];\n";
-        } else if (!kh) {
+        o << "  const int pn = pb + gridDim.x;\n"
+          << "  __syncthreads();\n  fz_stage_store(xn, X);\n";
+        if (last) o << "  if (tid < NA * 4) Sz[tid] = szn;\n";
+        if (!last) {
             o << "  const real *tb = Tb + (long)pb * BLK + lo;\n";
             for (int a = 0; a < na; ++a) o << "  const real t" << a << " = tb[" << a * 256 << "];\n";
         }
+        for (int a = 0; a < na; ++a) o << "  const real h" << a << " = hn" << a << ";\n";
+        o << "  if (pn < nblk) {\n";
+        next_ops("pn");
+        o << "  }\n  __syncthreads();\n";
+        o << "  __builtin_amdgcn_sched_barrier(0);\n";
+        if (last)
+            for (int a = 0; a < na; ++a) o << "  const real t" << a << " = wl * Sz[" << a * 4 << " + p];\n";
         decl_sb();
         k5_begin();
         emit_pair_sums(o, t, false, true, k5_hook);
         k5_end();
-        if (kh) o << "  if (pn < nblk) fz_stage_load(Tin + (long)pn * BLK, xn);\n";
-        // xa once more through the asm: K6 then forms its own LDS addresses
-        // instead of keeping K1's alive across the block
+        o << "  if (pn < nblk) fz_stage_load(Tin + (long)pn * BLK, xn);\n";
+        // xa once more through the asm: K6 forms its own LDS addresses
         o << "  __syncthreads();\n  {\n";
         for (int a = 0; a < na; ++a) o << "    X[xs + " << a << " * XS] = sb" << a << ";\n";
         o << "  }\n  __syncthreads();\n  FZ_OPAQUE(xa);\n  {\n    float v[16];\n"
           << "#pragma unroll\n    for (int s = 0; s < 16; ++s) v[s] = W[wv + 4 * s * 65];\n"
           << "    real *tbo = Tbout + (long)pb * BLK + lo;\n";
         for (int q = 0; q < nt; ++q) {
-            if (kh && q + 1 < nt) {
-                // kept h leaves the registers for two tiles at a time
+            if (q + 1 < nt) {
                 o << "    {\n      flt4 c, c1;\n      fz_tile2(X, xa, " << q << ", v, c, c1);\n";
                 for (int r = 0; r < 4 && 4 * q + r < na; ++r)
                     o << "      tbo[" << (4 * q + r) * 256 << "] = c[" << r << "];\n";
@@ -1287,47 +1336,34 @@ int gen_fused_load(const std::vector<char> &code, FusedKernels *f)
 {
     *f = FusedKernels();
     f->code = code;
-    static const char *const nm[FusedKernels::NR] = {"dnnf90_fz_fwd_in",   "dnnf90_fz_fwd", "dnnf90_fz_fwd_last",
+    static const char *const nm[FusedKernels::NK] = {"dnnf90_fz_fwd_in",   "dnnf90_fz_fwd", "dnnf90_fz_fwd_last",
                                                       "dnnf90_fz_bwd_last", "dnnf90_fz_bwd", "dnnf90_fz_bwd_in"};
     int st = hipModuleLoadData(&f->mod, f->code.data()) == hipSuccess ? 1 : -3;
-    int regs[FusedKernels::NK] = {}, scratch[FusedKernels::NK] = {};
     for (int i = 0; st == 1 && i < FusedKernels::NK; ++i) {
-        const int kh = i / FusedKernels::NR, r = i % FusedKernels::NR;
-        hipFunction_t *fn = &f->fn[kh][r];
-        // the first hidden layer's kernels are one form in both policies
-        const bool one = r == FusedKernels::FWD_IN || r == FusedKernels::BWD_IN;
-        const std::string name = std::string(nm[r]) + (kh && !one ? "_h" : "");
-        if (hipModuleGetFunction(fn, f->mod, name.c_str()) != hipSuccess ||
-            hipFuncGetAttribute(&regs[i], HIP_FUNC_ATTRIBUTE_NUM_REGS, *fn) != hipSuccess ||
-            hipFuncGetAttribute(&scratch[i], HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, *fn) != hipSuccess)
+        if (hipModuleGetFunction(&f->fn[i], f->mod, nm[i]) != hipSuccess ||
+            hipFuncGetAttribute(&f->regs[i], HIP_FUNC_ATTRIBUTE_NUM_REGS, f->fn[i]) != hipSuccess ||
+            hipFuncGetAttribute(&f->scratch[i], HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, f->fn[i]) != hipSuccess)
             st = -3;
         // a few loop-invariant values may live in scratch (reloaded a few
         // times a block); a kernel spilling more is not used (the unfused
         // kernels take the step)
-        else if (scratch[i] > FZ_MAX_SCRATCH)
+        else if (f->scratch[i] > FZ_MAX_SCRATCH)
             st = -4;
     }
     // the adjoint kernels stream their blocks in one resident round
-    int occ[2] = {}, dev = 0, ncu = 0;
+    int occ = 0, dev = 0, ncu = 0;
     if (st == 1 &&
-        !(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ[0], f->fn[0][FusedKernels::BWD], 256, 0) ==
-              hipSuccess &&
-          hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ[1], f->fn[1][FusedKernels::BWD], 256, 0) ==
-              hipSuccess &&
+        !(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ, f->fn[FusedKernels::BWD], 256, 0) == hipSuccess &&
           hipGetDevice(&dev) == hipSuccess &&
-          hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && occ[0] > 0 &&
-          occ[1] > 0))
+          hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && occ > 0))
         st = -5;
-    std::copy(regs, regs + FusedKernels::NK, f->regs);
-    std::copy(scratch, scratch + FusedKernels::NK, f->scratch);
     if (st != 1) {
         (void)hipGetLastError();
-        f->release();
+        f->release();   // the registers and scratch read stay for the report
         f->status = st;
         return -3;
     }
-    f->grid[0] = occ[0] * ncu;
-    f->grid[1] = occ[1] * ncu;
+    f->grid = occ * ncu;
     f->status = 1;
     f->ok = true;
     return 0;
@@ -1337,19 +1373,13 @@ void FusedKernels::release()
 {
     if (mod) (void)hipModuleUnload(mod);
     mod = nullptr;
-    for (auto &row : fn)
-        for (auto &k : row) k = nullptr;
+    for (auto &k : fn) k = nullptr;
     ok = false;
 }
 
-// the kernels taking h (H non-null) have it as their last argument
-static hipError_t fz_launch(const FusedKernels &f, int role, const float *H, unsigned grid, void **params, int np,
-                            hipStream_t s)
+static hipError_t fz_launch(const FusedKernels &f, int role, unsigned grid, void **params, hipStream_t s)
 {
-    void *pv[16];
-    std::copy(params, params + np, pv);
-    if (H) pv[np] = (void *)&H;
-    return hipModuleLaunchKernel(f.fn[H ? 1 : 0][role], grid, 1, 1, 256, 1, 1, 0, s, pv, nullptr);
+    return hipModuleLaunchKernel(f.fn[role], grid, 1, 1, 256, 1, 1, 0, s, params, nullptr);
 }
 
 hipError_t fz_launch_fwd_in(const FusedKernels &f, const double *x, int npt, const float *W1, float *Tout, int nblk,
@@ -1357,36 +1387,34 @@ hipError_t fz_launch_fwd_in(const FusedKernels &f, const double *x, int npt, con
 {
     ActParams a = ap;
     void *params[] = {&x, &npt, &W1, &Tout, &nblk, &a};
-    return fz_launch(f, FusedKernels::FWD_IN, nullptr, (unsigned)nblk, params, 6, s);
+    return fz_launch(f, FusedKernels::FWD_IN, (unsigned)nblk, params, s);
 }
 
 hipError_t fz_launch_fwd(const FusedKernels &f, const float *Tin, const float *W, float *Tout, float *H, int nblk,
                          const ActParams &ap, hipStream_t s)
 {
     ActParams a = ap;
-    void *params[] = {&Tin, &W, &Tout, &nblk, &a};
-    return fz_launch(f, FusedKernels::FWD, H, (unsigned)nblk, params, 5, s);
+    void *params[] = {&Tin, &W, &Tout, &nblk, &a, &H};
+    return fz_launch(f, FusedKernels::FWD, (unsigned)nblk, params, s);
 }
 
 hipError_t fz_launch_fwd_last(const FusedKernels &f, const float *Tin, const float *W, float *H, int nblk,
                               const ActParams &ap, const float *WL, int npt, const DevResidual &r, double fac,
-                              float *Sbo, double *gpart, double *rpart, hipStream_t s)
+                              float *Sbo, double *gpart, double *rpart, double *tout, hipStream_t s)
 {
     ActParams a = ap;
     DevResidual rr = r;
-    void *params[] = {&Tin, &W, &nblk, &a, &WL, &npt, &rr, &fac, &Sbo, &gpart, &rpart};
-    return fz_launch(f, FusedKernels::FWD_LAST, H, (unsigned)nblk, params, 11, s);
+    void *params[] = {&Tin, &W, &nblk, &a, &WL, &npt, &rr, &fac, &Sbo, &gpart, &rpart, &tout, &H};
+    return fz_launch(f, FusedKernels::FWD_LAST, (unsigned)nblk, params, s);
 }
 
-// the adjoint of a hidden layer l >= 2 (last: T-bar from the output row's
-// seed in Tb)
 hipError_t fz_launch_bwd(const FusedKernels &f, bool last, const float *Tin, const float *Tb, const float *H,
                          const float *W, const float *WL, float *Tbout, double *part, int nblk, const ActParams &ap,
                          int grid, hipStream_t s)
 {
     ActParams a = ap;
-    void *params[] = {&Tin, &Tb, &W, &WL, &Tbout, &part, &nblk, &a};
-    return fz_launch(f, last ? FusedKernels::BWD_LAST : FusedKernels::BWD, H, (unsigned)grid, params, 8, s);
+    void *params[] = {&Tin, &Tb, &W, &WL, &Tbout, &part, &nblk, &a, &H};
+    return fz_launch(f, last ? FusedKernels::BWD_LAST : FusedKernels::BWD, (unsigned)grid, params, s);
 }
 
 hipError_t fz_launch_bwd_in(const FusedKernels &f, const double *x, int npt, const float *Tb, const float *W1,
@@ -1394,7 +1422,7 @@ hipError_t fz_launch_bwd_in(const FusedKernels &f, const double *x, int npt, con
 {
     ActParams a = ap;
     void *params[] = {&x, &npt, &Tb, &W1, &part, &nblk, &a};
-    return fz_launch(f, FusedKernels::BWD_IN, nullptr, (unsigned)grid, params, 7, s);
+    return fz_launch(f, FusedKernels::BWD_IN, (unsigned)grid, params, s);
 }
 
 }  // namespace dnnf90

@@ -99,30 +99,24 @@ hipError_t gen_launch_pt_bwd(const GenKernels &g, const double *W, double *grad,
 
 // Fused layer kernels (FP32 storage; codegen.cpp): hidden layers FZ_W wide,
 // point blocks of FZ_P points, slots at most FZ_MAXA, inputs at most
-// FZ_MAXD0.  A staged block in LDS: rows of FZ_PJ floats (one point's 64
-// neurons and 4 of padding, 16-byte aligned for the DMA), slots of FZ_XS
-// (four rows), which keeps the linear maps' operand reads at most 2-way
-// bank-conflicted; FZ_FL: k-steps of the
-// forward map between FP64 flushes (four terms a step).
-constexpr int FZ_W = 64, FZ_P = 4, FZ_MAXA = 48, FZ_MAXD0 = 8, FZ_PJ = 68, FZ_XS = 272, FZ_FL = 4;
+// FZ_MAXD0.  The adjoint's staged block in LDS: rows of FZ_PJ floats (one
+// point's 64 neurons and 4 of padding), slots of FZ_XS (four rows), which
+// keeps the linear maps' operand reads at most 2-way bank-conflicted.
+constexpr int FZ_W = 64, FZ_P = 4, FZ_MAXA = 48, FZ_MAXD0 = 8, FZ_PJ = 68, FZ_XS = 272;
 // terms of the collocation residual the fused last forward holds in LDS
 constexpr int FZ_MAXT = 64;
 // scratch (bytes a lane) a fused kernel may hold: loop-invariant values the
 // register allocator parks there and reloads a few times a block
 constexpr int FZ_MAX_SCRATCH = 256;
 struct FusedKernels {
-    // kernel roles; each in two plane policies: [0] the adjoint recomputes S
-    // and h from T of the layer below, [1] the forward writes h of each
-    // hidden layer and the adjoint reads it
-    enum Role { FWD_IN, FWD, FWD_LAST, BWD_LAST, BWD, BWD_IN, NR };
-    static constexpr int NK = 2 * NR;
-    int grid[2] = {};   // workgroups of the adjoint kernels: one resident round
+    enum Role { FWD_IN, FWD, FWD_LAST, BWD_LAST, BWD, BWD_IN, NK };
+    int grid = 0;   // workgroups of the adjoint kernels: one resident round
     // 1 loaded; else why not: -2 hipRTC failed, -3 module or kernel missing,
     // -4 a kernel spills past FZ_MAX_SCRATCH, -5 no occupancy reported
     int status = 0;
     hipModule_t mod = nullptr;
-    hipFunction_t fn[2][NR] = {};
-    int regs[NK] = {}, scratch[NK] = {};   // [policy * NR + role]
+    hipFunction_t fn[NK] = {};
+    int regs[NK] = {}, scratch[NK] = {};
     std::vector<char> code;
     bool ok = false;
     void release();
@@ -132,20 +126,21 @@ std::string gen_fused_source(const TableSet &t, const FwdSchedule &s);
 int gen_fused_compile(const TableSet &t, const FwdSchedule &s, FusedKernels *f, std::string *log);
 // 0, or -3 (a kernel missing or spilling) with f released
 int gen_fused_load(const std::vector<char> &code, FusedKernels *f);
-// H: the layer's h plane (written by the forward, read by the adjoint), or
-// null for the recomputing kernels; the first hidden layer's adjoint always
-// recomputes
 hipError_t fz_launch_fwd_in(const FusedKernels &f, const double *x, int npt, const float *W1, float *Tout, int nblk,
                             const ActParams &ap, hipStream_t s);
-// forward of a hidden layer l >= 2 but the last, one block a workgroup
+// forward of a hidden layer l >= 2 but the last, one block a workgroup: T
+// and h of the layer
 hipError_t fz_launch_fwd(const FusedKernels &f, const float *Tin, const float *W, float *Tout, float *H, int nblk,
                          const ActParams &ap, hipStream_t s);
-// forward of the last hidden layer with the output row, the collocation
-// seed into Sbo ([block][slot][4 points]), the output layer's gradient
-// partials gpart[block][65] (bias first) and rpart[block] = sum of R^2
+// forward of the last hidden layer: h, the output row (into tout,
+// [point][slot]), the collocation seed into Sbo ([block][slot][4 points]),
+// the output layer's gradient partials gpart[block][65] (bias first) and
+// rpart[block] = sum of R^2
 hipError_t fz_launch_fwd_last(const FusedKernels &f, const float *Tin, const float *W, float *H, int nblk,
                               const ActParams &ap, const float *WL, int npt, const DevResidual &r, double fac,
-                              float *Sbo, double *gpart, double *rpart, hipStream_t s);
+                              float *Sbo, double *gpart, double *rpart, double *tout, hipStream_t s);
+// adjoint of a hidden layer l >= 2 from its h plane (last: T-bar from the
+// output row's seed in Tb)
 hipError_t fz_launch_bwd(const FusedKernels &f, bool last, const float *Tin, const float *Tb, const float *H,
                          const float *W, const float *WL, float *Tbout, double *part, int nblk, const ActParams &ap,
                          int grid, hipStream_t s);

@@ -428,6 +428,10 @@ struct BatchWork {
     std::vector<double> h_coeff;
     std::vector<int> h_slot, h_nonlin;
     bool fwd_valid = false;
+    // where the last collocation step left its output row: 0 nowhere (or
+    // used since), 1 the output plane tl, 2 d_tout ([point][slot], the
+    // fused kernels)
+    int step_out = 0;
     long fwd_ver = -1;
     KProf prof;
 
@@ -727,6 +731,7 @@ static PtNet point_net(const Net &nt, const BatchWork &b)
 template <typename T>
 static hipError_t forward_t(const Net &nt, BatchWork &b, const double *d_x, int npt)
 {
+    b.step_out = 0;
     const int na = nt.tab.na, L = nt.nlayer;
     const long Np = round_up(std::max(npt, 1), 64);
     const long ncol = (long)na * Np;
@@ -1203,14 +1208,16 @@ static int set_residual(BatchWork &b, int nterm, const double *coeff, const int 
 }
 
 // the fused layer kernels take a collocation step (not per-point rows) of an
-// FP32 work space when the network has at least two hidden layers (at most
-// RS_MAX layers in all: one reduction segment a layer), every one FZ_W
+// FP32 work space that keeps its planes (no per-layer recompute: h of each
+// hidden layer is kept) when the network has at least two hidden layers (at
+// most RS_MAX layers in all: one reduction segment a layer), every one FZ_W
 // wide, one output row, at most FZ_MAXD0 inputs, and the family's fused
 // module compiled (codegen.cpp)
 static const FusedKernels *fused_plan(Net &nt, const BatchWork &b)
 {
     const int L = nt.nlayer;
-    if (!g_fused_layers || !codegen_on() || b.prec != PREC_FP32 || L < 4 || L > RS_MAX) return nullptr;
+    if (!g_fused_layers || !codegen_on() || b.prec != PREC_FP32 || b.recompute || L < 4 || L > RS_MAX)
+        return nullptr;
     if (nt.dims[L - 1] != 1 || nt.dims[0] > FZ_MAXD0) return nullptr;
     for (int l = 1; l < L - 1; ++l)
         if (nt.dims[l] != FZ_W) return nullptr;
@@ -1239,14 +1246,13 @@ static int grow_partial(BatchWork &b, long n)
 // The collocation step on the fused layer kernels.  The work space's planes
 // hold the point-blocked layout ([block][slot][4 points][neuron]): T of the
 // hidden layers but the last (tpl), h of the hidden layers but the first
-// (hpl; not with per-layer recompute), the seed and the T-bar planes the adjoint passes
-// down (p0, p1).  Forward: the first hidden layer from the inputs, then one
-// kernel a layer (K1 + Faa di Bruno, T and h); the last one also forms the
-// output row, the residual, the seed, the output layer's gradient partials
-// and the sums of R^2.  Then per hidden layer from the last: one kernel (h
-// read, or with per-layer recompute S and h recomputed from T below; pair
-// sums, K5, T-bar of the layer below), the first hidden layer's forming its
-// gradient from the inputs.  The planes are left in that layout: no
+// (hpl), the seed and the T-bar planes the adjoint passes down (p0, p1).
+// Forward: the first hidden layer from the inputs, then one kernel a layer
+// (K1 + Faa di Bruno, T and h); the last one also forms the output row, the
+// residual, the seed, the output layer's gradient partials and the sums of
+// R^2.  Then per hidden layer from the last: one kernel (h read; pair sums,
+// K5, T-bar of the layer below), the first hidden layer's forming S and h
+// from the inputs and its gradient.  The planes are left in that layout: no
 // backward_batch may follow.
 static int run_residual_fused(Net &nt, BatchWork &b, const FusedKernels &f, int npt, const double *x,
                               const DevResidual &r, double fac, Grad &g, double *rsq, bool dev)
@@ -1263,9 +1269,8 @@ static int run_residual_fused(Net &nt, BatchWork &b, const FusedKernels &f, int 
     }
     const long Np = round_up(std::max(npt, 1), 64);
     const int nblk = (int)(Np / FZ_P);
-    const bool kh = !b.recompute;   // h planes kept (hpl)
-    const int grid = std::min(nblk, f.grid[kh ? 1 : 0]);
-    auto H = [&](int l) { return kh ? (float *)b.hpl[l] : nullptr; };
+    const int grid = std::min(nblk, f.grid);
+    auto H = [&](int l) { return (float *)b.hpl[l]; };
     b.np = Np;
     b.npt = npt;
     b.pt_used = false;
@@ -1294,7 +1299,7 @@ static int run_residual_fused(Net &nt, BatchWork &b, const FusedKernels &f, int 
     if (e == hipSuccess) {
         PScope p(b, CAT_BELL_FWD);
         e = fz_launch_fwd_last(f, (const float *)b.tpl[L - 3], W + nt.woff[L - 2], H(L - 2), nblk, act,
-                               W + nt.woff[L - 1], npt, r, fac, (float *)b.p0, gpart, rpart, s);
+                               W + nt.woff[L - 1], npt, r, fac, (float *)b.p0, gpart, rpart, b.d_tout, s);
     }
     ReduceSegs red;
     red.add(gpart, nblk, FZ_W + 1, g.d_g + nt.woff[L - 1]);
@@ -1322,6 +1327,7 @@ static int run_residual_fused(Net &nt, BatchWork &b, const FusedKernels &f, int 
     }
     if ((rc = g.after(s))) return rc;
     if (e) return hip_fail(e, "fused step launch");
+    b.step_out = 2;
     if (!b.defer_use) nt.used_on(s);
     if (rsq) {
         HIPCHK(hipMemcpyAsync(rsq, b.d_rsq, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1363,6 +1369,7 @@ static int run_residual(Net &nt, BatchWork &b, int npt, const double *x, int nte
         if ((rc = g->after(b.stream))) return rc;
     }
     if (e) return hip_fail(e, "adjoint launch");
+    b.step_out = b.pt_used ? 0 : 1;
     if (rsq) {
         launch_reduce_sum(b.d_rsq_part, nblk, b.d_rsq, b.stream);
         HIPCHK(hipMemcpyAsync(rsq, b.d_rsq, sizeof(double), hipMemcpyDeviceToHost, b.stream));
@@ -2183,6 +2190,28 @@ int dnnf90_bwork_info(int bid, double *info)
 }
 int dnnf90_bwork_free(int bid) { return free_work(g_bwork, bid); }
 
+int dnnf90_bwork_step_output(int bid, double *tout, int flags)
+{
+    BatchWork *b = g_bwork.get(bid);
+    if (!b || !tout) return fail(-1, "bad work-space handle or argument");
+    Net *nt = g_net.get(g_bwork.owner[bid]);
+    if (!nt || !b->step_out) return fail(-1, "no collocation step output held by this work space");
+    const bool dev = flags & DNNF90_DEVICE_PTRS;
+    const int nout = nt->dims[nt->nlayer - 1], na = nt->tab.na, npt = b->npt;
+    const size_t bytes = sizeof(double) * (size_t)npt * nout * na;
+    double *dt = dev ? tout : b->d_tout;
+    if (b->step_out == 1) {
+        if (b->prec == PREC_FP64) launch_output<double>((const double *)b->tl, nout, na, npt, b->np, dt, b->stream);
+        else launch_output<float>((const float *)b->tl, nout, na, npt, b->np, dt, b->stream);
+    } else if (dev && npt > 0) {
+        HIPCHK(hipMemcpyAsync(tout, b->d_tout, bytes, hipMemcpyDeviceToDevice, b->stream));
+    }
+    if (!dev && npt > 0) HIPCHK(hipMemcpyAsync(tout, b->d_tout, bytes, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 int dnnf90_bwork_set_stream(int bid, void *stream)
 {
     BatchWork *b = g_bwork.get(bid);
@@ -2360,7 +2389,7 @@ int dnnf90_net_fused_info(int nid, int *info)
     probe.prec = PREC_FP32;
     const FusedKernels *f = fused_plan(*nt, probe);
     constexpr int NK = FusedKernels::NK;
-    for (int i = 0; i < 3 + 2 * NK; ++i) info[i] = 0;
+    for (int i = 0; i < 2 + 2 * NK; ++i) info[i] = 0;
     if (!f) {
         // the network's shape or the switch rules them out (0), or the
         // module did not come up (FusedKernels::status)
@@ -2377,8 +2406,7 @@ int dnnf90_net_fused_info(int nid, int *info)
         info[1 + i] = f->regs[i];
         info[1 + NK + i] = f->scratch[i];
     }
-    info[1 + 2 * NK] = f->grid[0];
-    info[2 + 2 * NK] = f->grid[1];
+    info[1 + 2 * NK] = f->grid;
     return 0;
 }
 

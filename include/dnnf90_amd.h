@@ -112,16 +112,16 @@ int dnnf90_net_step_info(int nid, int prec, int *info);
  * relative).  On by default; returns the previous setting.  No counterpart
  * in the reference. */
 int dnnf90_set_fused_grad(int on);
-/* FP32 work spaces (DNNF90_PREC_FP32) whose network has at least two hidden
- * layers, all 64 wide, one output row and at most 8 inputs take the
+/* FP32 work spaces (DNNF90_PREC_FP32) that keep their planes (not per-layer
+ * recompute) and whose network has at least two hidden layers, all 64 wide,
+ * one output row, at most 8 inputs and at most 16 layers take the
  * collocation step (dnnf90_residual_step_batch) on fused layer kernels:
  * point-blocked planes ([block][slot][4 points][neuron]), each hidden
  * layer's linear map inside its Faa di Bruno forward (S never stored; T and
- * h written), and one adjoint kernel a layer that reads h (with per-layer
- * recompute, dnnf90_bwork_create_budget: recomputes S and h from T of the
- * layer below), forms the pair sums, the layer's weight gradient and Tbar
- * of the layer below (Sbar never stored); linear maps on FP32 MFMA, the
- * forward map moved into FP64 every 16 terms.
+ * h written; the map on BF16 three-way split products, moved into FP64
+ * every 32 terms), and one adjoint kernel a layer that reads h, forms the
+ * pair sums, the layer's weight gradient and Tbar of the layer below (Sbar
+ * never stored; FP32 MFMA).
  * The work space's planes are then in that layout, so a dnnf90_backward_batch
  * must follow a dnnf90_forward_batch, not the step (refused otherwise, as
  * always).  dnnf90_set_fused_layers(0) takes the layer-by-layer kernels
@@ -130,15 +130,14 @@ int dnnf90_set_fused_grad(int on);
  * point at a time). */
 int dnnf90_set_fused_layers(int on);
 /* whether the fused layer kernels take this network's FP32 collocation
- * step, info[27]: [0] 1 taken; 0 not (shape or switch); -1 the family is
- * too large to unroll, -2 hipRTC failed, -3 module or kernel missing, -4 a
- * kernel spills past its allowance, -5 no occupancy; [1..12] registers and
- * [13..24] scratch bytes a lane of the twelve kernels: first hidden layer
- * forward, forward, last hidden layer forward with the output row, last
- * hidden layer adjoint, adjoint, first hidden layer adjoint, first with S
- * and h recomputed in the adjoint, then with h kept; [25], [26] the
- * adjoint kernels' grid in the two.  Compiles the module on first use.  No
- * counterpart in the reference. */
+ * step (in a work space that keeps its planes), info[14]: [0] 1 taken; 0
+ * not (shape or switch); -1 the family is too large to unroll, -2 hipRTC
+ * failed, -3 module or kernel missing, -4 a kernel spills past its
+ * allowance, -5 no occupancy; [1..6] registers and [7..12] scratch bytes a
+ * lane of the six kernels (first hidden layer forward, forward, last hidden
+ * layer forward with the output row, last hidden layer adjoint, adjoint,
+ * first hidden layer adjoint); [13] the adjoint kernels' grid.  Compiles the
+ * module on first use.  No counterpart in the reference. */
 int dnnf90_net_fused_info(int nid, int *info);
 /* Debug mode for reads of unwritten memory: batched work spaces created
  * while it is on fill their planes, operand scratch and gradient partials
@@ -225,6 +224,13 @@ int dnnf90_residual_step_batch(int nid, int bid, int npt, const double *x,
                                int nterm, const double *coeff,
                                const int *slot, const int *nonlin,
                                double fac, int gid, double *rsq, int flags);
+/* the network's outputs and their derivatives (the output row R was formed
+ * from) at the points of the last dnnf90_residual_step_batch on this work
+ * space, tout[npt][nout * nderiv] as dnnf90_forward_batch writes them (host
+ * or, with DNNF90_DEVICE_PTRS, device); -1 if no such step left them (a
+ * forward, another call or a one-point step since).  No counterpart in the
+ * reference. */
+int dnnf90_bwork_step_output(int bid, double *tout, int flags);
 /* Gauss-Newton rows dR_n/dw of the same residual (seed dR/dT), written
  * transposed as FP32 rows_t[w][n], w < nweights (device pointer always).
  * The kernels write whole 64-point blocks, the rows of the padding points
