@@ -112,6 +112,14 @@ def test_argument_checks(L):
     assert L.dnnf90_net_free(nid) == 0
     assert L.dnnf90_net_free(nid) == -1
     assert L.dnnf90_net_set_layer(nid, 2, row.ctypes.data) == -1
+    # no work space, no output array
+    out = np.zeros(8)
+    assert L.dnnf90_bwork_step_output(0, out.ctypes.data, 0) == -1
+    assert L.dnnf90_bwork_step_output(63, out.ctypes.data, 0) == -1
+    assert L.dnnf90_bwork_step_output(1, None, 0) == -1
+    # the fused kernels' switch returns its previous setting
+    prev = L.dnnf90_set_fused_layers(0)
+    assert L.dnnf90_set_fused_layers(prev) == 0
 
 
 def test_weight_file_round_trip(L, tmp_path):
