@@ -38,8 +38,9 @@ CONFIGS = ("kdv", "dense4_k3", "gelu3_k4")
 
 METRIC = "fwd+adjoint points/s, mixed-prec vs FP64 on GPU, at stated err vs FP64 oracle"
 PEAK = {"hbm": 8000.0, "fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
-DTYPE = {"fp32": "f32 storage, FP32 MFMA linear maps (forward map flushed to FP64 every 16 terms), FP64 "
-                 "activation derivatives, residual and gradient sums",
+DTYPE = {"fp32": "f32 storage (fused layer kernels where the network's shape allows: forward map on bf16 "
+                 "three-way split MFMA moved into FP64 every 32 terms, adjoint maps FP32 MFMA), FP64 activation "
+                 "derivatives, residual and gradient sums",
          "bf16x3": "f32 storage, bf16 hi+lo split MFMA",
          "bf16x6": "f32 storage, bf16 three-way split MFMA (FP64-accumulated forward map)"}
 TRAFFIC = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic.json")
