@@ -612,13 +612,18 @@ __global__ __launch_bounds__(256) void k_reduce_fused(const double *__restrict__
 
 // several of k_reduce_fused's reductions in one launch (blockIdx.y picks
 // the segment), each with its own split count: the same bits as one launch
-// a segment; set: the segment's output receives the sum instead of adding it
+// a segment, but for segments of 1024 chunks or more, which take 64 slices
+// (short rows of many point blocks: the output layer's partials and the
+// sums of R^2); set: the segment's output receives the sum instead of
+// adding it
+__host__ __device__ static inline int seg_splits(int nchunk) { return nchunk >= 1024 ? 64 : grad_reduce_splits(nchunk); }
+
 __global__ __launch_bounds__(256) void k_reduce_segs(ReduceSegs r)
 {
     __shared__ double red[256];
     const int g = blockIdx.y;
     const long len = r.len[g];
-    const int nchunk = r.nchunk[g], nsplit = grad_reduce_splits(nchunk);
+    const int nchunk = r.nchunk[g], nsplit = seg_splits(nchunk);
     const int wpb = 256 / nsplit, t = threadIdx.x;
     if ((long)blockIdx.x * wpb >= len) return;   // the whole workgroup: this segment is shorter
     const int wi = t % wpb, sidx = t / wpb;
@@ -644,8 +649,10 @@ __global__ __launch_bounds__(256) void k_reduce_segs(ReduceSegs r)
 void launch_reduce_segs(const ReduceSegs &r, hipStream_t s)
 {
     long nb = 0;
-    for (int g = 0; g < r.n; ++g) nb = std::max(nb, (r.len[g] + 256 / grad_reduce_splits(r.nchunk[g]) - 1) /
-                                                        (256 / grad_reduce_splits(r.nchunk[g])));
+    for (int g = 0; g < r.n; ++g) {
+        const long wpb = 256 / seg_splits(r.nchunk[g]);
+        nb = std::max(nb, (r.len[g] + wpb - 1) / wpb);
+    }
     if (r.n > 0 && nb > 0)
         hipLaunchKernelGGL(k_reduce_segs, dim3((unsigned)nb, (unsigned)r.n), dim3(256), 0, s, r);
 }

@@ -88,7 +88,8 @@ void launch_gemm_grad(int prec, const T *A, const T *B, int M, int Kd, long ncol
 // grad[len] += sum over chunks of partial[chunk][len], in a fixed order
 void launch_grad_reduce(double *partial, int nchunk, long len, double *grad, hipStream_t s);
 // up to RS_MAX such reductions in one launch (set: out = the sum instead of
-// out += it); each segment's bits are launch_grad_reduce's
+// out += it); each segment's bits are launch_grad_reduce's, but for
+// segments of 1024 chunks or more (64 slices instead of 16)
 constexpr int RS_MAX = 16;
 struct ReduceSegs {
     int n = 0;
