@@ -13,7 +13,8 @@
 //   gram          G = R R^T, lower triangle, R the transposed FP32 rows (gemm.hip)
 //   k_damp        G <- G / nb + c I (lower triangle), b <- b / nb
 //   k_chol_diag   one workgroup: Cholesky of the PB x PB diagonal block in LDS,
-//                 then its inverse Linv (kept for the panel and the solves)
+//                 then its inverse Linv (kept for the panel and the solves;
+//                 k_linv_transpose forms the transposes after the last panel)
 //   k_chol_panel  L21 = A21 Linv^T for the rows under the block: an FP64
 //                 MFMA product instead of a row-sequential triangular solve;
 //                 also packs L21 row-contiguously for the update
@@ -24,6 +25,7 @@
 //                 update), and the workgroup that owns the next diagonal
 //                 block solves it with that block's Linv
 #include <algorithm>
+#include <vector>
 
 #include "kernels.hpp"
 
@@ -81,20 +83,25 @@ __device__ __forceinline__ double ldot(const double *__restrict__ x, const doubl
 }
 
 // Cholesky of the diagonal block at (k, k) (nb <= PB rows) in LDS, and
-// its inverse X = L^-1 and the inverse's transpose, written to Linv[PB][PB]
-// and LinvT[PB][PB] (zero off the triangle; rows past nb hold the
-// identity).  Right-looking over 16-column steps: wave 0 factors the
-// 16 x 16 diagonal block in registers (one row a lane, the pivot column
-// broadcast by readlane) and forms that block's inverse X_kk by forward
-// substitution (one column a lane); the rows below take L21 = A21 X_kk^T
-// and the trailing lower blocks the rank-16 update, both on
-// v_mfma_f64_16x16x4f64 (16 x 16 blocks, a wave a block).  The off-diagonal
-// blocks of X follow block row by block row, X_ij = -X_ii (sum_{j <= m < i}
-// L_im X_mj), on FP64 MFMA, a wave a block.  X (below the diagonal) is kept
-// transposed in the free upper triangle of S, its diagonal in dg.  The
-// block of G itself is left as the trailing updates made it (the solves
-// and the panel read Linv, LinvT and the rows below the block).  info > 0
-// marks the first non-positive pivot (index + 1).
+// its inverse X = L^-1, written to Linv[PB][PB] (zero above the diagonal;
+// rows past nb hold the identity).  The transposes of all the blocks'
+// inverses are formed once the factorization is done (k_linv_transpose),
+// off the panels' dependent chain.  Right-looking over 16-column steps:
+// wave 0 factors the 16 x 16 diagonal block in registers (one row a lane;
+// each pivot column goes to every lane through a 16-double LDS row) and
+// forms that block's inverse X_kk by forward substitution (one column a
+// lane, the rows of L read back from LDS); the rows below take
+// L21 = A21 X_kk^T and the trailing lower blocks the rank-16 update, both
+// on v_mfma_f64_16x16x4f64 (16 x 16 blocks, a wave a block).  The
+// off-diagonal blocks of X follow by block columns, which are independent
+// of each other: X_ij = -X_ii (sum_{j <= m < i} L_im X_mj) needs only the
+// blocks of column j above it, so a wave walks down whole columns (columns
+// j and 6 - j, and column 3, balance the four waves) on FP64 MFMA with no
+// barrier between block rows.  X (below the diagonal) is kept transposed
+// in the free upper triangle of S, its diagonal in dg.  The block of G
+// itself is left as the trailing updates made it (the solves and the panel
+// read Linv, LinvT and the rows below the block).  info > 0 marks the
+// first non-positive pivot (index + 1).
 static constexpr int DB = 16;         // step width inside the diagonal block
 static constexpr int ND = PB / DB;    // steps
 
@@ -104,95 +111,131 @@ __device__ __forceinline__ double xinv(const double (*S)[PBL], const double *dg,
     return R > C ? S[C][R] : (R == C ? dg[R] : 0.0);
 }
 
-// the value of lane src (a constant after unrolling) in every lane
-__device__ __forceinline__ double lane_bcast(double v, int src)
+// (a) of a step, wave 0: the 16 x 16 diagonal block at (o, o), lane
+// r + 16 q holding columns q, q + 4, q + 8, q + 12 of row o + r (a quarter
+// of the row's rank-1 updates a lane), then its inverse, lane b forming
+// column b of X_kk (lanes 16..63 repeat lanes 0..15).  LDS operations of
+// one wave complete in program order, so a value a lane writes is there
+// for the reads after it.
+__device__ __forceinline__ void diag16(double (*S)[PBL], double *dg, double *colb, int *info, int k, int o, int lane)
 {
-    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-    const int lo = __builtin_amdgcn_readlane((int)(unsigned)b, src);
-    const int hi = __builtin_amdgcn_readlane((int)(unsigned)(b >> 32), src);
-    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+    const int r = lane & 15, q = lane >> 4;
+    double d[4], il_[DB];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) d[m] = S[o + r][o + 4 * m + q];
+#pragma unroll
+    for (int j = 0; j < DB; ++j) {
+        const int jm = j >> 2, jq = j & 3;
+        // column j as the earlier steps left it, one element a row, to every lane
+        if (q == jq) colb[r] = d[jm];
+        __builtin_amdgcn_wave_barrier();
+        const double djj = colb[j], arj = colb[r];
+        double acj[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) acj[m] = colb[4 * m + q];
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0 && !(djj > 0.0) && *info == 0) *info = k + o + j + 1;
+        // 1/sqrt from v_rsq_f64 and two Newton steps (each doubles the
+        // correct bits), l = d * (1/sqrt d): no IEEE sqrt and division
+        // sequences on the column's dependent path
+        const double dd = djj > 0.0 ? djj : 1.0;
+        double il = __builtin_amdgcn_rsq(dd);
+        il = il * (1.5 - 0.5 * dd * il * il);
+        il = il * (1.5 - 0.5 * dd * il * il);
+        il_[j] = il;
+        if (lane == 0) dg[o + j] = il;
+        const double l = dd * il;
+        const double lrj = arj * il;   // L[o + r][o + j] (rows r > j)
+        if (q == jq) d[jm] = r == j ? l : (r > j ? lrj : d[jm]);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int c = 4 * m + q;
+            const double lcj = acj[m] * il;   // L[o + c][o + j]
+            if (c > j && c <= r) d[m] -= lrj * lcj;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const int c = 4 * m + q;
+        if (c <= r) S[o + r][o + c] = d[m];
+    }
+    __builtin_amdgcn_wave_barrier();
+    // X[a][b] = -X[a][a] sum_{b <= m < a} L[a][m] X[m][b] (row a of L read
+    // back from LDS, the same address in every lane)
+    const int b = r;
+    double x[DB];
+#pragma unroll
+    for (int a = 0; a < DB; ++a) {
+        double s = 0.0;
+#pragma unroll
+        for (int m = 0; m < a; ++m) s += S[o + a][o + m] * x[m];
+        x[a] = a < b ? 0.0 : (a == b ? il_[a] : -il_[a] * s);
+    }
+    if (lane < DB) {
+#pragma unroll
+        for (int a = 0; a < DB; ++a)
+            if (a > b) S[o + b][o + a] = x[a];   // X[a][b] at S[C][R]
+    }
+}
+
+// (c) of a step: trailing lower block u (row-major over the lower triangle
+// of blocks below the step's column o, from t0) -= L(ib, kb) L(jb, kb)^T
+__device__ __forceinline__ void trail16(double (*S)[PBL], int o, int t0, int u, int li, int lk)
+{
+    int ti = 0;
+    while ((ti + 1) * (ti + 2) / 2 <= u) ++ti;
+    const int tj = u - ti * (ti + 1) / 2;
+    const int ra = t0 + DB * ti, rc = t0 + DB * tj;
+    double av[4], bv[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        av[c] = S[ra + li][o + 4 * c + lk];
+        bv[c] = S[rc + li][o + 4 * c + lk];
+    }
+    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[c], bv[c], acc, 0, 0, 0);
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) S[ra + lk + 4 * rr][rc + li] -= acc[rr];
 }
 
 __global__ __launch_bounds__(256) void k_chol_diag(const double *__restrict__ A, long lda, int k, int nb,
-                                                   double *__restrict__ Linv, double *__restrict__ LinvT,
-                                                   int *__restrict__ info)
+                                                   double *__restrict__ Linv, int *__restrict__ info)
 {
     __shared__ double S[PB][PBL];    // L (lower), then X transposed (upper)
     __shared__ double dg[PB];        // 1 / L[j][j] = X[j][j]
+    __shared__ double colb[DB];      // a step's pivot column, one element a row
     const int tid = threadIdx.x;
-    // the block in sixteen loads a thread at a time, all in flight together
-    for (int u0 = 0; u0 < PB * PB; u0 += 256 * 16) {
-        double v[16];
+    // the block in one round: every load of a thread in flight together
+    // (only the lower triangle is read; the upper one is written before use)
+    {
+        constexpr int NL = PB * PB / 256;
+        double v[NL];
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int u = u0 + tid + 256 * i, r = u / PB, c = u % PB;
-            v[i] = (r < nb && c <= r) ? A[(long)(k + r) * lda + k + c] : (r == c ? 1.0 : 0.0);
+        for (int i = 0; i < NL; ++i) {
+            // unconditional loads (an element outside the triangle reads the
+            // block's first one), so that none waits for another
+            const int u = tid + 256 * i, r = u / PB, c = u % PB;
+            const bool in = r < nb && c <= r;
+            const double x = A[(long)(k + (in ? r : 0)) * lda + k + (in ? c : 0)];
+            v[i] = in ? x : (r == c ? 1.0 : 0.0);
         }
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int u = u0 + tid + 256 * i;
+        for (int i = 0; i < NL; ++i) {
+            const int u = tid + 256 * i;
             S[u / PB][u % PB] = v[i];
         }
     }
     __syncthreads();
     const int lane = tid & 63, li = lane & 15, lk = lane >> 4;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    for (int kb = 0; kb < ND; ++kb) {
-        const int o = kb * DB;
-        // (a) wave 0: the 16 x 16 diagonal block, lane r holding row o + r
-        // (lanes 16..63 repeat rows 0..15), then its inverse, lane b forming
-        // column b of X_kk
-        if (wave == 0) {
-            const int r = lane & 15;
-            double d[DB], il_[DB];
-#pragma unroll
-            for (int c = 0; c < DB; ++c) d[c] = S[o + r][o + c];
-#pragma unroll
-            for (int j = 0; j < DB; ++j) {
-                const double djj = lane_bcast(d[j], j);
-                if (lane == 0 && !(djj > 0.0) && *info == 0) *info = k + o + j + 1;
-                // 1/sqrt from v_rsq_f64 and two Newton steps (each doubles
-                // the correct bits), l = d * (1/sqrt d): no IEEE sqrt and
-                // division sequences on the column's dependent path
-                const double dd = djj > 0.0 ? djj : 1.0;
-                double il = __builtin_amdgcn_rsq(dd);
-                il = il * (1.5 - 0.5 * dd * il * il);
-                il = il * (1.5 - 0.5 * dd * il * il);
-                il_[j] = il;
-                if (lane == 0) dg[o + j] = il;
-                const double l = dd * il;
-                if (r == j) d[j] = l;
-                else if (r > j) d[j] = d[j] * il;
-#pragma unroll
-                for (int c = j + 1; c < DB; ++c) {
-                    const double lcj = lane_bcast(d[j], c);   // L[o + c][o + j]
-                    if (c <= r) d[c] -= d[j] * lcj;
-                }
-            }
-            if (lane < DB) {
-#pragma unroll
-                for (int c = 0; c < DB; ++c)
-                    if (c <= r) S[o + r][o + c] = d[c];
-            }
-            // X[a][b] = -X[a][a] sum_{b <= m < a} L[a][m] X[m][b]
-            const int b = r;
-            double x[DB];
-#pragma unroll
-            for (int a = 0; a < DB; ++a) {
-                double s = 0.0;
-#pragma unroll
-                for (int m = 0; m < a; ++m) s += lane_bcast(d[m], a) * x[m];   // L[o + a][o + m]
-                x[a] = a < b ? 0.0 : (a == b ? il_[a] : -il_[a] * s);
-            }
-            if (lane < DB) {
-#pragma unroll
-                for (int a = 0; a < DB; ++a)
-                    if (a > b) S[o + b][o + a] = x[a];   // X[a][b] at S[C][R]
-            }
-        }
-        __syncthreads();
-        const int t0 = o + DB, nrb = ND - 1 - kb;
-        if (nrb == 0) break;
+    // Step kb: (b) the rows below the block, then (c) the trailing update,
+    // in which wave 0 updates the next diagonal block first and factors it
+    // ((a) of step kb + 1) while waves 1-3 update the other blocks.
+    if (wave == 0) diag16(S, dg, colb, info, k, 0, lane);
+    __syncthreads();
+    for (int kb = 0; kb + 1 < ND; ++kb) {
+        const int o = kb * DB, t0 = o + DB, nrb = ND - 1 - kb;
         // (b) the rows below: L21 = A21 X_kk^T, a 16-row block a wave turn
         // (A operand: lane (i = li, k = lk); B: (k = lk, j = li); result
         // element r at row lk + 4 r, column li)
@@ -213,26 +256,15 @@ __global__ __launch_bounds__(256) void k_chol_diag(const double *__restrict__ A,
             }
         }
         __syncthreads();
-        // (c) trailing lower blocks (ib >= jb > kb) -= L(ib, kb) L(jb, kb)^T
-        {
+        // (c) trailing lower blocks (ib >= jb > kb) -= L(ib, kb) L(jb, kb)^T;
+        // block 0 is the next diagonal block
+        if (wave == 0) {
+            trail16(S, o, t0, 0, li, lk);
+            __builtin_amdgcn_wave_barrier();
+            diag16(S, dg, colb, info, k, t0, lane);
+        } else {
             const int nblk = nrb * (nrb + 1) / 2;
-            for (int u = wave; u < nblk; u += 4) {
-                int ti = 0;
-                while ((ti + 1) * (ti + 2) / 2 <= u) ++ti;
-                const int tj = u - ti * (ti + 1) / 2;
-                const int ra = t0 + DB * ti, rc = t0 + DB * tj;
-                double av[4], bv[4];
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    av[c] = S[ra + li][o + 4 * c + lk];
-                    bv[c] = S[rc + li][o + 4 * c + lk];
-                }
-                dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                for (int c = 0; c < 4; ++c) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[c], bv[c], acc, 0, 0, 0);
-#pragma unroll
-                for (int rr = 0; rr < 4; ++rr) S[ra + lk + 4 * rr][rc + li] -= acc[rr];
-            }
+            for (int u = wave; u < nblk; u += 3) trail16(S, o, t0, u, li, lk);
         }
         __syncthreads();
     }
@@ -271,8 +303,7 @@ __global__ __launch_bounds__(256) void k_chol_diag(const double *__restrict__ A,
             __syncthreads();
         }
     }
-    // X and its transpose, each in its own row order, two adjacent elements
-    // a thread at a time
+    // X in its row order, two adjacent elements a thread at a time
     typedef double dbl2 __attribute__((ext_vector_type(2)));
     for (int u = 2 * tid; u < PB * PB; u += 512) {
         const int r = u / PB, c = u % PB;
@@ -281,13 +312,21 @@ __global__ __launch_bounds__(256) void k_chol_diag(const double *__restrict__ A,
         v[1] = r == c + 1 ? dg[r] : (r > c + 1 ? S[c + 1][r] : 0.0);   // X[r][c + 1]
         *reinterpret_cast<dbl2 *>(Linv + u) = v;
     }
-    for (int u = 2 * tid; u < PB * PB; u += 512) {
-        const int c = u / PB, r = u % PB;
-        dbl2 v;
-        v[0] = r == c ? dg[r] : (r > c ? S[c][r] : 0.0);                   // X[r][c] at LinvT[c][r]
-        v[1] = r + 1 == c ? dg[r + 1] : (r + 1 > c ? S[c][r + 1] : 0.0);   // X[r + 1][c]
-        *reinterpret_cast<dbl2 *>(LinvT + u) = v;
-    }
+}
+
+// LinvT[p] = Linv[p]^T for every diagonal block p (blockIdx.y), a 32 x 32
+// tile a workgroup (blockIdx.x: tile row * 4 + tile column) through LDS
+__global__ __launch_bounds__(256) void k_linv_transpose(const double *__restrict__ Linv, double *__restrict__ LinvT)
+{
+    __shared__ double t[32][33];
+    const long base = (long)blockIdx.y * PB * PB;
+    const int bi = blockIdx.x >> 2, bj = blockIdx.x & 3;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+#pragma unroll
+    for (int q = 0; q < 32; q += 8) t[ty + q][tx] = Linv[base + (long)(32 * bi + ty + q) * PB + 32 * bj + tx];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 32; q += 8) LinvT[base + (long)(32 * bj + ty + q) * PB + 32 * bi + tx] = t[tx][ty + q];
 }
 
 // L21 = A21 Linv^T for 32 rows per workgroup (rows k+nb.. of the panel's
@@ -363,21 +402,29 @@ __global__ __launch_bounds__(256) void k_chol_panel(double *__restrict__ A, long
 }
 
 // A22 -= L21 L21^T over lower 64 x 64 tiles of the trailing matrix (rows
-// and columns >= base); L21 packed in P[row][PB], nb columns used.  The
+// and columns >= base), all of them (nct = 0) or those of the first nct
+// tile columns; L21 packed in P[row][PB], nb columns used.  The
 // panel goes through LDS in 32-column chunks, stored transposed with rows
 // padded to 65 doubles (the transposing stores of one wave then fall in
 // distinct banks); the next chunk's values are loaded into registers while
 // the current chunk is multiplied.
-__global__ __launch_bounds__(256) void k_chol_syrk(double *__restrict__ A, long lda, const double *__restrict__ P,
-                                                   int base, int nb, int n)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_chol_syrk(double *__restrict__ A, long lda, const double *__restrict__ P,
+                                                   int base, int nb, int n, int nct)
 {
     __shared__ double Pi[32][65];
     __shared__ double Pj[32][65];
     const long t = blockIdx.x;
-    int ti = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) / 2.0);
-    while ((long)(ti + 1) * (ti + 2) / 2 <= t) ++ti;
-    while ((long)ti * (ti + 1) / 2 > t) --ti;
-    const int tj = (int)(t - (long)ti * (ti + 1) / 2);
+    int ti, tj;
+    if (nct > 0) {   // the first nct tile columns only, every tile row
+        ti = (int)(t / nct);
+        tj = (int)(t % nct);
+        if (ti < tj) return;
+    } else {         // the whole lower triangle of tiles
+        ti = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) / 2.0);
+        while ((long)(ti + 1) * (ti + 2) / 2 <= t) ++ti;
+        while ((long)ti * (ti + 1) / 2 > t) --ti;
+        tj = (int)(t - (long)ti * (ti + 1) / 2);
+    }
     const int r0 = base + ti * 64, c0 = base + tj * 64;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
@@ -399,6 +446,20 @@ __global__ __launch_bounds__(256) void k_chol_syrk(double *__restrict__ A, long 
         }
     };
     fetch(0);
+    // the tile of A, loaded now so that the subtraction at the end waits for
+    // nothing (unconditional loads: outside the triangle, the tile's corner)
+    double cv[2][2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gr = r0 + wm * 32 + i * 16 + (lane >> 4) + 4 * r;
+                const int gc = c0 + wn * 32 + jj * 16 + (lane & 15);
+                const bool in = gr < n && gc <= gr;
+                cv[i][jj][r] = A[(long)(in ? gr : r0) * lda + (in ? gc : c0)];
+            }
     for (int k0 = 0; k0 < nb; k0 += 32) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
@@ -431,7 +492,7 @@ __global__ __launch_bounds__(256) void k_chol_syrk(double *__restrict__ A, long 
             for (int r = 0; r < 4; ++r) {
                 const int gr = r0 + wm * 32 + i * 16 + (lane >> 4) + 4 * r;
                 const int gc = c0 + wn * 32 + jj * 16 + (lane & 15);
-                if (gr < n && gc <= gr) A[(long)gr * lda + gc] -= acc[i][jj][r];
+                if (gr < n && gc <= gr) A[(long)gr * lda + gc] = cv[i][jj][r] - acc[i][jj][r];
             }
 }
 
@@ -562,32 +623,98 @@ __global__ void k_clear_info(int *info) { *info = 0; }
 long ngd_work_doubles(int n)
 {
     const long np = (n + PB - 1) / PB;
-    return 2 * np * PB * PB + (long)n * PB;
+    return 2 * np * PB * PB + 2L * n * PB;
 }
 
+// The trailing update's own stream leaves NGD_FREE_CUS compute units out of
+// its CU mask, so that the diagonal block of the next panel (one workgroup
+// with 130 KB of LDS, which finds no room on a CU running update tiles)
+// always has a CU to start on while the update of the previous panel runs.
+static constexpr int NGD_FREE_CUS = 8;
+
+int ngd_lookahead_init(NgdLookahead &la)
+{
+    if (la.u) return 0;
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
+        return -3;
+    const int use = ncu > 4 * NGD_FREE_CUS ? ncu - NGD_FREE_CUS : ncu;
+    std::vector<uint32_t> mask((size_t)(ncu + 31) / 32, 0u);
+    for (int c = 0; c < use; ++c) mask[(size_t)c >> 5] |= 1u << (c & 31);
+    if (hipExtStreamCreateWithCUMask(&la.u, (uint32_t)mask.size(), mask.data()) != hipSuccess) {
+        (void)hipGetLastError();   // no CU masking: an ordinary stream still overlaps
+        la.u = nullptr;
+        if (hipStreamCreateWithFlags(&la.u, hipStreamNonBlocking) != hipSuccess) return -3;
+    }
+    if (hipEventCreateWithFlags(&la.next, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&la.rest, hipEventDisableTiming) != hipSuccess) {
+        ngd_lookahead_free(la);
+        return -3;
+    }
+    return 0;
+}
+
+void ngd_lookahead_free(NgdLookahead &la)
+{
+    if (la.next) (void)hipEventDestroy(la.next);
+    if (la.rest) (void)hipEventDestroy(la.rest);
+    if (la.u) (void)hipStreamDestroy(la.u);
+    la = NgdLookahead{};
+}
+
+// Panel p on stream s: diagonal block, panel, then the update of the next
+// panel's block column only (syrk with nct tile columns); the rest of the
+// trailing update runs on la.u, beside the next panel's diagonal block and
+// panel.  The next block column's update waits for the previous panel's
+// rest (both write it); the panels alternate between two packed buffers, so
+// that panel p + 1 never writes the one the update of panel p still reads.
+// Without la every step runs on s in order.
 int ngd_factor_solve(double *G, int n, long lda, double nb, double mu, int mode, double *b, double *trace,
-                     int *info, double *work, hipStream_t s)
+                     int *info, double *work, hipStream_t s, NgdLookahead *la)
 {
     const int np = (n + PB - 1) / PB;
-    double *Linv = work, *LinvT = work + (long)np * PB * PB, *P = work + 2 * (long)np * PB * PB;
+    double *Linv = work, *LinvT = work + (long)np * PB * PB, *P0 = work + 2 * (long)np * PB * PB;
     hipLaunchKernelGGL(k_clear_info, dim3(1), dim3(1), 0, s, info);
     hipLaunchKernelGGL(k_trace, dim3(1), dim3(256), 0, s, G, n, lda, trace);
     hipLaunchKernelGGL(k_damp, dim3((unsigned)((n + 255) / 256), (unsigned)n), dim3(256), 0, s, G, n, lda, nb, mu,
                        mode, trace, b);
+    bool rest_pending = false;   // la->rest recorded after an update still to be waited for
     for (int p = 0; p < np; ++p) {
         const int k = p * PB, nb_ = std::min(PB, n - k);
         double *Li = Linv + (long)p * PB * PB;
-        hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(256), 0, s, G, lda, k, nb_, Li, LinvT + (long)p * PB * PB,
-                           info);
+        double *P = P0 + (long)(p & 1) * n * PB;
+        hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(256), 0, s, G, lda, k, nb_, Li, info);
         const int rest = n - k - nb_;
-        if (rest > 0) {
-            hipLaunchKernelGGL(k_chol_panel, dim3((unsigned)((rest + PR - 1) / PR)), dim3(256), 0, s, G, lda, k, nb_,
-                               n, (const double *)Li, P);
-            const long T = (rest + 63) / 64;
+        if (rest <= 0) break;
+        hipLaunchKernelGGL(k_chol_panel, dim3((unsigned)((rest + PR - 1) / PR)), dim3(256), 0, s, G, lda, k, nb_, n,
+                           (const double *)Li, P);
+        const long T = (rest + 63) / 64;
+        if (!la) {
             hipLaunchKernelGGL(k_chol_syrk, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, s, G, lda,
-                               (const double *)P, k + nb_, nb_, n);
+                               (const double *)P, k + nb_, nb_, n, 0);
+            continue;
+        }
+        const int nbn = std::min(PB, rest), nct = (nbn + 63) / 64;
+        if (rest_pending) {
+            if (hipStreamWaitEvent(s, la->rest, 0) != hipSuccess) return -3;
+            rest_pending = false;
+        }
+        hipLaunchKernelGGL(k_chol_syrk, dim3((unsigned)(T * nct)), dim3(256), 0, s, G, lda, (const double *)P,
+                           k + nb_, nb_, n, nct);
+        const int rest2 = rest - nbn;
+        if (rest2 > 0) {
+            const long T2 = (rest2 + 63) / 64;
+            if (hipEventRecord(la->next, s) != hipSuccess || hipStreamWaitEvent(la->u, la->next, 0) != hipSuccess)
+                return -3;
+            hipLaunchKernelGGL(k_chol_syrk, dim3((unsigned)(T2 * (T2 + 1) / 2)), dim3(256), 0, la->u, G, lda,
+                               (const double *)P, k + nb_ + nbn, nb_, n, 0);
+            if (hipEventRecord(la->rest, la->u) != hipSuccess) return -3;
+            rest_pending = true;
         }
     }
+    if (rest_pending && hipStreamWaitEvent(s, la->rest, 0) != hipSuccess) return -3;
+    hipLaunchKernelGGL(k_linv_transpose, dim3(16, (unsigned)np), dim3(256), 0, s, (const double *)Linv, LinvT);
     // L y = b, then L^T u = y (in place in b)
     hipLaunchKernelGGL(k_chol_solve_first, dim3(1), dim3(256), 0, s, (const double *)Linv, 0, std::min(PB, n), b);
     for (int p = 0; p + 1 < np; ++p) {
