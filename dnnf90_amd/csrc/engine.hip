@@ -425,7 +425,6 @@ struct BatchWork {
     double *comm_buf = nullptr;      // committee: running mean and M2
     long comm_cap = 0;
     int *ngd_info = nullptr;
-    NgdLookahead ngd_la;   // trailing-update stream of the NGD factorization
     std::vector<double> h_coeff;
     std::vector<int> h_slot, h_nonlin;
     bool fwd_valid = false;
@@ -473,7 +472,6 @@ struct BatchWork {
         dfree(partial); dfree(d_x); dfree(d_seed); dfree(d_tout); dfree(d_rsq); dfree(d_rsq_part);
         dfree(d_coeff); dfree(d_slot); dfree(d_nonlin);
         dfree(ngd_G); dfree(ngd_part); dfree(ngd_u); dfree(ngd_trace); dfree(ngd_info); dfree(ngd_work);
-        ngd_lookahead_free(ngd_la);
         dfree(ngd_J); dfree(ngd_v); dfree(comm_buf);
         if (own) (void)hipStreamDestroy(own);
         own = stream = nullptr;
@@ -2278,7 +2276,6 @@ int dnnf90_ngd_solve(int nid, int bid, int nrow, const float *rows_t, long ld, c
     if ((rc = grow(&bw.ngd_u, &bw.ngd_u_cap, (long)nw))) return rc;
     if ((rc = grow(&bw.ngd_work, &bw.ngd_work_cap, ngd_work_doubles(n)))) return rc;
     if (!bw.ngd_trace && ((rc = dalloc(&bw.ngd_trace, 1)) || (rc = dalloc(&bw.ngd_info, 1)))) return rc;
-    if (ngd_lookahead_init(bw.ngd_la)) return fail(-3, "NGD update stream creation failed");
     if (dual) {
         if ((rc = grow(&bw.ngd_J, &bw.ngd_J_cap, (long)nrow * ldj))) return rc;
         if ((rc = grow(&bw.ngd_v, &bw.ngd_v_cap, 2L * nw + nrow))) return rc;
@@ -2294,7 +2291,7 @@ int dnnf90_ngd_solve(int nid, int bid, int nrow, const float *rows_t, long ld, c
                               dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
         PScope p(bw, CAT_OTHER);
         if (ngd_factor_solve(bw.ngd_G, nw, nw, (double)nb, mu, mode, bw.ngd_u, bw.ngd_trace, bw.ngd_info,
-                             bw.ngd_work, s, &bw.ngd_la))
+                             bw.ngd_work, s))
             return fail(-3, "NGD factorization launch failed");
     } else {
         double *db = bw.ngd_v, *jv = bw.ngd_v + nw, *ry = bw.ngd_v + nw + nrow;
@@ -2307,7 +2304,7 @@ int dnnf90_ngd_solve(int nid, int bid, int nrow, const float *rows_t, long ld, c
         PScope p(bw, CAT_OTHER);
         launch_rows_matvec(bw.ngd_J, ldj, nrow, nw, db, jv, s);          // J b
         if (ngd_factor_solve(bw.ngd_G, nrow, nrow, (double)nb, mu, mode, jv, bw.ngd_trace, bw.ngd_info,
-                             bw.ngd_work, s, &bw.ngd_la))                // y = K^-1 J b / nb
+                             bw.ngd_work, s))                // y = K^-1 J b / nb
             return fail(-3, "NGD factorization launch failed");
         launch_rows_matvec(rows_t, ld, nw, nrow, jv, ry, s);            // R y = J^T y
         launch_dual_combine(nw, db, ry, (double)nb, mu, mode, bw.ngd_trace, bw.ngd_u, s);

@@ -153,17 +153,8 @@ __host__ __device__ inline int grad_reduce_splits(int nchunk) { return nchunk >=
 // NGD: G <- G/nb + cI, b <- b/nb, then G = L L^T (FP64 blocked Cholesky, in
 // place, lower) and b <- (L L^T)^{-1} b; *info > 0 flags a non-positive
 // pivot (1-based column); trace is one device double of scratch
-// The trailing update's stream of the factorization (CU-masked, so that a
-// few CUs stay free for the next diagonal block) and its two events; with
-// la == nullptr every step runs on s in order
-struct NgdLookahead {
-    hipStream_t u = nullptr;
-    hipEvent_t next = nullptr, rest = nullptr;
-};
-int ngd_lookahead_init(NgdLookahead &la);   // on the current device; 0 or -3
-void ngd_lookahead_free(NgdLookahead &la);
 int ngd_factor_solve(double *G, int n, long lda, double nb, double mu, int mode, double *b, double *trace,
-                     int *info, double *work, hipStream_t s, NgdLookahead *la);
+                     int *info, double *work, hipStream_t s);
 // NGD dual route helpers (ngd.hip): Jm[n][w] = R[w][n] widened to FP64 with
 // zero columns up to ldj; out = M x with FP64 sums; u = (b - R y)/nb / c
 void launch_rows_transpose(const float *R, long ldr, int nw, int nrow, double *Jm, long ldj, hipStream_t s);
@@ -176,7 +167,7 @@ void launch_gram64(const double *R, int n, long ld, double *G, hipStream_t s);
 void launch_dual_combine(int nw, const double *b, const double *ry, double nb, double mu, int mode,
                          const double *trace, double *u, hipStream_t s);
 // doubles of device work space ngd_factor_solve needs for order n (the
-// inverses of the diagonal blocks, their transposes and two packed panels)
+// inverses of the diagonal blocks, their transposes and one packed panel)
 long ngd_work_doubles(int n);
 
 // G[i][j] (+)= sum_n R[i][n] R[j][n] for the transposed FP32 rows R[nw][ld],

@@ -25,7 +25,6 @@
 //                 update), and the workgroup that owns the next diagonal
 //                 block solves it with that block's Linv
 #include <algorithm>
-#include <vector>
 
 #include "kernels.hpp"
 
@@ -402,29 +401,21 @@ __global__ __launch_bounds__(256) void k_chol_panel(double *__restrict__ A, long
 }
 
 // A22 -= L21 L21^T over lower 64 x 64 tiles of the trailing matrix (rows
-// and columns >= base), all of them (nct = 0) or those of the first nct
-// tile columns; L21 packed in P[row][PB], nb columns used.  The
+// and columns >= base); L21 packed in P[row][PB], nb columns used.  The
 // panel goes through LDS in 32-column chunks, stored transposed with rows
 // padded to 65 doubles (the transposing stores of one wave then fall in
 // distinct banks); the next chunk's values are loaded into registers while
 // the current chunk is multiplied.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_chol_syrk(double *__restrict__ A, long lda, const double *__restrict__ P,
-                                                   int base, int nb, int n, int nct)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_chol_syrk(
+    double *__restrict__ A, long lda, const double *__restrict__ P, int base, int nb, int n)
 {
     __shared__ double Pi[32][65];
     __shared__ double Pj[32][65];
     const long t = blockIdx.x;
-    int ti, tj;
-    if (nct > 0) {   // the first nct tile columns only, every tile row
-        ti = (int)(t / nct);
-        tj = (int)(t % nct);
-        if (ti < tj) return;
-    } else {         // the whole lower triangle of tiles
-        ti = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) / 2.0);
-        while ((long)(ti + 1) * (ti + 2) / 2 <= t) ++ti;
-        while ((long)ti * (ti + 1) / 2 > t) --ti;
-        tj = (int)(t - (long)ti * (ti + 1) / 2);
-    }
+    int ti = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) / 2.0);
+    while ((long)(ti + 1) * (ti + 2) / 2 <= t) ++ti;
+    while ((long)ti * (ti + 1) / 2 > t) --ti;
+    const int tj = (int)(t - (long)ti * (ti + 1) / 2);
     const int r0 = base + ti * 64, c0 = base + tj * 64;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
@@ -623,97 +614,30 @@ __global__ void k_clear_info(int *info) { *info = 0; }
 long ngd_work_doubles(int n)
 {
     const long np = (n + PB - 1) / PB;
-    return 2 * np * PB * PB + 2L * n * PB;
+    return 2 * np * PB * PB + (long)n * PB;
 }
 
-// The trailing update's own stream leaves NGD_FREE_CUS compute units out of
-// its CU mask, so that the diagonal block of the next panel (one workgroup
-// with 130 KB of LDS, which finds no room on a CU running update tiles)
-// always has a CU to start on while the update of the previous panel runs.
-static constexpr int NGD_FREE_CUS = 8;
-
-int ngd_lookahead_init(NgdLookahead &la)
-{
-    if (la.u) return 0;
-    int dev = 0, ncu = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
-        return -3;
-    const int use = ncu > 4 * NGD_FREE_CUS ? ncu - NGD_FREE_CUS : ncu;
-    std::vector<uint32_t> mask((size_t)(ncu + 31) / 32, 0u);
-    for (int c = 0; c < use; ++c) mask[(size_t)c >> 5] |= 1u << (c & 31);
-    if (hipExtStreamCreateWithCUMask(&la.u, (uint32_t)mask.size(), mask.data()) != hipSuccess) {
-        (void)hipGetLastError();   // no CU masking: an ordinary stream still overlaps
-        la.u = nullptr;
-        if (hipStreamCreateWithFlags(&la.u, hipStreamNonBlocking) != hipSuccess) return -3;
-    }
-    if (hipEventCreateWithFlags(&la.next, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&la.rest, hipEventDisableTiming) != hipSuccess) {
-        ngd_lookahead_free(la);
-        return -3;
-    }
-    return 0;
-}
-
-void ngd_lookahead_free(NgdLookahead &la)
-{
-    if (la.next) (void)hipEventDestroy(la.next);
-    if (la.rest) (void)hipEventDestroy(la.rest);
-    if (la.u) (void)hipStreamDestroy(la.u);
-    la = NgdLookahead{};
-}
-
-// Panel p on stream s: diagonal block, panel, then the update of the next
-// panel's block column only (syrk with nct tile columns); the rest of the
-// trailing update runs on la.u, beside the next panel's diagonal block and
-// panel.  The next block column's update waits for the previous panel's
-// rest (both write it); the panels alternate between two packed buffers, so
-// that panel p + 1 never writes the one the update of panel p still reads.
-// Without la every step runs on s in order.
 int ngd_factor_solve(double *G, int n, long lda, double nb, double mu, int mode, double *b, double *trace,
-                     int *info, double *work, hipStream_t s, NgdLookahead *la)
+                     int *info, double *work, hipStream_t s)
 {
     const int np = (n + PB - 1) / PB;
-    double *Linv = work, *LinvT = work + (long)np * PB * PB, *P0 = work + 2 * (long)np * PB * PB;
+    double *Linv = work, *LinvT = work + (long)np * PB * PB, *P = work + 2 * (long)np * PB * PB;
     hipLaunchKernelGGL(k_clear_info, dim3(1), dim3(1), 0, s, info);
     hipLaunchKernelGGL(k_trace, dim3(1), dim3(256), 0, s, G, n, lda, trace);
     hipLaunchKernelGGL(k_damp, dim3((unsigned)((n + 255) / 256), (unsigned)n), dim3(256), 0, s, G, n, lda, nb, mu,
                        mode, trace, b);
-    bool rest_pending = false;   // la->rest recorded after an update still to be waited for
     for (int p = 0; p < np; ++p) {
         const int k = p * PB, nb_ = std::min(PB, n - k);
         double *Li = Linv + (long)p * PB * PB;
-        double *P = P0 + (long)(p & 1) * n * PB;
         hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(256), 0, s, G, lda, k, nb_, Li, info);
         const int rest = n - k - nb_;
         if (rest <= 0) break;
         hipLaunchKernelGGL(k_chol_panel, dim3((unsigned)((rest + PR - 1) / PR)), dim3(256), 0, s, G, lda, k, nb_, n,
                            (const double *)Li, P);
         const long T = (rest + 63) / 64;
-        if (!la) {
-            hipLaunchKernelGGL(k_chol_syrk, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, s, G, lda,
-                               (const double *)P, k + nb_, nb_, n, 0);
-            continue;
-        }
-        const int nbn = std::min(PB, rest), nct = (nbn + 63) / 64;
-        if (rest_pending) {
-            if (hipStreamWaitEvent(s, la->rest, 0) != hipSuccess) return -3;
-            rest_pending = false;
-        }
-        hipLaunchKernelGGL(k_chol_syrk, dim3((unsigned)(T * nct)), dim3(256), 0, s, G, lda, (const double *)P,
-                           k + nb_, nb_, n, nct);
-        const int rest2 = rest - nbn;
-        if (rest2 > 0) {
-            const long T2 = (rest2 + 63) / 64;
-            if (hipEventRecord(la->next, s) != hipSuccess || hipStreamWaitEvent(la->u, la->next, 0) != hipSuccess)
-                return -3;
-            hipLaunchKernelGGL(k_chol_syrk, dim3((unsigned)(T2 * (T2 + 1) / 2)), dim3(256), 0, la->u, G, lda,
-                               (const double *)P, k + nb_ + nbn, nb_, n, 0);
-            if (hipEventRecord(la->rest, la->u) != hipSuccess) return -3;
-            rest_pending = true;
-        }
+        hipLaunchKernelGGL(k_chol_syrk, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, s, G, lda,
+                           (const double *)P, k + nb_, nb_, n);
     }
-    if (rest_pending && hipStreamWaitEvent(s, la->rest, 0) != hipSuccess) return -3;
     hipLaunchKernelGGL(k_linv_transpose, dim3(16, (unsigned)np), dim3(256), 0, s, (const double *)Linv, LinvT);
     // L y = b, then L^T u = y (in place in b)
     hipLaunchKernelGGL(k_chol_solve_first, dim3(1), dim3(256), 0, s, (const double *)Linv, 0, std::min(PB, n), b);
