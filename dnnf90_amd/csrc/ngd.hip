@@ -336,7 +336,8 @@ __global__ __launch_bounds__(256) void k_linv_transpose(const double *__restrict
 static constexpr int PR = 32;   // rows a workgroup
 
 __global__ __launch_bounds__(256) void k_chol_panel(double *__restrict__ A, long lda, int k, int nb, int n,
-                                                    const double *__restrict__ Linv, double *__restrict__ P)
+                                                    const double *__restrict__ Linv, double *__restrict__ P,
+                                                    int ldp)
 {
     __shared__ double Ar[PR][PBL];
     const int tid = threadIdx.x, lane = tid & 63;
@@ -395,27 +396,36 @@ __global__ __launch_bounds__(256) void k_chol_panel(double *__restrict__ A, long
             const int c = 16 * (NJ * hc + jb) + p;
             if (gr < n && c < nb) {
                 A[(long)gr * lda + k + c] = acc[jb][r];
-                P[(long)gr * PB + c] = acc[jb][r];
+                P[(long)gr * ldp + c] = acc[jb][r];
             }
         }
 }
 
 // A22 -= L21 L21^T over lower 64 x 64 tiles of the trailing matrix (rows
-// and columns >= base); L21 packed in P[row][PB], nb columns used.  The
+// and columns >= base), all of them (nct = 0) or those of the first nct
+// tile columns; L21 packed in P[row][ldp], nb columns used (one panel, or
+// two panels' L21 side by side).  The
 // panel goes through LDS in 32-column chunks, stored transposed with rows
 // padded to 65 doubles (the transposing stores of one wave then fall in
 // distinct banks); the next chunk's values are loaded into registers while
 // the current chunk is multiplied.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_chol_syrk(
-    double *__restrict__ A, long lda, const double *__restrict__ P, int base, int nb, int n)
+    double *__restrict__ A, long lda, const double *__restrict__ P, int ldp, int base, int nb, int n, int nct)
 {
     __shared__ double Pi[32][65];
     __shared__ double Pj[32][65];
     const long t = blockIdx.x;
-    int ti = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) / 2.0);
-    while ((long)(ti + 1) * (ti + 2) / 2 <= t) ++ti;
-    while ((long)ti * (ti + 1) / 2 > t) --ti;
-    const int tj = (int)(t - (long)ti * (ti + 1) / 2);
+    int ti, tj;
+    if (nct > 0) {   // the first nct tile columns only, every tile row
+        ti = (int)(t / nct);
+        tj = (int)(t % nct);
+        if (ti < tj) return;
+    } else {         // the whole lower triangle of tiles
+        ti = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) / 2.0);
+        while ((long)(ti + 1) * (ti + 2) / 2 <= t) ++ti;
+        while ((long)ti * (ti + 1) / 2 > t) --ti;
+        tj = (int)(t - (long)ti * (ti + 1) / 2);
+    }
     const int r0 = base + ti * 64, c0 = base + tj * 64;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
@@ -432,8 +442,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const int r = r_t + 8 * q;
-            vi[q] = (ok && r0 + r < n) ? P[(long)(r0 + r) * PB + k0 + kk_t] : 0.0;
-            vj[q] = (ok && c0 + r < n) ? P[(long)(c0 + r) * PB + k0 + kk_t] : 0.0;
+            vi[q] = (ok && r0 + r < n) ? P[(long)(r0 + r) * ldp + k0 + kk_t] : 0.0;
+            vj[q] = (ok && c0 + r < n) ? P[(long)(c0 + r) * ldp + k0 + kk_t] : 0.0;
         }
     };
     fetch(0);
@@ -614,29 +624,43 @@ __global__ void k_clear_info(int *info) { *info = 0; }
 long ngd_work_doubles(int n)
 {
     const long np = (n + PB - 1) / PB;
-    return 2 * np * PB * PB + (long)n * PB;
+    return 2 * np * PB * PB + 2L * n * PB;
 }
 
+// Panels in pairs: the first panel of a pair updates only the next block
+// column (the second panel's), the second panel then updates the trailing
+// matrix beyond it with both panels' L21 at once (a 256-column update):
+// half the passes over the trailing triangle, twice the products per tile
+// of it.  Same operations on every element as one update a panel, summed
+// in another order.
 int ngd_factor_solve(double *G, int n, long lda, double nb, double mu, int mode, double *b, double *trace,
                      int *info, double *work, hipStream_t s)
 {
     const int np = (n + PB - 1) / PB;
+    constexpr int LDP = 2 * PB;
     double *Linv = work, *LinvT = work + (long)np * PB * PB, *P = work + 2 * (long)np * PB * PB;
     hipLaunchKernelGGL(k_clear_info, dim3(1), dim3(1), 0, s, info);
     hipLaunchKernelGGL(k_trace, dim3(1), dim3(256), 0, s, G, n, lda, trace);
     hipLaunchKernelGGL(k_damp, dim3((unsigned)((n + 255) / 256), (unsigned)n), dim3(256), 0, s, G, n, lda, nb, mu,
                        mode, trace, b);
     for (int p = 0; p < np; ++p) {
-        const int k = p * PB, nb_ = std::min(PB, n - k);
+        const int k = p * PB, nb_ = std::min(PB, n - k), half = p & 1;
         double *Li = Linv + (long)p * PB * PB;
         hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(256), 0, s, G, lda, k, nb_, Li, info);
         const int rest = n - k - nb_;
         if (rest <= 0) break;
         hipLaunchKernelGGL(k_chol_panel, dim3((unsigned)((rest + PR - 1) / PR)), dim3(256), 0, s, G, lda, k, nb_, n,
-                           (const double *)Li, P);
+                           (const double *)Li, P + half * PB, LDP);
         const long T = (rest + 63) / 64;
-        hipLaunchKernelGGL(k_chol_syrk, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, s, G, lda,
-                           (const double *)P, k + nb_, nb_, n);
+        if (half == 0 && rest > PB) {   // the next block column only
+            const int nct = PB / 64;
+            hipLaunchKernelGGL(k_chol_syrk, dim3((unsigned)(T * nct)), dim3(256), 0, s, G, lda, (const double *)P,
+                               LDP, k + nb_, nb_, n, nct);
+        } else {                        // everything beyond this block, with this panel (and the pair's first)
+            hipLaunchKernelGGL(k_chol_syrk, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, s, G, lda,
+                               (const double *)P, LDP, k + nb_, half ? PB + nb_ : nb_, n,
+                               0);
+        }
     }
     hipLaunchKernelGGL(k_linv_transpose, dim3(16, (unsigned)np), dim3(256), 0, s, (const double *)Linv, LinvT);
     // L y = b, then L^T u = y (in place in b)
