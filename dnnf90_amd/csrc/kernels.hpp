@@ -167,7 +167,8 @@ void launch_gram64(const double *R, int n, long ld, double *G, hipStream_t s);
 void launch_dual_combine(int nw, const double *b, const double *ry, double nb, double mu, int mode,
                          const double *trace, double *u, hipStream_t s);
 // doubles of device work space ngd_factor_solve needs for order n (the
-// inverses of the diagonal blocks, their transposes and two packed panels)
+// inverses of the diagonal blocks, their transposes and a group of packed
+// panels)
 long ngd_work_doubles(int n);
 
 // G[i][j] (+)= sum_n R[i][n] R[j][n] for the transposed FP32 rows R[nw][ld],

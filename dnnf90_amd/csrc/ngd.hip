@@ -621,46 +621,51 @@ __global__ __launch_bounds__(64 * SW) void k_chol_bwd(const double *__restrict__
 
 __global__ void k_clear_info(int *info) { *info = 0; }
 
+// Panels in groups of NGD_GROUP: a panel of a group updates only the block
+// columns of the group's later panels; the last panel of the group updates
+// the trailing matrix beyond the group with every panel's L21 of the group
+// at once (an update over NGD_GROUP x 128 columns): fewer passes over the
+// trailing triangle, more products per tile of it.  Same operations on
+// every element as one update a panel, summed in another order.  Where the
+// matrix ends inside a group, each of its panels updates all that is left.
+static constexpr int NGD_GROUP = 4;
+
 long ngd_work_doubles(int n)
 {
     const long np = (n + PB - 1) / PB;
-    return 2 * np * PB * PB + 2L * n * PB;
+    return 2 * np * PB * PB + (long)NGD_GROUP * n * PB;
 }
 
-// Panels in pairs: the first panel of a pair updates only the next block
-// column (the second panel's), the second panel then updates the trailing
-// matrix beyond it with both panels' L21 at once (a 256-column update):
-// half the passes over the trailing triangle, twice the products per tile
-// of it.  Same operations on every element as one update a panel, summed
-// in another order.
 int ngd_factor_solve(double *G, int n, long lda, double nb, double mu, int mode, double *b, double *trace,
                      int *info, double *work, hipStream_t s)
 {
     const int np = (n + PB - 1) / PB;
-    constexpr int LDP = 2 * PB;
+    constexpr int LDP = NGD_GROUP * PB;
     double *Linv = work, *LinvT = work + (long)np * PB * PB, *P = work + 2 * (long)np * PB * PB;
     hipLaunchKernelGGL(k_clear_info, dim3(1), dim3(1), 0, s, info);
     hipLaunchKernelGGL(k_trace, dim3(1), dim3(256), 0, s, G, n, lda, trace);
     hipLaunchKernelGGL(k_damp, dim3((unsigned)((n + 255) / 256), (unsigned)n), dim3(256), 0, s, G, n, lda, nb, mu,
                        mode, trace, b);
     for (int p = 0; p < np; ++p) {
-        const int k = p * PB, nb_ = std::min(PB, n - k), half = p & 1;
+        const int k = p * PB, nb_ = std::min(PB, n - k), q = p % NGD_GROUP;
         double *Li = Linv + (long)p * PB * PB;
         hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(256), 0, s, G, lda, k, nb_, Li, info);
         const int rest = n - k - nb_;
         if (rest <= 0) break;
+        double *Pq = P + q * PB;
         hipLaunchKernelGGL(k_chol_panel, dim3((unsigned)((rest + PR - 1) / PR)), dim3(256), 0, s, G, lda, k, nb_, n,
-                           (const double *)Li, P + half * PB, LDP);
+                           (const double *)Li, Pq, LDP);
         const long T = (rest + 63) / 64;
-        if (half == 0 && rest > PB) {   // the next block column only
-            const int nct = PB / 64;
-            hipLaunchKernelGGL(k_chol_syrk, dim3((unsigned)(T * nct)), dim3(256), 0, s, G, lda, (const double *)P,
-                               LDP, k + nb_, nb_, n, nct);
-        } else {                        // everything beyond this block, with this panel (and the pair's first)
+        const int later = (NGD_GROUP - 1 - q) * PB;   // columns of the group's later panels
+        if (rest <= later)        // the matrix ends inside the group: all that is left, this panel
             hipLaunchKernelGGL(k_chol_syrk, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, s, G, lda,
-                               (const double *)P, LDP, k + nb_, half ? PB + nb_ : nb_, n,
-                               0);
-        }
+                               (const double *)Pq, LDP, k + nb_, nb_, n, 0);
+        else if (later > 0)       // the group's later block columns only
+            hipLaunchKernelGGL(k_chol_syrk, dim3((unsigned)(T * (later / 64))), dim3(256), 0, s, G, lda,
+                               (const double *)Pq, LDP, k + nb_, nb_, n, later / 64);
+        else                      // beyond the group, with every panel of it
+            hipLaunchKernelGGL(k_chol_syrk, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, s, G, lda,
+                               (const double *)P, LDP, k + nb_, q * PB + nb_, n, 0);
     }
     hipLaunchKernelGGL(k_linv_transpose, dim3(16, (unsigned)np), dim3(256), 0, s, (const double *)Linv, LinvT);
     // L y = b, then L^T u = y (in place in b)
