@@ -198,6 +198,35 @@ __device__ __forceinline__ void trail16(double (*S)[PBL], int o, int t0, int u, 
     for (int rr = 0; rr < 4; ++rr) S[ra + lk + 4 * rr][rc + li] -= acc[rr];
 }
 
+// Block (ib, jb) of X = L^-1 (jb < ib), one wave: Y = sum_{jb <= mb < ib}
+// L_(ib,mb) X_(mb,jb) on v_mfma_f64_16x16x4f64 (A operand: lane (i = lane %
+// 16, k = lane / 16); B: (k = lane / 16, j = lane % 16); accumulator element
+// r: row lane / 16 + 4 r, column lane % 16 -- so Y's element c is the B
+// operand of k-chunk c of the next product), then X_(ib,jb) = -X_(ib,ib) Y
+// into its transposed slot S[C][R] (rows in block jb, columns in block ib).
+__device__ __forceinline__ void xblock(double (*S)[PBL], const double *dg, int ib, int jb, int li, int lk)
+{
+    dbl4 y = {0.0, 0.0, 0.0, 0.0};
+    for (int mb = jb; mb < ib; ++mb) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int kk = 4 * c + lk;
+            const double a = S[DB * ib + li][DB * mb + kk];   // L[16 ib + li][16 mb + kk]
+            const int Rb = DB * mb + kk, Cb = DB * jb + li;   // X[Rb][Cb]
+            const double b = mb == jb ? xinv(S, dg, Rb, Cb) : S[Cb][Rb];
+            y = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, y, 0, 0, 0);
+        }
+    }
+    dbl4 z = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const double a = xinv(S, dg, DB * ib + li, DB * ib + 4 * c + lk);   // X_(ib,ib)[li][4 c + lk]
+        z = __builtin_amdgcn_mfma_f64_16x16x4f64(a, y[c], z, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) S[DB * jb + li][DB * ib + lk + 4 * r] = -z[r];
+}
+
 __global__ __launch_bounds__(256) void k_chol_diag(const double *__restrict__ A, long lda, int k, int nb,
                                                    double *__restrict__ Linv, int *__restrict__ info)
 {
@@ -262,46 +291,19 @@ __global__ __launch_bounds__(256) void k_chol_diag(const double *__restrict__ A,
             __builtin_amdgcn_wave_barrier();
             diag16(S, dg, colb, info, k, t0, lane);
         } else {
-            const int nblk = nrb * (nrb + 1) / 2;
-            for (int u = wave; u < nblk; u += 3) trail16(S, o, t0, u, li, lk);
+            // block row kb of X (its blocks longest first), then the other
+            // trailing blocks
+            const int nblk = nrb * (nrb + 1) / 2, nitem = kb + nblk - 1;
+            for (int it = wave - 1; it < nitem; it += 3) {
+                if (it < kb) xblock(S, dg, kb, it, li, lk);
+                else trail16(S, o, t0, it - kb + 1, li, lk);
+            }
         }
         __syncthreads();
     }
-    // (e) block row ib = 1 .. 7: each wave takes blocks jb < ib in turn and
-    // forms Y = sum_{jb <= mb < ib} L_(ib,mb) X_(mb,jb) on
-    // v_mfma_f64_16x16x4f64 (A operand: lane (i = lane % 16, k = lane / 16);
-    // B: (k = lane / 16, j = lane % 16); accumulator element r: row
-    // lane / 16 + 4 r, column lane % 16 -- so Y's element c is the B operand
-    // of k-chunk c of the next product), then X_(ib,jb) = -X_(ib,ib) Y into
-    // its transposed slot S[C][R].  Those slots (rows in block jb, columns
-    // in block ib) are read by nothing in the same block row; the blocks of
-    // row ib are complete at the barrier that ends it.
-    {
-        for (int ib = 1; ib < ND; ++ib) {
-            for (int jb = wave; jb < ib; jb += 4) {
-                dbl4 y = {0.0, 0.0, 0.0, 0.0};
-                for (int mb = jb; mb < ib; ++mb) {
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) {
-                        const int kk = 4 * c + lk;
-                        const double a = S[DB * ib + li][DB * mb + kk];   // L[16 ib + li][16 mb + kk]
-                        const int Rb = DB * mb + kk, Cb = DB * jb + li;   // X[Rb][Cb]
-                        const double b = mb == jb ? xinv(S, dg, Rb, Cb) : S[Cb][Rb];
-                        y = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, y, 0, 0, 0);
-                    }
-                }
-                dbl4 z = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    const double a = xinv(S, dg, DB * ib + li, DB * ib + 4 * c + lk);   // X_(ib,ib)[li][4 c + lk]
-                    z = __builtin_amdgcn_mfma_f64_16x16x4f64(a, y[c], z, 0, 0, 0);
-                }
-#pragma unroll
-                for (int r = 0; r < 4; ++r) S[DB * jb + li][DB * ib + lk + 4 * r] = -z[r];
-            }
-            __syncthreads();
-        }
-    }
+    // the last block row of X (its diagonal block was factored in the last step)
+    for (int jb = wave; jb < ND - 1; jb += 4) xblock(S, dg, ND - 1, jb, li, lk);
+    __syncthreads();
     // X in its row order, two adjacent elements a thread at a time
     typedef double dbl2 __attribute__((ext_vector_type(2)));
     for (int u = 2 * tid; u < PB * PB; u += 512) {
