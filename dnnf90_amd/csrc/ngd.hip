@@ -120,6 +120,7 @@ __device__ __forceinline__ void diag16(double (*S)[PBL], double *dg, double *col
 {
     const int r = lane & 15, q = lane >> 4;
     double d[4], il_[DB];
+    int bad = -1;   // the block's first non-positive pivot
 #pragma unroll
     for (int m = 0; m < 4; ++m) d[m] = S[o + r][o + 4 * m + q];
 #pragma unroll
@@ -133,7 +134,7 @@ __device__ __forceinline__ void diag16(double (*S)[PBL], double *dg, double *col
 #pragma unroll
         for (int m = 0; m < 4; ++m) acj[m] = colb[4 * m + q];
         __builtin_amdgcn_wave_barrier();
-        if (lane == 0 && !(djj > 0.0) && *info == 0) *info = k + o + j + 1;
+        if (!(djj > 0.0) && bad < 0) bad = j;   // reported after the loop
         // 1/sqrt from v_rsq_f64 and two Newton steps (each doubles the
         // correct bits), l = d * (1/sqrt d): no IEEE sqrt and division
         // sequences on the column's dependent path
@@ -142,7 +143,6 @@ __device__ __forceinline__ void diag16(double (*S)[PBL], double *dg, double *col
         il = il * (1.5 - 0.5 * dd * il * il);
         il = il * (1.5 - 0.5 * dd * il * il);
         il_[j] = il;
-        if (lane == 0) dg[o + j] = il;
         const double l = dd * il;
         const double lrj = arj * il;   // L[o + r][o + j] (rows r > j)
         if (q == jq) d[jm] = r == j ? l : (r > j ? lrj : d[jm]);
@@ -157,6 +157,11 @@ __device__ __forceinline__ void diag16(double (*S)[PBL], double *dg, double *col
     for (int m = 0; m < 4; ++m) {
         const int c = 4 * m + q;
         if (c <= r) S[o + r][o + c] = d[m];
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int j = 0; j < DB; ++j) dg[o + j] = il_[j];
+        if (bad >= 0 && *info == 0) *info = k + o + bad + 1;
     }
     __builtin_amdgcn_wave_barrier();
     // X[a][b] = -X[a][a] sum_{b <= m < a} L[a][m] X[m][b] (row a of L read

@@ -416,3 +416,25 @@ def test_write_c_ref(D, oracle_mod, tmp_path):
     e = rel(tr, ref)
     print("c_ref.dat values rel err", e)
     assert e <= TOL64
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("w0", [0, 130])
+def test_ngd_reports_first_bad_pivot(D, oracle_mod, w0):
+    """A NaN in the rows of weight w0 makes row and column w0 of the metric
+    NaN: the factorization reports column w0 + 1 (1-based, as the pivot
+    check of the diagonal block finds it: w0 = 130 lies in the second
+    panel) with -5, and the next solve on the same work space, with clean
+    rows, gives the step of a fresh one."""
+    dims = [2, 16, 16, 1]
+    npt = 200
+    net, bw, rows_t, ld, b, o, w, x, terms = _ngd_inputs(D, oracle_mod, dims, npt)
+    u0 = bw.ngd_solve(rows_t, ld, npt, b, npt, 1e-3, 1)
+    bad = rows_t.clone()
+    bad[w0, 3] = float("nan")
+    with pytest.raises(D.DNNF90Error) as e:
+        bw.ngd_solve(bad, ld, npt, b, npt, 1e-3, 1)
+    assert e.value.rc == -5
+    assert f"column {w0 + 1}" in str(e.value), str(e.value)
+    u1 = bw.ngd_solve(rows_t, ld, npt, b, npt, 1e-3, 1)
+    assert rel(u1, u0) == 0.0
