@@ -335,12 +335,15 @@ __global__ __launch_bounds__(256) void k_linv_transpose(const double *__restrict
     for (int q = 0; q < 32; q += 8) LinvT[base + (long)(32 * bj + ty + q) * PB + 32 * bi + tx] = t[tx][ty + q];
 }
 
-// L21 = A21 Linv^T for 32 rows per workgroup (rows k+nb.. of the panel's
+// L21 = A21 Linv^T for PR rows per workgroup (rows k+nb.. of the panel's
 // columns k..k+nb-1); the rows are staged in LDS (all of a thread's loads
 // in flight together), Linv is read through L2 four k-steps ahead of its
-// products; wave (b, h) forms rows 16 b .. +15 by columns 64 h .. +63.  The
-// result is written back into G and packed into P[row][PB].
-static constexpr int PR = 32;   // rows a workgroup
+// products; wave (b, h) forms rows 16 b .. +15 by columns 128 / NCG
+// starting at h 128 / NCG (16 rows: each wave a quarter of the columns,
+// so that a workgroup's chain of products is half that of 32 rows and the
+// panels, mostly under one round of workgroups, finish sooner).  The
+// result is written back into G and packed into P[row][ldp].
+static constexpr int PR = 16;   // rows a workgroup
 
 __global__ __launch_bounds__(256) void k_chol_panel(double *__restrict__ A, long lda, int k, int nb, int n,
                                                     const double *__restrict__ Linv, double *__restrict__ P,
@@ -349,7 +352,8 @@ __global__ __launch_bounds__(256) void k_chol_panel(double *__restrict__ A, long
     __shared__ double Ar[PR][PBL];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int rb = wave >> 1, hc = wave & 1;
+    constexpr int NCG = 4 / (PR / 16);   // column groups: the waves split rows, then columns
+    const int rb = wave / NCG, hc = wave % NCG;
     const int r0 = k + nb + blockIdx.x * PR;
     {
         constexpr int NL = PR * PB / 256;
@@ -366,7 +370,7 @@ __global__ __launch_bounds__(256) void k_chol_panel(double *__restrict__ A, long
         }
     }
     __syncthreads();
-    constexpr int NJ = PB / 32;   // 16-column blocks a wave
+    constexpr int NJ = PB / (16 * NCG);   // 16-column blocks a wave
     dbl4 acc[NJ];
 #pragma unroll
     for (int jb = 0; jb < NJ; ++jb) acc[jb] = dbl4{0, 0, 0, 0};
